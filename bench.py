@@ -1,0 +1,233 @@
+#!/usr/bin/env python
+"""Benchmark: profile-fitted reflections/s of the MI355X integration path on one GPU.
+
+A step is one pass of the integration hot path over the whole sweep with the frames resident in HBM: for every phi
+block, pass 1 with in-kernel shoebox extraction and fused profile accumulation, the adaptive-width fits, the float64
+profile normalisation and pass 2 (Poisson profile fit).  Host planning (prediction, box plan, neighbour lists) and
+the upload of the reflection table happen once before the timed region, like the frames.
+
+Prints one JSON line (metric, value = reflections/s, roofline of the dominant kernel from HIP events on the launch
+stream, CPU baseline = the float64 oracle on a bounded sample timed on this host's cores, parity of that sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+PEAK_HBM_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def per_refl_bytes(V, Dz, K, kind):
+    """Algorithmic HBM bytes of one reflection in one launch: its shoebox voxels read once from the frame store as
+    uint32 (4 V), its table row (S, pS: 48; phi, pphi: 16; ix0, iy0, iz0, fb, pxs, region, has_p: 28; neighbour
+    columns 20 K), and its outputs (pass 1: 14 float64 + 3 int32 + fz 4 Dz; pass 2: 9 float64; pass 2 also reads
+    bg, nbg: 16)."""
+    table = 48 + 16 + 28 + 20 * K
+    if kind == "pass1":
+        return 4 * V + table + 14 * 8 + 3 * 4 + 4 * Dz
+    return 4 * V + table + 16 + 9 * 8
+
+
+def cpu_baseline(sweep, hp, spec, n_sample, seconds):
+    """The float64 oracle (oracle/kernels.py, numpy, one core) on a bounded sample of the same workload: the
+    shoeboxes of n_sample reflections of one block, extracted on the device and copied to the host, run through
+    pass 1, profile accumulation, normalisation and pass 2 (IRLS per-reflection rule); also the HIP results of the
+    same rows for parity."""
+    import numpy as np
+    import torch
+    from oracle import kernels as ok
+    from xpg_amd import integrate_kernels as K
+    from xpg_amd.geometry import detector_consts
+
+    sp = sweep.sp
+    # the largest group of the middle block
+    b_mid = hp["block_order"][len(hp["block_order"]) // 2]
+    gi, (B, bx, s), start, m = max(sweep.by_block[b_mid], key=lambda t: t[3])
+    m = min(m, n_sample)
+    view = sp.refl_view(start, m, (B, bx, bx))
+    sb = K.run_extract(sweep.store, view, sweep.device).cpu().numpy().astype(np.float64)
+    rows = sp.perm[start:start + m]
+    refl = hp["refl"]
+    exp = sweep.exp_blocks[b_mid]
+    g = exp.geom
+    det = detector_consts(g)
+    det.update(nx=g.nx, ny=g.ny)
+    rc = dict(S=refl["S"][rows], phi=np.radians(refl["phi"][rows]), ix0=hp["ix0"][rows], iy0=hp["iy0"][rows],
+              iz0=hp["iz0"][rows], fb=hp["fb"][rows].astype(float), region=hp["region"][rows],
+              has_p=hp["has_partner"][rows].astype(float), pS=refl["S"][hp["pj"][rows]],
+              pphi=np.radians(refl["phi"][hp["pj"][rows]]))
+    nb = {k: v[rows] for k, v in hp["nb"].items()}
+    sd, sm = np.radians(sweep.model.sigma_d), np.radians(sweep.model.sigma_m)
+    frames_phi0, osc = np.radians(g.z_to_phi(0.0)), np.radians(g.osc)
+    rcw = ok.rc_default_weights()
+    a = (rc, det, g.axis, g.s0, frames_phi0, osc, sd, sm, 1.0)
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        o1 = ok.pass1(sb, *a, nb, rcw)
+        strong = ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10) & (o1["n_over"] == 0)
+                  & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0) & (o1["n_fg_masked"] == 0))
+        num, den = ok.accumulate_profiles(o1["img2"], o1["all_valid2"], o1["J"], o1["u1"], o1["u2"], rc["region"],
+                                          o1["isum"], strong, o1["seff"], 9)
+        prof = ok.normalise_profiles(num, den)
+        o2 = ok.pass2(sb, *a, prof, o1["bg"], o1["nbg"], nb, rcw, irls_rule="reflection")
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > seconds or reps >= 50:
+            break
+    rate = reps * m / el
+    # the same rows on the device with the same inputs (default widths, these profiles)
+    dev = sweep.device
+    detd = K.device_detector(det, dev)
+    call = K.make_call(g.axis, g.s0, frames_phi0, osc, sd, sm, 1.0, rcw)
+    r1 = K.run_pass1(detd, call, view, frames=sweep.store, pixel_outputs=False)
+    r2 = K.run_pass2(detd, call, view, torch.as_tensor(prof, device=dev), r1["bg"], r1["nbg"], frames=sweep.store)
+    torch.cuda.synchronize(dev)
+    hI = r2["iprf"].cpu().numpy()
+    dI = np.abs(hI - o2["iprf"]) / np.sqrt(np.maximum(o2["vprf"], 1e-30))
+    ok_ = np.isfinite(dI)
+    parity = dict(sample=int(m), max_dI_over_sigma=float(np.max(dI[ok_])) if ok_.any() else None,
+                  median_dI_over_sigma=float(np.median(dI[ok_])) if ok_.any() else None,
+                  n_over_equal=bool(np.array_equal(r1["n_over"].cpu().numpy(), o1["n_over"])),
+                  n_fg_nb_equal=bool(np.array_equal(r1["n_fg_nb"].cpu().numpy(), o1["n_fg_nb"])))
+    base = dict(value=round(rate, 2), unit="reflections/s", cores=1, kind="port",
+                sample=f"{m} reflections ({B}x{bx}x{bx} shoeboxes, block {b_mid}) x {reps} repetitions, float64 "
+                       f"oracle pass1 + profile accumulation + normalisation + pass2 on shoeboxes extracted on the "
+                       f"device; {el:.1f} s")
+    return base, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress"])
+    ap.add_argument("--frames", type=int, default=0, help="override the number of frames (smaller runs)")
+    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one MI355X: every box here has one GPU, so --gpus must be 1 (multi-GPU sharding is "
+              "described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from xpg_amd import synth
+    from xpg_amd.integrate import IntegrationModel, Sweep, _integrate
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
+                stress=synth.spec_stress)[args.workload]()
+    if args.frames:
+        spec = synth.replace(spec, nframes=args.frames, name=f"{spec.name}[:{args.frames}]")
+    t = time.time()
+    frames, spots = synth.generate(spec, dev, log=log)
+    log(f"generated {spec.nframes} frames {spec.ny}x{spec.nx} ({frames.nbytes() / 1e9:.1f} GB) and "
+        f"{len(spots['x'])} spots in {time.time() - t:.1f}s")
+    exp = spots["exp"]
+    edges = synth.block_edges(spec)
+    model = IntegrationModel(spec.sigma_d, spec.sigma_m, box_xy=spec.box_xy, box_z=spec.box_z)
+    t = time.time()
+    hp = _integrate(frames, [exp] * (len(edges) - 1), edges, model, spots["dmin"], plan_only=True, verbose=False)
+    from xpg_amd.integrate import DZ_BUCKETS  # noqa: F401
+    opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
+                bkg_template=None, moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
+                sm=np.radians(model.sigma_m))
+    sweep = Sweep(frames, [exp] * (len(edges) - 1), model, hp, opts)
+    N = sweep.sp.n
+    log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups over {len(edges) - 1} blocks "
+        f"({time.time() - t:.1f}s)")
+
+    for i in range(args.warmup):
+        r = sweep.run()
+        log(f"warmup {i + 1}/{args.warmup}: {r['t_gpu'] * 1e3:.1f} ms")
+    launches = {}
+    events = []
+    stream = torch.cuda.current_stream(dev)
+
+    class Timer:
+        def __init__(self, kind, b, gi):
+            self.kind, self.gi = kind, gi
+
+        def __enter__(self):
+            self.e0 = torch.cuda.Event(enable_timing=True)
+            self.e1 = torch.cuda.Event(enable_timing=True)
+            self.e0.record(stream)
+
+        def __exit__(self, *a):
+            self.e1.record(stream)
+            events.append((self.kind, self.gi, self.e0, self.e1))
+            return False
+
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    last = None
+    for i in range(args.steps):
+        last = sweep.run(timer=Timer)
+    torch.cuda.synchronize(dev)
+    el = time.perf_counter() - t0
+    ms = el / args.steps * 1e3
+    integrated = int(torch.isfinite(last["o2"]["iprf"]).sum().item())
+    value = integrated / (el / args.steps)
+    log(f"timed {args.steps} steps: {ms:.1f} ms/step, {integrated} profile-fitted reflections/step")
+
+    # dominant kernel and its roofline (HIP events on the launch stream, algorithmic bytes per launch)
+    per = {}
+    K_nb = sweep.sp.K
+    for kind, gi, e0, e1 in events:
+        b, (B, bx, s), start, m = sweep.sp.groups[gi]
+        V = B * bx * bx
+        d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
+        d["ms"] += e0.elapsed_time(e1)
+        d["bytes"] += m * per_refl_bytes(V, B, K_nb, kind)
+        d["n"] += 1
+    dom = max(per, key=lambda k: per[k]["ms"])
+    dd = per[dom]
+    avg_ms = dd["ms"] / dd["n"]
+    avg_bytes = dd["bytes"] / dd["n"]
+    achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{dom}_{args.workload}.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    roof = dict(bound="hbm", kernel=f"k_{dom}", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, avg_launch_ms=round(avg_ms, 4),
+                alg_bytes_per_launch=round(avg_bytes), launches_per_step=dd["n"] // args.steps,
+                kernels={k: dict(ms_per_step=round(v["ms"] / args.steps, 3),
+                                 GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)) for k, v in per.items()})
+
+    cpu, parity = (None, None)
+    if not args.no_cpu:
+        t = time.time()
+        cpu, parity = cpu_baseline(sweep, hp, spec, args.cpu_sample, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']} refl/s ({time.time() - t:.1f}s); parity {parity}")
+
+    out = dict(metric="profile-fitted reflections/s on 1 MI355X", value=round(value, 1), unit="reflections/s",
+               n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 3), higher_is_better=True,
+               scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+               config=dict(workload=spec.name, form="plain: one pass-1 and one pass-2 launch per shoebox group per "
+                                                    "phi block, extraction fused, frames resident in HBM",
+                           frames=spec.nframes, frame_shape=[spec.ny, spec.nx], reflections=N,
+                           integrated=integrated, blocks=len(edges) - 1, box=[spec.box_z, spec.box_xy, spec.box_xy]
+                           if spec.box_xy else "reference rules", parallelism="single GPU"),
+               roofline=roof, cpu_baseline=cpu, parity=parity)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

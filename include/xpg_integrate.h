@@ -1,0 +1,163 @@
+/* C ABI of the MI355X (gfx950) shoebox-integration kernels.
+ *
+ * Plain device pointers, sizes and one hipStream_t per call (passed as void*); no torch types.  Every entry point
+ * is asynchronous on the given stream, allocates nothing and returns 0 on success or a negative xpg_status; the text
+ * of the last error of the calling thread is available from xpg_last_error().
+ *
+ * Conventions shared by all entry points (they mirror the tensors of xpg/integrate_kernels.py):
+ *   - a "chunk" is n reflections of one shoebox shape (Dz frames or phi-bins, Dy rows, Dx columns); shoebox values
+ *     are float32 value codes: -3 unrecorded, -2 overloaded, -1 masked, >= 0 counts (xpg/shoebox.py:11,
+ *     xpg/integrate_kernels.py:291-293), laid out (n, Dz, Dy, Dx) row-major;
+ *   - angles are radians, per-reflection scalars and vectors are float64, neighbour columns float32 (n, K);
+ *   - profile grids are (n_regions, 41, 41).
+ */
+#ifndef XPG_INTEGRATE_H
+#define XPG_INTEGRATE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPG_PROFILE_GRID 41
+#define XPG_RC_K 21
+#define XPG_MAX_DZ 65
+#define XPG_MAX_K 16
+
+enum xpg_status {
+  XPG_OK = 0,
+  XPG_ERR_ARG = -1,        /* invalid argument (shape, null pointer, size beyond a limit) */
+  XPG_ERR_HIP = -2,        /* a HIP runtime call failed */
+  XPG_ERR_TOO_LARGE = -3   /* shoebox does not fit the LDS-staged kernels */
+};
+
+/* Detector description: xpg/integrate_kernels.py:48-65 (_detector_consts).  Segment arrays are device float64 (K,3);
+ * segmap is device int16 (seg_ny, seg_nx) or NULL for one panel; gx/gy device float32 geometric-correction tables or
+ * NULL; bkg the device float32 background template (bkg_ny, bkg_nx) or NULL (flat). */
+typedef struct xpg_detector {
+  int32_t n_seg;
+  const double *P0, *Bx, *By, *U1, *U2, *UN;
+  double orgx, orgy, qx, qy, wl, tsens, mu, plx;
+  const int16_t *segmap; int32_t seg_nx, seg_ny;
+  const float *gx, *gy; int32_t geo_nx, geo_ny;
+  const float *bkg; int32_t bkg_nx, bkg_ny;
+} xpg_detector;
+
+/* Scalars of one kernel call: axis, s0 (1/A), frames_phi0 = phi at frame coordinate 0, osc per frame, sigma_D,
+ * sigma_M (all radians), gain, the 21 rocking-curve mixture weights (use_rcw = 0: plain Gaussian). */
+typedef struct xpg_call {
+  double axis[3], s0[3];
+  double frames_phi0, osc, sd, sm, gain;
+  float rcw[XPG_RC_K];
+  int32_t use_rcw;
+} xpg_call;
+
+/* Per-reflection table of a chunk (device pointers), xpg/integrate.py:509-517.  pxs may be NULL (1); has_p, pS,
+ * pphi may be NULL (no partner); K = 0 means no neighbour columns. */
+typedef struct xpg_refl {
+  int32_t n, Dz, Dy, Dx;
+  const double *S, *phi;
+  const int32_t *ix0, *iy0, *iz0, *fb, *pxs, *region;
+  const float *has_p;
+  const double *pS, *pphi;
+  int32_t K;
+  const float *nbx, *nby, *nbz, *nbsx, *nbsz;
+} xpg_refl;
+
+/* pass 1 outputs (xpg/integrate_kernels.py:362-365).  Per-reflection float64 / int32 (n); fz (n, Dz); per-pixel
+ * arrays (n, Dy, Dx).  Any per-pixel pointer may be NULL (not written). */
+typedef struct xpg_pass1_out {
+  double *bg, *nbg, *isum, *vsum, *vsum0, *xo, *yo, *zo, *m_d, *m_m, *frac_rec, *nfg, *zeta, *seff;
+  int32_t *n_over, *n_fg_masked, *n_fg_nb;
+  float *fz;
+  float *img2, *J, *u1, *u2;
+  uint8_t *all_valid2;
+} xpg_pass1_out;
+
+/* pass 2 outputs (xpg/integrate_kernels.py:429, :482), float64 (n). */
+typedef struct xpg_pass2_out {
+  double *iprf, *vprf, *vprf0, *corr, *p_valid, *p_total, *ipartner, *rho, *p_fg;
+} xpg_pass2_out;
+
+/* Fixed-point profile accumulation grids: int64 (n_regions, 41, 41) each, value * 2^32, zeroed by the caller.
+ * Integer addition is associative, so the grids are bit-identical for any launch order. */
+typedef struct xpg_profile_acc {
+  int64_t *num, *den;
+  int32_t n_regions;
+} xpg_profile_acc;
+
+/* Device frame store: uint32 raw counts (n_frames, ny, nx) for 0-based frames frame0 .. frame0 + n_frames - 1, and the
+ * value codes of xpg/io/images.py:80-96 with the integrator's mask ceiling (xpg/integrate.py:648-702).
+ * available: per store frame 1 = read, 0 = excluded / missing (NULL: all read).  src_mask: 0 = untrusted pixel
+ * (-1, wins over overload; NULL: none).  ceil_mask: 0 = min(value, -1) (a masked overload keeps -2; NULL: none).
+ * Raw values < min_valid or == flag_value are -1; >= overload are -2 (overload 0: none).  Float32 frames (dtype 1)
+ * carry the codes already (xpg/io/images.py ImageSet.get) and only src_mask / ceil_mask apply. */
+typedef struct xpg_frames {
+  const void *data;            /* dtype 0: uint32 raw counts; dtype 1: float32 frames already value-coded */
+  int64_t frame_stride;        /* elements between frames (>= ny * nx) */
+  int32_t nx, ny, frame0, n_frames;
+  const uint8_t *available, *src_mask, *ceil_mask;
+  uint32_t min_valid, flag_value, overload;
+  int32_t use_flag_value;
+  int32_t dtype;
+} xpg_frames;
+
+/* ---- entry points ------------------------------------------------------------------------------------------- */
+
+/* Library version string and the last error of the calling thread. */
+const char *xpg_version(void);
+const char *xpg_last_error(void);
+
+/* Replaces the host shoebox gather of xpg/integrate.py:704-803 (_lean_task / frame_loop) and the bin decoding of
+ * xpg/shoebox.py:264-268 (_decode_bins) with xpg/integrate.py:484-488: writes the value-coded shoeboxes sb
+ * (n, Dz, Dy, Dx) of a chunk from the device frame store, with s x s super-pixels (pxs) and phi-bins of fb frames. */
+int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void *stream);
+
+/* Replaces _decode_bins (xpg/shoebox.py:264-268): raw IEEE-encoded bin sums (+inf masked, NaN overloaded) and bin
+ * counts -> stored codes (value + 3, 0 = unrecorded), in place.  arr (n, B, bx, bx) float32, cnt (n, B) int16. */
+int xpg_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int32_t B, int32_t bxbx, void *stream);
+
+/* Replaces pass1 of make_kernels (xpg/integrate_kernels.py:287-365).  If acc is not NULL the strong-reflection
+ * profile accumulation of xpg/integrate.py:525-543 (accumulate_profiles on rows with I/sigma > 10, no overload,
+ * frac_rec > 0.95, no foreground pixel masked or claimed by a neighbour) is fused in. */
+int xpg_pass1(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+              const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream);
+
+/* xpg_pass1 reading the shoeboxes straight from the device frame store (extraction fused into the kernel, no sb
+ * buffer): same outputs as xpg_extract followed by xpg_pass1. */
+int xpg_pass1_frames(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                     const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream);
+
+/* Replaces accumulate_profiles of make_kernels (xpg/integrate_kernels.py:367-380) for the rows with sel != 0.
+ * img2, J, u1, u2 (n, Dy, Dx) float32; ok2 (n, Dy, Dx) uint8; region int32, isum / seff float64, sel uint8 (n). */
+int xpg_accumulate_profiles(const float *img2, const uint8_t *ok2, const float *J, const float *u1, const float *u2,
+                            const int32_t *region, const double *isum, const uint8_t *sel, const double *seff,
+                            int32_t n, int32_t Dy, int32_t Dx, const xpg_profile_acc *acc, void *stream);
+
+/* Replaces the float64 profile normalisation of xpg/integrate.py:546-556: fixed-point grids -> float32 profiles
+ * (n_regions, 41, 41) regularised towards gauss (41, 41) with 3 pseudo-counts; num64/den64 (may be NULL) receive
+ * the float64 grids. */
+int xpg_normalise_profiles(const xpg_profile_acc *acc, const float *gauss, float *prof, double *num64, double *den64,
+                           void *stream);
+
+/* Replaces pass2 of make_kernels (xpg/integrate_kernels.py:382-483): Poisson-weighted profile fit with the 2x2
+ * overlap deconvolution.  bg, nbg from pass 1 (float64, n); prof (n_regions, 41, 41) float32.  irls_rule 0: each
+ * reflection stops on its own convergence test (independent of which reflections share the call); 1: the chunk-global
+ * rule of xpg/integrate_kernels.py:464-467 evaluated over this call's n reflections. */
+int xpg_pass2(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+              const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+              const xpg_pass2_out *out, void *stream);
+
+/* xpg_pass2 reading the shoeboxes straight from the device frame store: same outputs as xpg_extract + xpg_pass2. */
+int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                     const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+                     const xpg_pass2_out *out, void *stream);
+
+/* Bytes of LDS one wavefront of pass 1 / pass 2 uses for a (Dz, Dy, Dx) shoebox (launch planning, autotuning). */
+int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

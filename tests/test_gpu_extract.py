@@ -1,0 +1,73 @@
+"""Device shoebox extraction (xpg_extract and the extraction fused into pass 1 / pass 2) against the extraction oracle:
+bit-exact value codes, with dead / overloaded pixels, the integrator's mask, an excluded frame, phi-bins and
+super-pixels."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import extract as ox
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(seed=4, ny=23, nx=31, nfr=14):
+    rng = np.random.default_rng(seed)
+    raw = rng.poisson(3.0, (nfr, ny, nx)).astype(np.uint32)
+    raw[rng.random(raw.shape) < 0.03] = 2 ** 32 - 1
+    raw[rng.random(raw.shape) < 0.02] = 70000
+    mask = rng.random((ny, nx)) > 0.05
+    coded = {fi: ox.frame_codes(raw[fi], flag_value=2 ** 32 - 1, overload=65000) for fi in range(nfr)}
+    return rng, raw, mask, coded
+
+
+@pytest.mark.parametrize("fbv,s,B,bx", [(1, 1, 5, 7), (2, 1, 3, 5), (1, 2, 3, 5), (3, 2, 3, 5), (1, 1, 9, 9)])
+def test_extract_matches_oracle(fbv, s, B, bx):
+    from xpg_amd.frames import DeviceFrames
+    from xpg_amd.integrate_kernels import DeviceRefl, run_extract
+    rng, raw, mask, coded = _setup()
+    nfr, ny, nx = raw.shape
+    dev = torch.device("cuda:0")
+    frames = DeviceFrames(torch.as_tensor(raw.view(np.int32), device=dev), first=1, overload=65000,
+                          flag_value=2 ** 32 - 1)
+    store = frames.store(mask=mask, exclude_frames=(7,))          # 1-based frame 7 = index 6
+    n, h = 40, bx // 2
+    iz0 = rng.integers(-3, nfr - 2, n)
+    ix0 = rng.integers(0, (nx - 1) // s + 1, n) - h
+    iy0 = rng.integers(0, (ny - 1) // s + 1, n) - h
+    fb = np.full(n, fbv)
+    rc = dict(S=np.zeros((n, 3)), phi=np.zeros(n), ix0=ix0, iy0=iy0, iz0=iz0, fb=fb, pxs=np.full(n, s),
+              region=np.zeros(n, int))
+    refl = DeviceRefl(rc, None, (B, bx, bx), dev)
+    print(f"extract fb={fbv} s={s} B={B} bx={bx} n={n}", flush=True)
+    sb = run_extract(store, refl, dev).cpu().numpy()
+    read = [f for f in range(nfr) if f != 6]
+    want = ox.gather(coded, read, ix0, iy0, iz0, fb, s, B, bx, mask)
+    assert np.array_equal(sb, want)
+
+
+def test_fused_extraction_gives_identical_results():
+    """pass 1 and pass 2 reading the frame store directly == extraction into sb followed by the sb kernels."""
+    import sys
+    from chunks import make_chunk
+    from xpg_amd import integrate_kernels as K
+    from xpg_amd.frames import DeviceFrames
+    dev = torch.device("cuda:0")
+    c = make_chunk(n=64, seed=21)
+    # frames that contain the chunk's boxes: place each box's voxels into a frame store
+    nfr, ny, nx = 420, 1024, 1024
+    store_np = np.random.default_rng(1).poisson(2.0, (nfr, ny, nx)).astype(np.uint32)
+    frames = DeviceFrames(torch.as_tensor(store_np.view(np.int32), device=dev), first=1)
+    store = frames.store()
+    det = K.device_detector(c["det"], dev)
+    call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"], c["rcw"])
+    refl = K.DeviceRefl(c["rc"], c["nb"], (7, 11, 11), dev)
+    sb = K.run_extract(store, refl, dev)
+    a = K.run_pass1(det, call, refl, sb=sb)
+    b = K.run_pass1(det, call, refl, frames=store)
+    for k in K.P1_F64 + K.P1_I32 + ("fz", "img2"):
+        assert torch.equal(a[k], b[k]), k
+    prof = torch.as_tensor(c["prof"], device=dev)
+    a2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], sb=sb)
+    b2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], frames=store)
+    for k in K.P2_F64:
+        assert torch.equal(a2[k], b2[k]), k

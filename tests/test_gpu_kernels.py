@@ -1,0 +1,154 @@
+"""Parity of the HIP kernels (through the C ABI) with the FP64 oracle, on the golden chunks and seeded synthetic ones.
+
+Bars: integer masks and counts bit-exact; |dI|/sigma <= 1e-3 for the profile-fitted and summation intensities
+(observed ~1e-6); other float outputs within the relative tolerances stated per key.
+"""
+import numpy as np
+import pytest
+import torch
+
+from golden_io import cases, load
+from oracle import kernels as ok
+
+pytestmark = pytest.mark.gpu
+
+P1_TOL = dict(bg=1e-6, nbg=1e-9, isum=None, vsum=1e-6, vsum0=1e-6, xo=1e-5, yo=1e-5, zo=1e-5, frac_rec=1e-9,
+              zeta=1e-12, seff=1e-12)
+P2_TOL = dict(vprf=1e-5, vprf0=1e-5, corr=1e-4, p_valid=1e-5, p_total=1e-5, rho=1e-5, p_fg=1e-5)
+
+
+def _args(c):
+    return (c["rc"], c["det"], c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"])
+
+
+def _rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = np.maximum(np.abs(b), np.max(np.abs(b)) * 1e-3 + 1e-300)
+    return float(np.max(np.abs(a - b) / scale))
+
+
+def _run(c, dev):
+    from xpg_amd.integrate_kernels import make_kernels
+    p1, acc, p2 = make_kernels(c["Dz"], c["Dy"], c["Dx"], 9)
+    sb = torch.as_tensor(c["sb"], dtype=torch.float32, device=dev)
+    r1 = p1(sb, *_args(c), c["prof"], c["nb"], c["rcw"])
+    r2 = p2(sb, *_args(c), c["prof"], r1["bg"], r1["nbg"], c["nb"], c["rcw"])
+    torch.cuda.synchronize()
+    return {k: v.cpu().numpy() for k, v in r1.items()}, {k: v.cpu().numpy() for k, v in r2.items()}
+
+
+def _check_pass1(h1, o1, name):
+    for k in ("n_over", "n_fg_masked", "n_fg_nb"):
+        assert np.array_equal(h1[k], np.asarray(o1[k])), (name, k)
+    assert np.array_equal(h1["nfg"], o1["nfg"]), name
+    assert np.array_equal(h1["all_valid2"], o1["all_valid2"]), name
+    for k, tol in P1_TOL.items():
+        if tol is not None:
+            assert _rel(h1[k], o1[k]) < tol, (name, k, _rel(h1[k], o1[k]))
+    sig = np.sqrt(np.maximum(o1["vsum"], 1e-6))
+    assert np.max(np.abs(h1["isum"] - o1["isum"]) / sig) < 1e-4, name
+    for k in ("J", "u1", "u2"):
+        assert _rel(h1[k], o1[k]) < 1e-6, (name, k)
+    scale = np.max(np.abs(o1["img2"]))
+    assert np.max(np.abs(h1["img2"] - o1["img2"])) < 1e-5 * scale, name
+    assert np.max(np.abs(h1["fz"] - o1["fz"])) < 1e-5 * np.max(np.abs(o1["fz"])), name
+    for k in ("m_d", "m_m"):
+        assert np.max(np.abs(h1[k] - o1[k])) < 1e-4 * np.max(np.abs(o1[k])), (name, k)
+
+
+def _check_pass2(h2, o2, name):
+    sig = np.sqrt(o2["vprf"])
+    dI = np.abs(h2["iprf"] - o2["iprf"]) / sig
+    assert dI.max() < 1e-3, (name, dI.max())
+    for k, tol in P2_TOL.items():
+        assert _rel(h2[k], o2[k]) < tol, (name, k, _rel(h2[k], o2[k]))
+    hp = np.asarray(o2["ipartner"]) != 0
+    if hp.any():
+        assert np.max(np.abs(h2["ipartner"][hp] - o2["ipartner"][hp]) / sig[hp]) < 1e-3, name
+    return dI.max()
+
+
+@pytest.mark.parametrize("name", cases())
+def test_kernels_match_oracle_on_golden_chunks(name):
+    c, _ = load(name)
+    h1, h2 = _run(c, torch.device("cuda:0"))
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    _check_pass1(h1, o1, name)
+    o2 = ok.pass2(c["sb"], *_args(c), c["prof"], o1["bg"], o1["nbg"], c["nb"], c["rcw"], irls_rule="reflection")
+    d = _check_pass2(h2, o2, name)
+    print(f"{name}: max |dI|/sigma {d:.2e}")
+
+
+@pytest.mark.parametrize("seed,shape,kw", [
+    (101, (7, 11, 11), {}),
+    (102, (9, 9, 9), dict(fb=3)),
+    (103, (5, 13, 13), dict(pxs=3)),
+    (104, (17, 21, 21), dict(intensity=(10.0, 1e6))),
+    (105, (33, 9, 9), dict(sm_deg=0.4)),
+    (106, (5, 9, 9), dict(intensity=(1.0, 2 ** 20 * 20), background=0.5)),
+])
+def test_kernels_match_oracle_synthetic(seed, shape, kw):
+    from chunks import make_chunk
+    c = make_chunk(n=96, Dz=shape[0], Dy=shape[1], Dx=shape[2], seed=seed, **kw)
+    h1, h2 = _run(c, torch.device("cuda:0"))
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    _check_pass1(h1, o1, seed)
+    o2 = ok.pass2(c["sb"], *_args(c), c["prof"], o1["bg"], o1["nbg"], c["nb"], c["rcw"], irls_rule="reflection")
+    _check_pass2(h2, o2, seed)
+
+
+def test_accumulate_and_normalise_match_oracle():
+    from xpg_amd.integrate_kernels import make_kernels, new_profile_acc, normalise_profiles
+    c, _ = load("a_7x11x11")
+    dev = torch.device("cuda:0")
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    sel = np.asarray(o1["isum"]) > 0
+    f32 = lambda a: np.asarray(a, np.float32)
+    _, acc, _ = make_kernels(c["Dz"], c["Dy"], c["Dx"], 9)
+    T = lambda a, dt: torch.as_tensor(np.asarray(a), dtype=dt, device=dev)
+    num, den = acc(T(f32(o1["img2"]), torch.float32), T(o1["all_valid2"], torch.uint8), T(f32(o1["J"]), torch.float32),
+                   T(f32(o1["u1"]), torch.float32), T(f32(o1["u2"]), torch.float32),
+                   T(c["rc"]["region"], torch.int32), T(o1["isum"], torch.float64), T(sel, torch.uint8),
+                   T(o1["seff"], torch.float64))
+    # oracle on the same float32-rounded per-pixel inputs
+    onum, oden = ok.accumulate_profiles(f32(o1["img2"]), o1["all_valid2"], f32(o1["J"]), f32(o1["u1"]), f32(o1["u2"]),
+                                        c["rc"]["region"], o1["isum"], sel, o1["seff"], 9)
+    num, den = num.cpu().numpy(), den.cpu().numpy()
+    assert np.max(np.abs(num - onum)) < 1e-8 * max(np.max(np.abs(onum)), 1)
+    assert np.max(np.abs(den - oden)) < 1e-8 * max(np.max(np.abs(oden)), 1)
+    # normalisation on the device against the float64 recipe
+    num_fx = torch.as_tensor(np.rint(onum * 2 ** 32).astype(np.int64), device=dev)
+    den_fx = torch.as_tensor(np.rint(oden * 2 ** 32).astype(np.int64), device=dev)
+    prof, _, _ = normalise_profiles(num_fx, den_fx, dev)
+    oprof = ok.normalise_profiles(onum, oden)
+    assert np.max(np.abs(prof.cpu().numpy() - oprof)) < 1e-6 * np.max(oprof)
+
+
+def test_fused_accumulation_matches_strong_selection():
+    """pass1 with the profile grids fused in: the strong rows of xpg/integrate.py:530-541 splatted on the device."""
+    from chunks import make_chunk
+    from xpg_amd import integrate_kernels as K
+    c = make_chunk(n=200, seed=77, intensity=(500.0, 1e5))
+    dev = torch.device("cuda:0")
+    det = K.device_detector(c["det"], dev)
+    call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"], c["rcw"])
+    refl = K.DeviceRefl(c["rc"], c["nb"], (7, 11, 11), dev)
+    num_fx, den_fx, acc = K.new_profile_acc(9, dev)
+    sb = torch.as_tensor(c["sb"], dtype=torch.float32, device=dev)
+    K.run_pass1(det, call, refl, sb=sb, acc=acc)
+    torch.cuda.synchronize()
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    strong = ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10) & (o1["n_over"] == 0)
+              & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0) & (o1["n_fg_masked"] == 0))
+    assert strong.sum() > 20
+    onum, oden = ok.accumulate_profiles(o1["img2"], o1["all_valid2"], o1["J"], o1["u1"], o1["u2"], c["rc"]["region"],
+                                        o1["isum"], strong, o1["seff"], 9)
+    num = num_fx.cpu().numpy() / 2 ** 32
+    den = den_fx.cpu().numpy() / 2 ** 32
+    assert np.max(np.abs(den - oden)) < 1e-6 * np.max(oden)
+    assert np.max(np.abs(num - onum)) < 1e-5 * np.max(np.abs(onum))
+    # bit-identical on a second launch (integer accumulation is order-independent)
+    num2, den2, acc2 = K.new_profile_acc(9, dev)
+    K.run_pass1(det, call, refl, sb=sb, acc=acc2)
+    torch.cuda.synchronize()
+    assert torch.equal(num2, num_fx) and torch.equal(den2, den_fx)

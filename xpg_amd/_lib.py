@@ -1,0 +1,131 @@
+"""ctypes binding of libxpg_integrate.so (include/xpg_integrate.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
+is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
+below carry their raw pointers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+import torch
+
+LIB_PATH = Path(__file__).resolve().parent / "libxpg_integrate.so"
+RC_K = 21
+
+
+class XpgError(RuntimeError):
+    pass
+
+
+class Detector(C.Structure):
+    _fields_ = [("n_seg", C.c_int32), ("P0", C.c_void_p), ("Bx", C.c_void_p), ("By", C.c_void_p),
+                ("U1", C.c_void_p), ("U2", C.c_void_p), ("UN", C.c_void_p),
+                ("orgx", C.c_double), ("orgy", C.c_double), ("qx", C.c_double), ("qy", C.c_double),
+                ("wl", C.c_double), ("tsens", C.c_double), ("mu", C.c_double), ("plx", C.c_double),
+                ("segmap", C.c_void_p), ("seg_nx", C.c_int32), ("seg_ny", C.c_int32),
+                ("gx", C.c_void_p), ("gy", C.c_void_p), ("geo_nx", C.c_int32), ("geo_ny", C.c_int32),
+                ("bkg", C.c_void_p), ("bkg_nx", C.c_int32), ("bkg_ny", C.c_int32)]
+
+
+class Call(C.Structure):
+    _fields_ = [("axis", C.c_double * 3), ("s0", C.c_double * 3), ("frames_phi0", C.c_double), ("osc", C.c_double),
+                ("sd", C.c_double), ("sm", C.c_double), ("gain", C.c_double), ("rcw", C.c_float * RC_K),
+                ("use_rcw", C.c_int32)]
+
+
+class Refl(C.Structure):
+    _fields_ = [("n", C.c_int32), ("Dz", C.c_int32), ("Dy", C.c_int32), ("Dx", C.c_int32),
+                ("S", C.c_void_p), ("phi", C.c_void_p), ("ix0", C.c_void_p), ("iy0", C.c_void_p),
+                ("iz0", C.c_void_p), ("fb", C.c_void_p), ("pxs", C.c_void_p), ("region", C.c_void_p),
+                ("has_p", C.c_void_p), ("pS", C.c_void_p), ("pphi", C.c_void_p), ("K", C.c_int32),
+                ("nbx", C.c_void_p), ("nby", C.c_void_p), ("nbz", C.c_void_p), ("nbsx", C.c_void_p),
+                ("nbsz", C.c_void_p)]
+
+
+class Pass1Out(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m",
+                                          "frac_rec", "nfg", "zeta", "seff", "n_over", "n_fg_masked", "n_fg_nb",
+                                          "fz", "img2", "J", "u1", "u2", "all_valid2")]
+
+
+class Pass2Out(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("iprf", "vprf", "vprf0", "corr", "p_valid", "p_total", "ipartner", "rho",
+                                          "p_fg")]
+
+
+class ProfileAcc(C.Structure):
+    _fields_ = [("num", C.c_void_p), ("den", C.c_void_p), ("n_regions", C.c_int32)]
+
+
+class Frames(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("frame_stride", C.c_int64), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("frame0", C.c_int32), ("n_frames", C.c_int32), ("available", C.c_void_p),
+                ("src_mask", C.c_void_p), ("ceil_mask", C.c_void_p), ("min_valid", C.c_uint32),
+                ("flag_value", C.c_uint32), ("overload", C.c_uint32), ("use_flag_value", C.c_int32),
+                ("dtype", C.c_int32)]
+
+
+P = C.POINTER
+_SIGS = {
+    "xpg_version": ([], C.c_char_p),
+    "xpg_last_error": ([], C.c_char_p),
+    "xpg_extract": ([P(Frames), P(Refl), C.c_void_p, C.c_void_p], C.c_int),
+    "xpg_decode_bins": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "xpg_pass1": ([P(Detector), P(Call), P(Refl), C.c_void_p, P(Pass1Out), P(ProfileAcc), C.c_void_p], C.c_int),
+    "xpg_pass1_frames": ([P(Detector), P(Call), P(Refl), P(Frames), P(Pass1Out), P(ProfileAcc), C.c_void_p],
+                         C.c_int),
+    "xpg_accumulate_profiles": ([C.c_void_p] * 9 + [C.c_int32] * 3 + [P(ProfileAcc), C.c_void_p], C.c_int),
+    "xpg_normalise_profiles": ([P(ProfileAcc), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "xpg_pass2": ([P(Detector), P(Call), P(Refl), C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                   C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
+    "xpg_pass2_frames": ([P(Detector), P(Call), P(Refl), P(Frames), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                          C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
+    "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
+    "xpg_synth_background": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "xpg_synth_spots_add": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+}
+EXPORTS = tuple(_SIGS)
+
+_lib = None
+
+
+def lib():
+    """The loaded library (raises when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise XpgError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = h
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise XpgError(f"xpg kernel call failed ({rc}): {lib().xpg_last_error().decode()}")
+
+
+def ptr(t):
+    """Raw device pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    if not t.is_contiguous():
+        raise XpgError("kernel arguments must be contiguous")
+    return C.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_device(t):
+    if not (torch.is_tensor(t) and t.is_cuda):
+        raise XpgError("the integration kernels run on the GPU: device tensors expected")

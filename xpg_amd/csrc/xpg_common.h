@@ -1,0 +1,361 @@
+// Device helpers shared by the integration kernels (gfx950, wave64).
+//
+// Precision plan: per-reflection, per-pixel and per-frame geometry (Kabsch frame, eps, pixel area J, rocking-curve
+// fractions R, masks) in float64, in the operation order of the float64 oracle so that every threshold-derived mask
+// is identical; per-voxel arithmetic in float32 with compensated (TwoSum) accumulation.  Compiled with
+// -ffp-contract=off: no fused multiply-adds change the float64 roundings.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "../../include/xpg_integrate.h"
+
+namespace xpg {
+
+constexpr int WAVE = 64;
+constexpr double PIX0 = 1.0;        // coordinate of the centre of array element 0
+constexpr double UMAX = 4.5;        // N_BOX * 0.9: half-width of the profile grid (sigma units)
+constexpr int G = XPG_PROFILE_GRID;
+constexpr int RCK = XPG_RC_K;
+constexpr double FX_SCALE = 4294967296.0;   // 2^32: fixed-point profile grids
+constexpr double VAR_FLOOR = 1e-3;
+constexpr int N_IRLS = 6;
+constexpr double IRLS_TOL = 1e-3;
+
+// rocking-curve mixture centres linspace(-3.5, 3.5, 21) as float32 values (the reference's device tensor)
+__device__ __constant__ static const float RC_C32[RCK] = {
+    -3.5f, -3.1500000953674316f, -2.799999952316284f, -2.450000047683716f, -2.0999999046325684f, -1.75f,
+    -1.399999976158142f, -1.0499999523162842f, -0.699999988079071f, -0.3499999940395355f, 0.0f,
+    0.3499999940395355f, 0.699999988079071f, 1.0499999523162842f, 1.399999976158142f, 1.75f,
+    2.0999999046325684f, 2.450000047683716f, 2.799999952316284f, 3.1500000953674316f, 3.5f};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---- compensated float32 sums -----------------------------------------------------------------------------------
+
+struct csum {
+  float s, c;
+};
+__device__ __forceinline__ csum cz() { return csum{0.f, 0.f}; }
+__device__ __forceinline__ void cadd(csum &a, float x) {
+  float t = a.s + x;
+  float bp = t - a.s;
+  float e = (a.s - (t - bp)) + (x - bp);
+  a.s = t;
+  a.c += e;
+}
+__device__ __forceinline__ double creduce(csum a) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) {
+    float s2 = __shfl_xor(a.s, off);
+    float c2 = __shfl_xor(a.c, off);
+    float t = a.s + s2;
+    float bp = t - a.s;
+    float e = (a.s - (t - bp)) + (s2 - bp);
+    a.s = t;
+    a.c = (a.c + c2) + e;
+  }
+  return (double)a.s + (double)a.c;
+}
+__device__ __forceinline__ double wsum_d(double v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ int wsum_i(int v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ double wmax_d(double v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ double wmin_d(double v) {
+  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0; }
+__device__ __forceinline__ bool wall(bool b) { return __ballot(!b) == 0; }
+
+// ---- float64 vector helpers (left-to-right order) ---------------------------------------------------------------
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// ---- detector: S of a detector point (xpg/integrate_kernels.py:78-105) -------------------------------------------
+
+__device__ __forceinline__ double bilinear_table(const float *tab, int nx, int ny, double x, double y) {
+  double fx = fmin(fmax(x - 1.0, 0.0), (double)nx - 1.000001);
+  double fy = fmin(fmax(y - 1.0, 0.0), (double)ny - 1.000001);
+  int ix = (int)floor(fx), iy = (int)floor(fy);
+  int ix1 = min(ix + 1, nx - 1), iy1 = min(iy + 1, ny - 1);
+  double tx = fx - ix, ty = fy - iy;
+  double a = (double)tab[(int64_t)iy * nx + ix], b = (double)tab[(int64_t)iy * nx + ix1];
+  double c = (double)tab[(int64_t)iy1 * nx + ix], d = (double)tab[(int64_t)iy1 * nx + ix1];
+  return (((a * (1 - tx)) * (1 - ty) + (b * tx) * (1 - ty)) + (c * (1 - tx)) * ty) + (d * tx) * ty;
+}
+
+__device__ void pixel_S(const xpg_detector &d, double x, double y, double *S) {
+  int k = 0;
+  if (d.segmap) {
+    long ix = (long)floor(x - 0.5), iy = (long)floor(y - 0.5);
+    ix = ix < 0 ? 0 : (ix > d.seg_nx - 1 ? d.seg_nx - 1 : ix);
+    iy = iy < 0 ? 0 : (iy > d.seg_ny - 1 ? d.seg_ny - 1 : iy);
+    k = max((int)d.segmap[iy * d.seg_nx + ix], 0);
+  }
+  const double *P0 = d.P0 + 3 * k, *Bx = d.Bx + 3 * k, *By = d.By + 3 * k;
+  double xc = x, yc = y;
+  if (d.gx) {
+    xc = x + bilinear_table(d.gx, d.geo_nx, d.geo_ny, x, y);
+    yc = y + bilinear_table(d.gy, d.geo_nx, d.geo_ny, x, y);
+  }
+  double X[3];
+  {
+    double a = xc - d.orgx, b = yc - d.orgy;
+    for (int i = 0; i < 3; ++i) X[i] = (P0[i] + a * Bx[i]) + b * By[i];
+  }
+  if (d.plx != 0.0) {
+    const double *U1 = d.U1 + 3 * k, *U2 = d.U2 + 3 * k, *UN = d.UN + 3 * k;
+    for (int it = 0; it < 2; ++it) {
+      double nrm = sqrt(dot3(X, X));
+      double e[3] = {X[0] / nrm, X[1] / nrm, X[2] / nrm};
+      double cosa = fmin(fmax(dot3(e, UN), 1e-3), 1.0);
+      double L = d.tsens / cosa;
+      double muL = d.mu * L;
+      double lbar = 1.0 / d.mu - (L * exp(-muL)) / fmax(-expm1(-muL), 1e-12);
+      double d1 = lbar * dot3(e, U1), d2 = lbar * dot3(e, U2);
+      double a = (xc - (d.plx * d1) / d.qx) - d.orgx, b = (yc - (d.plx * d2) / d.qy) - d.orgy;
+      for (int i = 0; i < 3; ++i) X[i] = (P0[i] + a * Bx[i]) + b * By[i];
+    }
+  }
+  double den = sqrt(dot3(X, X)) * d.wl;
+  S[0] = X[0] / den;
+  S[1] = X[1] / den;
+  S[2] = X[2] / den;
+}
+
+// Kabsch frame of one reflection: e1 = S x s0 / |.|, e2 = S x e1 / |.|, zeta = e1 . axis (xpg/integrate_kernels.py:234-238)
+struct Frame {
+  double S[3], e1[3], e2[3], invS, zeta;
+};
+__device__ __forceinline__ void kabsch(const double *S, const double *s0, const double *axis, Frame &f) {
+  double c1[3], c2[3];
+  for (int i = 0; i < 3; ++i) f.S[i] = S[i];
+  cross3(S, s0, c1);
+  double n1 = sqrt(dot3(c1, c1));
+  for (int i = 0; i < 3; ++i) f.e1[i] = c1[i] / n1;
+  cross3(S, f.e1, c2);
+  double n2 = sqrt(dot3(c2, c2));
+  for (int i = 0; i < 3; ++i) f.e2[i] = c2[i] / n2;
+  f.zeta = dot3(f.e1, axis);
+  f.invS = 1.0 / sqrt(dot3(S, S));
+}
+
+// eps1, eps2 of detector point (x, y) (xpg/integrate_kernels.py:108-115)
+__device__ __forceinline__ void eps12(const xpg_detector &d, const Frame &f, double x, double y, double &e1,
+                                      double &e2) {
+  double Sp[3], dd[3];
+  pixel_S(d, x, y, Sp);
+  for (int i = 0; i < 3; ++i) dd[i] = Sp[i] - f.S[i];
+  e1 = dot3(dd, f.e1) * f.invS;
+  e2 = dot3(dd, f.e2) * f.invS;
+}
+
+// ---- rocking curve (xpg/integrate_kernels.py:130-141), one mixture term ------------------------------------------
+
+// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when use_rcw == 0 (j == 0 only)
+__device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double hi, double phi, double zeta, int j) {
+  double s = c.sm / fmax(fabs(zeta), 1e-3);
+  if (!c.use_rcw) {
+    double a = (hi - phi) / (s * sqrt(2.0));
+    double b = (lo - phi) / (s * sqrt(2.0));
+    return 0.5 * (erf(a) - erf(b));
+  }
+  double sw = (s * 0.25) * sqrt(2.0);
+  double cj = (double)RC_C32[j];
+  double a = ((hi - phi) - cj * s) / sw;
+  double b = ((lo - phi) - cj * s) / sw;
+  return ((double)c.rcw[j] * 0.5) * (erf(a) - erf(b));
+}
+
+__device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb, int j, double &lo, double &hi) {
+  double k = (double)iz0 + (double)fb * (double)j;
+  double plo = c.frames_phi0 + c.osc * k;
+  double phi_ = plo + c.osc * (double)fb;
+  lo = fmin(plo, phi_);
+  hi = fmax(plo, phi_);
+}
+
+// R[j] for all frames of a reflection into R (LDS), using scratch (>= Dz * 21 doubles); the mixture terms are summed
+// in component order as the oracle does.
+__device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, double zeta, int Dz, double *scratch,
+                              double *R, int lane) {
+  const int nt = c.use_rcw ? RCK : 1;
+  for (int t = lane; t < Dz * nt; t += WAVE) {
+    int j = t / nt, q = t - j * nt;
+    double lo, hi;
+    frame_bounds(c, iz0, fb, j, lo, hi);
+    scratch[t] = rc_term(c, lo, hi, phi, zeta, q);
+  }
+  wave_sync();
+  for (int j = lane; j < Dz; j += WAVE) {
+    double out = 0.0;
+    if (nt == 1) {
+      out = scratch[j];
+    } else {
+      for (int q = 0; q < nt; ++q) out = out + scratch[j * nt + q];
+    }
+    R[j] = out;
+  }
+  wave_sync();
+}
+
+// ---- per-reflection eps / J / seff on the box grid (xpg/integrate_kernels.py:224-272) -----------------------------
+
+__device__ __forceinline__ double pix_x(int ix0, int i, int s) {
+  return ((double)ix0 + (double)i) * (double)s + ((double)s - 1.0) * 0.5 + PIX0;
+}
+
+// Writes u1 = eps1/seff, u2 = eps2/seff (float64) into U1, U2 (P each) and J into Jd (float64, P); returns seff.
+// Small boxes (a side < 3): J by central differences at the pixel edges.
+__device__ double box_geometry(const xpg_detector &d, const Frame &f, int ix0, int iy0, int s, int Dy, int Dx,
+                               double sd, double *U1, double *U2, double *Jd, int lane) {
+  const int P = Dy * Dx;
+  for (int p = lane; p < P; p += WAVE) {
+    int iy = p / Dx, ix = p - iy * Dx;
+    double x = pix_x(ix0, ix, s), y = pix_x(iy0, iy, s);
+    double a, b;
+    eps12(d, f, x, y, a, b);
+    U1[p] = a;
+    U2[p] = b;
+  }
+  wave_sync();
+  if (Dx >= 3 && Dy >= 3) {
+    for (int p = lane; p < P; p += WAVE) {
+      int iy = p / Dx, ix = p - iy * Dx;
+      double g1x, g1y, g2x, g2y;
+      // d/dx
+      if (ix == 0) {
+        g1x = (-1.5 * U1[p] + 2.0 * U1[p + 1]) + -0.5 * U1[p + 2];
+        g2x = (-1.5 * U2[p] + 2.0 * U2[p + 1]) + -0.5 * U2[p + 2];
+      } else if (ix == Dx - 1) {
+        g1x = (0.5 * U1[p - 2] + -2.0 * U1[p - 1]) + 1.5 * U1[p];
+        g2x = (0.5 * U2[p - 2] + -2.0 * U2[p - 1]) + 1.5 * U2[p];
+      } else {
+        g1x = (U1[p + 1] - U1[p - 1]) / 2.0;
+        g2x = (U2[p + 1] - U2[p - 1]) / 2.0;
+      }
+      if (iy == 0) {
+        g1y = (-1.5 * U1[p] + 2.0 * U1[p + Dx]) + -0.5 * U1[p + 2 * Dx];
+        g2y = (-1.5 * U2[p] + 2.0 * U2[p + Dx]) + -0.5 * U2[p + 2 * Dx];
+      } else if (iy == Dy - 1) {
+        g1y = (0.5 * U1[p - 2 * Dx] + -2.0 * U1[p - Dx]) + 1.5 * U1[p];
+        g2y = (0.5 * U2[p - 2 * Dx] + -2.0 * U2[p - Dx]) + 1.5 * U2[p];
+      } else {
+        g1y = (U1[p + Dx] - U1[p - Dx]) / 2.0;
+        g2y = (U2[p + Dx] - U2[p - Dx]) / 2.0;
+      }
+      Jd[p] = fabs(g1x * g2y - g2x * g1y);
+    }
+  } else {
+    for (int p = lane; p < P; p += WAVE) {
+      int iy = p / Dx, ix = p - iy * Dx;
+      double x = pix_x(ix0, ix, s), y = pix_x(iy0, iy, s);
+      double a1, a2, b1, b2, c1, c2, d1, d2;
+      eps12(d, f, x + 0.5, y, a1, a2);
+      eps12(d, f, x - 0.5, y, b1, b2);
+      eps12(d, f, x, y + 0.5, c1, c2);
+      eps12(d, f, x, y - 0.5, d1, d2);
+      Jd[p] = fabs((a1 - b1) * (c2 - d2) - (a2 - b2) * (c1 - d1));
+    }
+  }
+  wave_sync();
+  double psz = sqrt(Jd[(Dy / 2) * Dx + Dx / 2]);
+  double t = psz / sd;
+  double seff = sd * sqrt(1.0 + (t * t) / 12.0);
+  for (int p = lane; p < P; p += WAVE) {
+    U1[p] = U1[p] / seff;
+    U2[p] = U2[p] / seff;
+  }
+  wave_sync();
+  return seff;
+}
+
+// ---- bilinear profile grid coordinates (xpg/integrate_kernels.py:154-166) ---------------------------------------
+
+struct Bil {
+  bool inside;
+  int i1, i2;
+  double t1, t2;
+};
+__device__ __forceinline__ Bil bilinear(double u1, double u2) {
+  Bil b;
+  double f1 = (u1 + UMAX) / (2 * UMAX) * (double)(G - 1);
+  double f2 = (u2 + UMAX) / (2 * UMAX) * (double)(G - 1);
+  b.inside = (f1 >= 0) && (f1 <= G - 1) && (f2 >= 0) && (f2 <= G - 1);
+  f1 = fmin(fmax(f1, 0.0), (double)(G - 1) - 1e-4);
+  f2 = fmin(fmax(f2, 0.0), (double)(G - 1) - 1e-4);
+  b.i1 = (int)floor(f1);
+  b.i2 = (int)floor(f2);
+  b.t1 = f1 - b.i1;
+  b.t2 = f2 - b.i2;
+  return b;
+}
+
+// ---- neighbour ellipsoids (xpg/integrate_kernels.py:186-213) ----------------------------------------------------
+// For each neighbour column that reaches the box (separable nearest-point test, exact), ORs bit `bit` of vflag for the
+// voxels with ((dx^2 + dy^2) + dz^2) < 1.  tab: scratch of Dx + Dy + Dz doubles.
+__device__ void neighbour_mask(const xpg_refl &R, int r, const double *xs, const double *ys, const double *zs,
+                               int Dz, int Dy, int Dx, int skip_first, uint8_t *vflag, uint8_t bit, double *tab,
+                               int lane) {
+  const int K = R.K;
+  if (K <= 0) return;
+  double lo_x = xs[0], hi_x = xs[Dx - 1], lo_y = ys[0], hi_y = ys[Dy - 1], lo_z = zs[0], hi_z = zs[Dz - 1];
+  bool hit = false;
+  if (lane < K && !(lane == 0 && skip_first)) {
+    double nx = (double)R.nbx[(int64_t)r * K + lane], ny = (double)R.nby[(int64_t)r * K + lane];
+    double nz = (double)R.nbz[(int64_t)r * K + lane];
+    double sx = (double)R.nbsx[(int64_t)r * K + lane], sz = (double)R.nbsz[(int64_t)r * K + lane];
+    double mx = (fmin(fmax(nx, lo_x), hi_x) - nx) / sx;
+    double my = (fmin(fmax(ny, lo_y), hi_y) - ny) / sx;
+    double mz = (fmin(fmax(nz, lo_z), hi_z) - nz) / sz;
+    hit = ((mx * mx + my * my) + mz * mz) < 1.0;
+  }
+  uint64_t hits = __ballot(hit);
+  const int P = Dy * Dx, V = Dz * P;
+  double *dx2 = tab, *dy2 = tab + Dx, *dz2 = tab + Dx + Dy;
+  while (hits) {
+    int k = __ffsll((unsigned long long)hits) - 1;
+    hits &= hits - 1;
+    double nx = (double)R.nbx[(int64_t)r * K + k], ny = (double)R.nby[(int64_t)r * K + k];
+    double nz = (double)R.nbz[(int64_t)r * K + k];
+    double sx = (double)R.nbsx[(int64_t)r * K + k], sz = (double)R.nbsz[(int64_t)r * K + k];
+    for (int i = lane; i < Dx + Dy + Dz; i += WAVE) {
+      double v;
+      if (i < Dx) {
+        v = (xs[i] - nx) / sx;
+      } else if (i < Dx + Dy) {
+        v = (ys[i - Dx] - ny) / sx;
+      } else {
+        v = (zs[i - Dx - Dy] - nz) / sz;
+      }
+      tab[i] = v * v;
+    }
+    wave_sync();
+    for (int v = lane; v < V; v += WAVE) {
+      int j = v / P, p = v - j * P;
+      int iy = p / Dx, ix = p - iy * Dx;
+      if ((dx2[ix] + dy2[iy]) + dz2[j] < 1.0) vflag[v] |= bit;
+    }
+    wave_sync();
+  }
+}
+
+}  // namespace xpg

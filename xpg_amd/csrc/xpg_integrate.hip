@@ -1,0 +1,983 @@
+// Shoebox integration kernels for MI355X (gfx950): extraction from a device frame store, pass 1 (masks, robust
+// background, summation, centroids, moments, projected images, fused profile accumulation), profile normalisation
+// and pass 2 (Poisson-weighted profile fit with 2x2 overlap deconvolution), behind the C ABI of
+// include/xpg_integrate.h.
+//
+// Work decomposition: one wavefront per reflection, W wavefronts per workgroup, each with its own slice of dynamic LDS
+// holding the shoebox values, per-voxel flags and the per-pixel / per-frame geometry.  No workgroup barriers: a
+// wavefront synchronises its lanes through LDS with wave_sync().  Reductions are cross-lane.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "xpg_common.h"
+
+using namespace xpg;
+
+// ---------------------------------------------------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------------------------------------------------
+
+static thread_local char g_err[512] = "";
+
+static int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess) return fail(XPG_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+  } while (0)
+
+// ---------------------------------------------------------------------------------------------------------------------
+// LDS layout of one wavefront (pass 1 and pass 2 share it)
+//   vals  float[V]    shoebox values                 vflag u8[V]   per-voxel flags
+//   scr   doubles     max(21 Dz, 3 P): rocking-curve terms, then the partner's u1, u2, J in pass 2
+//   U1,U2 double[P]   u1, u2                          Jd    double[P] pixel area
+//   pa    8P bytes    pass 1: bref, thr (float)  /  pass 2: p2 (double)
+//   pb    8P bytes    pass 1: e1v, e2v (float)   /  pass 2: pp2 (double)
+//   pc    4P bytes    pass 2: bref (float)            pflag u8[P]
+//   R,Rp  double[Dz]  rocking-curve fractions         fr    16 Dz bytes: zz, e3 (float), frame flags
+//   co    double[Dx + Dy + Dz] box coordinates        tab   double[Dx + Dy + Dz] neighbour terms
+// ---------------------------------------------------------------------------------------------------------------------
+
+__host__ __device__ static inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
+
+struct Layout {
+  size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, pflag, R, Rp, fr, co, tab, bytes;
+};
+
+__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
+  const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
+  Layout L;
+  size_t o = 0;
+  L.vals = o; o = al8(o + 4 * V);
+  L.vflag = o; o = al8(o + V);
+  L.scr = o; {
+    size_t a = 21 * (size_t)Dz, b = 3 * P;
+    o = al8(o + 8 * (a > b ? a : b));
+  }
+  L.U1 = o; o += 8 * P;
+  L.U2 = o; o += 8 * P;
+  L.Jd = o; o += 8 * P;
+  L.pa = o; o += 8 * P;
+  L.pb = o; o += 8 * P;
+  L.pc = o; o = al8(o + 4 * P);
+  L.pflag = o; o = al8(o + P);
+  L.R = o; o += 8 * Dz;
+  L.Rp = o; o += 8 * Dz;
+  L.fr = o; o = al8(o + 16 * Dz);
+  L.co = o; o += 8 * C;
+  L.tab = o; o += 8 * C;
+  L.bytes = al8(o);
+  return L;
+}
+
+enum : uint8_t { VF_NBM = 1, VF_USE = 2 };      // per-voxel flags
+enum : uint8_t { PF_FG2 = 1, PF_BGR = 2 };      // per-pixel flags: r^2 <= FG_R^2, r^2 >= BG_R^2
+
+// ---------------------------------------------------------------------------------------------------------------------
+// shoebox values: from a chunk buffer, or extracted from the frame store
+// ---------------------------------------------------------------------------------------------------------------------
+
+// value code of one detector pixel: xpg/io/images.py:80-96 (min_valid / flagged / static mask -> -1 and it wins,
+// overload -> -2) and the integrator's mask ceiling of xpg/integrate.py:661-664 (min(value, -1))
+__device__ __forceinline__ float frame_code(const xpg_frames &F, int64_t t, int iy, int ix) {
+  const int64_t p = (int64_t)iy * F.nx + ix;
+  float v;
+  if (F.dtype == 1) {
+    v = ((const float *)F.data)[t * F.frame_stride + p];
+    if (F.src_mask && !F.src_mask[p]) v = -1.0f;
+  } else {
+    const uint32_t raw = ((const uint32_t *)F.data)[t * F.frame_stride + p];
+    bool bad = raw < F.min_valid || (F.use_flag_value && raw == F.flag_value);
+    if (F.src_mask && !F.src_mask[p]) bad = true;
+    v = (float)raw;
+    if (F.overload && raw >= F.overload) v = -2.0f;
+    if (bad) v = -1.0f;
+  }
+  if (F.ceil_mask && !F.ceil_mask[p]) v = fminf(v, -1.0f);
+  return v;
+}
+
+// (super-)pixel (sy, sx) of store frame t: the padded frame's -1 border, s x s super-pixel sums flagged -2 if any
+// pixel is overloaded else -1 if any is negative, incomplete super-pixels at the far edges masked (xpg/integrate.py:
+// 648-702)
+__device__ __forceinline__ float pixel_value(const xpg_frames &F, int64_t t, int sy, int sx, int s) {
+  if (s == 1) {
+    if (sy < 0 || sx < 0 || sy >= F.ny || sx >= F.nx) return -1.0f;
+    return frame_code(F, t, sy, sx);
+  }
+  if (sy < 0 || sx < 0 || sy >= F.ny / s || sx >= F.nx / s) return -1.0f;
+  float sum = 0.f, mn = 0.f;
+  for (int a = 0; a < s; ++a)
+    for (int b = 0; b < s; ++b) {
+      float v = frame_code(F, t, sy * s + a, sx * s + b);
+      if (a == 0 && b == 0) {
+        sum = v;
+        mn = v;
+      } else {
+        sum = sum + v;
+        mn = fminf(mn, v);
+      }
+    }
+  if (mn < 0.f) return mn <= -1.5f ? -2.0f : -1.0f;
+  return sum;
+}
+
+__device__ __forceinline__ bool frame_read(const xpg_frames &F, int fi, int64_t &t) {
+  t = (int64_t)fi - F.frame0;
+  if (t < 0 || t >= F.n_frames) return false;
+  return F.available == nullptr || F.available[t] != 0;
+}
+
+// voxel (j, sy, sx): unbinned frames are read or unrecorded (-3); phi-bins of fb frames are summed with the
+// reference's IEEE encoding (an overload in any frame wins, then masked), a bin missing a frame is unrecorded
+// (xpg/integrate.py:704-716, xpg/shoebox.py:264-268, xpg/integrate.py:484-488)
+__device__ float voxel_value(const xpg_frames &F, int iz0, int fb, int j, int sy, int sx, int s) {
+  int64_t t;
+  if (fb == 1) {
+    if (!frame_read(F, iz0 + j, t)) return -3.0f;
+    return pixel_value(F, t, sy, sx, s);
+  }
+  float sum = 0.f;
+  bool any_over = false, any_mask = false, first = true;
+  for (int q = 0; q < fb; ++q) {
+    if (!frame_read(F, iz0 + fb * j + q, t)) return -3.0f;
+    float v = pixel_value(F, t, sy, sx, s);
+    if (v < 0.f) {
+      if (v == -2.0f) any_over = true;
+      else any_mask = true;
+    } else {
+      sum = first ? v : sum + v;
+      first = false;
+    }
+  }
+  if (any_over) return -2.0f;
+  if (any_mask) return -1.0f;
+  return sum;
+}
+
+__global__ void k_extract(xpg_frames F, xpg_refl R, float *sb) {
+  const int P = R.Dy * R.Dx, V = R.Dz * P;
+  const int64_t total = (int64_t)R.n * V;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+    int r = (int)(g / V);
+    int v = (int)(g - (int64_t)r * V);
+    int j = v / P, p = v - j * P;
+    int iy = p / R.Dx, ix = p - iy * R.Dx;
+    int s = R.pxs ? R.pxs[r] : 1;
+    sb[g] = voxel_value(F, R.iz0[r], R.fb[r], j, R.iy0[r] + iy, R.ix0[r] + ix, s);
+  }
+}
+
+__global__ void k_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int B, int bxbx) {
+  const int64_t total = n_rows * B * (int64_t)bxbx;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+    int64_t rb = g / bxbx;
+    float v = arr[g];
+    if (isnan(v)) v = -2.0f;
+    else if (isinf(v)) v = -1.0f;
+    v = v + 3.0f;
+    if (cnt[rb] == 0) v = 0.0f;
+    arr[g] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// per-reflection setup shared by pass 1 and pass 2
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct Box {
+  int r, Dz, Dy, Dx, P, V, ix0, iy0, iz0, fb, s, region;
+  double phi;
+};
+
+__device__ __forceinline__ Box box_of(const xpg_refl &R, int r) {
+  Box b;
+  b.r = r;
+  b.Dz = R.Dz;
+  b.Dy = R.Dy;
+  b.Dx = R.Dx;
+  b.P = R.Dy * R.Dx;
+  b.V = R.Dz * b.P;
+  b.ix0 = R.ix0[r];
+  b.iy0 = R.iy0[r];
+  b.iz0 = R.iz0[r];
+  b.fb = R.fb[r];
+  b.s = R.pxs ? R.pxs[r] : 1;
+  b.region = R.region ? R.region[r] : 0;
+  b.phi = R.phi[r];
+  return b;
+}
+
+__device__ __forceinline__ void stage_values(const Box &b, const float *sb, const xpg_frames &F, float *vals,
+                                             uint8_t *vflag, int lane) {
+  if (sb) {
+    const float *src = sb + (int64_t)b.r * b.V;
+    for (int v = lane; v < b.V; v += WAVE) vals[v] = src[v];
+  } else {
+    for (int v = lane; v < b.V; v += WAVE) {
+      int j = v / b.P, p = v - j * b.P;
+      int iy = p / b.Dx, ix = p - iy * b.Dx;
+      vals[v] = voxel_value(F, b.iz0, b.fb, j, b.iy0 + iy, b.ix0 + ix, b.s);
+    }
+  }
+  for (int v = lane; v < b.V; v += WAVE) vflag[v] = 0;
+}
+
+// box coordinates: co[0..Dx) pixel-centre x, co[Dx..Dx+Dy) y, then the Dz frame centres iz0 + fb (j + 1/2)
+__device__ __forceinline__ void box_coords(const Box &b, double *co, int lane) {
+  for (int i = lane; i < b.Dx + b.Dy + b.Dz; i += WAVE) {
+    double v;
+    if (i < b.Dx) v = pix_x(b.ix0, i, b.s);
+    else if (i < b.Dx + b.Dy) v = pix_x(b.iy0, i - b.Dx, b.s);
+    else v = (double)b.iz0 + (double)b.fb * ((double)(i - b.Dx - b.Dy) + 0.5);
+    co[i] = v;
+  }
+}
+
+// background template at the pixel centres (xpg/integrate_kernels.py:68-75; numpy round = half to even = rint)
+__device__ __forceinline__ float template_at(const xpg_detector &det, const double *co, int Dx, int p) {
+  if (!det.bkg) return 1.0f;
+  int iy = p / Dx, ix = p - iy * Dx;
+  int kx = min(max((int)rint(co[ix] - 1.0), 0), det.bkg_nx - 1);
+  int ky = min(max((int)rint(co[Dx + iy] - 1.0), 0), det.bkg_ny - 1);
+  return det.bkg[(int64_t)ky * det.bkg_nx + kx];
+}
+
+__device__ __forceinline__ double frames_fg(const double *Rd, int Dz, uint8_t *fflag, int lane) {
+  double Rmax = -INFINITY;
+  for (int j = lane; j < Dz; j += WAVE) Rmax = fmax(Rmax, Rd[j]);
+  Rmax = wmax_d(Rmax);
+  const double thr = 1e-3 * Rmax + 1e-6;
+  for (int j = lane; j < Dz; j += WAVE) fflag[j] = (Rd[j] >= thr) ? 1 : 0;
+  return Rmax;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass 1 (xpg/integrate_kernels.py:287-365)
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct P1Args {
+  xpg_detector det;
+  xpg_call call;
+  xpg_refl R;
+  xpg_frames F;
+  const float *sb;
+  xpg_pass1_out out;
+  xpg_profile_acc acc;
+  int use_acc;
+  int waves;
+  int wave_bytes;
+};
+
+__device__ __forceinline__ void splat(int64_t *grid, int idx, double v) {
+  long long q = llrint(v * FX_SCALE);
+  if (q != 0) atomicAdd((unsigned long long *)(grid + idx), (unsigned long long)q);
+}
+
+__device__ __forceinline__ void splat4(const xpg_profile_acc &acc, int region, double u1, double u2, double dens) {
+  Bil q = bilinear(u1, u2);
+  if (!q.inside) return;
+  region = min(max(region, 0), acc.n_regions - 1);   // memory safety; the host validates region
+  const int i00 = region * G * G + q.i2 * G + q.i1;
+  const double w00 = (1 - q.t1) * (1 - q.t2), w10 = q.t1 * (1 - q.t2), w01 = (1 - q.t1) * q.t2, w11 = q.t1 * q.t2;
+  splat(acc.num, i00, w00 * dens);
+  splat(acc.den, i00, w00);
+  splat(acc.num, i00 + 1, w10 * dens);
+  splat(acc.den, i00 + 1, w10);
+  splat(acc.num, i00 + G, w01 * dens);
+  splat(acc.den, i00 + G, w01);
+  splat(acc.num, i00 + G + 1, w11 * dens);
+  splat(acc.den, i00 + G + 1, w11);
+}
+
+__global__ void __launch_bounds__(256) k_pass1(P1Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wid = threadIdx.x / WAVE;
+  const int r = blockIdx.x * a.waves + wid;
+  if (r >= a.R.n) return;
+  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx);
+  unsigned char *base = smem + (size_t)wid * a.wave_bytes;
+  const int P = a.R.Dy * a.R.Dx;
+  float *vals = (float *)(base + L.vals);
+  uint8_t *vflag = base + L.vflag;
+  double *scr = (double *)(base + L.scr);
+  double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
+  float *bref = (float *)(base + L.pa), *thr = bref + P;
+  float *e1v = (float *)(base + L.pb), *e2v = e1v + P;
+  uint8_t *pflag = base + L.pflag;
+  double *Rd = (double *)(base + L.R);
+  float *zzf = (float *)(base + L.fr), *e3f = zzf + a.R.Dz;
+  uint8_t *fflag = (uint8_t *)(e3f + a.R.Dz);
+  double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
+
+  const Box b = box_of(a.R, r);
+  const int V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  const xpg_detector &det = a.det;
+  const xpg_call &call = a.call;
+
+  stage_values(b, a.sb, a.F, vals, vflag, lane);
+  Frame f;
+  kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
+  rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
+  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, U1, U2, Jd, lane);
+  box_coords(b, co, lane);
+  wave_sync();
+  for (int p = lane; p < P; p += WAVE) {
+    const double u1 = U1[p], u2 = U2[p];
+    const double r2 = u1 * u1 + u2 * u2;
+    pflag[p] = (r2 <= 9.0 ? PF_FG2 : 0) | (r2 >= 12.25 ? PF_BGR : 0);
+    bref[p] = template_at(det, co, Dx, p);
+    e1v[p] = (float)(u1 * seff);
+    e2v[p] = (float)(u2 * seff);
+  }
+  frames_fg(Rd, Dz, fflag, lane);
+  for (int j = lane; j < Dz; j += WAVE) {
+    double lo, hi;
+    frame_bounds(call, b.iz0, b.fb, j, lo, hi);
+    e3f[j] = (float)(f.zeta * (0.5 * (lo + hi) - b.phi));
+    zzf[j] = (float)co[Dx + Dy + j];
+  }
+  wave_sync();
+  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, lane);
+
+  // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
+  double bk = 0.0, nbg = 0.0;
+  for (int round = 0; round < 5; ++round) {
+    csum sc = cz(), sT = cz();
+    for (int v = lane; v < V; v += WAVE) {
+      const int j = v / P, p = v - j * P;
+      const float c = vals[v];
+      const uint8_t pf = pflag[p];
+      const bool fg = (pf & PF_FG2) && fflag[j];
+      const bool bgm = c >= 0.f && !fg && !(vflag[v] & VF_NBM) && (pf & PF_BGR);
+      if (bgm && (round == 0 || c <= thr[p])) {
+        cadd(sc, c);
+        cadd(sT, bref[p]);
+      }
+    }
+    const double Sc = creduce(sc);
+    nbg = creduce(sT);
+    bk = Sc / fmax(nbg, 1e-6);
+    if (round < 4) {
+      for (int p = lane; p < P; p += WAVE) {
+        const double bp = bk * (double)bref[p];
+        const double t = (bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0;
+        thr[p] = __double2float_rd(t);     // c <= t  <=>  c <= rd(t) for float32 c
+      }
+      wave_sync();
+    }
+  }
+
+  // ---- sums over the foreground (fg & valid & ~nbm) ----
+  csum s_fgc = cz(), s_fgT = cz(), s_wpos = cz(), s_wx = cz(), s_wy = cz(), s_wz = cz();
+  csum s_m1 = cz(), s_m2 = cz(), s_mdd = cz(), s_m3 = cz(), s_m33 = cz();
+  int n_fg = 0, n_over = 0, n_fgm = 0, n_fgnb = 0;
+  for (int v = lane; v < V; v += WAVE) {
+    const int j = v / P, p = v - j * P;
+    const float c = vals[v];
+    const bool valid = c >= 0.f;
+    const bool nbm = vflag[v] & VF_NBM;
+    const bool fg = (pflag[p] & PF_FG2) && fflag[j];
+    if (!fg) continue;
+    n_over += (c == -2.0f);
+    n_fgm += (!valid && c > -2.5f);
+    n_fgnb += nbm;
+    if (!valid || nbm) continue;
+    const int iy = p / Dx, ix = p - iy * Dx;
+    const double net = (double)c - bk * (double)bref[p];
+    const float netf = (float)net;
+    n_fg += 1;
+    cadd(s_fgc, c);
+    cadd(s_fgT, bref[p]);
+    if (net > 0.0) {
+      cadd(s_wpos, netf);
+      cadd(s_wx, netf * (float)co[ix]);
+      cadd(s_wy, netf * (float)co[Dx + iy]);
+      cadd(s_wz, netf * zzf[j]);
+    }
+    const float a1 = e1v[p], a2 = e2v[p], a3 = e3f[j];
+    cadd(s_m1, netf * a1);
+    cadd(s_m2, netf * a2);
+    cadd(s_mdd, netf * (a1 * a1 + a2 * a2));
+    cadd(s_m3, netf * a3);
+    cadd(s_m33, netf * (a3 * a3));
+  }
+  const double Sfgc = creduce(s_fgc), sfb = creduce(s_fgT);
+  const double Swpos = creduce(s_wpos), Swx = creduce(s_wx), Swy = creduce(s_wy), Swz = creduce(s_wz);
+  const double Sm1 = creduce(s_m1), Sm2 = creduce(s_m2), Smdd = creduce(s_mdd), Sm3 = creduce(s_m3);
+  const double Sm33 = creduce(s_m33);
+  n_fg = wsum_i(n_fg);
+  n_over = wsum_i(n_over);
+  n_fgm = wsum_i(n_fgm);
+  n_fgnb = wsum_i(n_fgnb);
+
+  const double gain = call.gain;
+  const double isum = Sfgc - bk * sfb;
+  const double vb = ((sfb * sfb) * gain * fmax(bk, VAR_FLOOR)) / fmax(nbg, 1.0);
+  const double vsum = gain * Sfgc + vb;
+  const double vsum0 = gain * (bk * sfb) + vb;
+  const double sw = fmax(Swpos, 1e-6);
+  const double sws = fmax(isum, 1e-6);
+  const double mu1 = Sm1 / sws, mu2 = Sm2 / sws;
+  const double m_d = (Smdd / sws - mu1 * mu1 - mu2 * mu2) / 2.0;
+  const double mu3 = Sm3 / sws;
+  const double m_m = Sm33 / sws - mu3 * mu3;
+
+  // ---- per frame: fz (net counts in the foreground ellipse), recorded frames -> frac_rec ----
+  float *fz_out = a.out.fz ? a.out.fz + (int64_t)r * Dz : nullptr;
+  for (int j = lane; j < Dz; j += WAVE) {
+    double fzj = 0.0;
+    bool anyrec = false;
+    for (int p = 0; p < P; ++p) {
+      const float c = vals[j * P + p];
+      anyrec |= c > -2.5f;
+      if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) fzj += (double)c - bk * (double)bref[p];
+    }
+    if (fz_out) fz_out[j] = (float)fzj;
+    scr[j] = anyrec ? Rd[j] : 0.0;
+  }
+  wave_sync();
+  double frac_rec = 0.0;
+  for (int j = 0; j < Dz; ++j) frac_rec = frac_rec + scr[j];
+
+  // ---- per pixel: projected image over the foreground frames, all-valid flag, fused profile accumulation ----
+  const bool strong = a.use_acc && (isum > 0.0) && (isum / sqrt(fmax(vsum, 1e-6)) > 10.0) && (n_over == 0) &&
+                      (frac_rec > 0.95) && (n_fgnb == 0) && (n_fgm == 0);
+  const double seff2 = seff * seff;
+  for (int p = lane; p < P; p += WAVE) {
+    double img = 0.0;
+    bool okv = true;
+    const double bpx = bk * (double)bref[p];
+    for (int j = 0; j < Dz; ++j) {
+      if (!fflag[j]) continue;
+      const float c = vals[j * P + p];
+      if (c >= 0.f && !(vflag[j * P + p] & VF_NBM)) img += (double)c - bpx;
+      else okv = false;
+    }
+    const int64_t o = (int64_t)r * P + p;
+    if (a.out.img2) a.out.img2[o] = (float)img;
+    if (a.out.all_valid2) a.out.all_valid2[o] = okv ? 1 : 0;
+    if (a.out.J) a.out.J[o] = (float)Jd[p];
+    if (a.out.u1) a.out.u1[o] = (float)U1[p];
+    if (a.out.u2) a.out.u2[o] = (float)U2[p];
+    if (strong && okv) {
+      const double dens = ((double)(float)img / fmax(isum, 1e-6)) / fmax(Jd[p] / seff2, 1e-12);
+      splat4(a.acc, b.region, U1[p], U2[p], dens);
+    }
+  }
+
+  if (lane == 0) {
+    const xpg_pass1_out &o = a.out;
+    o.bg[r] = bk;
+    o.nbg[r] = nbg;
+    o.isum[r] = isum;
+    o.vsum[r] = vsum;
+    o.vsum0[r] = vsum0;
+    o.xo[r] = Swx / sw;
+    o.yo[r] = Swy / sw;
+    o.zo[r] = Swz / sw;
+    o.m_d[r] = m_d;
+    o.m_m[r] = m_m;
+    o.frac_rec[r] = frac_rec;
+    o.nfg[r] = (double)n_fg;
+    o.zeta[r] = f.zeta;
+    o.seff[r] = seff;
+    o.n_over[r] = n_over;
+    o.n_fg_masked[r] = n_fgm;
+    o.n_fg_nb[r] = n_fgnb;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// accumulate_profiles (xpg/integrate_kernels.py:367-380), standalone form
+// ---------------------------------------------------------------------------------------------------------------------
+
+__global__ void k_accumulate(const float *img2, const uint8_t *ok2, const float *J, const float *u1, const float *u2,
+                             const int32_t *region, const double *isum, const uint8_t *sel, const double *seff, int n,
+                             int P, xpg_profile_acc acc) {
+  const int64_t total = (int64_t)n * P;
+  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(g / P);
+    if (!sel[r] || !ok2[g]) continue;
+    const double s = seff[r];
+    const double dens = ((double)img2[g] / fmax(isum[r], 1e-6)) / fmax((double)J[g] / (s * s), 1e-12);
+    splat4(acc, region[r], (double)u1[g], (double)u2[g], dens);
+  }
+}
+
+// profile normalisation in float64 (xpg/integrate.py:546-556): one workgroup of 256 per region
+__global__ void __launch_bounds__(256) k_normalise(xpg_profile_acc acc, const float *gauss, float *prof, double *num64,
+                                                   double *den64) {
+  __shared__ double red[256];
+  const int reg = blockIdx.x;
+  const int GG = G * G;
+  const double du = (2 * UMAX / (G - 1)) * (2 * UMAX / (G - 1));
+  double part = 0.0;
+  for (int i = threadIdx.x; i < GG; i += blockDim.x) {
+    const double num = (double)acc.num[reg * GG + i] / FX_SCALE;
+    const double den = (double)acc.den[reg * GG + i] / FX_SCALE;
+    if (num64) num64[reg * GG + i] = num;
+    if (den64) den64[reg * GG + i] = den;
+    const double dens = den > 0 ? num / fmax(den, 1e-12) : 0.0;
+    const double pr = (den * fmax(dens, 0.0) + 3.0 * (double)gauss[i]) / (den + 3.0);
+    part += pr;
+  }
+  red[threadIdx.x] = part;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  const double tot = fmax(red[0] * du, 1e-12);
+  for (int i = threadIdx.x; i < GG; i += blockDim.x) {
+    const double num = (double)acc.num[reg * GG + i] / FX_SCALE;
+    const double den = (double)acc.den[reg * GG + i] / FX_SCALE;
+    const double dens = den > 0 ? num / fmax(den, 1e-12) : 0.0;
+    const double pr = (den * fmax(dens, 0.0) + 3.0 * (double)gauss[i]) / (den + 3.0);
+    prof[reg * GG + i] = (float)(pr / tot);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// pass 2 (xpg/integrate_kernels.py:382-483)
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct P2Args {
+  xpg_detector det;
+  xpg_call call;
+  xpg_refl R;
+  xpg_frames F;
+  const float *sb;
+  const float *prof;
+  int n_regions;
+  const double *bg, *nbg;
+  xpg_pass2_out out;
+  int waves;
+  int wave_bytes;
+};
+
+// p2 = P2(u) J / seff^2 per pixel, float64 (xpg/integrate_kernels.py:278-285)
+__device__ __forceinline__ void profile_pixels(const float *prof, int region, const double *U1, const double *U2,
+                                               const double *Jd, double seff, int P, double *p2, int lane) {
+  const float *pr = prof + (int64_t)region * G * G;   // region in [0, n_regions): clamped by the caller
+  const double s2 = seff * seff;
+  for (int p = lane; p < P; p += WAVE) {
+    const Bil q = bilinear(U1[p], U2[p]);
+    const double a = (double)pr[q.i2 * G + q.i1], bq = (double)pr[q.i2 * G + q.i1 + 1];
+    const double c = (double)pr[(q.i2 + 1) * G + q.i1], d = (double)pr[(q.i2 + 1) * G + q.i1 + 1];
+    const double v = ((((1 - q.t1) * (1 - q.t2)) * a + (q.t1 * (1 - q.t2)) * bq) + ((1 - q.t1) * q.t2) * c) +
+                     (q.t1 * q.t2) * d;
+    p2[p] = (q.inside ? v : 0.0) * (Jd[p] / s2);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pass2(P2Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wid = threadIdx.x / WAVE;
+  const int r = blockIdx.x * a.waves + wid;
+  if (r >= a.R.n) return;
+  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx);
+  unsigned char *base = smem + (size_t)wid * a.wave_bytes;
+  float *vals = (float *)(base + L.vals);
+  uint8_t *vflag = base + L.vflag;
+  double *scr = (double *)(base + L.scr);
+  double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
+  double *p2 = (double *)(base + L.pa), *pp2 = (double *)(base + L.pb);
+  float *bref = (float *)(base + L.pc);
+  uint8_t *pflag = base + L.pflag;
+  double *Rd = (double *)(base + L.R), *Rpd = (double *)(base + L.Rp);
+  uint8_t *fflag = base + L.fr;
+  double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
+
+  Box b = box_of(a.R, r);
+  b.region = min(max(b.region, 0), a.n_regions - 1);   // memory safety; the host validates region
+  const int P = b.P, V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  const xpg_detector &det = a.det;
+  const xpg_call &call = a.call;
+  const double has_p = a.R.has_p ? (double)a.R.has_p[r] : 0.0;
+  const bool partner = has_p > 0.5;
+  const double bgr = a.bg[r];
+  const double gain = call.gain;
+
+  stage_values(b, a.sb, a.F, vals, vflag, lane);
+  Frame f;
+  kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
+  rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
+  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, U1, U2, Jd, lane);
+  box_coords(b, co, lane);
+  profile_pixels(a.prof, b.region, U1, U2, Jd, seff, P, p2, lane);
+  wave_sync();
+  for (int p = lane; p < P; p += WAVE) {
+    const double r2 = U1[p] * U1[p] + U2[p] * U2[p];
+    pflag[p] = r2 <= 9.0 ? PF_FG2 : 0;
+    bref[p] = template_at(det, co, Dx, p);
+  }
+  frames_fg(Rd, Dz, fflag, lane);
+  if (partner) {
+    // the overlapping partner's profile on this box's grid, fresh geometry (xpg/integrate_kernels.py:393-400)
+    Frame fp;
+    kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
+    rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], fp.zeta, Dz, scr, Rpd, lane);
+    double *PU1 = scr, *PU2 = scr + P, *PJ = scr + 2 * P;
+    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, PU1, PU2, PJ, lane);
+    profile_pixels(a.prof, b.region, PU1, PU2, PJ, seffp, P, pp2, lane);
+  }
+  wave_sync();
+  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, lane);
+
+  // ---- fit mask: valid & ~nbm & (p > 0.01 max p | pp > 0.01 max(max pp, 1e-12)), float64 products ----
+  double pmax = -INFINITY, ppmax = -INFINITY;
+  for (int v = lane; v < V; v += WAVE) {
+    const int j = v / P, p = v - j * P;
+    pmax = fmax(pmax, p2[p] * Rd[j]);
+    ppmax = fmax(ppmax, partner ? (pp2[p] * Rpd[j]) * has_p : 0.0);
+  }
+  pmax = wmax_d(pmax);
+  ppmax = wmax_d(ppmax);
+  const double pthr = 0.01 * pmax, ppthr = 0.01 * fmax(ppmax, 1e-12);
+  csum s_net = cz(), s_pw = cz();
+  for (int v = lane; v < V; v += WAVE) {
+    const int j = v / P, p = v - j * P;
+    const float c = vals[v];
+    const double pd = p2[p] * Rd[j];
+    const double ppd = partner ? (pp2[p] * Rpd[j]) * has_p : 0.0;
+    if (c >= 0.f && !(vflag[v] & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
+      vflag[v] |= VF_USE;
+      cadd(s_net, (float)((double)c - bgr * (double)bref[p]));
+      cadd(s_pw, (float)pd + (float)ppd);
+    }
+  }
+  wave_sync();
+  double I = creduce(s_net) / fmax(creduce(s_pw), 1e-6);
+  double Ip = I * has_p;
+
+  // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
+  double a11 = 0.0, a12 = 0.0, a22 = 1.0, det2 = 1.0, Iw = I, Ipw = Ip;
+  for (int it = 0; it < N_IRLS; ++it) {
+    Iw = I;
+    Ipw = Ip;
+    const float Ii = (float)fmax(I, 0.0), Ipi = (float)fmax(Ip, 0.0);
+    csum s11 = cz(), s12 = cz(), s22 = cz(), sb1 = cz(), sb2 = cz();
+    for (int v = lane; v < V; v += WAVE) {
+      if (!(vflag[v] & VF_USE)) continue;
+      const int j = v / P, p = v - j * P;
+      const double bpd = bgr * (double)bref[p];
+      const float bpx = (float)bpd;
+      const float net = (float)((double)vals[v] - bpd);
+      const float pf = (float)(p2[p] * Rd[j]);
+      const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
+      const float mu = (bpx + Ii * pf) + Ipi * ppf;
+      const float iv = 1.0f / ((float)gain * fmaxf(mu, (float)VAR_FLOOR));
+      cadd(s11, (pf * pf) * iv);
+      cadd(sb1, (pf * net) * iv);
+      if (partner) {
+        cadd(s12, (pf * ppf) * iv);
+        cadd(s22, (ppf * ppf) * iv);
+        cadd(sb2, (ppf * net) * iv);
+      }
+    }
+    a11 = creduce(s11);
+    const double b1 = creduce(sb1);
+    a12 = partner ? creduce(s12) : 0.0;
+    a22 = (partner ? creduce(s22) : 0.0) + (1.0 - has_p) * 1.0;
+    const double b2 = partner ? creduce(sb2) : 0.0;
+    det2 = fmax(a11 * a22 - a12 * a12, 1e-30);
+    const double I_old = I;
+    I = (a22 * b1 - a12 * b2) / det2;
+    Ip = (a11 * b2 - a12 * b1) / det2 * has_p;
+    if (IRLS_TOL > 0 && it >= 1 && it < N_IRLS - 1) {
+      const double dz = fabs(I - I_old) * sqrt(fmax(det2 / fmax(a22, 1e-30), 0.0));
+      if (dz < IRLS_TOL) break;     // uniform across the wavefront: every lane holds the same reduced values
+    }
+  }
+
+  // ---- tail (xpg/integrate_kernels.py:405-429): weights of the last pass (intensities Iw, Ipw) ----
+  const float Ii = (float)fmax(Iw, 0.0), Ipi = (float)fmax(Ipw, 0.0);
+  const float Ipos = (float)fmax(Ip, 0.0);
+  csum sC = cz(), s110 = cz(), sC0 = cz(), swp = cz(), sres = cz();
+  int nuse = 0;
+  for (int v = lane; v < V; v += WAVE) {
+    if (!(vflag[v] & VF_USE)) continue;
+    const int j = v / P, p = v - j * P;
+    const double bpd = bgr * (double)bref[p];
+    const float bpx = (float)bpd;
+    const float net = (float)((double)vals[v] - bpd);
+    const float pf = (float)(p2[p] * Rd[j]);
+    const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
+    const float mu = (bpx + Ii * pf) + Ipi * ppf;
+    const float iv = 1.0f / ((float)gain * fmaxf(mu, (float)VAR_FLOOR));
+    const float iv0 = 1.0f / ((float)gain * fmaxf(bpx, (float)VAR_FLOOR));
+    cadd(sC, (pf * bref[p]) * iv);
+    cadd(s110, (pf * pf) * iv0);
+    cadd(sC0, (pf * bref[p]) * iv0);
+    cadd(swp, pf);
+    cadd(sres, net - Ipos * ppf);
+    nuse += 1;
+  }
+  const double C = creduce(sC), A110 = creduce(s110), C0 = creduce(sC0), Swp = creduce(swp), Sres = creduce(sres);
+  nuse = wsum_i(nuse);
+  const double nu = fmax((double)nuse, 1.0);
+  const double pm = Swp / nu, nm = Sres / nu;
+  csum sdd = cz(), sdp = cz(), snn = cz();
+  csum s_pv = cz(), s_pt = cz(), s_pfg = cz();
+  for (int v = lane; v < V; v += WAVE) {
+    const int j = v / P, p = v - j * P;
+    const float pf = (float)(p2[p] * Rd[j]);
+    const float c = vals[v];
+    cadd(s_pt, pf);
+    if (c >= 0.f && !(vflag[v] & VF_NBM)) cadd(s_pv, pf);
+    if ((pflag[p] & PF_FG2) && fflag[j]) cadd(s_pfg, pf);
+    if (!(vflag[v] & VF_USE)) continue;
+    const float net = (float)((double)c - bgr * (double)bref[p]);
+    const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
+    const float dp = (float)((double)pf - pm);
+    const float dn = (float)((double)(net - Ipos * ppf) - nm);
+    cadd(sdp, dp * dn);
+    cadd(sdd, dp * dp);
+    cadd(snn, dn * dn);
+  }
+  const double Sdp = creduce(sdp), Sdd = creduce(sdd), Snn = creduce(snn);
+  const double Spv = creduce(s_pv), Spt = creduce(s_pt), Spfg = creduce(s_pfg);
+  if (lane == 0) {
+    const double var_b = gain * fmax(bgr, VAR_FLOOR) / fmax(a.nbg[r], 1.0);
+    const double a11i = a22 / det2;
+    const double varI = a11i + (C * a11i) * (C * a11i) * var_b;
+    const double a110 = fmax(A110, 1e-12);
+    const double infl = a11i * a11;
+    const double var0 = (1.0 / a110 + (C0 / a110) * (C0 / a110) * var_b) * fmax(infl, 1.0);
+    const xpg_pass2_out &o = a.out;
+    o.iprf[r] = I;
+    o.vprf[r] = varI;
+    o.vprf0[r] = var0;
+    o.corr[r] = Sdp / sqrt(fmax(Sdd * Snn, 1e-20));
+    o.p_valid[r] = Spv;
+    o.p_total[r] = Spt;
+    o.ipartner[r] = Ip;
+    o.rho[r] = partner ? a12 / sqrt(fmax(a11 * a22, 1e-30)) : 0.0;
+    o.p_fg[r] = Spfg / fmax(Spt, 1e-9);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------------------------------------
+
+static int lds_limit() {
+  static int cached[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 65536;
+  if (!cached[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) v = 65536;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+static int check_refl(const xpg_refl *R, bool need_partner_arrays) {
+  if (!R) return fail(XPG_ERR_ARG, "refl is NULL");
+  if (R->n < 0 || R->Dz < 1 || R->Dy < 1 || R->Dx < 1) return fail(XPG_ERR_ARG, "bad chunk shape");
+  if (R->Dz > XPG_MAX_DZ) return fail(XPG_ERR_ARG, "Dz %d > %d", R->Dz, XPG_MAX_DZ);
+  if (R->K < 0 || R->K > XPG_MAX_K) return fail(XPG_ERR_ARG, "K %d outside [0, %d]", R->K, XPG_MAX_K);
+  if ((R->Dx < 3 || R->Dy < 3) && (R->Dx < 1 || R->Dy < 1)) return fail(XPG_ERR_ARG, "bad box side");
+  if (R->n == 0) return XPG_OK;
+  if (!R->S || !R->phi || !R->ix0 || !R->iy0 || !R->iz0 || !R->fb) return fail(XPG_ERR_ARG, "missing refl array");
+  if (R->K > 0 && (!R->nbx || !R->nby || !R->nbz || !R->nbsx || !R->nbsz))
+    return fail(XPG_ERR_ARG, "missing neighbour columns");
+  if (need_partner_arrays && R->has_p && (!R->pS || !R->pphi)) return fail(XPG_ERR_ARG, "has_p without pS/pphi");
+  return XPG_OK;
+}
+
+static int launch_shape(const xpg_refl *R, int *waves, int *wave_bytes, size_t *lds) {
+  const Layout L = make_layout(R->Dz, R->Dy, R->Dx);
+  const int lim = lds_limit();
+  if ((long long)L.bytes > lim)
+    return fail(XPG_ERR_TOO_LARGE, "shoebox %dx%dx%d needs %zu bytes of LDS > %d", R->Dz, R->Dy, R->Dx, L.bytes, lim);
+  int w = 4;
+  while (w > 1 && (long long)w * L.bytes > lim) w >>= 1;
+  // keep several workgroups per CU: at most 32 KB per workgroup when the shoebox is small
+  *waves = w;
+  *wave_bytes = (int)L.bytes;
+  *lds = (size_t)w * L.bytes;
+  return XPG_OK;
+}
+
+template <typename K>
+static int set_lds(K kernel, size_t lds) {
+  if (lds > 65536) HIP_TRY(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return XPG_OK;
+}
+
+static int grid_for(int64_t total) {
+  int64_t g = (total + 255) / 256;
+  if (g > (1 << 20)) g = 1 << 20;
+  return (int)(g < 1 ? 1 : g);
+}
+
+static const xpg_frames kNoFrames = {};
+
+extern "C" {
+
+const char *xpg_version(void) { return "xpg_amd 0.1 (gfx950)"; }
+const char *xpg_last_error(void) { return g_err; }
+
+int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void *stream) {
+  int rc = check_refl(refl, false);
+  if (rc) return rc;
+  if (!frames || !frames->data || !sb) return fail(XPG_ERR_ARG, "frames / sb is NULL");
+  if (frames->nx <= 0 || frames->ny <= 0 || frames->frame_stride < (int64_t)frames->nx * frames->ny ||
+      (frames->dtype != 0 && frames->dtype != 1))
+    return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
+  const int64_t total = (int64_t)refl->n * refl->Dz * refl->Dy * refl->Dx;
+  if (total == 0) return XPG_OK;
+  hipLaunchKernelGGL(k_extract, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream, *frames, *refl, sb);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+int xpg_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int32_t B, int32_t bxbx, void *stream) {
+  if (!arr || !cnt || n_rows < 0 || B < 1 || bxbx < 1) return fail(XPG_ERR_ARG, "bad decode_bins arguments");
+  const int64_t total = n_rows * B * (int64_t)bxbx;
+  if (total == 0) return XPG_OK;
+  hipLaunchKernelGGL(k_decode_bins, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream, arr, cnt, n_rows, B, bxbx);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+                      const xpg_frames *frames, const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
+  int rc = check_refl(refl, false);
+  if (rc) return rc;
+  if (!det || !call || !out) return fail(XPG_ERR_ARG, "det / call / out is NULL");
+  if (!sb && (!frames || !frames->data)) return fail(XPG_ERR_ARG, "neither sb nor a frame store");
+  if (!out->bg || !out->nbg || !out->isum || !out->vsum || !out->vsum0 || !out->xo || !out->yo || !out->zo ||
+      !out->m_d || !out->m_m || !out->frac_rec || !out->nfg || !out->zeta || !out->seff || !out->n_over ||
+      !out->n_fg_masked || !out->n_fg_nb)
+    return fail(XPG_ERR_ARG, "missing per-reflection output");
+  if (acc && (!acc->num || !acc->den || acc->n_regions < 1)) return fail(XPG_ERR_ARG, "bad accumulation grids");
+  if (det->n_seg < 1 || !det->P0 || !det->Bx || !det->By || !det->U1 || !det->U2 || !det->UN)
+    return fail(XPG_ERR_ARG, "bad detector");
+  if (refl->n == 0) return XPG_OK;
+  if (acc && refl->region == nullptr) return fail(XPG_ERR_ARG, "accumulation needs region");
+  P1Args a;
+  a.det = *det;
+  a.call = *call;
+  a.R = *refl;
+  a.F = frames ? *frames : kNoFrames;
+  a.sb = sb;
+  a.out = *out;
+  a.use_acc = acc != nullptr;
+  if (acc) a.acc = *acc;
+  else memset(&a.acc, 0, sizeof(a.acc));
+  size_t lds;
+  rc = launch_shape(refl, &a.waves, &a.wave_bytes, &lds);
+  if (rc) return rc;
+  rc = set_lds(k_pass1, lds);
+  if (rc) return rc;
+  const int blocks = (refl->n + a.waves - 1) / a.waves;
+  hipLaunchKernelGGL(k_pass1, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+int xpg_pass1(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+              const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
+  if (!sb) return fail(XPG_ERR_ARG, "sb is NULL");
+  return pass1_impl(det, call, refl, sb, nullptr, out, acc, stream);
+}
+
+int xpg_pass1_frames(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                     const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
+  if (!frames || !frames->data) return fail(XPG_ERR_ARG, "frames is NULL");
+  return pass1_impl(det, call, refl, nullptr, frames, out, acc, stream);
+}
+
+int xpg_accumulate_profiles(const float *img2, const uint8_t *ok2, const float *J, const float *u1, const float *u2,
+                            const int32_t *region, const double *isum, const uint8_t *sel, const double *seff,
+                            int32_t n, int32_t Dy, int32_t Dx, const xpg_profile_acc *acc, void *stream) {
+  if (!img2 || !ok2 || !J || !u1 || !u2 || !region || !isum || !sel || !seff || !acc || !acc->num || !acc->den)
+    return fail(XPG_ERR_ARG, "NULL argument");
+  if (n < 0 || Dy < 1 || Dx < 1) return fail(XPG_ERR_ARG, "bad shape");
+  const int64_t total = (int64_t)n * Dy * Dx;
+  if (total == 0) return XPG_OK;
+  hipLaunchKernelGGL(k_accumulate, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream, img2, ok2, J, u1, u2,
+                     region, isum, sel, seff, n, Dy * Dx, *acc);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+int xpg_normalise_profiles(const xpg_profile_acc *acc, const float *gauss, float *prof, double *num64, double *den64,
+                           void *stream) {
+  if (!acc || !acc->num || !acc->den || acc->n_regions < 1 || !gauss || !prof) return fail(XPG_ERR_ARG, "NULL argument");
+  hipLaunchKernelGGL(k_normalise, dim3(acc->n_regions), dim3(256), 0, (hipStream_t)stream, *acc, gauss, prof, num64,
+                     den64);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+                      const xpg_frames *frames, const float *prof, int32_t n_regions, const double *bg,
+                      const double *nbg, int32_t irls_rule, const xpg_pass2_out *out, void *stream) {
+  int rc = check_refl(refl, true);
+  if (rc) return rc;
+  if (!det || !call || !out || !prof || !bg || !nbg) return fail(XPG_ERR_ARG, "NULL argument");
+  if (!sb && (!frames || !frames->data)) return fail(XPG_ERR_ARG, "neither sb nor a frame store");
+  if (!out->iprf || !out->vprf || !out->vprf0 || !out->corr || !out->p_valid || !out->p_total || !out->ipartner ||
+      !out->rho || !out->p_fg)
+    return fail(XPG_ERR_ARG, "missing output");
+  if (irls_rule != 0) return fail(XPG_ERR_ARG, "irls_rule %d: only the per-reflection rule (0) is built", irls_rule);
+  if (n_regions < 1) return fail(XPG_ERR_ARG, "n_regions < 1");
+  if (det->n_seg < 1 || !det->P0 || !det->Bx || !det->By || !det->U1 || !det->U2 || !det->UN)
+    return fail(XPG_ERR_ARG, "bad detector");
+  if (refl->n == 0) return XPG_OK;
+  P2Args a;
+  a.det = *det;
+  a.call = *call;
+  a.R = *refl;
+  a.F = frames ? *frames : kNoFrames;
+  a.sb = sb;
+  a.prof = prof;
+  a.n_regions = n_regions;
+  a.bg = bg;
+  a.nbg = nbg;
+  a.out = *out;
+  size_t lds;
+  rc = launch_shape(refl, &a.waves, &a.wave_bytes, &lds);
+  if (rc) return rc;
+  rc = set_lds(k_pass2, lds);
+  if (rc) return rc;
+  const int blocks = (refl->n + a.waves - 1) / a.waves;
+  hipLaunchKernelGGL(k_pass2, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
+int xpg_pass2(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+              const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+              const xpg_pass2_out *out, void *stream) {
+  if (!sb) return fail(XPG_ERR_ARG, "sb is NULL");
+  return pass2_impl(det, call, refl, sb, nullptr, prof, n_regions, bg, nbg, irls_rule, out, stream);
+}
+
+int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                     const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+                     const xpg_pass2_out *out, void *stream) {
+  if (!frames || !frames->data) return fail(XPG_ERR_ARG, "frames is NULL");
+  return pass2_impl(det, call, refl, nullptr, frames, prof, n_regions, bg, nbg, irls_rule, out, stream);
+}
+
+int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) { return (int)make_layout(Dz, Dy, Dx).bytes; }
+
+}  // extern "C"

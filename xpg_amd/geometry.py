@@ -1,0 +1,265 @@
+"""Host-side diffraction geometry consumed by the integration path (float64 numpy).
+
+Follows the conventions of xpg/geometry.py:1-14 (Kabsch 1988/2010): right-handed lab frame, S0 = beam / lambda,
+pixel (x, y) continuous with array element ix centred at x = ix + 1 (PIX0), lab position
+X = QX (x - ORGX) ED1 + QY (y - ORGY) ED2 + F ED3, frame i covering z in [i - 1, i), phi(z) = phi0 + osc (z - (first - 1)),
+A* columns = a*, b*, c* at phi = 0.  Only what the integrator reads is provided: a flat single panel with
+optional sensor parallax; the integration kernels themselves also accept segmented detectors through
+``detector_consts``.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PIX0 = 1.0
+
+
+def unit(v):
+    v = np.asarray(v, dtype=float)
+    return v / np.linalg.norm(v)
+
+
+def rotation_matrices(axis, angles):
+    """Right-handed rotations about unit ``axis`` by ``angles`` (radians), shape angles.shape + (3, 3)."""
+    u = unit(axis)
+    a = np.asarray(angles, float)
+    c, s = np.cos(a)[..., None, None], np.sin(a)[..., None, None]
+    K = np.array([[0, -u[2], u[1]], [u[2], 0, -u[0]], [-u[1], u[0], 0]])
+    return c * np.eye(3) + s * K + (1 - c) * np.outer(u, u)
+
+
+def cell_to_B(cell):
+    """Reciprocal basis (columns a*, b*, c*) of a cell in the Busing-Levy frame: a along x, b in the xy plane."""
+    a, b, c, al, be, ga = cell
+    al, be, ga = np.radians([al, be, ga])
+    cx = c * np.cos(be)
+    cy = c * (np.cos(al) - np.cos(be) * np.cos(ga)) / np.sin(ga)
+    real = np.array([[a, 0.0, 0.0], [b * np.cos(ga), b * np.sin(ga), 0.0],
+                     [cx, cy, np.sqrt(max(c * c - cx * cx - cy * cy, 1e-12))]])
+    return np.linalg.inv(real)
+
+
+def random_rotation(rng):
+    """Uniform random rotation from a unit quaternion."""
+    q = rng.normal(size=4)
+    w, x, y, z = q / np.linalg.norm(q)
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@dataclass
+class Geometry:
+    wavelength: float
+    beam: np.ndarray
+    axis: np.ndarray
+    orgx: float
+    orgy: float
+    distance: float
+    ed1: np.ndarray
+    ed2: np.ndarray
+    qx: float
+    qy: float
+    nx: int
+    ny: int
+    phi0: float                  # degrees at the start of starting_frame
+    osc: float                   # degrees per frame
+    starting_frame: int = 1
+    polarization_fraction: float = 0.5
+    polarization_normal: np.ndarray = field(default_factory=lambda: np.array([0.0, 1.0, 0.0]))
+    sensor_thickness: float = 0.0
+    mu: float = 0.0
+    segments: list | None = None
+    geo_corr: tuple | None = None
+
+    @property
+    def ed3(self):
+        return np.cross(self.ed1, self.ed2)
+
+    @property
+    def s0(self):
+        return unit(self.beam) / self.wavelength
+
+    @property
+    def has_parallax(self):
+        return self.sensor_thickness > 0 and self.mu > 0
+
+    @property
+    def segmented(self):
+        return bool(self.segments)
+
+    def z_to_phi(self, z):
+        return self.phi0 + self.osc * (np.asarray(z) - (self.starting_frame - 1))
+
+    def phi_to_z(self, phi):
+        return (np.asarray(phi) - self.phi0) / self.osc + (self.starting_frame - 1)
+
+    def panels(self):
+        """Affine lab maps (P0, Bx, By), each (1, 3), of the single panel: X = P0 + (x - orgx) Bx + (y - orgy) By."""
+        e1, e2 = unit(self.ed1), unit(self.ed2)
+        return (self.distance * np.cross(e1, e2))[None], (self.qx * e1)[None], (self.qy * e2)[None]
+
+    def _parallax(self, e):
+        """Shift (panel axes, mm) of the absorption-weighted mean interaction depth along unit rays e (sensor of
+        thickness t, attenuation mu): lbar = 1/mu - L e^{-mu L} / (1 - e^{-mu L}), L = t / cos(incidence)."""
+        un = self.ed3
+        cosa = np.clip(e @ un, 1e-3, 1.0)
+        L = self.sensor_thickness / cosa
+        muL = self.mu * L
+        lbar = 1.0 / self.mu - L * np.exp(-muL) / np.maximum(-np.expm1(-muL), 1e-12)
+        return lbar * (e @ unit(self.ed1)), lbar * (e @ unit(self.ed2))
+
+    def _plane(self, x, y):
+        x, y = np.asarray(x, float), np.asarray(y, float)
+        return (self.qx * (x[..., None] - self.orgx) * unit(self.ed1) + self.qy * (y[..., None] - self.orgy)
+                * unit(self.ed2) + self.distance * self.ed3)
+
+    def pixel_to_lab(self, x, y):
+        if self.segmented or self.geo_corr is not None:
+            raise NotImplementedError("host geometry supports a single flat panel")
+        X = self._plane(x, y)
+        if self.has_parallax:
+            x, y = np.asarray(x, float), np.asarray(y, float)
+            for _ in range(4):
+                d1, d2 = self._parallax(X / np.linalg.norm(X, axis=-1, keepdims=True))
+                X = self._plane(x - d1 / self.qx, y - d2 / self.qy)
+        return X
+
+    def pixel_to_s(self, x, y):
+        X = self.pixel_to_lab(x, y)
+        return X / (np.linalg.norm(X, axis=-1, keepdims=True) * self.wavelength)
+
+    def s_to_pixel(self, S):
+        """Pixel coordinates where rays S hit the panel (nan when they miss the plane)."""
+        S = np.asarray(S, float)
+        n = self.ed3
+        sn = S @ n
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = self.distance / sn
+        P = t[..., None] * S - self.distance * n
+        x = self.orgx + (P @ unit(self.ed1)) / self.qx
+        y = self.orgy + (P @ unit(self.ed2)) / self.qy
+        if self.has_parallax:
+            d1, d2 = self._parallax(S / np.linalg.norm(S, axis=-1, keepdims=True))
+            x, y = x + d1 / self.qx, y + d2 / self.qy
+        ok = np.isfinite(t) & (t > 0)
+        return np.where(ok, x, np.nan), np.where(ok, y, np.nan)
+
+    def lp_factor(self, S):
+        """(Lorentz zeta, polarisation P) of diffracted beams S."""
+        e = S / np.linalg.norm(S, axis=-1, keepdims=True)
+        e0 = unit(self.beam)
+        zeta = np.cross(e, e0) @ unit(self.axis)
+        n = self.polarization_normal - np.dot(self.polarization_normal, e0) * e0
+        n = n / np.linalg.norm(n)
+        emain = np.cross(n, e0)
+        f = self.polarization_fraction
+        return zeta, f * (1 - (e @ emain) ** 2) + (1 - f) * (1 - (e @ n) ** 2)
+
+    def resolution_of_pixel(self, x, y):
+        return 1.0 / np.linalg.norm(self.pixel_to_s(x, y) - self.s0, axis=-1)
+
+
+@dataclass
+class Crystal:
+    Astar: np.ndarray
+    space_group: str = "P 1"
+
+
+@dataclass
+class Experiment:
+    geom: Geometry
+    crystal: Crystal
+
+
+def detector_consts(g: Geometry, bkg_template=None):
+    """Detector constants for the kernels (float64 host arrays), the fields of xpg/integrate_kernels.py:48-65."""
+    P0, Bx, By = g.panels()
+    U1, U2 = Bx / g.qx, By / g.qy
+    return dict(orgx=g.orgx, orgy=g.orgy, P0=P0, Bx=Bx, By=By, U1=U1, U2=U2, UN=np.cross(U1, U2), qx=g.qx, qy=g.qy,
+                wl=g.wavelength, tsens=float(max(g.sensor_thickness, 1e-6)), mu=float(max(g.mu, 1e-6)),
+                plx=float(g.has_parallax), segmap=None, gx=None, gy=None,
+                bkg=None if bkg_template is None else np.asarray(bkg_template, np.float32))
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# prediction (rotation method)
+# ---------------------------------------------------------------------------------------------------------------------
+
+def generate_hkl(Astar, dmin, dmax=1e9):
+    """All Miller indices with dmin <= d < dmax (not 000)."""
+    R = np.linalg.inv(Astar)
+    hmax = [int(np.ceil(np.linalg.norm(R[i]) / dmin)) for i in range(3)]
+    H = np.stack(np.meshgrid(*[np.arange(-m, m + 1) for m in hmax], indexing="ij"), -1).reshape(-1, 3)
+    p = H @ Astar.T
+    d2 = np.einsum("ij,ij->i", p, p)
+    keep = (d2 <= 1.0 / dmin ** 2) & (d2 > 1.0 / dmax ** 2)
+    return H[keep].astype(np.int32)
+
+
+def centring_allowed(hkl, space_group):
+    """Reflections allowed by the lattice centring of the space-group symbol's first letter (h.t integral for every
+    centring vector t): P, A, B, C, I, F, R (hexagonal axes, obverse)."""
+    c = (space_group.strip()[:1] or "P").upper()
+    h, k, l = (hkl[:, i].astype(np.int64) for i in range(3))
+    if c == "P":
+        return np.ones(len(hkl), bool)
+    if c == "A":
+        return (k + l) % 2 == 0
+    if c == "B":
+        return (h + l) % 2 == 0
+    if c == "C":
+        return (h + k) % 2 == 0
+    if c == "I":
+        return (h + k + l) % 2 == 0
+    if c == "F":
+        return ((h + k) % 2 == 0) & ((h + l) % 2 == 0) & ((k + l) % 2 == 0)
+    if c == "R":
+        return (-h + k + l) % 3 == 0
+    return np.ones(len(hkl), bool)
+
+
+def predict_reflections(g: Geometry, Astar, dmin, dmax=1e9, z_range=None, hkl=None):
+    """Rotation-method predictions with z in z_range: hkl, phi (deg), S, x, y, z, zeta (Lorentz), polarization.
+    Both roots of S0 . R(phi) p0 = -|p0|^2 / 2 are kept."""
+    if hkl is None:
+        hkl = generate_hkl(Astar, dmin, dmax)
+    p0 = hkl @ Astar.T
+    m, s0 = unit(g.axis), g.s0
+    par = (p0 @ m)[:, None] * m
+    perp = p0 - par
+    a = perp @ s0
+    b = np.cross(m, p0) @ s0
+    c = -0.5 * np.einsum("ij,ij->i", p0, p0) - par @ s0
+    r = np.hypot(a, b)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ratio = c / r
+    valid = np.abs(ratio) <= 1.0
+    base = np.arctan2(b, a)
+    delta = np.arccos(np.clip(ratio, -1, 1))
+    if z_range is None:
+        z_range = (g.starting_frame - 1, g.starting_frame - 1 + 360.0 / abs(g.osc))
+    lo, hi = sorted(g.z_to_phi(np.array(z_range, float)))
+    out = {k: [] for k in ("hkl", "phi", "S")}
+    for root in (base + delta, base - delta):
+        ph = lo + np.mod(np.degrees(root[valid]) - lo, 360.0)
+        for shift in (0.0, 360.0):
+            phs = ph + shift
+            ok = (phs >= lo) & (phs < hi)
+            if not ok.any():
+                continue
+            R = rotation_matrices(g.axis, np.radians(phs[ok]))
+            out["hkl"].append(hkl[valid][ok])
+            out["phi"].append(phs[ok])
+            out["S"].append(np.einsum("nij,nj->ni", R, p0[valid][ok]) + s0)
+    if not out["hkl"]:
+        z = np.zeros(0)
+        return dict(hkl=np.zeros((0, 3), np.int32), phi=z, S=np.zeros((0, 3)), x=z, y=z, z=z, zeta=z, polarization=z)
+    res = {k: np.concatenate(v) for k, v in out.items()}
+    res["x"], res["y"] = g.s_to_pixel(res["S"])
+    res["z"] = g.phi_to_z(res["phi"])
+    res["zeta"], res["polarization"] = g.lp_factor(res["S"])
+    ok = np.isfinite(res["x"])
+    return {k: v[ok] for k, v in res.items()}

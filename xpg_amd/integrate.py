@@ -1,0 +1,733 @@
+"""3D profile-fitting integration on MI355X: the call surface of xpg/integrate.py (integrate, estimate_sigmas,
+IntegrationModel, IntegrationResult, N_BOX) over the HIP kernels.
+
+Model (Kabsch 1988/2010), as xpg/integrate.py:1-21: each shoebox voxel maps to the reflection frame
+eps1 = e1.(S'-S)/|S|, eps2 = e2.(S'-S)/|S|, eps3 = zeta (phi' - phi); the expected fraction of the reflection in voxel
+(i, j, k) is p = P2(eps1, eps2) J_ij R_k with P2 a learnt 2D profile per detector region and phi block, J the pixel
+area and R_k the frame-integrated rocking curve.
+
+What runs where:
+  host (numpy/scipy, before the device work): prediction, box planning, neighbour lists, block order, and the small
+      per-block fits of the adaptive widths (xpg/integrate.py:404-461);
+  device (HIP, libxpg_integrate.so): shoebox extraction from the frame store fused into pass 1 and pass 2, the masks,
+      background, sums and moments, the profile accumulation (fused into pass 1), the float64 profile normalisation and
+      the Poisson profile fit.  There is no CPU fallback: without a GPU and the built library the calls raise.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+from scipy.spatial import cKDTree
+
+from . import integrate_kernels as K
+from ._lib import XpgError
+from .frames import DeviceFrames, from_imageset
+from .geometry import PIX0, Geometry, centring_allowed, detector_consts, generate_hkl, predict_reflections
+from .integrate_kernels import (FG_R, N_BOX, N_FG, PROFILE_GRID, RC_C, gaussian_profile, rc_basis_np,
+                                rc_default_weights)
+
+N_BOX_Z = 4.0
+DZ_BUCKETS = (3, 5, 9, 17, 33, 65)
+BIN_SLOTS = 33
+BIN_MAX = 8
+XY_BUCKETS = (9, 13, 17, 23, 31, 41, 55, 73, 97)
+SUPERPIXEL = 2.5
+SUPERPIXEL_MAX = 4
+
+
+@dataclass
+class IntegrationModel:
+    """xpg/integrate.py:60-79.  ``box_z`` (odd, 0 = the reference's depth rule) fixes the shoebox depth in frames, for
+    workloads specified with fixed box shapes; it has no counterpart in the reference."""
+    sigma_d: float
+    sigma_m: float
+    gain: float = 1.0
+    box_xy: int = 0
+    box_z: int = 0
+
+    def box(self, geom: Geometry, dmin=None):
+        if self.box_xy:
+            return self.box_xy
+        corners = geom.pixel_to_lab(np.array([0.5, geom.nx + .5, 0.5, geom.nx + .5]),
+                                    np.array([0.5, 0.5, geom.ny + .5, geom.ny + .5]))
+        L = np.linalg.norm(corners, axis=1).max()
+        cos2t = geom.distance / L
+        spx = L * np.radians(self.sigma_d) / min(geom.qx, geom.qy) / max(cos2t, 0.4)
+        spx = math.sqrt(spx ** 2 + 1.0 / 12.0)
+        h = int(math.ceil(N_BOX * spx)) + 1
+        return 2 * max(h, 2) + 1
+
+
+@dataclass
+class IntegrationResult:
+    refl: dict
+    model: IntegrationModel
+    profiles: dict = field(default_factory=dict)
+    timing: dict = field(default_factory=dict)
+
+
+def _bucket(dz):
+    for b in DZ_BUCKETS:
+        if dz <= b:
+            return b
+    return None
+
+
+def _neighbours(x, y, z, sx, sz, k=8):
+    """k nearest predicted centres in scaled (x/sx, y/sx, z/sz) space within 4 N_FG (xpg/integrate.py:164-172)."""
+    pts = np.c_[x / sx, y / sx, z / sz]
+    tree = cKDTree(pts)
+    d, j = tree.query(pts, k=k + 1, distance_upper_bound=4.0 * N_FG)
+    j, d = j[:, 1:], d[:, 1:]
+    has = np.isfinite(d)
+    return np.where(has, j, 0), has, np.where(has, d, np.inf)
+
+
+def predict_for_integration(exps_by_block, block_edges, dmin, dmax, z_lo, z_hi, minimum_zeta=0.05, lattice_of=None):
+    """Predictions per phi block (xpg/integrate.py:101-122).  Blocks sharing one Experiment object are predicted in
+    one call over their union of z and split by the block edges (the same reflections)."""
+    out = []
+    nb0 = len(block_edges) - 1
+    groups = {}
+    for b, exp in enumerate(exps_by_block):
+        groups.setdefault(id(exp), (exp, []))[1].append(b)
+    hkl_cache = {}
+    for exp, blocks in groups.values():
+        key = (id(exp.crystal.Astar), exp.crystal.space_group)
+        if key not in hkl_cache:
+            hkl = generate_hkl(exp.crystal.Astar, dmin, dmax)
+            hkl_cache[key] = hkl[centring_allowed(hkl, exp.crystal.space_group)]
+        edges = [(block_edges[b % nb0], block_edges[b % nb0 + 1]) for b in blocks]
+        za, zb = min(e[0] for e in edges), max(e[1] for e in edges)
+        pr = predict_reflections(exp.geom, exp.crystal.Astar, dmin, dmax, z_range=(za, zb), hkl=hkl_cache[key])
+        for b, (a, c) in zip(blocks, edges):
+            m = (pr["z"] >= a) & (pr["z"] < c)
+            q = {k: v[m] for k, v in pr.items()}
+            q["block"] = np.full(int(m.sum()), b, np.int32)
+            q["lattice"] = np.full(int(m.sum()), 0 if lattice_of is None else lattice_of[b], np.int32)
+            out.append((b, q))
+    out.sort(key=lambda t: t[0])
+    res = {k: np.concatenate([o[k] for _, o in out]) for k in out[0][1]}
+    g = exps_by_block[0].geom
+    e1 = np.cross(res["S"], g.s0)
+    e1 /= np.maximum(np.linalg.norm(e1, axis=1, keepdims=True), 1e-12)
+    res["zeta_k"] = e1 @ g.axis
+    ok = (np.abs(res["zeta_k"]) >= minimum_zeta) & (res["x"] >= 0.5) & (res["x"] < g.nx + 0.5) & \
+         (res["y"] >= 0.5) & (res["y"] < g.ny + 0.5) & (res["z"] >= z_lo) & (res["z"] < z_hi)
+    return {k: v[ok] for k, v in res.items()}
+
+
+def plan_boxes(refl, g0, model, adaptive=True):
+    """Pixel-to-reflection assignment (xpg/integrate.py:230-287): shoebox depth, phi-binning, side, super-pixels and
+    origin of every reflection.  Integer results from float64 floor / ceil; returns (keep, plan dict)."""
+    box = (IntegrationModel(model.sigma_d * (1.3 if adaptive else 1.0), model.sigma_m).box(g0)
+           if not model.box_xy else model.box_xy)
+    sz_frames = model.sigma_m / np.abs(refl["zeta_k"]) / abs(g0.osc)
+    if model.box_z:
+        n = refl["x"].size
+        hz = np.full(n, model.box_z // 2)
+        fb = np.ones(n, int)
+        iz0 = np.floor(refl["z"]).astype(int) - hz
+        dz = 2 * hz + 1
+        bucket = dz.copy()
+        cov = np.ones(n, np.float32)
+        wide = trunc = np.zeros(n, bool)
+    else:
+        from scipy.special import erf as _erf
+        hz = np.ceil(N_BOX_Z * sz_frames + 0.5).astype(int)
+        hmax = (max(DZ_BUCKETS) - 1) // 2
+        wide = hz > hmax
+        fb = np.where(wide, np.clip(np.ceil((2 * hz + 1) / BIN_SLOTS), 2, BIN_MAX), 1).astype(int)
+        hz = np.where(wide, np.minimum(np.ceil(N_BOX_Z * sz_frames / fb + 0.5).astype(int), (BIN_SLOTS - 1) // 2),
+                      hz).astype(int)
+        trunc = wide & (np.ceil(N_BOX_Z * sz_frames / fb + 0.5) > (BIN_SLOTS - 1) // 2)
+        iz_c = np.floor(refl["z"] - (fb - 1) / 2.0).astype(int)
+        iz0 = iz_c - hz * fb
+        dz = 2 * hz + 1
+        bucket = np.array([_bucket(d) or -1 for d in dz], int)
+        zc = refl["z"]
+        cov = 0.5 * (_erf((iz0 + dz * fb - zc) / (np.sqrt(2) * sz_frames)) - _erf((iz0 - zc) / (np.sqrt(2) * sz_frames)))
+        cov = np.where(trunc, np.clip(cov, 0, 1), 1.0).astype(np.float32)
+    keep = bucket > 0
+    plan = dict(box=box, iz0=iz0, dz=dz, fb=fb, bucket=bucket, coverage=cov, wide=wide, trunc=trunc)
+    return keep, plan
+
+
+def plan_sides(refl, g0, model, box, adaptive=True):
+    """Per-reflection shoebox side, super-pixel size and origin (xpg/integrate.py:263-287)."""
+    n = refl["x"].size
+    if model.box_xy:
+        bxs = np.full(n, box, int)
+        spx = None
+    else:
+        Xv = g0.pixel_to_lab(refl["x"], refl["y"])
+        Lr = np.linalg.norm(Xv, axis=1)
+        cosr = np.abs(Xv @ g0.ed3) / Lr
+        spx = Lr * np.radians(model.sigma_d * (1.3 if adaptive else 1.0)) / min(g0.qx, g0.qy) / np.maximum(cosr, 0.4)
+        spx = np.sqrt(spx ** 2 + 1.0 / 12.0)
+        need = 2 * np.maximum(np.ceil(N_BOX * spx).astype(int) + 1, 2) + 1
+        xyb = np.array([b_ for b_ in XY_BUCKETS if b_ < box] + [box])
+        bxs = xyb[np.minimum(np.searchsorted(xyb, need), len(xyb) - 1)]
+    pxs = np.ones(n, int)
+    if SUPERPIXEL > 0 and not model.box_xy and n:
+        pxs = np.clip(np.floor(spx / SUPERPIXEL), 1, SUPERPIXEL_MAX).astype(int)
+        need_s = 2 * np.maximum(np.ceil(N_BOX * spx / pxs).astype(int) + 1, 2) + 1
+        bxs = np.where(pxs > 1, xyb[np.minimum(np.searchsorted(xyb, need_s), len(xyb) - 1)], bxs)
+    halfs = bxs // 2
+    ix0 = np.floor(refl["x"] - PIX0 + 0.5).astype(int) // pxs - halfs
+    iy0 = np.floor(refl["y"] - PIX0 + 0.5).astype(int) // pxs - halfs
+    return bxs, pxs, ix0, iy0
+
+
+def block_order(blocks_refl, need_lo, need_hi, all_frames):
+    """Order in which the reference's frame loop hands completed blocks to the block worker (xpg/integrate.py:
+    729-803): blocks open in need_lo order as the loop reaches them and are submitted, in open order, after the first
+    processed frame >= need_hi; blocks still open at the end follow in open order."""
+    pending = sorted((b for b in range(len(blocks_refl)) if blocks_refl[b].size), key=lambda b: need_lo[b])
+    opened, order = [], []
+    for f in all_frames:
+        fi = f - 1
+        while pending and need_lo[pending[0]] <= fi:
+            opened.append(pending.pop(0))
+        for b in list(opened):
+            if need_hi[b] <= fi:
+                opened.remove(b)
+                order.append(b)
+    return order + opened
+
+
+def _kernel_device(images):
+    dev = getattr(images, "device", None)
+    if dev is not None and torch.device(dev).type == "cuda":
+        return torch.device(dev)
+    if not torch.cuda.is_available():
+        raise XpgError("the MI355X integration path needs a GPU (torch.cuda / ROCm device)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def integrate(*args, **kw):
+    """3D profile-fitting integration of the predicted reflections (see _integrate)."""
+    return _integrate(*args, **kw)
+
+
+def estimate_sigmas(*args, **kw):
+    """sigma_D / sigma_M from strong spots (see _estimate_sigmas)."""
+    return _estimate_sigmas(*args, **kw)
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# the sweep
+# ---------------------------------------------------------------------------------------------------------------------
+
+OUT_F64 = ("bg", "isum", "vsum", "vsum0", "iprf", "vprf", "vprf0", "corr", "xo", "yo", "zo", "p_valid", "p_total",
+           "frac_rec", "rho", "m_d", "m_m", "p_fg")
+P1_KEEP = ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "frac_rec", "m_d", "m_m", "n_over", "n_fg_masked",
+           "n_fg_nb", "nfg", "zeta", "seff")
+
+
+class SweepPlan:
+    """Everything the device needs for one integrate() call: the reflection table in processing order (blocks, then
+    shoebox-shape groups with overlapping-partner rows first, xpg/integrate.py:372-383), uploaded once."""
+
+    def __init__(self, refl, iz0, fb, bucket, bxs, pxs, ix0, iy0, region, has_partner, pj, nb, blocks_refl, order,
+                 device):
+        self.groups = []                 # (block, (B, bx, sp), start, n)
+        perm = []
+        for b in order:
+            rows = blocks_refl[b]
+            keys = sorted({(int(bucket[r]), int(bxs[r]), int(pxs[r])) for r in rows},
+                          key=lambda k: (DZ_BUCKETS.index(k[0]) if k[0] in DZ_BUCKETS else k[0], k[1], k[2]))
+            for gk in keys:
+                sel = rows[(bucket[rows] == gk[0]) & (bxs[rows] == gk[1]) & (pxs[rows] == gk[2])]
+                sel = sel[np.argsort(~has_partner[sel], kind="stable")]
+                if sel.size:
+                    self.groups.append((b, gk, len(perm), int(sel.size)))
+                    perm.extend(sel.tolist())
+        self.perm = np.asarray(perm, np.int64)
+        p = self.perm
+        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        self.t = dict(S=T(refl["S"][p], torch.float64), phi=T(np.radians(refl["phi"][p]), torch.float64),
+                      ix0=T(ix0[p], torch.int32), iy0=T(iy0[p], torch.int32), iz0=T(iz0[p], torch.int32),
+                      fb=T(fb[p], torch.int32), pxs=T(pxs[p], torch.int32), region=T(region[p], torch.int32),
+                      has_p=T(has_partner[p], torch.float32), pS=T(refl["S"][pj[p]], torch.float64),
+                      pphi=T(np.radians(refl["phi"][pj[p]]), torch.float64))
+        for k in ("x", "y", "z", "sx", "sz"):
+            self.t["nb" + k] = T(nb[k][p], torch.float32)
+        self.K = int(nb["x"].shape[1])
+        self.n = int(p.size)
+        self.device = device
+
+    def refl_view(self, start, n, shape):
+        """ctypes table of rows [start, start + n) (pointer offsets into the uploaded table)."""
+        import ctypes as C
+        from . import _lib as L
+        s = L.Refl()
+        s.n, (s.Dz, s.Dy, s.Dx) = n, shape
+        for k in ("S", "pS"):
+            setattr(s, k, C.c_void_p(self.t[k].data_ptr() + 24 * start))
+        for k in ("phi", "pphi"):
+            setattr(s, k, C.c_void_p(self.t[k].data_ptr() + 8 * start))
+        for k in ("ix0", "iy0", "iz0", "fb", "pxs", "region", "has_p"):
+            setattr(s, k, C.c_void_p(self.t[k].data_ptr() + 4 * start))
+        s.K = self.K
+        for k in ("x", "y", "z", "sx", "sz"):
+            setattr(s, "nb" + k, C.c_void_p(self.t["nb" + k].data_ptr() + 4 * self.K * start))
+        v = K.DeviceRefl.__new__(K.DeviceRefl)
+        v.s, v.n, v.t = s, n, {}
+        return v
+
+
+def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, dmax=1e9, frames=None, mask=None,
+               minimum_zeta=0.05, verbose=True, log=print, n_regions_xy=3, learn_profiles=True, exclude_frames=(),
+               adaptive=True, deconvolve_overlaps=True, extra_lattices=None, bkg_template=None, foreign_spots=None,
+               n_foreign=4, moments_only=False, plan_only=False, timing=None):
+    """Integrate a sweep (xpg/integrate.py:209-999).  ``images``: a DeviceFrames store, or an ImageSet-like object
+    (first, last, nx, ny, iter_frames) whose frames are uploaded once.  ``plan_only`` returns the host plan without
+    device work (planning tests); ``timing`` (dict) receives host / device times."""
+    lattice_of = [0] * len(exp_blocks)
+    if extra_lattices:
+        exp_blocks = list(exp_blocks)
+        for li, lat in enumerate(extra_lattices, start=1):
+            exp_blocks += list(lat)
+            lattice_of += [li] * len(lat)
+    t0 = time.time()
+    g0 = exp_blocks[0].geom
+    if frames is None:
+        frames = (images.first, images.last)
+    f_first, f_last = frames
+    z_lo, z_hi = f_first - 1.0, float(f_last)
+    refl = predict_for_integration(exp_blocks, block_edges, dmin, dmax, z_lo, z_hi, minimum_zeta, lattice_of)
+    keep, pl = plan_boxes(refl, g0, model, adaptive)
+    box = pl["box"]
+    sd = np.radians(model.sigma_d)
+    sm = np.radians(model.sigma_m)
+    n0 = refl["x"].size
+    if verbose:
+        log(f"  INTEGRATE: {n0} predicted reflections, box {box}x{box} px, sigma_D {model.sigma_d:.4f} "
+            f"sigma_M {model.sigma_m:.4f} deg; {int(pl['wide'].sum())} wide in phi (> {max(DZ_BUCKETS)} frames): "
+            f"integrated in bins of 2-{BIN_MAX} frames, {int(pl['trunc'].sum())} of them still truncated")
+    refl["coverage"] = pl["coverage"]
+    refl = {k: v[keep] for k, v in refl.items()}
+    iz0, dz, bucket, fb = (pl[k][keep] for k in ("iz0", "dz", "bucket", "fb"))
+    n = refl["x"].size
+    bxs, pxs, ix0, iy0 = plan_sides(refl, g0, model, box, adaptive)
+    # neighbours for overlap masking (xpg/integrate.py:288-302)
+    Xlen = np.linalg.norm(g0.pixel_to_lab(refl["x"], refl["y"]), axis=1)
+    sx_px = np.maximum(Xlen * sd / g0.qx, 0.5) * N_FG
+    sz_f = np.sqrt((model.sigma_m / np.abs(refl["zeta_k"])) ** 2 + (g0.osc ** 2) / 12.0) / abs(g0.osc) * N_FG
+    if n:
+        nbj, nbh, nbd = _neighbours(refl["x"], refl["y"], refl["z"], sx_px.mean(), sz_f.mean())
+    else:
+        nbj, nbh, nbd = np.zeros((0, 8), int), np.zeros((0, 8), bool), np.zeros((0, 8))
+    has_partner = (nbh[:, 0] & (nbd[:, 0] < 2.0)) if deconvolve_overlaps else np.zeros(len(nbd), bool)
+    nb = dict(x=np.where(nbh, refl["x"][nbj], -1e6).astype(np.float32),
+              y=np.where(nbh, refl["y"][nbj], -1e6).astype(np.float32),
+              z=np.where(nbh, refl["z"][nbj], -1e6).astype(np.float32),
+              sx=np.where(nbh, sx_px[nbj], 1.0).astype(np.float32),
+              sz=np.where(nbh, sz_f[nbj], 1.0).astype(np.float32))
+    if foreign_spots is not None and len(foreign_spots[0]) and n:
+        # spots of other crystals masked at 2 sigma unless within 2.5 sigma of the own centre (xpg/integrate.py:303-327)
+        fx, fy, fz_ = (np.asarray(a, float) for a in foreign_spots)
+        sxm, szm = float(sx_px.mean()), float(sz_f.mean())
+        ft = cKDTree(np.c_[fx / sxm, fy / sxm, fz_ / szm])
+        d_f, j_f = ft.query(np.c_[refl["x"] / sxm, refl["y"] / sxm, refl["z"] / szm], k=n_foreign + 2,
+                            distance_upper_bound=4.0 * N_FG)
+        d_f, j_f = np.atleast_2d(d_f), np.atleast_2d(j_f)
+        hf = np.isfinite(d_f) & (d_f > 2.5 / N_FG)
+        order_f = np.argsort(np.where(hf, d_f, np.inf), axis=1)[:, :n_foreign]
+        rows_f = np.arange(d_f.shape[0])[:, None]
+        hf, j_f = hf[rows_f, order_f], np.where(np.isfinite(d_f), j_f, 0)[rows_f, order_f]
+        rad = 2.0 / N_FG
+        nb["x"] = np.concatenate([nb["x"], np.where(hf, fx[j_f], -1e6).astype(np.float32)], 1)
+        nb["y"] = np.concatenate([nb["y"], np.where(hf, fy[j_f], -1e6).astype(np.float32)], 1)
+        nb["z"] = np.concatenate([nb["z"], np.where(hf, fz_[j_f], -1e6).astype(np.float32)], 1)
+        nb["sx"] = np.concatenate([nb["sx"], np.full(hf.shape, sxm * rad, np.float32)], 1)
+        nb["sz"] = np.concatenate([nb["sz"], np.full(hf.shape, szm * rad, np.float32)], 1)
+    rx = np.clip((refl["x"] / g0.nx * n_regions_xy).astype(int), 0, n_regions_xy - 1)
+    ry = np.clip((refl["y"] / g0.ny * n_regions_xy).astype(int), 0, n_regions_xy - 1)
+    region = (ry * n_regions_xy + rx).astype(np.int32)
+    nreg = n_regions_xy * n_regions_xy
+    nblocks = len(exp_blocks)
+    blk = refl["block"]
+    zstart = np.maximum(iz0, f_first - 1)
+    zend = np.minimum(iz0 + dz * fb - 1, f_last - 1)
+    order_b = np.argsort(blk, kind="stable")
+    blocks_refl = [order_b[blk[order_b] == b] for b in range(nblocks)]
+    need_lo = [int(zstart[r].min()) if r.size else 10 ** 9 for r in blocks_refl]
+    need_hi = [int(zend[r].max()) if r.size else -1 for r in blocks_refl]
+    excl = set(exclude_frames)
+    all_frames = [f for f in range(f_first, f_last + 1) if f not in excl]
+    border = block_order(blocks_refl, need_lo, need_hi, all_frames)
+    pj = nbj[:, 0] if n else np.zeros(0, int)
+    host_plan = dict(refl=refl, iz0=iz0, dz=dz, fb=fb, bucket=bucket, bxs=bxs, pxs=pxs, ix0=ix0, iy0=iy0,
+                     region=region, has_partner=has_partner, nb=nb, block_order=border, blocks_refl=blocks_refl,
+                     need_lo=need_lo, need_hi=need_hi, pj=pj)
+    if plan_only:
+        return host_plan
+    sw = Sweep(images, exp_blocks, model, host_plan, dict(
+        frames=frames, mask=mask, exclude_frames=exclude_frames, learn_profiles=learn_profiles, adaptive=adaptive,
+        bkg_template=bkg_template, moments_only=moments_only, nreg=nreg, sd=sd, sm=sm))
+    sw.t_plan = time.time() - t0
+    run = sw.run()
+    res = sw.finish(run, t0)
+    if timing is not None:
+        timing.update(res.timing)
+    if verbose:
+        ok = np.isfinite(res.refl["iprf"])
+        log(f"  INTEGRATE: {ok.sum()} reflections integrated in {res.timing['total']:.1f}s (plan "
+            f"{res.timing['plan']:.1f}s, device {res.timing['gpu']:.2f}s)")
+    return res
+
+
+class Sweep:
+    """A planned sweep on the device: the reflection table uploaded once (SweepPlan), the frame store view, and the
+    per-block loop of pass 1 -> adaptive widths -> profile normalisation -> pass 2 (xpg/integrate.py:463-574), which
+    ``run`` executes (repeatably: benchmarks time it with the inputs resident in HBM)."""
+
+    def __init__(self, images, exp_blocks, model, hp, opts):
+        self.images, self.exp_blocks, self.model, self.hp, self.o = images, exp_blocks, model, hp, opts
+        device = _kernel_device(images)
+        self.device = device
+        src = images if isinstance(images, DeviceFrames) else from_imageset(images, opts["frames"], device)
+        self.frames_src = src
+        self.store = src.store(mask=opts["mask"], exclude_frames=opts["exclude_frames"], frames=opts["frames"])
+        self.sp = SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"],
+                            hp["iy0"], hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"],
+                            hp["block_order"], device)
+        self.by_block = {}
+        for gi, (b, gk, start, m) in enumerate(self.sp.groups):
+            self.by_block.setdefault(b, []).append((gi, gk, start, m))
+        self.dets = {}
+        self.t_plan = 0.0
+
+    def det_for(self, g):
+        key = id(g)
+        if key not in self.dets:
+            self.dets[key] = (g, K.DeviceDetector(detector_consts(g, self.o["bkg_template"]), self.device))
+        return self.dets[key][1]
+
+    def launches(self):
+        """(kernel, block, group index, n reflections, (Dz, Dy, Dx)) of every launch, in order."""
+        out = []
+        for b in self.hp["block_order"]:
+            for gi, (B, bx, _), start, m in self.by_block.get(b, []):
+                out.append(("pass1", b, gi, m, (B, bx, bx)))
+            if not self.o["moments_only"]:
+                for gi, (B, bx, _), start, m in self.by_block.get(b, []):
+                    out.append(("pass2", b, gi, m, (B, bx, bx)))
+        return out
+
+    def run(self, timer=None):
+        """Device execution of the whole sweep; ``timer(kind, b, gi)`` may return a context that brackets launches."""
+        o, sp, device, model = self.o, self.sp, self.device, self.model
+        N = sp.n
+        f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
+        o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
+        o2 = {k: f64() for k in K.P2_F64}
+        fz_groups, prof_by_block, block_params = {}, {}, {}
+        gauss = gaussian_profile()
+        adapt = dict(sd=o["sd"], sm=o["sm"], rcw=rc_default_weights())
+        hp = self.hp
+        nreg = o["nreg"]
+        t_dev0 = time.time()
+        nullctx = _Null()
+        for b in hp["block_order"]:
+            groups = self.by_block.get(b, [])
+            if not groups:
+                continue
+            sd_b = adapt["sd"] if o["adaptive"] else o["sd"]
+            sm_b, rcw_b = (adapt["sm"], adapt["rcw"]) if o["adaptive"] else (o["sm"], rc_default_weights())
+            g = self.exp_blocks[b].geom
+            det = self.det_for(g)
+            frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
+            osc = float(np.radians(g.osc))
+            call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, sd_b, sm_b, model.gain, rcw_b)
+            acc = None
+            if o["learn_profiles"]:
+                num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
+            for gi, (B, bx, _), start, m in groups:
+                view = sp.refl_view(start, m, (B, bx, bx))
+                out = {k: v[start:start + m] for k, v in o1.items()}
+                out["fz"] = fz_groups[gi] = torch.empty((m, B), dtype=torch.float32, device=device)
+                with (timer("pass1", b, gi) if timer else nullctx):
+                    _launch_pass1(det, call1, view, self.store, out, acc)
+            if o["adaptive"]:
+                sm_b, rcw_b = _adapt_block(b, g, groups, o1, fz_groups, sp, hp["refl"], hp["iz0"], hp["fb"], sd_b,
+                                           sm_b, rcw_b, o["sd"], o["sm"], adapt, block_params)
+            if o["learn_profiles"]:
+                prof, _, _ = K.normalise_profiles(num_fx, den_fx, device)
+            else:
+                prof = torch.as_tensor(np.repeat(gauss[None], nreg, 0), device=device)
+            prof_by_block[b] = prof
+            if o["moments_only"]:
+                continue
+            call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, sd_b, sm_b, model.gain, rcw_b)
+            for gi, (B, bx, _), start, m in groups:
+                view = sp.refl_view(start, m, (B, bx, bx))
+                out = {k: v[start:start + m] for k, v in o2.items()}
+                with (timer("pass2", b, gi) if timer else nullctx):
+                    _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
+                                  o1["nbg"][start:start + m], out)
+        torch.cuda.synchronize(device)
+        return dict(o1=o1, o2=o2, fz_groups=fz_groups, prof_by_block=prof_by_block, block_params=block_params,
+                    t_gpu=time.time() - t_dev0)
+
+    def finish(self, run, t0=None):
+        """Host results in prediction order (xpg/integrate.py:967-999)."""
+        sp, hp = self.sp, self.hp
+        N = sp.n
+        inv = np.empty(N, np.int64)
+        inv[sp.perm] = np.arange(N)
+        host = {k: v.cpu().numpy()[inv] for k, v in {**run["o1"], **run["o2"]}.items()}
+        refl = dict(hp["refl"])
+        for k in OUT_F64:
+            refl[k] = host[k].astype(np.float64)
+        refl["n_over"] = host["n_over"].astype(np.int32)
+        refl["n_fg_masked"] = host["n_fg_masked"].astype(np.int32)
+        refl["overlap"] = hp["has_partner"].copy()
+        if not self.o["learn_profiles"]:
+            fz_out = np.zeros((N, max(DZ_BUCKETS)), np.float32)
+            for gi, (b, (B, _, _), start, m) in enumerate(sp.groups):
+                fz_out[sp.perm[start:start + m], :B] = run["fz_groups"][gi].cpu().numpy()
+            refl["fz"] = fz_out
+        refl["iz0"], refl["dz"], refl["fb"] = hp["iz0"], hp["dz"], hp["fb"]
+        refl["region"] = hp["region"]
+        refl["ix0"], refl["iy0"] = hp["ix0"] * hp["pxs"], hp["iy0"] * hp["pxs"]
+        if (hp["pxs"] > 1).any():
+            refl["pxs"] = hp["pxs"]
+        profiles = {b: p.cpu().numpy() for b, p in run["prof_by_block"].items()}
+        tm = dict(total=(time.time() - t0) if t0 else float("nan"), gpu=run["t_gpu"], plan=self.t_plan)
+        res = IntegrationResult(refl, self.model, profiles, tm)
+        res.block_params = run["block_params"]
+        return res
+
+
+class _Null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+def _launch_pass1(det, call, view, store, out, acc):
+    import ctypes as C
+    from . import _lib as L
+    o = L.Pass1Out()
+    for k in K.P1_F64 + K.P1_I32 + ("fz",):
+        setattr(o, k, L.ptr(out[k]) if k in out else None)
+    for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m", "frac_rec", "nfg", "zeta", "seff"):
+        if k not in out:
+            raise XpgError(f"missing output {k}")
+    L.check(L.lib().xpg_pass1_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), C.byref(o),
+                                     C.byref(acc) if acc is not None else None, L.stream_ptr(det.device)))
+
+
+def _launch_pass2(det, call, view, store, prof, bg, nbg, out):
+    import ctypes as C
+    from . import _lib as L
+    o = L.Pass2Out()
+    for k in K.P2_F64:
+        setattr(o, k, L.ptr(out[k]))
+    L.check(L.lib().xpg_pass2_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), L.ptr(prof),
+                                     int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
+                                     L.stream_ptr(det.device)))
+
+
+# ---------------------------------------------------------------------------------------------------------------------
+# adaptive widths per block (host fits on the strong reflections' pass-1 results)
+# ---------------------------------------------------------------------------------------------------------------------
+
+def _adapt_block(b, g, groups, o1, fz_groups, sp, refl, iz0, fb, sd_b, sm_b, rcw_b, sd, sm, adapt, block_params):
+    """Refit sigma_M and the rocking-curve weights for this block's profile fit and sigma_D for the next block
+    (xpg/integrate.py:404-461)."""
+    F, PH, ZK, IZ, MD, XL = [], [], [], [], [], []
+    for gi, (B, _, _), start, m in groups:
+        rows = sp.perm[start:start + m]
+        h = {k: o1[k][start:start + m].cpu().numpy() for k in ("isum", "vsum", "n_over", "frac_rec", "m_d")}
+        st = (h["isum"] > 0) & (h["isum"] / np.sqrt(np.maximum(h["vsum"], 1e-6)) > 15) & (h["n_over"] == 0) \
+            & (h["frac_rec"] > 0.98) & (fb[rows] == 1)
+        if not st.any():
+            continue
+        fz = np.zeros((int(st.sum()), max(DZ_BUCKETS)))
+        fz[:, :B] = fz_groups[gi].cpu().numpy()[st]
+        F.append(fz)
+        PH.append(refl["phi"][rows][st])
+        ZK.append(refl["zeta_k"][rows][st])
+        IZ.append(iz0[rows][st])
+        MD.append(h["m_d"][st])
+        XL.append(np.c_[refl["x"][rows][st], refl["y"][rows][st]])
+    n = sum(len(f) for f in F)
+    if n < 100:
+        block_params[b] = dict(sigma_d=float(np.degrees(sd_b)), sigma_m=float(np.degrees(sm_b)), n=n,
+                               rcw=np.asarray(rcw_b).round(3).tolist())
+        return sm_b, rcw_b
+    F, PH, ZK, IZ, MD, XL = map(np.concatenate, (F, PH, ZK, IZ, MD, XL))
+    sm_fit = np.radians(_fit_rocking_width([(F, IZ, PH, ZK)], g, np.degrees(sm), 0.5, 2.0))
+    sm_new = float(np.exp((n * np.log(sm_fit) + 300.0 * np.log(sm)) / (n + 300.0)))
+    k = IZ[:, None] + np.arange(F.shape[1])[None]
+    lo = g.z_to_phi(k.astype(float))
+    hi = g.z_to_phi(k + 1.0)
+    lo, hi = np.radians(np.minimum(lo, hi)), np.radians(np.maximum(lo, hi))
+    Bm = rc_basis_np(lo, hi, np.radians(PH), ZK, sm_new)
+    tot = np.maximum(F.sum(1, keepdims=True), 1e-9)
+    A = (Bm * np.sqrt(tot)[:, :, None]).reshape(-1, len(RC_C))
+    y = (F / tot * np.sqrt(tot)).reshape(-1)
+    w = _nnls_gram(A, y)
+    lam = 200.0 / n
+    w = (w / max(w.sum(), 1e-9) + lam * rc_default_weights()) / (1 + lam)
+    Xv = g.pixel_to_lab(XL[:, 0], XL[:, 1])
+    Xlen = np.linalg.norm(Xv, axis=1)
+    cosa = np.abs(Xv @ g.ed3) / Xlen
+    t2 = 1.0 - (FG_R ** 2 / 2) * np.exp(-FG_R ** 2 / 2) / (1 - np.exp(-FG_R ** 2 / 2))
+    md = MD / t2 - (g.qx / Xlen) ** 2 * (1 + cosa ** 2) / 24.0
+    sd_fit = float(np.clip(np.sqrt(max(np.median(md), 1e-12)), 0.5 * sd, 2.0 * sd))
+    sd_new = float(np.exp((n * np.log(sd_fit) + 300.0 * np.log(sd)) / (n + 300.0)))
+    adapt.update(sd=sd_new, sm=float(sm_new), rcw=w.astype(np.float32))
+    block_params[b] = dict(sigma_d=float(np.degrees(sd_b)), sigma_m=float(np.degrees(sm_new)), n=int(n),
+                           rcw=w.round(3).tolist(), sigma_d_fit=float(np.degrees(sd_fit)),
+                           sigma_m_fit=float(np.degrees(sm_fit)))
+    return float(sm_new), w.astype(np.float32)
+
+
+def _nnls_gram(A, y):
+    """argmin ||A w - y||, w >= 0, through the Cholesky factor of A^T A (xpg/integrate.py:1095-1104)."""
+    from scipy.linalg import LinAlgError, cholesky, solve_triangular
+    from scipy.optimize import nnls
+    try:
+        R = cholesky(A.T @ A)
+        return nnls(R, solve_triangular(R.T, A.T @ y, lower=True))[0]
+    except (LinAlgError, ValueError):
+        return nnls(A, y)[0]
+
+
+def _rocking_resolved(r, max_edge=0.5):
+    """Reflections whose net counts show a rocking curve within the box (xpg/integrate.py:1107-1127)."""
+    fz = r.get("fz")
+    if fz is None:
+        return np.ones(len(r["x"]), bool)
+    n = len(fz)
+    dz = np.minimum(np.asarray(r["dz"], int), fz.shape[1])
+    rows = np.arange(n)
+    c = np.clip(np.floor((r["z"] - r["iz0"]) / np.maximum(r["fb"], 1)).astype(int), 0, np.maximum(dz - 1, 0))
+    centre = fz[rows, c]
+    k = np.arange(fz.shape[1])[None]
+    rec = (fz != 0) & (k < dz[:, None])
+    first = np.where(rec.any(1), np.argmax(rec, 1), 0)
+    last = np.where(rec.any(1), fz.shape[1] - 1 - np.argmax(rec[:, ::-1], 1), 0)
+    edge = 0.5 * (fz[rows, first] + fz[rows, last])
+    return (dz < 3) | ((centre > 0) & (edge < max_edge * centre) & (last - first >= 2))
+
+
+def _fit_rocking_width(sel_fit, g: Geometry, sm_guess, lo_fac=0.2, hi_fac=3.0):
+    """Grid search (61 points in log sigma_M) of the frame-integrated Gaussian rocking curve against the per-frame
+    fractions of strong reflections (xpg/integrate.py:1130-1154)."""
+    from scipy.special import erf
+    fz = np.concatenate([s[0] for s in sel_fit])
+    iz0 = np.concatenate([s[1] for s in sel_fit])
+    phi = np.concatenate([s[2] for s in sel_fit])
+    zeta = np.abs(np.concatenate([s[3] for s in sel_fit]))
+    tot = fz.sum(1, keepdims=True)
+    ok = tot[:, 0] > 0
+    fz, iz0, phi, zeta, tot = fz[ok], iz0[ok], phi[ok], zeta[ok], tot[ok]
+    frac = fz / tot
+    k = iz0[:, None] + np.arange(fz.shape[1])[None]
+    lo = g.z_to_phi(k.astype(float))
+    hi = g.z_to_phi(k + 1.0)
+    lo, hi = np.minimum(lo, hi), np.maximum(lo, hi)
+    best = (np.inf, sm_guess)
+    for smv in sm_guess * np.exp(np.linspace(np.log(lo_fac), np.log(hi_fac), 61)):
+        s = smv / zeta[:, None] * np.sqrt(2)
+        R = 0.5 * (erf((hi - phi[:, None]) / s) - erf((lo - phi[:, None]) / s))
+        R = R / np.maximum(R.sum(1, keepdims=True), 1e-9)
+        cost = np.sum(tot * (frac - R) ** 2)
+        if cost < best[0]:
+            best = (cost, smv)
+    return float(best[1])
+
+
+def _estimate_sigmas(images, exp, model0: IntegrationModel, dmin, frames, n_sample_frames=30, verbose=True, log=print,
+                     extra_exps=(), bkg_template=None, foreign_spots=None, max_iter=None, guard=None):
+    """sigma_D / sigma_M from truncation-corrected second moments of strong spots in five phi ranges, iterated
+    (xpg/integrate.py:1002-1092)."""
+    g = exp.geom
+    f0, f1 = frames
+    nf = f1 - f0 + 1
+    nr = 5
+    span = max(int(round(n_sample_frames / 3)), int(np.ceil(5.0 / max(abs(g.osc), 1e-6))), 3)
+    span = min(span, max(nf // nr, 3))
+    ranges = []
+    for i in range(nr):
+        a = f0 + int(i * (nf - span) / max(nr - 1, 1))
+        ranges.append((a, min(a + span - 1, f1)))
+    model = prev = model0
+    n_iter = max_iter or 3
+    history = []
+    for it in range(n_iter):
+        sel_md, sel_mm, sel_fit, per_range = [], [], [], []
+        for a, b in ranges:
+            edges = np.array([a - 1.0, float(b)])
+            res = integrate(images, [exp], edges, model, dmin, frames=(max(f0, a - 5), min(f1, b + 5)), verbose=False,
+                            learn_profiles=False, adaptive=False, bkg_template=bkg_template,
+                            extra_lattices=[[e] for e in extra_exps] or None, foreign_spots=foreign_spots,
+                            moments_only=True)
+            r = res.refl
+            z_ok = (r["z"] >= a - 1) & (r["z"] < b) & (r.get("lattice", 0) == 0) & ~r["overlap"]
+            isg = r["isum"] / np.sqrt(np.maximum(r["vsum"], 1e-6))
+            base = z_ok & np.isfinite(r["isum"]) & (r["isum"] > 0) & (r["n_over"] == 0) & (r["frac_rec"] > 0.98)
+            base &= _rocking_resolved(r)
+            per_range.append((r, base, isg))
+        sels = []
+        for r, base, isg in per_range:
+            strong = base & (isg > 20)
+            if strong.sum() < 150:
+                strong = base & (isg > 10)
+            sels.append(strong if strong.sum() >= 20 else None)
+        if all(x is None for x in sels) and sum(int((bs & (ig > 5.0)).sum()) for _, bs, ig in per_range) >= 30:
+            sels = [bs & (ig > 5.0) for _, bs, ig in per_range]
+        for (r, base, isg), strong in zip(per_range, sels):
+            if strong is None or not strong.any():
+                continue
+            Xv = g.pixel_to_lab(r["x"][strong], r["y"][strong])
+            Xlen = np.linalg.norm(Xv, axis=1)
+            cosa = np.abs(Xv @ g.ed3) / Xlen
+            pix = (g.qx / Xlen) ** 2 * (1 + cosa ** 2) / 24.0
+            if "pxs" in r:
+                pix = pix * r["pxs"][strong] ** 2
+            sel_md.append(np.c_[r["m_d"][strong], pix])
+            fbs = r["fb"][strong] if "fb" in r else 1
+            sel_mm.append(r["m_m"][strong] - (np.radians(g.osc) * fbs * r["zeta_k"][strong]) ** 2 / 12.0)
+            one = strong & (r["fb"] == 1) if "fb" in r else strong
+            sel_fit.append((r["fz"][one], r["iz0"][one], r["phi"][one], r["zeta_k"][one]))
+        if not sel_md:
+            break
+        md2 = np.concatenate(sel_md)
+        mm = np.concatenate(sel_mm)
+        t2 = 1.0 - (FG_R ** 2 / 2) * np.exp(-FG_R ** 2 / 2) / (1 - np.exp(-FG_R ** 2 / 2))
+        md = md2[:, 0] / t2 - md2[:, 1]
+        sd_new = np.degrees(np.sqrt(max(np.median(md), 1e-12)))
+        sm_new = _fit_rocking_width(sel_fit, g, max(np.degrees(np.sqrt(max(np.median(mm), 1e-12))),
+                                                    0.5 * model.sigma_m))
+        new = IntegrationModel(sigma_d=float(np.clip(sd_new, 0.005, 1.0)), sigma_m=float(np.clip(sm_new, 0.005, 5.0)),
+                               gain=model.gain)
+        history.append((float(new.sigma_d), float(new.sigma_m), int(md.size)))
+        if verbose:
+            log(f"  INTEGRATE: estimated sigma_D {new.sigma_d:.4f} deg, sigma_M {new.sigma_m:.4f} deg "
+                f"from {md.size} strong spots (iteration {it + 1})")
+        if (guard is None or guard) and it >= 1 and (new.sigma_d > 1.15 * model.sigma_d or
+                                                     new.sigma_m > 1.15 * model.sigma_m) and \
+                (model.sigma_d > 1.15 * prev.sigma_d or model.sigma_m > 1.15 * prev.sigma_m):
+            if verbose:
+                log("  INTEGRATE: spot-width estimate keeps growing with the box (contamination): keeping iteration "
+                    f"{it} values")
+            break
+        prev, model = model, new
+        if max_iter and it >= 2 and abs(model.sigma_d / prev.sigma_d - 1) < 0.03 and \
+                abs(model.sigma_m / prev.sigma_m - 1) < 0.03:
+            break
+    model.history = history
+    return model
