@@ -51,7 +51,17 @@ typedef struct xpg_call {
   double frames_phi0, osc, sd, sm, gain;
   float rcw[XPG_RC_K];
   int32_t use_rcw;
+  const struct xpg_widths *widths;   /* device pointer or NULL: when set, sd, sm and rcw are read from it */
 } xpg_call;
+
+/* Adaptive widths of one launch, in device memory (written by xpg_adapt_block, read by the kernels when
+ * xpg_call.widths is set): sigma_D, sigma_M (radians) and the rocking-curve mixture weights. */
+typedef struct xpg_widths {
+  double sd, sm;
+  float rcw[XPG_RC_K];
+  int32_t n_strong;        /* strong reflections the widths were fitted on (0: carried over) */
+  double sd_fit, sm_fit;   /* this block's own estimates before shrinkage (radians; 0 when not fitted) */
+} xpg_widths;
 
 /* Per-reflection table of a chunk (device pointers), xpg/integrate.py:509-517.  pxs may be NULL (1); has_p, pS,
  * pphi may be NULL (no partner); K = 0 means no neighbour columns. */
@@ -70,7 +80,8 @@ typedef struct xpg_refl {
 typedef struct xpg_pass1_out {
   double *bg, *nbg, *isum, *vsum, *vsum0, *xo, *yo, *zo, *m_d, *m_m, *frac_rec, *nfg, *zeta, *seff;
   int32_t *n_over, *n_fg_masked, *n_fg_nb;
-  float *fz;
+  float *fz;               /* (n, fz_stride): frames 0..Dz-1 written, the rest of each row zeroed */
+  int32_t fz_stride;       /* 0: Dz */
   float *img2, *J, *u1, *u2;
   uint8_t *all_valid2;
 } xpg_pass1_out;
@@ -153,6 +164,25 @@ int xpg_pass2(const xpg_detector *det, const xpg_call *call, const xpg_refl *ref
 int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
                      const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
                      const xpg_pass2_out *out, void *stream);
+
+/* Replaces _adapt_block (xpg/integrate.py:404-461) on the device, without a host round trip: from the pass-1 results
+ * of one phi block (rows [0, n) of the arrays below, in any fixed order) it selects the strong reflections (I/sigma >
+ * 15, no overload, frac_rec > 0.98, fb == 1), fits sigma_M on the 61-point log grid of _fit_rocking_width
+ * (xpg/integrate.py:1130-1154) against their per-frame net counts fz (n, 65), shrinks it towards the run's sigma_M with
+ * a 300-reflection prior, fits the 21 rocking-curve weights by non-negative least squares on the Cholesky factor of
+ * the Gram matrix (_nnls_gram, :1095-1104) regularised towards the Gaussian, and sigma_D from the median of the
+ * truncation-corrected second moments.  in: the widths the block's pass 1 used; out_pass2: widths for this block's
+ * pass 2; out_next: widths for the next block's pass 1 (in copied when fewer than 100 strong reflections).
+ * Per-row inputs (device): isum, vsum, frac_rec, m_d, phi (deg), zeta_k, Xlen (|X| mm), cosa (|X.ed3|/|X|) float64;
+ * n_over, fb, iz0 int32.  Scalars: run widths sd_run, sm_run (radians), frames_phi0_deg + osc_deg (phi of frame
+ * coordinate z = phi0 + osc z), qx.  workspace: xpg_adapt_workspace_bytes(n) bytes of device memory. */
+int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int32_t *n_over, const double *frac_rec,
+                    const double *m_d, const float *fz, const double *phi_deg, const double *zeta_k,
+                    const int32_t *iz0, const int32_t *fb, const double *Xlen, const double *cosa, double sd_run,
+                    double sm_run, double phi0_deg, double osc_deg, double qx, const xpg_widths *in,
+                    xpg_widths *out_pass2, xpg_widths *out_next, void *workspace, int64_t workspace_bytes,
+                    void *stream);
+int64_t xpg_adapt_workspace_bytes(int32_t n);
 
 /* Bytes of LDS one wavefront of pass 1 / pass 2 uses for a (Dz, Dy, Dx) shoebox (launch planning, autotuning). */
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx);

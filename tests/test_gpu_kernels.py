@@ -63,8 +63,9 @@ def _check_pass2(h2, o2, name):
     for k, tol in P2_TOL.items():
         assert _rel(h2[k], o2[k]) < tol, (name, k, _rel(h2[k], o2[k]))
     hp = np.asarray(o2["ipartner"]) != 0
-    if hp.any():
-        assert np.max(np.abs(h2["ipartner"][hp] - o2["ipartner"][hp]) / sig[hp]) < 1e-3, name
+    if hp.any():      # the partner's own scale: float32-level relative agreement
+        err = np.abs(h2["ipartner"][hp] - o2["ipartner"][hp])
+        assert np.all(err <= 1e-5 * np.abs(o2["ipartner"][hp]) + 1e-3 * sig[hp]), name
     return dI.max()
 
 
@@ -128,7 +129,7 @@ def test_fused_accumulation_matches_strong_selection():
     """pass1 with the profile grids fused in: the strong rows of xpg/integrate.py:530-541 splatted on the device."""
     from chunks import make_chunk
     from xpg_amd import integrate_kernels as K
-    c = make_chunk(n=200, seed=77, intensity=(500.0, 1e5))
+    c = make_chunk(n=200, seed=77, intensity=(500.0, 1e5), codes=False, neighbours=False)
     dev = torch.device("cuda:0")
     det = K.device_detector(c["det"], dev)
     call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"], c["rcw"])

@@ -34,7 +34,12 @@ class Detector(C.Structure):
 class Call(C.Structure):
     _fields_ = [("axis", C.c_double * 3), ("s0", C.c_double * 3), ("frames_phi0", C.c_double), ("osc", C.c_double),
                 ("sd", C.c_double), ("sm", C.c_double), ("gain", C.c_double), ("rcw", C.c_float * RC_K),
-                ("use_rcw", C.c_int32)]
+                ("use_rcw", C.c_int32), ("widths", C.c_void_p)]
+
+
+class Widths(C.Structure):
+    _fields_ = [("sd", C.c_double), ("sm", C.c_double), ("rcw", C.c_float * RC_K), ("n_strong", C.c_int32),
+                ("sd_fit", C.c_double), ("sm_fit", C.c_double)]
 
 
 class Refl(C.Structure):
@@ -49,7 +54,8 @@ class Refl(C.Structure):
 class Pass1Out(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m",
                                           "frac_rec", "nfg", "zeta", "seff", "n_over", "n_fg_masked", "n_fg_nb",
-                                          "fz", "img2", "J", "u1", "u2", "all_valid2")]
+                                          "fz")] + [("fz_stride", C.c_int32)] + \
+               [(k, C.c_void_p) for k in ("img2", "J", "u1", "u2", "all_valid2")]
 
 
 class Pass2Out(C.Structure):
@@ -85,6 +91,9 @@ _SIGS = {
     "xpg_pass2_frames": ([P(Detector), P(Call), P(Refl), P(Frames), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                           C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
+    "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 13 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
+                                                                                         C.c_void_p], C.c_int),
+    "xpg_adapt_workspace_bytes": ([C.c_int32], C.c_int64),
     "xpg_synth_background": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "xpg_synth_spots_add": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
 }

@@ -7,7 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-#include "../../include/xpg_integrate.h"
+#include "xpg_integrate.h"
 
 namespace xpg {
 
@@ -34,12 +34,40 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ---- cross-lane reductions (DPP within rows of 16 lanes, then the four row results by readlane) -----------------
+// Every lane returns the same value; call with all 64 lanes active.
+
+template <int CTRL>
+__device__ __forceinline__ float dppf(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ double dppd(double v) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float rlf(float v, int l) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double rld(double v, int l) {
+  long long b = __builtin_bit_cast(long long, v);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+constexpr int DPP_Q1 = 0xB1, DPP_Q2 = 0x4E, DPP_R4 = 0x124, DPP_R8 = 0x128;
+
 // ---- compensated float32 sums -----------------------------------------------------------------------------------
 
 struct csum {
   float s, c;
 };
 __device__ __forceinline__ csum cz() { return csum{0.f, 0.f}; }
+// TwoSum (Knuth): t + e == a.s + x exactly
 __device__ __forceinline__ void cadd(csum &a, float x) {
   float t = a.s + x;
   float bp = t - a.s;
@@ -47,36 +75,68 @@ __device__ __forceinline__ void cadd(csum &a, float x) {
   a.s = t;
   a.c += e;
 }
+__device__ __forceinline__ void cmerge(csum &a, float s2, float c2) {
+  float t = a.s + s2;
+  float bp = t - a.s;
+  float e = (a.s - (t - bp)) + (s2 - bp);
+  a.s = t;
+  a.c = (a.c + c2) + e;
+}
 __device__ __forceinline__ double creduce(csum a) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) {
-    float s2 = __shfl_xor(a.s, off);
-    float c2 = __shfl_xor(a.c, off);
-    float t = a.s + s2;
-    float bp = t - a.s;
-    float e = (a.s - (t - bp)) + (s2 - bp);
-    a.s = t;
-    a.c = (a.c + c2) + e;
-  }
-  return (double)a.s + (double)a.c;
+  cmerge(a, dppf<DPP_Q1>(a.s), dppf<DPP_Q1>(a.c));
+  cmerge(a, dppf<DPP_Q2>(a.s), dppf<DPP_Q2>(a.c));
+  cmerge(a, dppf<DPP_R4>(a.s), dppf<DPP_R4>(a.c));
+  cmerge(a, dppf<DPP_R8>(a.s), dppf<DPP_R8>(a.c));
+  // the four row sums, combined in float64 in a fixed order
+  double r0 = (double)rlf(a.s, 0) + (double)rlf(a.c, 0), r1 = (double)rlf(a.s, 16) + (double)rlf(a.c, 16);
+  double r2 = (double)rlf(a.s, 32) + (double)rlf(a.c, 32), r3 = (double)rlf(a.s, 48) + (double)rlf(a.c, 48);
+  return (r0 + r1) + (r2 + r3);
 }
 __device__ __forceinline__ double wsum_d(double v) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += dppd<DPP_Q1>(v);
+  v += dppd<DPP_Q2>(v);
+  v += dppd<DPP_R4>(v);
+  v += dppd<DPP_R8>(v);
+  return (rld(v, 0) + rld(v, 16)) + (rld(v, 32) + rld(v, 48));
 }
 __device__ __forceinline__ int wsum_i(int v) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  v += dppi<DPP_Q1>(v);
+  v += dppi<DPP_Q2>(v);
+  v += dppi<DPP_R4>(v);
+  v += dppi<DPP_R8>(v);
+  return (__builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16)) +
+         (__builtin_amdgcn_readlane(v, 32) + __builtin_amdgcn_readlane(v, 48));
 }
 __device__ __forceinline__ double wmax_d(double v) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  return v;
-}
-__device__ __forceinline__ double wmin_d(double v) {
-  for (int off = WAVE / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off));
-  return v;
+  v = fmax(v, dppd<DPP_Q1>(v));
+  v = fmax(v, dppd<DPP_Q2>(v));
+  v = fmax(v, dppd<DPP_R4>(v));
+  v = fmax(v, dppd<DPP_R8>(v));
+  return fmax(fmax(rld(v, 0), rld(v, 16)), fmax(rld(v, 32), rld(v, 48)));
 }
 __device__ __forceinline__ bool wany(bool b) { return __ballot(b) != 0; }
-__device__ __forceinline__ bool wall(bool b) { return __ballot(!b) == 0; }
+
+// Voxels of a (Dz, P) box visited by one lane: v = lane, lane + 64, ... with (j, p) = divmod(v, P) kept incrementally
+struct VoxIter {
+  int v, j, p, dj, dp, P, V;
+  __device__ __forceinline__ VoxIter(int lane, int P_, int V_) : P(P_), V(V_) {
+    v = lane;
+    j = lane / P_;
+    p = lane - j * P_;
+    dj = WAVE / P_;
+    dp = WAVE - dj * P_;
+  }
+  __device__ __forceinline__ bool ok() const { return v < V; }
+  __device__ __forceinline__ void next() {
+    v += WAVE;
+    j += dj;
+    p += dp;
+    if (p >= P) {
+      p -= P;
+      ++j;
+    }
+  }
+};
 
 // ---- float64 vector helpers (left-to-right order) ---------------------------------------------------------------
 
@@ -170,10 +230,16 @@ __device__ __forceinline__ void eps12(const xpg_detector &d, const Frame &f, dou
 
 // ---- rocking curve (xpg/integrate_kernels.py:130-141), one mixture term ------------------------------------------
 
-// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when use_rcw == 0 (j == 0 only)
+// effective widths of a launch: from the device widths record when set (adaptive chain on the device)
+__device__ __forceinline__ double sd_of(const xpg_call &c) { return c.widths ? c.widths->sd : c.sd; }
+__device__ __forceinline__ double sm_of(const xpg_call &c) { return c.widths ? c.widths->sm : c.sm; }
+__device__ __forceinline__ bool use_rcw_of(const xpg_call &c) { return c.widths != nullptr || c.use_rcw != 0; }
+__device__ __forceinline__ float rcw_of(const xpg_call &c, int j) { return c.widths ? c.widths->rcw[j] : c.rcw[j]; }
+
+// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when no mixture weights (j == 0 only)
 __device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double hi, double phi, double zeta, int j) {
-  double s = c.sm / fmax(fabs(zeta), 1e-3);
-  if (!c.use_rcw) {
+  double s = sm_of(c) / fmax(fabs(zeta), 1e-3);
+  if (!use_rcw_of(c)) {
     double a = (hi - phi) / (s * sqrt(2.0));
     double b = (lo - phi) / (s * sqrt(2.0));
     return 0.5 * (erf(a) - erf(b));
@@ -182,7 +248,7 @@ __device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double h
   double cj = (double)RC_C32[j];
   double a = ((hi - phi) - cj * s) / sw;
   double b = ((lo - phi) - cj * s) / sw;
-  return ((double)c.rcw[j] * 0.5) * (erf(a) - erf(b));
+  return ((double)rcw_of(c, j) * 0.5) * (erf(a) - erf(b));
 }
 
 __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb, int j, double &lo, double &hi) {
@@ -197,7 +263,7 @@ __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb,
 // in component order as the oracle does.
 __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, double zeta, int Dz, double *scratch,
                               double *R, int lane) {
-  const int nt = c.use_rcw ? RCK : 1;
+  const int nt = use_rcw_of(c) ? RCK : 1;
   for (int t = lane; t < Dz * nt; t += WAVE) {
     int j = t / nt, q = t - j * nt;
     double lo, hi;
@@ -314,7 +380,7 @@ __device__ __forceinline__ Bil bilinear(double u1, double u2) {
 // voxels with ((dx^2 + dy^2) + dz^2) < 1.  tab: scratch of Dx + Dy + Dz doubles.
 __device__ void neighbour_mask(const xpg_refl &R, int r, const double *xs, const double *ys, const double *zs,
                                int Dz, int Dy, int Dx, int skip_first, uint8_t *vflag, uint8_t bit, double *tab,
-                               int lane) {
+                               const uint8_t *pix_row, int lane) {
   const int K = R.K;
   if (K <= 0) return;
   double lo_x = xs[0], hi_x = xs[Dx - 1], lo_y = ys[0], hi_y = ys[Dy - 1], lo_z = zs[0], hi_z = zs[Dz - 1];
@@ -349,10 +415,9 @@ __device__ void neighbour_mask(const xpg_refl &R, int r, const double *xs, const
       tab[i] = v * v;
     }
     wave_sync();
-    for (int v = lane; v < V; v += WAVE) {
-      int j = v / P, p = v - j * P;
-      int iy = p / Dx, ix = p - iy * Dx;
-      if ((dx2[ix] + dy2[iy]) + dz2[j] < 1.0) vflag[v] |= bit;
+    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+      const int iy = pix_row[it.p], ix = it.p - iy * Dx;
+      if ((dx2[ix] + dy2[iy]) + dz2[it.j] < 1.0) vflag[it.v] |= bit;
     }
     wave_sync();
   }

@@ -41,17 +41,19 @@ static int fail(int code, const char *fmt, ...) {
 //   vals  float[V]    shoebox values                 vflag u8[V]   per-voxel flags
 //   scr   doubles     max(21 Dz, 3 P): rocking-curve terms, then the partner's u1, u2, J in pass 2
 //   U1,U2 double[P]   u1, u2                          Jd    double[P] pixel area
-//   pa    8P bytes    pass 1: bref, thr (float)  /  pass 2: p2 (double)
-//   pb    8P bytes    pass 1: e1v, e2v (float)   /  pass 2: pp2 (double)
-//   pc    4P bytes    pass 2: bref (float)            pflag u8[P]
-//   R,Rp  double[Dz]  rocking-curve fractions         fr    16 Dz bytes: zz, e3 (float), frame flags
+//   pa    8P bytes    pass 1: bref, thr (float)    /  pass 2: p2 (double)
+//   pb    8P bytes    pass 1: e1v, e2v (float)     /  pass 2: pp2 (double)
+//   pc    8P bytes    pass 1: x, y (float)         /  pass 2: bref, b*bref (float)
+//   pd    8P bytes                                     pass 2: p2, pp2 (float)
+//   prow  u8[P]       row of each pixel               pflag u8[P]
+//   R,Rp  double[Dz]  rocking-curve fractions         fr    16 Dz bytes: per-frame floats and flags
 //   co    double[Dx + Dy + Dz] box coordinates        tab   double[Dx + Dy + Dz] neighbour terms
 // ---------------------------------------------------------------------------------------------------------------------
 
 __host__ __device__ static inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 struct Layout {
-  size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, pflag, R, Rp, fr, co, tab, bytes;
+  size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, pd, prow, pflag, R, Rp, fr, co, tab, bytes;
 };
 
 __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
@@ -69,7 +71,9 @@ __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
   L.Jd = o; o += 8 * P;
   L.pa = o; o += 8 * P;
   L.pb = o; o += 8 * P;
-  L.pc = o; o = al8(o + 4 * P);
+  L.pc = o; o += 8 * P;
+  L.pd = o; o += 8 * P;
+  L.prow = o; o = al8(o + P);
   L.pflag = o; o = al8(o + P);
   L.R = o; o += 8 * Dz;
   L.Rp = o; o += 8 * Dz;
@@ -80,7 +84,9 @@ __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
   return L;
 }
 
-enum : uint8_t { VF_NBM = 1, VF_USE = 2 };      // per-voxel flags
+// per-voxel flags: neighbour ellipsoid, fit mask (pass 2), background candidate, foreground (fg & valid & ~nbm),
+// foreground ellipse x foreground frame
+enum : uint8_t { VF_NBM = 1, VF_USE = 2, VF_BGM = 4, VF_FGV = 8, VF_FG = 16 };
 enum : uint8_t { PF_FG2 = 1, PF_BGR = 2 };      // per-pixel flags: r^2 <= FG_R^2, r^2 >= BG_R^2
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -218,16 +224,19 @@ __device__ __forceinline__ Box box_of(const xpg_refl &R, int r) {
   return b;
 }
 
+// per-pixel row index (prow), then the shoebox of reflection r into vals (from sb, or extracted from the frame store
+// when sb == nullptr), flags cleared
 __device__ __forceinline__ void stage_values(const Box &b, const float *sb, const xpg_frames &F, float *vals,
-                                             uint8_t *vflag, int lane) {
+                                             uint8_t *vflag, uint8_t *prow, int lane) {
+  for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
+  wave_sync();
   if (sb) {
     const float *src = sb + (int64_t)b.r * b.V;
     for (int v = lane; v < b.V; v += WAVE) vals[v] = src[v];
   } else {
-    for (int v = lane; v < b.V; v += WAVE) {
-      int j = v / b.P, p = v - j * b.P;
-      int iy = p / b.Dx, ix = p - iy * b.Dx;
-      vals[v] = voxel_value(F, b.iz0, b.fb, j, b.iy0 + iy, b.ix0 + ix, b.s);
+    for (VoxIter it(lane, b.P, b.V); it.ok(); it.next()) {
+      const int iy = prow[it.p], ix = it.p - iy * b.Dx;
+      vals[it.v] = voxel_value(F, b.iz0, b.fb, it.j, b.iy0 + iy, b.ix0 + ix, b.s);
     }
   }
   for (int v = lane; v < b.V; v += WAVE) vflag[v] = 0;
@@ -315,7 +324,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   float *bref = (float *)(base + L.pa), *thr = bref + P;
   float *e1v = (float *)(base + L.pb), *e2v = e1v + P;
-  uint8_t *pflag = base + L.pflag;
+  float *xf = (float *)(base + L.pc), *yf = xf + P;
+  uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R);
   float *zzf = (float *)(base + L.fr), *e3f = zzf + a.R.Dz;
   uint8_t *fflag = (uint8_t *)(e3f + a.R.Dz);
@@ -326,20 +336,23 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
 
-  stage_values(b, a.sb, a.F, vals, vflag, lane);
+  stage_values(b, a.sb, a.F, vals, vflag, prow, lane);
   Frame f;
   kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
   rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
-  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, U1, U2, Jd, lane);
+  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   wave_sync();
   for (int p = lane; p < P; p += WAVE) {
+    const int iy = prow[p], ix = p - iy * Dx;
     const double u1 = U1[p], u2 = U2[p];
     const double r2 = u1 * u1 + u2 * u2;
     pflag[p] = (r2 <= 9.0 ? PF_FG2 : 0) | (r2 >= 12.25 ? PF_BGR : 0);
     bref[p] = template_at(det, co, Dx, p);
     e1v[p] = (float)(u1 * seff);
     e2v[p] = (float)(u2 * seff);
+    xf[p] = (float)co[ix];
+    yf[p] = (float)co[Dx + iy];
   }
   frames_fg(Rd, Dz, fflag, lane);
   for (int j = lane; j < Dz; j += WAVE) {
@@ -349,21 +362,31 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     zzf[j] = (float)co[Dx + Dy + j];
   }
   wave_sync();
-  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, lane);
+  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, prow, lane);
+
+  // per-voxel classes: fg = fg2 & fgz; fgv = fg & valid & ~nbm; background candidates (xpg/integrate_kernels.py:291-305)
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    const float c = vals[it.v];
+    uint8_t fl = vflag[it.v];
+    const bool valid = c >= 0.f, nbm = fl & VF_NBM;
+    const bool fg = (pflag[it.p] & PF_FG2) && fflag[it.j];
+    if (fg) fl |= VF_FG;
+    if (fg && valid && !nbm) fl |= VF_FGV;
+    if (!fg && valid && !nbm && (pflag[it.p] & PF_BGR)) fl |= VF_BGM;
+    vflag[it.v] = fl;
+  }
+  wave_sync();
 
   // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
   double bk = 0.0, nbg = 0.0;
   for (int round = 0; round < 5; ++round) {
     csum sc = cz(), sT = cz();
-    for (int v = lane; v < V; v += WAVE) {
-      const int j = v / P, p = v - j * P;
-      const float c = vals[v];
-      const uint8_t pf = pflag[p];
-      const bool fg = (pf & PF_FG2) && fflag[j];
-      const bool bgm = c >= 0.f && !fg && !(vflag[v] & VF_NBM) && (pf & PF_BGR);
-      if (bgm && (round == 0 || c <= thr[p])) {
+    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+      if (!(vflag[it.v] & VF_BGM)) continue;
+      const float c = vals[it.v];
+      if (round == 0 || c <= thr[it.p]) {
         cadd(sc, c);
-        cadd(sT, bref[p]);
+        cadd(sT, bref[it.p]);
       }
     }
     const double Sc = creduce(sc);
@@ -379,22 +402,19 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     }
   }
 
-  // ---- sums over the foreground (fg & valid & ~nbm) ----
+  // ---- sums over the foreground ----
   csum s_fgc = cz(), s_fgT = cz(), s_wpos = cz(), s_wx = cz(), s_wy = cz(), s_wz = cz();
   csum s_m1 = cz(), s_m2 = cz(), s_mdd = cz(), s_m3 = cz(), s_m33 = cz();
   int n_fg = 0, n_over = 0, n_fgm = 0, n_fgnb = 0;
-  for (int v = lane; v < V; v += WAVE) {
-    const int j = v / P, p = v - j * P;
-    const float c = vals[v];
-    const bool valid = c >= 0.f;
-    const bool nbm = vflag[v] & VF_NBM;
-    const bool fg = (pflag[p] & PF_FG2) && fflag[j];
-    if (!fg) continue;
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    const uint8_t fl = vflag[it.v];
+    if (!(fl & VF_FG)) continue;
+    const float c = vals[it.v];
     n_over += (c == -2.0f);
-    n_fgm += (!valid && c > -2.5f);
-    n_fgnb += nbm;
-    if (!valid || nbm) continue;
-    const int iy = p / Dx, ix = p - iy * Dx;
+    n_fgm += (c < 0.f && c > -2.5f);
+    n_fgnb += (fl & VF_NBM) ? 1 : 0;
+    if (!(fl & VF_FGV)) continue;
+    const int p = it.p, j = it.j;
     const double net = (double)c - bk * (double)bref[p];
     const float netf = (float)net;
     n_fg += 1;
@@ -402,8 +422,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     cadd(s_fgT, bref[p]);
     if (net > 0.0) {
       cadd(s_wpos, netf);
-      cadd(s_wx, netf * (float)co[ix]);
-      cadd(s_wy, netf * (float)co[Dx + iy]);
+      cadd(s_wx, netf * xf[p]);
+      cadd(s_wy, netf * yf[p]);
       cadd(s_wz, netf * zzf[j]);
     }
     const float a1 = e1v[p], a2 = e2v[p], a3 = e3f[j];
@@ -435,17 +455,38 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const double m_m = Sm33 / sws - mu3 * mu3;
 
   // ---- per frame: fz (net counts in the foreground ellipse), recorded frames -> frac_rec ----
-  float *fz_out = a.out.fz ? a.out.fz + (int64_t)r * Dz : nullptr;
-  for (int j = lane; j < Dz; j += WAVE) {
-    double fzj = 0.0;
-    bool anyrec = false;
-    for (int p = 0; p < P; ++p) {
-      const float c = vals[j * P + p];
-      anyrec |= c > -2.5f;
-      if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) fzj += (double)c - bk * (double)bref[p];
+  const int fzs = a.out.fz_stride > 0 ? a.out.fz_stride : Dz;
+  float *fz_out = a.out.fz ? a.out.fz + (int64_t)r * fzs : nullptr;
+  if (fz_out)
+    for (int j = Dz + lane; j < fzs; j += WAVE) fz_out[j] = 0.f;
+  if (Dz <= 16) {
+    for (int j = 0; j < Dz; ++j) {               // lanes over the pixels of frame j, then a wave reduction
+      double part = 0.0;
+      bool rec = false;
+      for (int p = lane; p < P; p += WAVE) {
+        const float c = vals[j * P + p];
+        rec |= c > -2.5f;
+        if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) part += (double)c - bk * (double)bref[p];
+      }
+      const double fzj = wsum_d(part);
+      const bool anyrec = wany(rec);
+      if (lane == 0) {
+        if (fz_out) fz_out[j] = (float)fzj;
+        scr[j] = anyrec ? Rd[j] : 0.0;
+      }
     }
-    if (fz_out) fz_out[j] = (float)fzj;
-    scr[j] = anyrec ? Rd[j] : 0.0;
+  } else {
+    for (int j = lane; j < Dz; j += WAVE) {      // one lane per frame
+      double fzj = 0.0;
+      bool anyrec = false;
+      for (int p = 0; p < P; ++p) {
+        const float c = vals[j * P + p];
+        anyrec |= c > -2.5f;
+        if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) fzj += (double)c - bk * (double)bref[p];
+      }
+      if (fz_out) fz_out[j] = (float)fzj;
+      scr[j] = anyrec ? Rd[j] : 0.0;
+    }
   }
   wave_sync();
   double frac_rec = 0.0;
@@ -590,96 +631,110 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   if (r >= a.R.n) return;
   const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx);
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
+  const int P = a.R.Dy * a.R.Dx;
   float *vals = (float *)(base + L.vals);
   uint8_t *vflag = base + L.vflag;
   double *scr = (double *)(base + L.scr);
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   double *p2 = (double *)(base + L.pa), *pp2 = (double *)(base + L.pb);
-  float *bref = (float *)(base + L.pc);
-  uint8_t *pflag = base + L.pflag;
+  float *bref = (float *)(base + L.pc), *bpx = bref + P;
+  float *p2f = (float *)(base + L.pd), *pp2f = p2f + P;
+  uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R), *Rpd = (double *)(base + L.Rp);
-  uint8_t *fflag = base + L.fr;
+  float *Rf = (float *)(base + L.fr), *Rpf = Rf + a.R.Dz;
+  uint8_t *fflag = (uint8_t *)(Rpf + a.R.Dz);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
 
   Box b = box_of(a.R, r);
   b.region = min(max(b.region, 0), a.n_regions - 1);   // memory safety; the host validates region
-  const int P = b.P, V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  const int V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
   const double has_p = a.R.has_p ? (double)a.R.has_p[r] : 0.0;
   const bool partner = has_p > 0.5;
+  const float hpf = (float)has_p;
   const double bgr = a.bg[r];
   const double gain = call.gain;
 
-  stage_values(b, a.sb, a.F, vals, vflag, lane);
+  stage_values(b, a.sb, a.F, vals, vflag, prow, lane);
   Frame f;
   kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
   rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
-  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, U1, U2, Jd, lane);
+  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   profile_pixels(a.prof, b.region, U1, U2, Jd, seff, P, p2, lane);
   wave_sync();
   for (int p = lane; p < P; p += WAVE) {
     const double r2 = U1[p] * U1[p] + U2[p] * U2[p];
     pflag[p] = r2 <= 9.0 ? PF_FG2 : 0;
-    bref[p] = template_at(det, co, Dx, p);
+    const float br = template_at(det, co, Dx, p);
+    bref[p] = br;
+    bpx[p] = (float)(bgr * (double)br);
+    p2f[p] = (float)p2[p];
+    pp2f[p] = 0.f;
   }
   frames_fg(Rd, Dz, fflag, lane);
+  for (int j = lane; j < Dz; j += WAVE) {
+    Rf[j] = (float)Rd[j];
+    Rpf[j] = 0.f;
+  }
   if (partner) {
     // the overlapping partner's profile on this box's grid, fresh geometry (xpg/integrate_kernels.py:393-400)
     Frame fp;
     kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
     rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], fp.zeta, Dz, scr, Rpd, lane);
     double *PU1 = scr, *PU2 = scr + P, *PJ = scr + 2 * P;
-    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, call.sd, PU1, PU2, PJ, lane);
+    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), PU1, PU2, PJ, lane);
     profile_pixels(a.prof, b.region, PU1, PU2, PJ, seffp, P, pp2, lane);
+    for (int p = lane; p < P; p += WAVE) pp2f[p] = (float)pp2[p];
+    for (int j = lane; j < Dz; j += WAVE) Rpf[j] = (float)Rpd[j];
   }
   wave_sync();
-  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, lane);
+  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, prow, lane);
 
   // ---- fit mask: valid & ~nbm & (p > 0.01 max p | pp > 0.01 max(max pp, 1e-12)), float64 products ----
   double pmax = -INFINITY, ppmax = -INFINITY;
-  for (int v = lane; v < V; v += WAVE) {
-    const int j = v / P, p = v - j * P;
-    pmax = fmax(pmax, p2[p] * Rd[j]);
-    ppmax = fmax(ppmax, partner ? (pp2[p] * Rpd[j]) * has_p : 0.0);
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    pmax = fmax(pmax, p2[it.p] * Rd[it.j]);
+    ppmax = fmax(ppmax, partner ? (pp2[it.p] * Rpd[it.j]) * has_p : 0.0);
   }
   pmax = wmax_d(pmax);
   ppmax = wmax_d(ppmax);
   const double pthr = 0.01 * pmax, ppthr = 0.01 * fmax(ppmax, 1e-12);
   csum s_net = cz(), s_pw = cz();
-  for (int v = lane; v < V; v += WAVE) {
-    const int j = v / P, p = v - j * P;
-    const float c = vals[v];
-    const double pd = p2[p] * Rd[j];
-    const double ppd = partner ? (pp2[p] * Rpd[j]) * has_p : 0.0;
-    if (c >= 0.f && !(vflag[v] & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
-      vflag[v] |= VF_USE;
-      cadd(s_net, (float)((double)c - bgr * (double)bref[p]));
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    const float c = vals[it.v];
+    const double pd = p2[it.p] * Rd[it.j];
+    const double ppd = partner ? (pp2[it.p] * Rpd[it.j]) * has_p : 0.0;
+    uint8_t fl = vflag[it.v];
+    if ((pflag[it.p] & PF_FG2) && fflag[it.j]) fl |= VF_FG;
+    if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
+      fl |= VF_USE;
+      cadd(s_net, c - bpx[it.p]);
       cadd(s_pw, (float)pd + (float)ppd);
     }
+    vflag[it.v] = fl;
   }
   wave_sync();
   double I = creduce(s_net) / fmax(creduce(s_pw), 1e-6);
   double Ip = I * has_p;
 
   // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
+  const float gainf = (float)gain;
   double a11 = 0.0, a12 = 0.0, a22 = 1.0, det2 = 1.0, Iw = I, Ipw = Ip;
-  for (int it = 0; it < N_IRLS; ++it) {
+  for (int it_ = 0; it_ < N_IRLS; ++it_) {
     Iw = I;
     Ipw = Ip;
     const float Ii = (float)fmax(I, 0.0), Ipi = (float)fmax(Ip, 0.0);
     csum s11 = cz(), s12 = cz(), s22 = cz(), sb1 = cz(), sb2 = cz();
-    for (int v = lane; v < V; v += WAVE) {
-      if (!(vflag[v] & VF_USE)) continue;
-      const int j = v / P, p = v - j * P;
-      const double bpd = bgr * (double)bref[p];
-      const float bpx = (float)bpd;
-      const float net = (float)((double)vals[v] - bpd);
-      const float pf = (float)(p2[p] * Rd[j]);
-      const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
-      const float mu = (bpx + Ii * pf) + Ipi * ppf;
-      const float iv = 1.0f / ((float)gain * fmaxf(mu, (float)VAR_FLOOR));
+    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+      if (!(vflag[it.v] & VF_USE)) continue;
+      const float b0 = bpx[it.p];
+      const float net = vals[it.v] - b0;
+      const float pf = p2f[it.p] * Rf[it.j];
+      const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
+      const float mu = (b0 + Ii * pf) + Ipi * ppf;
+      const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
       cadd(s11, (pf * pf) * iv);
       cadd(sb1, (pf * net) * iv);
       if (partner) {
@@ -697,9 +752,9 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     const double I_old = I;
     I = (a22 * b1 - a12 * b2) / det2;
     Ip = (a11 * b2 - a12 * b1) / det2 * has_p;
-    if (IRLS_TOL > 0 && it >= 1 && it < N_IRLS - 1) {
+    if (IRLS_TOL > 0 && it_ >= 1 && it_ < N_IRLS - 1) {
       const double dz = fabs(I - I_old) * sqrt(fmax(det2 / fmax(a22, 1e-30), 0.0));
-      if (dz < IRLS_TOL) break;     // uniform across the wavefront: every lane holds the same reduced values
+      if (dz < IRLS_TOL) break;     // uniform: every lane holds the same reduced values
     }
   }
 
@@ -708,20 +763,19 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const float Ipos = (float)fmax(Ip, 0.0);
   csum sC = cz(), s110 = cz(), sC0 = cz(), swp = cz(), sres = cz();
   int nuse = 0;
-  for (int v = lane; v < V; v += WAVE) {
-    if (!(vflag[v] & VF_USE)) continue;
-    const int j = v / P, p = v - j * P;
-    const double bpd = bgr * (double)bref[p];
-    const float bpx = (float)bpd;
-    const float net = (float)((double)vals[v] - bpd);
-    const float pf = (float)(p2[p] * Rd[j]);
-    const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
-    const float mu = (bpx + Ii * pf) + Ipi * ppf;
-    const float iv = 1.0f / ((float)gain * fmaxf(mu, (float)VAR_FLOOR));
-    const float iv0 = 1.0f / ((float)gain * fmaxf(bpx, (float)VAR_FLOOR));
-    cadd(sC, (pf * bref[p]) * iv);
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    if (!(vflag[it.v] & VF_USE)) continue;
+    const float b0 = bpx[it.p];
+    const float net = vals[it.v] - b0;
+    const float pf = p2f[it.p] * Rf[it.j];
+    const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
+    const float mu = (b0 + Ii * pf) + Ipi * ppf;
+    const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
+    const float iv0 = 1.0f / (gainf * fmaxf(b0, (float)VAR_FLOOR));
+    const float pb = pf * bref[it.p];
+    cadd(sC, pb * iv);
     cadd(s110, (pf * pf) * iv0);
-    cadd(sC0, (pf * bref[p]) * iv0);
+    cadd(sC0, pb * iv0);
     cadd(swp, pf);
     cadd(sres, net - Ipos * ppf);
     nuse += 1;
@@ -729,21 +783,21 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double C = creduce(sC), A110 = creduce(s110), C0 = creduce(sC0), Swp = creduce(swp), Sres = creduce(sres);
   nuse = wsum_i(nuse);
   const double nu = fmax((double)nuse, 1.0);
-  const double pm = Swp / nu, nm = Sres / nu;
+  const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
   csum sdd = cz(), sdp = cz(), snn = cz();
   csum s_pv = cz(), s_pt = cz(), s_pfg = cz();
-  for (int v = lane; v < V; v += WAVE) {
-    const int j = v / P, p = v - j * P;
-    const float pf = (float)(p2[p] * Rd[j]);
-    const float c = vals[v];
+  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
+    const uint8_t fl = vflag[it.v];
+    const float pf = p2f[it.p] * Rf[it.j];
+    const float c = vals[it.v];
     cadd(s_pt, pf);
-    if (c >= 0.f && !(vflag[v] & VF_NBM)) cadd(s_pv, pf);
-    if ((pflag[p] & PF_FG2) && fflag[j]) cadd(s_pfg, pf);
-    if (!(vflag[v] & VF_USE)) continue;
-    const float net = (float)((double)c - bgr * (double)bref[p]);
-    const float ppf = partner ? (float)((pp2[p] * Rpd[j]) * has_p) : 0.f;
-    const float dp = (float)((double)pf - pm);
-    const float dn = (float)((double)(net - Ipos * ppf) - nm);
+    if (c >= 0.f && !(fl & VF_NBM)) cadd(s_pv, pf);
+    if (fl & VF_FG) cadd(s_pfg, pf);
+    if (!(fl & VF_USE)) continue;
+    const float net = c - bpx[it.p];
+    const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
+    const float dp = pf - pm;
+    const float dn = (net - Ipos * ppf) - nm;
     cadd(sdp, dp * dn);
     cadd(sdd, dp * dp);
     cadd(snn, dn * dn);
@@ -862,6 +916,7 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   if (rc) return rc;
   if (!det || !call || !out) return fail(XPG_ERR_ARG, "det / call / out is NULL");
   if (!sb && (!frames || !frames->data)) return fail(XPG_ERR_ARG, "neither sb nor a frame store");
+  if (out->fz && out->fz_stride != 0 && out->fz_stride < refl->Dz) return fail(XPG_ERR_ARG, "fz_stride < Dz");
   if (!out->bg || !out->nbg || !out->isum || !out->vsum || !out->vsum0 || !out->xo || !out->yo || !out->zo ||
       !out->m_d || !out->m_m || !out->frac_rec || !out->nfg || !out->zeta || !out->seff || !out->n_over ||
       !out->n_fg_masked || !out->n_fg_nb)

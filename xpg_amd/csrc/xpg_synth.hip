@@ -3,7 +3,7 @@
 #include <math.h>
 #include <stdint.h>
 
-#include "../../include/xpg_synth.h"
+#include "xpg_synth.h"
 
 namespace {
 

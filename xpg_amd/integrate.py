@@ -15,6 +15,7 @@ What runs where:
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import time
 from dataclasses import dataclass, field
@@ -386,9 +387,14 @@ def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, d
 class Sweep:
     """A planned sweep on the device: the reflection table uploaded once (SweepPlan), the frame store view, and the
     per-block loop of pass 1 -> adaptive widths -> profile normalisation -> pass 2 (xpg/integrate.py:463-574), which
-    ``run`` executes (repeatably: benchmarks time it with the inputs resident in HBM)."""
+    ``run`` executes (repeatably: benchmarks time it with the inputs resident in HBM).
+
+    The adaptive chain stays on the device: pass 1 of a block reads its widths record (sigma_D, sigma_M, rocking-curve
+    weights) from device memory, xpg_adapt_block writes the records for the block's pass 2 and for the next block, so
+    a whole sweep is a sequence of launches on one stream with no host round trip."""
 
     def __init__(self, images, exp_blocks, model, hp, opts):
+        from . import _lib as L
         self.images, self.exp_blocks, self.model, self.hp, self.o = images, exp_blocks, model, hp, opts
         device = _kernel_device(images)
         self.device = device
@@ -401,8 +407,31 @@ class Sweep:
         self.by_block = {}
         for gi, (b, gk, start, m) in enumerate(self.sp.groups):
             self.by_block.setdefault(b, []).append((gi, gk, start, m))
+        self.order = [b for b in hp["block_order"] if b in self.by_block]
         self.dets = {}
         self.t_plan = 0.0
+        # per-row inputs of the device adaptive fits, in table order
+        refl, perm = hp["refl"], self.sp.perm
+        Xlen = np.empty(perm.size)
+        cosa = np.empty(perm.size)
+        for b in self.order:
+            g = exp_blocks[b].geom
+            s0 = self.by_block[b][0][2]
+            n_b = sum(m for *_, m in self.by_block[b])
+            rows = perm[s0:s0 + n_b]
+            Xv = g.pixel_to_lab(refl["x"][rows], refl["y"][rows])
+            Xlen[s0:s0 + n_b] = np.linalg.norm(Xv, axis=1)
+            cosa[s0:s0 + n_b] = np.abs(Xv @ g.ed3) / Xlen[s0:s0 + n_b]
+        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
+        self.ad = dict(phi=T(refl["phi"][perm], torch.float64), zeta_k=T(refl["zeta_k"][perm], torch.float64),
+                       Xlen=T(Xlen, torch.float64), cosa=T(cosa, torch.float64))
+        nmax = max([sum(m for *_, m in self.by_block[b]) for b in self.order] or [1])
+        self.ws_bytes = int(L.lib().xpg_adapt_workspace_bytes(int(nmax)))
+        self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
+        self.wsz = C.sizeof(L.Widths)
+        nb = len(self.order)
+        self.widths = torch.zeros((2 * nb + 2) * self.wsz, dtype=torch.uint8, device=device)
+        self.fz = torch.zeros((self.sp.n, max(DZ_BUCKETS)), dtype=torch.float32, device=device)
 
     def det_for(self, g):
         key = id(g)
@@ -413,51 +442,69 @@ class Sweep:
     def launches(self):
         """(kernel, block, group index, n reflections, (Dz, Dy, Dx)) of every launch, in order."""
         out = []
-        for b in self.hp["block_order"]:
-            for gi, (B, bx, _), start, m in self.by_block.get(b, []):
+        for b in self.order:
+            for gi, (B, bx, _), start, m in self.by_block[b]:
                 out.append(("pass1", b, gi, m, (B, bx, bx)))
             if not self.o["moments_only"]:
-                for gi, (B, bx, _), start, m in self.by_block.get(b, []):
+                for gi, (B, bx, _), start, m in self.by_block[b]:
                     out.append(("pass2", b, gi, m, (B, bx, bx)))
         return out
 
+    def _widths_ptr(self, slot):
+        return self.widths.data_ptr() + slot * self.wsz
+
+    def _init_widths(self):
+        from . import _lib as L
+        w = L.Widths()
+        w.sd, w.sm = float(self.o["sd"]), float(self.o["sm"])
+        w.rcw[:] = [float(v) for v in rc_default_weights()]
+        raw = torch.frombuffer(bytearray(bytes(w)), dtype=torch.uint8)
+        nb = len(self.order)
+        # slot 2i: pass 1 of the i-th block, slot 2i + 1: its pass 2 (every slot starts at the run's widths)
+        self.widths.copy_(raw.repeat(2 * nb + 2).to(self.device))
+
     def run(self, timer=None):
         """Device execution of the whole sweep; ``timer(kind, b, gi)`` may return a context that brackets launches."""
+        from . import _lib as L
         o, sp, device, model = self.o, self.sp, self.device, self.model
         N = sp.n
         f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
         o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
         o2 = {k: f64() for k in K.P2_F64}
-        fz_groups, prof_by_block, block_params = {}, {}, {}
+        prof_by_block = {}
         gauss = gaussian_profile()
-        adapt = dict(sd=o["sd"], sm=o["sm"], rcw=rc_default_weights())
-        hp = self.hp
         nreg = o["nreg"]
+        self._init_widths()
         t_dev0 = time.time()
         nullctx = _Null()
-        for b in hp["block_order"]:
-            groups = self.by_block.get(b, [])
-            if not groups:
-                continue
-            sd_b = adapt["sd"] if o["adaptive"] else o["sd"]
-            sm_b, rcw_b = (adapt["sm"], adapt["rcw"]) if o["adaptive"] else (o["sm"], rc_default_weights())
+        st = L.stream_ptr(device)
+        for i, b in enumerate(self.order):
+            groups = self.by_block[b]
             g = self.exp_blocks[b].geom
             det = self.det_for(g)
             frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
             osc = float(np.radians(g.osc))
-            call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, sd_b, sm_b, model.gain, rcw_b)
+            call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+            call1.widths = C.c_void_p(self._widths_ptr(2 * i))
             acc = None
             if o["learn_profiles"]:
                 num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
             for gi, (B, bx, _), start, m in groups:
                 view = sp.refl_view(start, m, (B, bx, bx))
                 out = {k: v[start:start + m] for k, v in o1.items()}
-                out["fz"] = fz_groups[gi] = torch.empty((m, B), dtype=torch.float32, device=device)
+                out["fz"] = self.fz[start:start + m]
                 with (timer("pass1", b, gi) if timer else nullctx):
-                    _launch_pass1(det, call1, view, self.store, out, acc)
+                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS))
+            s0 = groups[0][2]
+            n_b = sum(m for *_, m in groups)
             if o["adaptive"]:
-                sm_b, rcw_b = _adapt_block(b, g, groups, o1, fz_groups, sp, hp["refl"], hp["iz0"], hp["fb"], sd_b,
-                                           sm_b, rcw_b, o["sd"], o["sm"], adapt, block_params)
+                a = lambda t: C.c_void_p(t.data_ptr() + t.element_size() * s0 * (t.numel() // N))
+                L.check(L.lib().xpg_adapt_block(
+                    n_b, a(o1["isum"]), a(o1["vsum"]), a(o1["n_over"]), a(o1["frac_rec"]), a(o1["m_d"]), a(self.fz),
+                    a(self.ad["phi"]), a(self.ad["zeta_k"]), a(sp.t["iz0"]), a(sp.t["fb"]), a(self.ad["Xlen"]),
+                    a(self.ad["cosa"]), float(o["sd"]), float(o["sm"]), float(g.z_to_phi(0.0)), float(g.osc),
+                    float(g.qx), C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
+                    C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes, st))
             if o["learn_profiles"]:
                 prof, _, _ = K.normalise_profiles(num_fx, den_fx, device)
             else:
@@ -465,7 +512,8 @@ class Sweep:
             prof_by_block[b] = prof
             if o["moments_only"]:
                 continue
-            call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, sd_b, sm_b, model.gain, rcw_b)
+            call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+            call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
             for gi, (B, bx, _), start, m in groups:
                 view = sp.refl_view(start, m, (B, bx, bx))
                 out = {k: v[start:start + m] for k, v in o2.items()}
@@ -473,8 +521,22 @@ class Sweep:
                     _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
                                   o1["nbg"][start:start + m], out)
         torch.cuda.synchronize(device)
-        return dict(o1=o1, o2=o2, fz_groups=fz_groups, prof_by_block=prof_by_block, block_params=block_params,
-                    t_gpu=time.time() - t_dev0)
+        return dict(o1=o1, o2=o2, prof_by_block=prof_by_block, t_gpu=time.time() - t_dev0)
+
+    def block_params(self):
+        """Per-block widths (xpg/integrate.py:427-429, :458-460) from the device records of the last run."""
+        from . import _lib as L
+        raw = self.widths.cpu().numpy().tobytes()
+        out = {}
+        for i, b in enumerate(self.order):
+            w1 = L.Widths.from_buffer_copy(raw[2 * i * self.wsz:(2 * i + 1) * self.wsz])
+            w2 = L.Widths.from_buffer_copy(raw[(2 * i + 1) * self.wsz:(2 * i + 2) * self.wsz])
+            rec = dict(sigma_d=float(np.degrees(w1.sd)), sigma_m=float(np.degrees(w2.sm)), n=int(w2.n_strong),
+                       rcw=np.round(np.array(w2.rcw[:], np.float64), 3).tolist())
+            if w2.sm_fit > 0:
+                rec.update(sigma_d_fit=float(np.degrees(w2.sd_fit)), sigma_m_fit=float(np.degrees(w2.sm_fit)))
+            out[b] = rec
+        return out
 
     def finish(self, run, t0=None):
         """Host results in prediction order (xpg/integrate.py:967-999)."""
@@ -490,10 +552,7 @@ class Sweep:
         refl["n_fg_masked"] = host["n_fg_masked"].astype(np.int32)
         refl["overlap"] = hp["has_partner"].copy()
         if not self.o["learn_profiles"]:
-            fz_out = np.zeros((N, max(DZ_BUCKETS)), np.float32)
-            for gi, (b, (B, _, _), start, m) in enumerate(sp.groups):
-                fz_out[sp.perm[start:start + m], :B] = run["fz_groups"][gi].cpu().numpy()
-            refl["fz"] = fz_out
+            refl["fz"] = self.fz.cpu().numpy()[inv]
         refl["iz0"], refl["dz"], refl["fb"] = hp["iz0"], hp["dz"], hp["fb"]
         refl["region"] = hp["region"]
         refl["ix0"], refl["iy0"] = hp["ix0"] * hp["pxs"], hp["iy0"] * hp["pxs"]
@@ -502,7 +561,7 @@ class Sweep:
         profiles = {b: p.cpu().numpy() for b, p in run["prof_by_block"].items()}
         tm = dict(total=(time.time() - t0) if t0 else float("nan"), gpu=run["t_gpu"], plan=self.t_plan)
         res = IntegrationResult(refl, self.model, profiles, tm)
-        res.block_params = run["block_params"]
+        res.block_params = self.block_params() if self.o["adaptive"] else {}
         return res
 
 
@@ -514,10 +573,10 @@ class _Null:
         return False
 
 
-def _launch_pass1(det, call, view, store, out, acc):
-    import ctypes as C
+def _launch_pass1(det, call, view, store, out, acc, fz_stride=0):
     from . import _lib as L
     o = L.Pass1Out()
+    o.fz_stride = int(fz_stride)
     for k in K.P1_F64 + K.P1_I32 + ("fz",):
         setattr(o, k, L.ptr(out[k]) if k in out else None)
     for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m", "frac_rec", "nfg", "zeta", "seff"):
@@ -528,7 +587,6 @@ def _launch_pass1(det, call, view, store, out, acc):
 
 
 def _launch_pass2(det, call, view, store, prof, bg, nbg, out):
-    import ctypes as C
     from . import _lib as L
     o = L.Pass2Out()
     for k in K.P2_F64:
@@ -539,60 +597,8 @@ def _launch_pass2(det, call, view, store, prof, bg, nbg, out):
 
 
 # ---------------------------------------------------------------------------------------------------------------------
-# adaptive widths per block (host fits on the strong reflections' pass-1 results)
+# host fits of estimate_sigmas (the per-block adaptive fits run on the device: xpg_adapt_block)
 # ---------------------------------------------------------------------------------------------------------------------
-
-def _adapt_block(b, g, groups, o1, fz_groups, sp, refl, iz0, fb, sd_b, sm_b, rcw_b, sd, sm, adapt, block_params):
-    """Refit sigma_M and the rocking-curve weights for this block's profile fit and sigma_D for the next block
-    (xpg/integrate.py:404-461)."""
-    F, PH, ZK, IZ, MD, XL = [], [], [], [], [], []
-    for gi, (B, _, _), start, m in groups:
-        rows = sp.perm[start:start + m]
-        h = {k: o1[k][start:start + m].cpu().numpy() for k in ("isum", "vsum", "n_over", "frac_rec", "m_d")}
-        st = (h["isum"] > 0) & (h["isum"] / np.sqrt(np.maximum(h["vsum"], 1e-6)) > 15) & (h["n_over"] == 0) \
-            & (h["frac_rec"] > 0.98) & (fb[rows] == 1)
-        if not st.any():
-            continue
-        fz = np.zeros((int(st.sum()), max(DZ_BUCKETS)))
-        fz[:, :B] = fz_groups[gi].cpu().numpy()[st]
-        F.append(fz)
-        PH.append(refl["phi"][rows][st])
-        ZK.append(refl["zeta_k"][rows][st])
-        IZ.append(iz0[rows][st])
-        MD.append(h["m_d"][st])
-        XL.append(np.c_[refl["x"][rows][st], refl["y"][rows][st]])
-    n = sum(len(f) for f in F)
-    if n < 100:
-        block_params[b] = dict(sigma_d=float(np.degrees(sd_b)), sigma_m=float(np.degrees(sm_b)), n=n,
-                               rcw=np.asarray(rcw_b).round(3).tolist())
-        return sm_b, rcw_b
-    F, PH, ZK, IZ, MD, XL = map(np.concatenate, (F, PH, ZK, IZ, MD, XL))
-    sm_fit = np.radians(_fit_rocking_width([(F, IZ, PH, ZK)], g, np.degrees(sm), 0.5, 2.0))
-    sm_new = float(np.exp((n * np.log(sm_fit) + 300.0 * np.log(sm)) / (n + 300.0)))
-    k = IZ[:, None] + np.arange(F.shape[1])[None]
-    lo = g.z_to_phi(k.astype(float))
-    hi = g.z_to_phi(k + 1.0)
-    lo, hi = np.radians(np.minimum(lo, hi)), np.radians(np.maximum(lo, hi))
-    Bm = rc_basis_np(lo, hi, np.radians(PH), ZK, sm_new)
-    tot = np.maximum(F.sum(1, keepdims=True), 1e-9)
-    A = (Bm * np.sqrt(tot)[:, :, None]).reshape(-1, len(RC_C))
-    y = (F / tot * np.sqrt(tot)).reshape(-1)
-    w = _nnls_gram(A, y)
-    lam = 200.0 / n
-    w = (w / max(w.sum(), 1e-9) + lam * rc_default_weights()) / (1 + lam)
-    Xv = g.pixel_to_lab(XL[:, 0], XL[:, 1])
-    Xlen = np.linalg.norm(Xv, axis=1)
-    cosa = np.abs(Xv @ g.ed3) / Xlen
-    t2 = 1.0 - (FG_R ** 2 / 2) * np.exp(-FG_R ** 2 / 2) / (1 - np.exp(-FG_R ** 2 / 2))
-    md = MD / t2 - (g.qx / Xlen) ** 2 * (1 + cosa ** 2) / 24.0
-    sd_fit = float(np.clip(np.sqrt(max(np.median(md), 1e-12)), 0.5 * sd, 2.0 * sd))
-    sd_new = float(np.exp((n * np.log(sd_fit) + 300.0 * np.log(sd)) / (n + 300.0)))
-    adapt.update(sd=sd_new, sm=float(sm_new), rcw=w.astype(np.float32))
-    block_params[b] = dict(sigma_d=float(np.degrees(sd_b)), sigma_m=float(np.degrees(sm_new)), n=int(n),
-                           rcw=w.round(3).tolist(), sigma_d_fit=float(np.degrees(sd_fit)),
-                           sigma_m_fit=float(np.degrees(sm_fit)))
-    return float(sm_new), w.astype(np.float32)
-
 
 def _nnls_gram(A, y):
     """argmin ||A w - y||, w >= 0, through the Cholesky factor of A^T A (xpg/integrate.py:1095-1104)."""

@@ -195,7 +195,8 @@ def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=Tru
         out["all_valid2"] = torch.empty((n, Dy, Dx), dtype=torch.uint8, device=device)
     o = L.Pass1Out()
     for k, _ in L.Pass1Out._fields_:
-        setattr(o, k, L.ptr(out.get(k)))
+        if k != "fz_stride":
+            setattr(o, k, L.ptr(out.get(k)))
     st = L.stream_ptr(device)
     accp = C.byref(acc) if acc is not None else None
     if frames is not None:
