@@ -17,6 +17,38 @@ def _declared():
     return names
 
 
+def _prototypes():
+    """{name: [parameter kinds]} of every function the headers declare ('ptr', 'f64', 'i32', 'i64', 'u32')."""
+    protos = {}
+    for h in (ROOT / "include").glob("*.h"):
+        txt = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?\w+\s*\*?\s*(xpg_\w+)\s*\(([^)]*)\)\s*;", txt, flags=re.M):
+            params = [q.strip() for q in m.group(2).split(",") if q.strip() and q.strip() != "void"]
+            kinds = []
+            for q in params:
+                if "*" in q:
+                    kinds.append("ptr")
+                else:
+                    kinds.append({"double": "f64", "int32_t": "i32", "int64_t": "i64", "uint32_t": "u32",
+                                  "int": "i32"}[q.split()[0]])
+            protos[m.group(1)] = kinds
+    return protos
+
+
+def test_ctypes_signatures_match_prototypes():
+    import ctypes as C
+    from xpg_amd import _lib
+
+    def kind(t):
+        if t in (C.c_void_p, C.c_char_p) or isinstance(t, type(C.POINTER(C.c_int))):
+            return "ptr"
+        return {C.c_double: "f64", C.c_int32: "i32", C.c_int64: "i64", C.c_uint32: "u32"}[t]
+    protos = _prototypes()
+    assert set(protos) == set(_lib.EXPORTS)
+    for name, (args, _) in _lib._SIGS.items():
+        assert [kind(t) for t in args] == protos[name], name
+
+
 def test_library_exports_declared_symbols():
     from xpg_amd import _lib
     h = _lib.lib()

@@ -91,7 +91,7 @@ _SIGS = {
     "xpg_pass2_frames": ([P(Detector), P(Call), P(Refl), P(Frames), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                           C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
-    "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 13 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
+    "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 12 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
                                                                                          C.c_void_p], C.c_int),
     "xpg_adapt_workspace_bytes": ([C.c_int32], C.c_int64),
     "xpg_synth_background": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
