@@ -1,15 +1,19 @@
 // Adaptive widths per phi block on the device (xpg_adapt_block, include/xpg_integrate.h): the small fits of
-// xpg/integrate.py:404-461 as a chain of five launches on the caller's stream, so the per-block chain
+// xpg/integrate.py:404-461 as a chain of six launches on the caller's stream, so the per-block chain
 // pass 1 -> fits -> pass 2 -> next block runs without a host round trip.
 //
-//   k_ad_select  (1 x 1024)  strong reflections in row order (block-wide scan), their fz totals and sigma_D terms
-//   k_ad_grid    (61 x 256)  cost of each sigma_M of the 61-point log grid (_fit_rocking_width, :1130-1154)
-//   k_ad_finish  (1 x 1024)  argmin -> sigma_M fit and shrinkage; median of the sigma_D terms (bitonic sort)
-//   k_ad_gram    (NG x 256)  Gram matrix A^T A and A^T y of the rocking-curve basis (rc_basis_np, :437-443)
-//   k_ad_solve   (1 x 256)   fixed-order reduction, non-negative least squares on the Gram form (_nnls_gram,
-//                            :1095-1104), regularisation towards the Gaussian weights, the widths records
+//   k_ad_prep    (n/256 x 256)   strong-row flags, fz totals and sigma_D terms per row; strong rows per workgroup
+//   k_ad_grid    (61 x NCH x 256) partial cost of each sigma_M of the 61-point log grid over a fixed row range
+//                                (_fit_rocking_width, :1130-1154)
+//   k_ad_finish  (1 x 1024)      argmin -> sigma_M fit and shrinkage; exact median of the sigma_D terms by radix
+//                                selection of the two middle ranks
+//   k_ad_gram    (NG x 256)      Gram matrix A^T A and A^T y of the rocking-curve basis over a fixed row range
+//                                (rc_basis_np, :437-443)
+//   k_ad_gsum    (1 x 1024)      fixed-order reduction of the Gram partials
+//   k_ad_solve   (1 x 64)        non-negative least squares on the Gram form (_nnls_gram, :1095-1104) in one
+//                                wavefront, regularisation towards the Gaussian weights, the widths records
 //
-// All sums are float64 in fixed orders: results do not depend on scheduling.
+// Every row range and every reduction order is fixed by the row count alone: results do not depend on scheduling.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
@@ -20,7 +24,8 @@ namespace {
 
 constexpr int K = XPG_RC_K;               // 21 mixture components
 constexpr int NF = 65;                     // frames of a per-frame profile row (max(DZ_BUCKETS))
-constexpr int NG = 64;                     // workgroups of the Gram accumulation
+constexpr int NG = 256;                    // workgroups of the Gram accumulation
+constexpr int NCH = 32;                    // row ranges of the grid costs
 constexpr int NE = K * (K + 1) / 2 + K;    // unique Gram entries + right-hand side
 constexpr int NGRID = 61;
 constexpr double PI = 3.141592653589793;
@@ -32,16 +37,16 @@ __device__ __constant__ static const double RC_C[K] = {
     2.4500000000000006, 2.8000000000000007, 3.1500000000000004, 3.5};
 
 struct Ws {
-  int32_t *idx;
-  double *tot, *md, *sortbuf, *cost, *gram, *scal;   // scal: n, sm_new, sm_fit, sd_new, sd_fit
-  int32_t npow;
+  uint8_t *st;        // n: strong-row flag
+  uint64_t *key;      // n: order-preserving bits of the sigma_D term
+  double *tot;        // n: fz total
+  int32_t *cnt;       // nprep: strong rows per k_ad_prep workgroup
+  double *cost;       // NGRID x NCH partial grid costs
+  double *gram;       // NG x NE partial Gram entries
+  double *gtot;       // NE Gram totals
+  double *scal;       // n_strong, sm_new, sm_fit, sd_new, sd_fit
+  int32_t nprep;
 };
-
-__host__ __device__ inline int32_t pow2ceil(int32_t n) {
-  int32_t p = 1;
-  while (p < n) p <<= 1;
-  return p;
-}
 
 __host__ __device__ inline Ws carve(void *base, int32_t n) {
   Ws w;
@@ -51,13 +56,15 @@ __host__ __device__ inline Ws carve(void *base, int32_t n) {
     p += (bytes + 255) & ~(size_t)255;
     return q;
   };
-  w.npow = pow2ceil(n > 0 ? n : 1);
-  w.idx = (int32_t *)take(4 * (size_t)(n > 0 ? n : 1));
-  w.tot = (double *)take(8 * (size_t)(n > 0 ? n : 1));
-  w.md = (double *)take(8 * (size_t)(n > 0 ? n : 1));
-  w.sortbuf = (double *)take(8 * (size_t)w.npow);
-  w.cost = (double *)take(8 * NGRID);
+  const size_t m = (size_t)(n > 0 ? n : 1);
+  w.nprep = (int32_t)((m + 255) / 256);
+  w.st = (uint8_t *)take(m);
+  w.key = (uint64_t *)take(8 * m);
+  w.tot = (double *)take(8 * m);
+  w.cnt = (int32_t *)take(4 * (size_t)w.nprep);
+  w.cost = (double *)take(8 * NGRID * NCH);
   w.gram = (double *)take(8 * (size_t)NG * NE);
+  w.gtot = (double *)take(8 * (size_t)NE);
   w.scal = (double *)take(8 * 8);
   return w;
 }
@@ -72,131 +79,239 @@ struct In {
 
 __device__ __forceinline__ double bound_deg(const In &a, int iz0, int k) { return a.phi0 + a.osc * (double)(iz0 + k); }
 
-// ---- 1. strong reflections in row order ----------------------------------------------------------------------------
-__global__ void __launch_bounds__(1024) k_ad_select(In a, Ws w) {
-  __shared__ int32_t cnt[1024];
-  const int t = threadIdx.x, T = blockDim.x;
-  const int chunk = (a.n + T - 1) / T;
-  const int r0 = min(t * chunk, a.n), r1 = min(r0 + chunk, a.n);
-  int c = 0;
-  for (int r = r0; r < r1; ++r) {
-    const bool st = (a.isum[r] > 0) && (a.isum[r] / sqrt(fmax(a.vsum[r], 1e-6)) > 15.0) && (a.n_over[r] == 0) &&
-                    (a.frac_rec[r] > 0.98) && (a.fb[r] == 1);
-    c += st;
-  }
-  cnt[t] = c;
-  __syncthreads();
-  // inclusive scan (Hillis-Steele)
-  for (int off = 1; off < T; off <<= 1) {
-    int v = t >= off ? cnt[t - off] : 0;
-    __syncthreads();
-    cnt[t] += v;
-    __syncthreads();
-  }
-  int o = cnt[t] - c;
-  const double t2 = 1.0 - (9.0 / 2) * exp(-9.0 / 2) / (1 - exp(-9.0 / 2));
-  for (int r = r0; r < r1; ++r) {
-    const bool st = (a.isum[r] > 0) && (a.isum[r] / sqrt(fmax(a.vsum[r], 1e-6)) > 15.0) && (a.n_over[r] == 0) &&
-                    (a.frac_rec[r] > 0.98) && (a.fb[r] == 1);
-    if (!st) continue;
-    w.idx[o] = r;
-    double s = 0.0;
-    for (int k = 0; k < NF; ++k) s += (double)a.fz[(int64_t)r * NF + k];
-    w.tot[o] = s;
-    const double q = a.qx / a.Xlen[r];
-    w.md[o] = a.m_d[r] / t2 - (q * q) * (1 + a.cosa[r] * a.cosa[r]) / 24.0;
-    ++o;
-  }
-  if (t == T - 1) w.scal[0] = (double)cnt[T - 1];
+// rows [r0, r1) of range c of nc equal ranges
+__device__ __forceinline__ void row_range(int n, int c, int nc, int &r0, int &r1) {
+  const int per = (n + nc - 1) / nc;
+  r0 = min(c * per, n);
+  r1 = min(r0 + per, n);
 }
 
-// ---- 2. grid of sigma_M candidates -------------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
-  __shared__ double red[256];
-  const int n = (int)w.scal[0];
-  const int c = blockIdx.x;
-  double part = 0.0;
-  if (n >= 100) {
-    const double guess = a.sm_run * (180.0 / PI);
-    const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
-    const double smv = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-      const double tot = w.tot[i];
-      if (!(tot > 0)) continue;
-      const int r = w.idx[i];
-      const double phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
-      const double s = smv / zeta * sqrt(2.0);
-      const float *fz = a.fz + (int64_t)r * NF;
-      double R[NF];
-      double Rs = 0.0;
-      double Eprev = erf((bound_deg(a, a.iz0[r], 0) - phi) / s);
-      for (int k = 0; k < NF; ++k) {
-        const double E = erf((bound_deg(a, a.iz0[r], k + 1) - phi) / s);
-        const double v = a.osc >= 0 ? 0.5 * (E - Eprev) : 0.5 * (Eprev - E);
-        R[k] = v;
-        Rs = Rs + v;
-        Eprev = E;
-      }
-      const double den = fmax(Rs, 1e-9);
-      double cst = 0.0;
-      for (int k = 0; k < NF; ++k) {
-        const double d = (double)fz[k] / tot - R[k] / den;
-        cst += tot * (d * d);
-      }
-      part += cst;
-    }
-  }
-  red[threadIdx.x] = part;
+__device__ __forceinline__ double rld(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// fixed-shape tree sum over a workgroup of 256 threads
+__device__ __forceinline__ double block_sum256(double v, double *red) {
+  red[threadIdx.x] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) w.cost[c] = red[0];
+  return red[0];
+}
+
+// ---- 1. strong rows -------------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
+  __shared__ int32_t wc[4];
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  bool st = false;
+  if (r < a.n) {
+    st = (a.isum[r] > 0) && (a.isum[r] / sqrt(fmax(a.vsum[r], 1e-6)) > 15.0) && (a.n_over[r] == 0) &&
+         (a.frac_rec[r] > 0.98) && (a.fb[r] == 1);
+    double s = 0.0, md = 0.0;
+    if (st) {
+      for (int k = 0; k < NF; ++k) s += (double)a.fz[(int64_t)r * NF + k];
+      const double t2 = 1.0 - (9.0 / 2) * exp(-9.0 / 2) / (1 - exp(-9.0 / 2));
+      const double q = a.qx / a.Xlen[r];
+      md = a.m_d[r] / t2 - (q * q) * (1 + a.cosa[r] * a.cosa[r]) / 24.0;
+    }
+    const uint64_t b = __builtin_bit_cast(uint64_t, md);
+    w.key[r] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);      // unsigned order = numeric order
+    w.tot[r] = s;
+    w.st[r] = st;
+  }
+  const uint64_t bal = __ballot(st);
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(bal);
+  __syncthreads();
+  if (threadIdx.x == 0) w.cnt[blockIdx.x] = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+}
+
+// ---- 2. grid of sigma_M candidates -------------------------------------------------------------------------------
+
+// erf(x), with |x| > ERF_SAT taken as saturated (float64 erf is +-1 to the last bit there)
+constexpr double ERF_SAT = 6.5;
+__device__ __forceinline__ double erf_sat(double x) { return fabs(x) > ERF_SAT ? copysign(1.0, x) : erf(x); }
+
+__global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
+  __shared__ double red[256];
+  const int c = blockIdx.x, ch = blockIdx.y;
+  int r0, r1;
+  row_range(a.n, ch, NCH, r0, r1);
+  const double guess = a.sm_run * (180.0 / PI);
+  const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
+  const double smv = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
+  double part = 0.0;
+  for (int r = r0 + threadIdx.x; r < r1; r += 256) {
+    if (!w.st[r]) continue;
+    const double tot = w.tot[r];
+    if (!(tot > 0)) continue;
+    const double phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
+    const double s = smv / zeta * sqrt(2.0);
+    const float *fz = a.fz + (int64_t)r * NF;
+    const int iz0 = a.iz0[r];
+    // frame k spans [lo, hi] = sorted(phi0 + osc (iz0 + k), phi0 + osc (iz0 + k + 1)); the row total of R telescopes
+    const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) / s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) / s);
+    const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9);
+    double cst = 0.0, Eprev = e0;
+    for (int k = 0; k < NF; ++k) {
+      const double E = k == NF - 1 ? e1 : erf_sat((bound_deg(a, iz0, k + 1) - phi) / s);
+      const double R = a.osc >= 0 ? 0.5 * (E - Eprev) : 0.5 * (Eprev - E);
+      const double d = (double)fz[k] / tot - R / den;
+      cst += tot * (d * d);
+      Eprev = E;
+    }
+    part += cst;
+  }
+  part = block_sum256(part, red);
+  if (threadIdx.x == 0) w.cost[c * NCH + ch] = part;
 }
 
 // ---- 3. sigma_M from the grid, sigma_D from the median ----------------------------------------------------------
+
+// the bin of a 256-bin histogram holding rank k (one wavefront): returns the bin, k reduced to the rank inside it
+__device__ __forceinline__ int select_bin(const uint32_t *h, uint32_t &k) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t c0 = h[4 * lane], c1 = h[4 * lane + 1], c2 = h[4 * lane + 2], c3 = h[4 * lane + 3];
+  const uint32_t s = (c0 + c1) + (c2 + c3);
+  uint32_t incl = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += v;
+  }
+  const uint64_t hit = __ballot(incl > k);
+  const int L = __ffsll((long long)hit) - 1;        // the rank exists: L >= 0
+  uint32_t excl = __shfl(incl - s, L, 64);
+  const uint32_t cc[4] = {__shfl(c0, L, 64), __shfl(c1, L, 64), __shfl(c2, L, 64), __shfl(c3, L, 64)};
+  int bin = 4 * L + 3;
+  for (int i = 0; i < 4; ++i) {
+    if (excl + cc[i] > k) {
+      bin = 4 * L + i;
+      break;
+    }
+    excl += cc[i];
+  }
+  k -= excl;
+  return bin;
+}
+
+// histogram increment of a wavefront (all 64 lanes call it): one atomic when every live lane hits the same bin
+__device__ __forceinline__ void hist_add(uint32_t *h, uint32_t bin, bool live) {
+  const uint64_t m = __ballot(live);
+  if (!m) return;
+  const int first = __ffsll((long long)m) - 1;
+  const uint32_t lead = __shfl(bin, first, 64);
+  const uint64_t same = __ballot(live && bin == lead);
+  if (same == m) {
+    if ((int)(threadIdx.x & 63) == first) atomicAdd(&h[lead], (uint32_t)__popcll(m));
+  } else if (live) {
+    atomicAdd(&h[bin], 1u);
+  }
+}
+
 __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
-  const int n = (int)w.scal[0];
+  __shared__ int32_t s_n;
+  __shared__ double s_part[NGRID * NCH];
+  __shared__ double s_cost[NGRID];
+  __shared__ uint32_t hist[2][256];
+  __shared__ uint64_t s_pre[2];
+  __shared__ uint32_t s_k[2];
+  const int t = threadIdx.x;
+  if (t == 0) s_n = 0;
+  for (int i = t; i < NGRID * NCH; i += 1024) s_part[i] = w.cost[i];
+  __syncthreads();
+  int cn = 0;
+  for (int b = t; b < w.nprep; b += 1024) cn += w.cnt[b];
+  if (cn) atomicAdd(&s_n, cn);           // integer counts: order-free
+  if (t < NGRID) {
+    double s = 0.0;
+    for (int ch = 0; ch < NCH; ++ch) s += s_part[t * NCH + ch];
+    s_cost[t] = s;
+  }
+  __syncthreads();
+  const int n = s_n;
+  if (t == 0) w.scal[0] = (double)n;
   if (n < 100) return;
-  const int t = threadIdx.x, T = blockDim.x;
   if (t == 0) {
     const double guess = a.sm_run * (180.0 / PI);
     const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
     double best = INFINITY, bsm = guess;
     for (int c = 0; c < NGRID; ++c) {
-      if (w.cost[c] < best) {
-        best = w.cost[c];
+      if (s_cost[c] < best) {
+        best = s_cost[c];
         bsm = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
       }
     }
     const double sm_fit = bsm * (PI / 180.0);
-    const double sm_new = exp(((double)n * log(sm_fit) + 300.0 * log(a.sm_run)) / ((double)n + 300.0));
-    w.scal[1] = sm_new;
+    w.scal[1] = exp(((double)n * log(sm_fit) + 300.0 * log(a.sm_run)) / ((double)n + 300.0));
     w.scal[2] = sm_fit;
+    s_pre[0] = s_pre[1] = 0;
+    s_k[0] = (uint32_t)((n - 1) / 2);     // the two middle ranks (equal for odd n)
+    s_k[1] = (uint32_t)(n / 2);
   }
-  // bitonic sort of the sigma_D terms (padded with +inf)
-  const int N = w.npow;
-  for (int i = t; i < N; i += T) w.sortbuf[i] = i < n ? w.md[i] : INFINITY;
-  __syncthreads();
-  for (int k = 2; k <= N; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = t; i < N; i += T) {
-        const int l = i ^ j;
-        if (l > i) {
-          const double x = w.sortbuf[i], y = w.sortbuf[l];
-          const bool up = (i & k) == 0;
-          if ((x > y) == up) {
-            w.sortbuf[i] = y;
-            w.sortbuf[l] = x;
-          }
-        }
+  // radix selection, 8 bits per round, of both ranks among the strong rows' keys; the keys stay in registers
+  // (KR per thread) when the block has at most 1024 KR rows
+  constexpr int KR = 32;
+  const bool in_regs = a.n <= 1024 * KR;
+  uint64_t kr[KR];
+  uint32_t kv = 0;                         // bit i: kr[i] is a strong row's key
+  if (in_regs) {
+#pragma unroll
+    for (int i = 0; i < KR; ++i) {
+      const int r = t + 1024 * i;
+      kr[i] = 0;
+      if (r < a.n && w.st[r]) {
+        kr[i] = w.key[r];
+        kv |= 1u << i;
       }
-      __syncthreads();
     }
   }
+  for (int shift = 56; shift >= 0; shift -= 8) {
+    for (int i = t; i < 512; i += 1024) (&hist[0][0])[i] = 0;
+    __syncthreads();
+    const uint64_t hm = shift == 56 ? 0ull : (~0ull << (shift + 8));
+    const uint64_t p0 = s_pre[0], p1 = s_pre[1];
+    if (in_regs) {
+#pragma unroll
+      for (int i = 0; i < KR; ++i) {
+        const uint64_t key = kr[i];
+        const uint32_t bin = (uint32_t)(key >> shift) & 255u;
+        const bool live = (kv >> i) & 1u;
+        hist_add(hist[0], bin, live && (key & hm) == p0);
+        hist_add(hist[1], bin, live && (key & hm) == p1);
+      }
+    } else {
+      for (int r0 = 0; r0 < a.n; r0 += 1024) {
+        const int r = r0 + t;
+        const bool live = r < a.n && w.st[r];
+        const uint64_t key = live ? w.key[r] : 0;
+        const uint32_t bin = (uint32_t)(key >> shift) & 255u;
+        hist_add(hist[0], bin, live && (key & hm) == p0);
+        hist_add(hist[1], bin, live && (key & hm) == p1);
+      }
+    }
+    __syncthreads();
+    if (t < 128) {
+      const int q = t >> 6;
+      uint32_t k = s_k[q];
+      const int bin = select_bin(hist[q], k);
+      if ((t & 63) == 0) {
+        s_pre[q] |= (uint64_t)bin << shift;
+        s_k[q] = k;
+      }
+    }
+    __syncthreads();
+  }
   if (t == 0) {
-    const double med = (n & 1) ? w.sortbuf[n / 2] : (w.sortbuf[n / 2 - 1] + w.sortbuf[n / 2]) / 2.0;
+    double v[2];
+    for (int q = 0; q < 2; ++q) {
+      const uint64_t b = s_pre[q];
+      v[q] = __builtin_bit_cast(double, (b >> 63) ? (b & 0x7fffffffffffffffull) : ~b);
+    }
+    const double med = (n & 1) ? v[1] : (v[0] + v[1]) / 2.0;
     double sd_fit = sqrt(fmax(med, 1e-12));
     sd_fit = fmin(fmax(sd_fit, 0.5 * a.sd_run), 2.0 * a.sd_run);
     w.scal[3] = exp(((double)n * log(sd_fit) + 300.0 * log(a.sd_run)) / ((double)n + 300.0));
@@ -205,100 +320,250 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
 }
 
 // ---- 4. Gram matrix of the rocking-curve basis -------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_ad_gram(In a, Ws w) {
-  __shared__ double A[NF][K + 1];          // A[k][j] = B_kj sqrt(tot), A[k][K] = y_k
-  __shared__ double E[NF + 1][K];
+//
+// One wavefront per strong row (rows of a fixed range per wavefront, in order).  Frame k contributes to the Gram
+// entries only where some basis function B_kj is non-zero; B_kj = 0 exactly when both frame boundaries saturate every
+// component's erf to the same sign, so the accumulation runs over the frames between the first and last live one.
+constexpr int GW = 4;                      // wavefronts per workgroup
+__global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
+  __shared__ double A[GW][NF + 1][K + 1];  // boundary erfs E[b][j], then in place A[k][j] = B_kj sqrt(tot), A[k][K] = y
+  __shared__ double red[GW][NE];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double (*Aw)[K + 1] = A[wv];
+  double *Af = &A[wv][0][0];                // the same storage, flat
   const int n = (int)w.scal[0];
-  double acc = 0.0;
-  const int e = threadIdx.x;               // entry: (p, q) with p <= q, then the right-hand side
-  int p = 0, q = 0;
-  if (e < NE) {
+  int ep[4], eq[4];                        // this lane's entries e = lane + 64 i: (p, q), p <= q; q = K: right-hand side
+  for (int i = 0; i < 4; ++i) {
+    const int e = lane + 64 * i;
+    int p = 0, q = 0;
     if (e < K * (K + 1) / 2) {
       int rem = e;
-      p = 0;
       while (rem >= K - p) {
         rem -= K - p;
         ++p;
       }
       q = p + rem;
-    } else {
+    } else if (e < NE) {
       p = e - K * (K + 1) / 2;
       q = K;
     }
+    ep[i] = p;
+    eq[i] = q;
   }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   if (n >= 100) {
     const double sm = w.scal[1];
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
-      const int r = w.idx[i];
+    int r0, r1;
+    row_range(a.n, blockIdx.x * GW + wv, NG * GW, r0, r1);
+    for (int r = r0; r < r1; ++r) {
+      if (!w.st[r]) continue;              // uniform over the wavefront
       const double phi = a.phi[r] * (PI / 180.0);
       const double s = sm / fmax(fabs(a.zeta_k[r]), 1e-3);
       const double sw = s * 0.25 * sqrt(2.0);
-      const double tot = fmax(w.tot[i], 1e-9), st = sqrt(tot);
-      // erf at the 66 frame boundaries for each component (lo / hi of frame k share boundaries)
-      for (int u = threadIdx.x; u < (NF + 1) * K; u += blockDim.x) {
-        const int b = u / K, j = u - b * K;
-        const double bd = bound_deg(a, a.iz0[r], b) * (PI / 180.0);
-        E[b][j] = erf(((bd - phi) - RC_C[j] * s) / sw);
+      const double tot = fmax(w.tot[r], 1e-9), st = sqrt(tot);
+      const int iz0 = a.iz0[r];
+      // boundary b saturates every component to +1 when (bd - phi) - 3.5 s > ERF_SAT sw, to -1 when
+      // (bd - phi) + 3.5 s < -ERF_SAT sw (|RC_C| <= 3.5)
+      uint64_t pos_lo, neg_lo;
+      bool pos64, neg64, pos65, neg65;
+      {
+        const double bd = bound_deg(a, iz0, lane) * (PI / 180.0) - phi;
+        pos_lo = __ballot(bd - 3.5 * s > ERF_SAT * sw);
+        neg_lo = __ballot(bd + 3.5 * s < -ERF_SAT * sw);
+        const double b64 = bound_deg(a, iz0, 64) * (PI / 180.0) - phi, b65 = bound_deg(a, iz0, 65) * (PI / 180.0) - phi;
+        pos64 = b64 - 3.5 * s > ERF_SAT * sw;
+        neg64 = b64 + 3.5 * s < -ERF_SAT * sw;
+        pos65 = b65 - 3.5 * s > ERF_SAT * sw;
+        neg65 = b65 + 3.5 * s < -ERF_SAT * sw;
       }
-      __syncthreads();
-      for (int u = threadIdx.x; u < NF * (K + 1); u += blockDim.x) {
-        const int k = u / (K + 1), j = u - k * (K + 1);
-        if (j < K) {
-          const double Bk = a.osc >= 0 ? 0.5 * (E[k + 1][j] - E[k][j]) : 0.5 * (E[k][j] - E[k + 1][j]);
-          A[k][j] = Bk * st;
-        } else {
-          A[k][K] = (double)a.fz[(int64_t)r * NF + k] / tot * st;
+      auto pos = [&](int b) { return b < 64 ? ((pos_lo >> b) & 1) != 0 : (b == 64 ? pos64 : pos65); };
+      auto neg = [&](int b) { return b < 64 ? ((neg_lo >> b) & 1) != 0 : (b == 64 ? neg64 : neg65); };
+      int kl = NF, kh = -1;
+      for (int k = 0; k < NF; ++k) {       // uniform scalar loop
+        const bool dead = (pos(k) && pos(k + 1)) || (neg(k) && neg(k + 1));
+        if (!dead) {
+          kl = min(kl, k);
+          kh = k;
         }
       }
-      __syncthreads();
-      if (e < NE)
-        for (int k = 0; k < NF; ++k) acc += A[k][p] * A[k][q];
-      __syncthreads();
+      if (kh < kl) continue;
+      const int nb = kh - kl + 2;          // boundaries kl .. kh + 1
+      for (int u = lane; u < nb * K; u += 64) {
+        const int b = u / K, j = u - b * K;
+        const double bd = bound_deg(a, iz0, kl + b) * (PI / 180.0);
+        Aw[b][j] = erf_sat(((bd - phi) - RC_C[j] * s) / sw);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      double v[8];                         // (nb - 1) (K + 1) <= 66 x 22 / 64 -> at most 23 per lane: in chunks of 8
+      for (int u0 = 0; u0 < (nb - 1) * (K + 1); u0 += 8 * 64) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int u = u0 + lane + 64 * i;
+          v[i] = 0.0;
+          if (u < (nb - 1) * (K + 1)) {
+            const int k = u / (K + 1), j = u - k * (K + 1);
+            if (j < K) {
+              const double Bk = a.osc >= 0 ? 0.5 * (Aw[k + 1][j] - Aw[k][j]) : 0.5 * (Aw[k][j] - Aw[k + 1][j]);
+              v[i] = Bk * st;
+            } else {
+              v[i] = (double)a.fz[(int64_t)r * NF + kl + k] / tot * st;
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int u = u0 + lane + 64 * i;
+          if (u < (nb - 1) * (K + 1)) Af[u] = v[i];      // row-major (K + 1): A[k][j] at k (K + 1) + j
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+      for (int k = 0; k < nb - 1; ++k) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] += Aw[k][ep[i]] * Aw[k][eq[i]];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
-  if (e < NE) w.gram[(int64_t)blockIdx.x * NE + e] = acc;
+  for (int i = 0; i < 4; ++i)
+    if (lane + 64 * i < NE) red[wv][lane + 64 * i] = acc[i];
+  __syncthreads();
+  for (int e = threadIdx.x; e < NE; e += 64 * GW) {
+    double t = red[0][e];
+    for (int q = 1; q < GW; ++q) t += red[q][e];
+    w.gram[(int64_t)blockIdx.x * NE + e] = t;
+  }
 }
 
 // ---- 5. non-negative least squares and the widths records --------------------------------------------------------
 
-// solve G[P,P] s = h[P] by Cholesky (|P| <= 21); returns false when not positive definite
-__device__ bool chol_solve(const double G[K][K], const double *h, const bool *inP, double *s) {
-  int idx[K], m = 0;
-  for (int j = 0; j < K; ++j)
-    if (inP[j]) idx[m++] = j;
-  double L[K][K];
-  for (int i = 0; i < m; ++i)
-    for (int j = 0; j <= i; ++j) {
-      double v = G[idx[i]][idx[j]];
-      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
-      if (i == j) {
-        if (!(v > 0)) return false;
-        L[i][i] = sqrt(v);
-      } else {
-        L[i][j] = v / L[j][j];
+// Gram totals: entry e of each of four quarters of the NG partials summed in order (loads 16 at a time), then the
+// quarters in order
+__global__ void __launch_bounds__(1024) k_ad_gsum(Ws w) {
+  __shared__ double part4[4][NE];
+  const int t = threadIdx.x;
+  if ((int)w.scal[0] < 100) return;
+  if (t < 4 * 256) {
+    const int e = t & 255, qt = t >> 8;
+    if (e < NE) {
+      const double *src = w.gram + (int64_t)qt * (NG / 4) * NE + e;
+      double s = 0.0;
+      for (int g0 = 0; g0 < NG / 4; g0 += 16) {
+        double v[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = src[(int64_t)(g0 + i) * NE];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += v[i];
       }
+      part4[qt][e] = s;
     }
-  double z[K];
-  for (int i = 0; i < m; ++i) {
-    double v = h[idx[i]];
-    for (int k = 0; k < i; ++k) v -= L[i][k] * z[k];
-    z[i] = v / L[i][i];
   }
-  for (int j = 0; j < K; ++j) s[j] = 0.0;
-  for (int i = m - 1; i >= 0; --i) {
-    double v = z[i];
-    for (int k = i + 1; k < m; ++k) v -= L[k][i] * s[idx[k]];
-    s[idx[i]] = v / L[i][i];
+  __syncthreads();
+  if (t < NE) w.gtot[t] = ((part4[0][t] + part4[1][t]) + part4[2][t]) + part4[3][t];
+}
+
+// Lawson-Hanson active-set NNLS in Gram form (min w^T G w - 2 w^T h, w >= 0) on one wavefront.
+//
+// Lane j holds row j of G and x_j.  The active block G[P,P] is kept as a Cholesky factor in insertion order: lane i
+// holds row i (a) and column i (ct) of L, and z = L^-1 h[P].  Adding a variable appends one row (a forward
+// substitution); removing variables refactors by re-appending the remaining ones in order.  Every index into a
+// per-lane array is static; broadcasts are readlanes.
+typedef double dvec32 __attribute__((ext_vector_type(32)));   // per-lane vectors kept in registers
+struct Nnls {
+  dvec32 a, ct;
+  double zv;       // lane i < m: z_i
+  int ordv;        // lane i < m: variable at position i
+  int posv;        // lane j: position of variable j, or -1
+  int m;
+};
+
+__device__ __forceinline__ void nnls_reset(Nnls &S) {
+  S.a = 0.0;
+  S.ct = 0.0;
+  S.zv = 0.0;
+  S.ordv = -1;
+  S.posv = -1;
+  S.m = 0;
+}
+
+// append variable t (uniform); false (state unchanged) when G[P+t, P+t] is not numerically positive definite
+__device__ __forceinline__ bool nnls_append(Nnls &S, const double *Gs, const double *hs, int t) {
+  const int lane = threadIdx.x & 63;
+  const int m = S.m;
+  double b = lane < m ? Gs[S.ordv * K + t] : 0.0;
+  // L l = G[P, t], column-oriented: after step k lane k holds l_k
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k < m) {
+      const double lk = rld(b, k) / rld(S.a[k], k);
+      if (lane == k) b = lk;
+      else if (lane > k) b -= S.a[k] * lk;
+    }
   }
+  double d2 = Gs[t * K + t], zm = hs[t];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k < m) {
+      const double lk = rld(b, k);
+      d2 -= lk * lk;
+      zm -= lk * rld(S.zv, k);
+    }
+  }
+  if (!(d2 > 0)) return false;
+  const double d = sqrt(d2);
+  zm = zm / d;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if (k < m) {
+      const double lk = rld(b, k);
+      if (lane == m) S.a[k] = lk;
+    } else if (k == m) {
+      if (lane == m) S.a[k] = d;
+      if (lane < m) S.ct[k] = b;
+      else if (lane == m) S.ct[k] = d;
+    }
+  }
+  if (lane == m) {
+    S.zv = zm;
+    S.ordv = t;
+  }
+  if (lane == t) S.posv = m;
+  S.m = m + 1;
   return true;
 }
 
-__global__ void __launch_bounds__(256) k_ad_solve(In a, Ws w, const xpg_widths *in, xpg_widths *out2,
-                                                  xpg_widths *outn) {
-  __shared__ double tot[NE];
+// L^T s = z; returns s_j on lane j (0 for inactive variables)
+__device__ __forceinline__ double nnls_solve(const Nnls &S) {
+  const int lane = threadIdx.x & 63;
+  const int m = S.m;
+  double c = lane < m ? S.zv : 0.0;
+#pragma unroll
+  for (int k = K - 1; k >= 0; --k) {
+    if (k < m) {
+      const double sk = rld(c, k) / rld(S.ct[k], k);
+      if (lane == k) c = sk;
+      else if (lane < k) c -= S.ct[k] * sk;
+    }
+  }
+  const double sv = __shfl(c, S.posv < 0 ? 0 : S.posv, 64);
+  return S.posv < 0 ? 0.0 : sv;
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1))) k_ad_solve(In a, Ws w, const xpg_widths *in, xpg_widths *out2,
+                                                 xpg_widths *outn) {
+  __shared__ double Gs[K * K], hs[K];
   const int n = (int)w.scal[0];
+  const int lane = threadIdx.x;
   if (n < 100) {
-    if (threadIdx.x == 0) {
+    if (lane == 0) {
       xpg_widths v = *in;
       v.n_strong = n;
       v.sd_fit = 0.0;
@@ -308,93 +573,96 @@ __global__ void __launch_bounds__(256) k_ad_solve(In a, Ws w, const xpg_widths *
     }
     return;
   }
-  for (int e = threadIdx.x; e < NE; e += blockDim.x) {
-    double s = 0.0;
-    for (int g = 0; g < NG; ++g) s += w.gram[(int64_t)g * NE + e];
-    tot[e] = s;
+  for (int u = lane; u < K * K; u += 64) {
+    const int i = u / K, k = u - i * K;
+    const int p = i < k ? i : k, q = i < k ? k : i;
+    Gs[u] = w.gtot[p * K - p * (p - 1) / 2 + (q - p)];
   }
+  if (lane < K) hs[lane] = w.gtot[K * (K + 1) / 2 + lane];
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double G[K][K], h[K];
-  {
-    int e = 0;
-    for (int p = 0; p < K; ++p)
-      for (int q = p; q < K; ++q, ++e) G[p][q] = G[q][p] = tot[e];
-    for (int p = 0; p < K; ++p) h[p] = tot[K * (K + 1) / 2 + p];
-  }
-  // active-set NNLS (Lawson-Hanson, Gram form): min w^T G w - 2 w^T h, w >= 0
+  double grow[K];
+  const int il = lane < K ? lane : 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) grow[k] = lane < K ? Gs[il * K + k] : 0.0;
+  const double hl = lane < K ? hs[il] : 0.0;
   double gmax = 0.0;
-  for (int p = 0; p < K; ++p)
-    for (int q = 0; q < K; ++q) gmax = fmax(gmax, fabs(G[p][q]));
+#pragma unroll
+  for (int k = 0; k < K; ++k) gmax = fmax(gmax, fabs(grow[k]));
+  for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o, 64));
   const double tol = 10.0 * 2.220446049250313e-16 * gmax * K;
-  bool inP[K];
-  double x[K], s[K], g[K];
-  for (int j = 0; j < K; ++j) {
-    inP[j] = false;
-    x[j] = 0.0;
-  }
+  Nnls S;
+  nnls_reset(S);
+  double x = 0.0;                          // lane j: x_j
   for (int outer = 0; outer < 3 * K; ++outer) {
-    for (int j = 0; j < K; ++j) {
-      double v = h[j];
-      for (int k = 0; k < K; ++k) v -= G[j][k] * x[k];
-      g[j] = v;
-    }
-    int t = -1;
-    double best = tol;
-    for (int j = 0; j < K; ++j)
-      if (!inP[j] && g[j] > best) {
-        best = g[j];
-        t = j;
-      }
-    if (t < 0) break;
-    inP[t] = true;
+    double g = hl;
+#pragma unroll
+    for (int k = 0; k < K; ++k) g -= grow[k] * rld(x, k);
+    // the first variable outside P with the largest gradient above tol
+    const bool cand = lane < K && S.posv < 0 && g > tol;
+    double gb = cand ? g : -INFINITY;
+    for (int o = 32; o > 0; o >>= 1) gb = fmax(gb, __shfl_xor(gb, o, 64));
+    const uint64_t hit = __ballot(cand && g == gb);
+    if (!hit) break;
+    const int t = __ffsll((long long)hit) - 1;
+    if (!nnls_append(S, Gs, hs, t)) break;   // the same t would be chosen again with x unchanged
     for (int inner = 0; inner < 3 * K; ++inner) {
-      if (!chol_solve(G, h, inP, s)) {
-        inP[t] = false;
+      const double sj = nnls_solve(S);
+      const bool act = S.posv >= 0;
+      if (!__ballot(act && sj <= 0)) {
+        x = act ? sj : 0.0;
         break;
       }
-      bool allpos = true;
-      for (int j = 0; j < K; ++j)
-        if (inP[j] && s[j] <= 0) allpos = false;
-      if (allpos) {
-        for (int j = 0; j < K; ++j) x[j] = s[j];
-        break;
-      }
-      double alpha = INFINITY;
-      for (int j = 0; j < K; ++j)
-        if (inP[j] && s[j] <= 0) alpha = fmin(alpha, x[j] / (x[j] - s[j]));
-      for (int j = 0; j < K; ++j) {
-        x[j] = x[j] + alpha * (s[j] - x[j]);
-        if (inP[j] && x[j] <= tol) {
-          inP[j] = false;
-          x[j] = 0.0;
+      double al = (act && sj <= 0) ? x / (x - sj) : INFINITY;
+      for (int o = 32; o > 0; o >>= 1) al = fmin(al, __shfl_xor(al, o, 64));
+      x = x + al * (sj - x);
+      const bool drop = act && x <= tol;
+      if (drop) x = 0.0;
+      if (!__ballot(drop)) continue;
+      // refactor: re-append the remaining variables in their previous order
+      const int mo = S.m;
+      const int ord_old = S.ordv;
+      const bool keep_l = lane < K && S.posv >= 0 && !drop;
+      const uint64_t keep = __ballot(keep_l);
+      nnls_reset(S);
+      for (int i = 0; i < mo; ++i) {
+        const int v = __builtin_amdgcn_readlane(ord_old, i);
+        if ((keep >> v) & 1ull) {
+          if (!nnls_append(S, Gs, hs, v) && lane == v) x = 0.0;
         }
       }
     }
   }
+  double xs = x;
   double sw = 0.0;
-  for (int j = 0; j < K; ++j) sw += x[j];
+#pragma unroll
+  for (int j = 0; j < K; ++j) sw += rld(xs, j);
   sw = fmax(sw, 1e-9);
   // default weights: exp(-c^2 / (2 (1 - RC_W^2))) normalised, as float32 values
-  double w0[K], s0 = 0.0;
-  for (int j = 0; j < K; ++j) {
-    w0[j] = exp(-0.5 * RC_C[j] * RC_C[j] / (1 - 0.25 * 0.25));
-    s0 += w0[j];
-  }
+  double s0 = 0.0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) s0 += exp(-0.5 * RC_C[j] * RC_C[j] / (1 - 0.25 * 0.25));
   const double lam = 200.0 / (double)n;
-  xpg_widths o2 = *in, on;
-  for (int j = 0; j < K; ++j) {
-    const double d = (double)(float)(w0[j] / s0);
-    o2.rcw[j] = (float)((x[j] / sw + lam * d) / (1 + lam));
+  float wj = 0.f;
+  if (lane < K) {
+    const double d = (double)(float)(exp(-0.5 * RC_C[lane] * RC_C[lane] / (1 - 0.25 * 0.25)) / s0);
+    wj = (float)((xs / sw + lam * d) / (1 + lam));
   }
-  o2.sm = w.scal[1];
-  o2.n_strong = n;
-  o2.sm_fit = w.scal[2];
-  o2.sd_fit = w.scal[4];
-  on = o2;
-  on.sd = w.scal[3];
-  *out2 = o2;
-  *outn = on;
+  if (lane < K) {
+    out2->rcw[lane] = wj;
+    outn->rcw[lane] = wj;
+  }
+  if (lane == 0) {
+    out2->sd = in->sd;
+    out2->sm = w.scal[1];
+    out2->n_strong = n;
+    out2->sm_fit = w.scal[2];
+    out2->sd_fit = w.scal[4];
+    outn->sd = w.scal[3];
+    outn->sm = w.scal[1];
+    outn->n_strong = n;
+    outn->sm_fit = w.scal[2];
+    outn->sd_fit = w.scal[4];
+  }
 }
 
 }  // namespace
@@ -438,11 +706,12 @@ int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int
   a.qx = qx;
   Ws w = carve(workspace, n);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_ad_select, dim3(1), dim3(1024), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_grid, dim3(NGRID), dim3(256), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_prep, dim3(w.nprep), dim3(256), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_grid, dim3(NGRID, NCH), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(1024), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_gram, dim3(NG), dim3(256), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_solve, dim3(1), dim3(256), 0, st, a, w, in, out_pass2, out_next);
+  hipLaunchKernelGGL(k_ad_gram, dim3(NG), dim3(64 * GW), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_gsum, dim3(1), dim3(1024), 0, st, w);
+  hipLaunchKernelGGL(k_ad_solve, dim3(1), dim3(64), 0, st, a, w, in, out_pass2, out_next);
   return hipGetLastError() == hipSuccess ? XPG_OK : XPG_ERR_HIP;
 }
 

@@ -395,7 +395,7 @@ __device__ void neighbour_mask(const xpg_refl &R, int r, const double *xs, const
     hit = ((mx * mx + my * my) + mz * mz) < 1.0;
   }
   uint64_t hits = __ballot(hit);
-  const int P = Dy * Dx, V = Dz * P;
+  const int P = Dy * Dx;
   double *dx2 = tab, *dy2 = tab + Dx, *dz2 = tab + Dx + Dy;
   while (hits) {
     int k = __ffsll((unsigned long long)hits) - 1;
@@ -415,9 +415,14 @@ __device__ void neighbour_mask(const xpg_refl &R, int r, const double *xs, const
       tab[i] = v * v;
     }
     wave_sync();
-    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-      const int iy = pix_row[it.p], ix = it.p - iy * Dx;
-      if ((dx2[ix] + dy2[iy]) + dz2[it.j] < 1.0) vflag[it.v] |= bit;
+    for (int p0 = 0; p0 < P; p0 += WAVE) {       // lanes over pixels, frames in order
+      const int p = p0 + lane;
+      if (p < P) {
+        const int iy = pix_row[p], ix = p - iy * Dx;
+        const double dxy = dx2[ix] + dy2[iy];
+        for (int j = 0; j < Dz; ++j)
+          if (dxy + dz2[j] < 1.0) vflag[j * P + p] |= bit;
+      }
     }
     wave_sync();
   }

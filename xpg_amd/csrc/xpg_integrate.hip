@@ -46,7 +46,7 @@ static int fail(int code, const char *fmt, ...) {
 //   pc    8P bytes    pass 1: x, y (float)         /  pass 2: bref, b*bref (float)
 //   pd    8P bytes                                     pass 2: p2, pp2 (float)
 //   prow  u8[P]       row of each pixel               pflag u8[P]
-//   R,Rp  double[Dz]  rocking-curve fractions         fr    16 Dz bytes: per-frame floats and flags
+//   R,Rp  double[Dz]  rocking-curve fractions         fr    24 Dz bytes: per-frame values and flags
 //   co    double[Dx + Dy + Dz] box coordinates        tab   double[Dx + Dy + Dz] neighbour terms
 // ---------------------------------------------------------------------------------------------------------------------
 
@@ -77,7 +77,7 @@ __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
   L.pflag = o; o = al8(o + P);
   L.R = o; o += 8 * Dz;
   L.Rp = o; o += 8 * Dz;
-  L.fr = o; o = al8(o + 16 * Dz);
+  L.fr = o; o = al8(o + 24 * (size_t)Dz);
   L.co = o; o += 8 * C;
   L.tab = o; o += 8 * C;
   L.bytes = al8(o);
@@ -224,15 +224,89 @@ __device__ __forceinline__ Box box_of(const xpg_refl &R, int r) {
   return b;
 }
 
+// value code of a raw frame word whose pixel lies on the detector, m: bit 0 untrusted (source mask), bit 1 outside the
+// integrator's mask (the same rules as frame_code)
+__device__ __forceinline__ float code_raw(const xpg_frames &F, uint32_t raw, uint8_t m) {
+  float v;
+  if (F.dtype == 1) {
+    v = __builtin_bit_cast(float, raw);
+    if (m & 1) v = -1.0f;
+  } else {
+    const bool bad = raw < F.min_valid || (F.use_flag_value && raw == F.flag_value) || (m & 1);
+    v = (float)raw;
+    if (F.overload && raw >= F.overload) v = -2.0f;
+    if (bad) v = -1.0f;
+  }
+  if (m & 2) v = fminf(v, -1.0f);
+  return v;
+}
+
 // per-pixel row index (prow), then the shoebox of reflection r into vals (from sb, or extracted from the frame store
-// when sb == nullptr), flags cleared
+// when sb == nullptr), flags cleared.  tmp (8 P bytes) and ftmp (4 Dz bytes): LDS free during staging.
+//
+// Unbinned boxes of single pixels take the streaming path: the per-pixel detector offsets and mask bits and the
+// per-frame store indices are resolved once, then each lane issues NB frame loads before it consumes any, so a
+// wavefront waits for memory about V / (64 NB) times instead of V / 64.
 __device__ __forceinline__ void stage_values(const Box &b, const float *sb, const xpg_frames &F, float *vals,
-                                             uint8_t *vflag, uint8_t *prow, int lane) {
+                                             uint8_t *vflag, uint8_t *prow, unsigned char *tmp, unsigned char *ftmp,
+                                             int lane) {
   for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
   wave_sync();
   if (sb) {
     const float *src = sb + (int64_t)b.r * b.V;
     for (int v = lane; v < b.V; v += WAVE) vals[v] = src[v];
+  } else if (b.fb == 1 && b.s == 1) {
+    int32_t *poff = (int32_t *)tmp;                 // P: y nx + x, or -1 off the detector
+    uint8_t *pm = tmp + 4 * (size_t)b.P;            // P: mask bits
+    int32_t *ft = (int32_t *)ftmp;                  // Dz: store frame, or -1 (not read)
+    for (int p = lane; p < b.P; p += WAVE) {
+      const int y = b.iy0 + prow[p], x = b.ix0 + (p - prow[p] * b.Dx);
+      int32_t off = -1;
+      uint8_t m = 0;
+      if (y >= 0 && x >= 0 && y < F.ny && x < F.nx) {
+        off = y * F.nx + x;
+        if (F.src_mask && !F.src_mask[off]) m |= 1;
+        if (F.ceil_mask && !F.ceil_mask[off]) m |= 2;
+      }
+      poff[p] = off;
+      pm[p] = m;
+    }
+    for (int j = lane; j < b.Dz; j += WAVE) {
+      int64_t t;
+      ft[j] = frame_read(F, b.iz0 + j, t) ? (int32_t)t : -1;
+    }
+    wave_sync();
+    const uint32_t *src = (const uint32_t *)F.data;
+    constexpr int NB = 8;
+    VoxIter it(lane, b.P, b.V);
+    while (it.ok()) {
+      uint32_t raw[NB];
+      int vv[NB], pp[NB], kind[NB];            // kind: 0 loaded, 1 frame not read (-3), 2 off the detector (-1), 3 none
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        vv[u] = it.v;
+        pp[u] = it.p;
+        raw[u] = 0;
+        kind[u] = 3;
+        if (it.ok()) {
+          const int32_t t = ft[it.j], off = poff[it.p];
+          if (t < 0) {
+            kind[u] = 1;
+          } else if (off < 0) {
+            kind[u] = 2;
+          } else {
+            kind[u] = 0;
+            raw[u] = src[(int64_t)t * F.frame_stride + off];
+          }
+          it.next();
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NB; ++u) {
+        if (kind[u] == 3) continue;
+        vals[vv[u]] = kind[u] == 1 ? -3.0f : (kind[u] == 2 ? -1.0f : code_raw(F, raw[u], pm[pp[u]]));
+      }
+    }
   } else {
     for (VoxIter it(lane, b.P, b.V); it.ok(); it.next()) {
       const int iy = prow[it.p], ix = it.p - iy * b.Dx;
@@ -240,6 +314,7 @@ __device__ __forceinline__ void stage_values(const Box &b, const float *sb, cons
     }
   }
   for (int v = lane; v < b.V; v += WAVE) vflag[v] = 0;
+  wave_sync();
 }
 
 // box coordinates: co[0..Dx) pixel-centre x, co[Dx..Dx+Dy) y, then the Dz frame centres iz0 + fb (j + 1/2)
@@ -323,20 +398,18 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   double *scr = (double *)(base + L.scr);
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   float *bref = (float *)(base + L.pa), *thr = bref + P;
-  float *e1v = (float *)(base + L.pb), *e2v = e1v + P;
-  float *xf = (float *)(base + L.pc), *yf = xf + P;
   uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R);
-  float *zzf = (float *)(base + L.fr), *e3f = zzf + a.R.Dz;
-  uint8_t *fflag = (uint8_t *)(e3f + a.R.Dz);
+  double *zzd = (double *)(base + L.fr), *e3d = zzd + a.R.Dz;
+  uint8_t *fflag = (uint8_t *)(e3d + a.R.Dz);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
 
   const Box b = box_of(a.R, r);
-  const int V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  const int Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
 
-  stage_values(b, a.sb, a.F, vals, vflag, prow, lane);
+  stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
   Frame f;
   kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
   rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
@@ -344,53 +417,71 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   box_coords(b, co, lane);
   wave_sync();
   for (int p = lane; p < P; p += WAVE) {
-    const int iy = prow[p], ix = p - iy * Dx;
     const double u1 = U1[p], u2 = U2[p];
     const double r2 = u1 * u1 + u2 * u2;
     pflag[p] = (r2 <= 9.0 ? PF_FG2 : 0) | (r2 >= 12.25 ? PF_BGR : 0);
     bref[p] = template_at(det, co, Dx, p);
-    e1v[p] = (float)(u1 * seff);
-    e2v[p] = (float)(u2 * seff);
-    xf[p] = (float)co[ix];
-    yf[p] = (float)co[Dx + iy];
   }
   frames_fg(Rd, Dz, fflag, lane);
   for (int j = lane; j < Dz; j += WAVE) {
     double lo, hi;
     frame_bounds(call, b.iz0, b.fb, j, lo, hi);
-    e3f[j] = (float)(f.zeta * (0.5 * (lo + hi) - b.phi));
-    zzf[j] = (float)co[Dx + Dy + j];
+    e3d[j] = f.zeta * (0.5 * (lo + hi) - b.phi);
+    zzd[j] = co[Dx + Dy + j];
   }
   wave_sync();
   neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, prow, lane);
 
+  // Lanes run over pixels (p = lane, lane + 64, ...) and each lane over the frames of its pixel: per-pixel terms stay
+  // in registers, per-frame terms are broadcasts.  Sums are float64; sums of counts and of template values are exact.
+
   // per-voxel classes: fg = fg2 & fgz; fgv = fg & valid & ~nbm; background candidates (xpg/integrate_kernels.py:291-305)
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    const float c = vals[it.v];
-    uint8_t fl = vflag[it.v];
-    const bool valid = c >= 0.f, nbm = fl & VF_NBM;
-    const bool fg = (pflag[it.p] & PF_FG2) && fflag[it.j];
-    if (fg) fl |= VF_FG;
-    if (fg && valid && !nbm) fl |= VF_FGV;
-    if (!fg && valid && !nbm && (pflag[it.p] & PF_BGR)) fl |= VF_BGM;
-    vflag[it.v] = fl;
+  for (int p0 = 0; p0 < P; p0 += WAVE) {
+    const int p = p0 + lane;
+    if (p < P) {
+      const uint8_t pfl = pflag[p];
+      const bool fg2 = pfl & PF_FG2, bgr = pfl & PF_BGR;
+      for (int j = 0; j < Dz; ++j) {
+        const int v = j * P + p;
+        const float c = vals[v];
+        uint8_t fl = vflag[v];
+        const bool valid = c >= 0.f, nbm = fl & VF_NBM;
+        const bool fg = fg2 && fflag[j];
+        if (fg) fl |= VF_FG;
+        if (fg && valid && !nbm) fl |= VF_FGV;
+        if (!fg && valid && !nbm && bgr) fl |= VF_BGM;
+        vflag[v] = fl;
+      }
+    }
   }
   wave_sync();
 
   // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
   double bk = 0.0, nbg = 0.0;
   for (int round = 0; round < 5; ++round) {
-    csum sc = cz(), sT = cz();
-    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-      if (!(vflag[it.v] & VF_BGM)) continue;
-      const float c = vals[it.v];
-      if (round == 0 || c <= thr[it.p]) {
-        cadd(sc, c);
-        cadd(sT, bref[it.p]);
+    double sc = 0.0, sT = 0.0;
+    for (int p0 = 0; p0 < P; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const float th = round == 0 ? INFINITY : thr[p];
+        double cs = 0.0;
+        int cnt = 0;
+        for (int j = 0; j < Dz; ++j) {
+          const int v = j * P + p;
+          if (vflag[v] & VF_BGM) {
+            const float c = vals[v];
+            if (c <= th) {
+              cs += (double)c;
+              ++cnt;
+            }
+          }
+        }
+        sc += cs;
+        sT += (double)cnt * (double)bref[p];
       }
     }
-    const double Sc = creduce(sc);
-    nbg = creduce(sT);
+    const double Sc = wsum_d(sc);
+    nbg = wsum_d(sT);
     bk = Sc / fmax(nbg, 1e-6);
     if (round < 4) {
       for (int p = lane; p < P; p += WAVE) {
@@ -403,40 +494,63 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   }
 
   // ---- sums over the foreground ----
-  csum s_fgc = cz(), s_fgT = cz(), s_wpos = cz(), s_wx = cz(), s_wy = cz(), s_wz = cz();
-  csum s_m1 = cz(), s_m2 = cz(), s_mdd = cz(), s_m3 = cz(), s_m33 = cz();
+  double Sfgc = 0.0, sfb = 0.0, Swpos = 0.0, Swx = 0.0, Swy = 0.0, Swz = 0.0;
+  double Sm1 = 0.0, Sm2 = 0.0, Smdd = 0.0, Sm3 = 0.0, Sm33 = 0.0;
   int n_fg = 0, n_over = 0, n_fgm = 0, n_fgnb = 0;
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    const uint8_t fl = vflag[it.v];
-    if (!(fl & VF_FG)) continue;
-    const float c = vals[it.v];
-    n_over += (c == -2.0f);
-    n_fgm += (c < 0.f && c > -2.5f);
-    n_fgnb += (fl & VF_NBM) ? 1 : 0;
-    if (!(fl & VF_FGV)) continue;
-    const int p = it.p, j = it.j;
-    const double net = (double)c - bk * (double)bref[p];
-    const float netf = (float)net;
-    n_fg += 1;
-    cadd(s_fgc, c);
-    cadd(s_fgT, bref[p]);
-    if (net > 0.0) {
-      cadd(s_wpos, netf);
-      cadd(s_wx, netf * xf[p]);
-      cadd(s_wy, netf * yf[p]);
-      cadd(s_wz, netf * zzf[j]);
+  for (int p0 = 0; p0 < P; p0 += WAVE) {
+    const int p = p0 + lane;
+    if (p < P) {
+      const double bT = bk * (double)bref[p];
+      double cs = 0.0, npos = 0.0, nzz = 0.0, nall = 0.0, n3 = 0.0, n33 = 0.0;
+      int cnt = 0;
+      for (int j = 0; j < Dz; ++j) {
+        const int v = j * P + p;
+        const uint8_t fl = vflag[v];
+        if (!(fl & VF_FG)) continue;
+        const float c = vals[v];
+        n_over += (c == -2.0f);
+        n_fgm += (c < 0.f && c > -2.5f);
+        n_fgnb += (fl & VF_NBM) ? 1 : 0;
+        if (!(fl & VF_FGV)) continue;
+        const double net = (double)c - bT;
+        ++cnt;
+        cs += (double)c;
+        if (net > 0.0) {
+          npos += net;
+          nzz += net * zzd[j];
+        }
+        const double e3 = e3d[j];
+        nall += net;
+        n3 += net * e3;
+        n33 += net * (e3 * e3);
+      }
+      const int iy = prow[p], ix = p - iy * Dx;
+      const double a1 = U1[p] * seff, a2 = U2[p] * seff;
+      n_fg += cnt;
+      Sfgc += cs;
+      sfb += (double)cnt * (double)bref[p];
+      Swpos += npos;
+      Swx += npos * co[ix];
+      Swy += npos * co[Dx + iy];
+      Swz += nzz;
+      Sm1 += nall * a1;
+      Sm2 += nall * a2;
+      Smdd += nall * (a1 * a1 + a2 * a2);
+      Sm3 += n3;
+      Sm33 += n33;
     }
-    const float a1 = e1v[p], a2 = e2v[p], a3 = e3f[j];
-    cadd(s_m1, netf * a1);
-    cadd(s_m2, netf * a2);
-    cadd(s_mdd, netf * (a1 * a1 + a2 * a2));
-    cadd(s_m3, netf * a3);
-    cadd(s_m33, netf * (a3 * a3));
   }
-  const double Sfgc = creduce(s_fgc), sfb = creduce(s_fgT);
-  const double Swpos = creduce(s_wpos), Swx = creduce(s_wx), Swy = creduce(s_wy), Swz = creduce(s_wz);
-  const double Sm1 = creduce(s_m1), Sm2 = creduce(s_m2), Smdd = creduce(s_mdd), Sm3 = creduce(s_m3);
-  const double Sm33 = creduce(s_m33);
+  Sfgc = wsum_d(Sfgc);
+  sfb = wsum_d(sfb);
+  Swpos = wsum_d(Swpos);
+  Swx = wsum_d(Swx);
+  Swy = wsum_d(Swy);
+  Swz = wsum_d(Swz);
+  Sm1 = wsum_d(Sm1);
+  Sm2 = wsum_d(Sm2);
+  Smdd = wsum_d(Smdd);
+  Sm3 = wsum_d(Sm3);
+  Sm33 = wsum_d(Sm33);
   n_fg = wsum_i(n_fg);
   n_over = wsum_i(n_over);
   n_fgm = wsum_i(n_fgm);
@@ -637,8 +751,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   double *scr = (double *)(base + L.scr);
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   double *p2 = (double *)(base + L.pa), *pp2 = (double *)(base + L.pb);
-  float *bref = (float *)(base + L.pc), *bpx = bref + P;
-  float *p2f = (float *)(base + L.pd), *pp2f = p2f + P;
+  float *bref = (float *)(base + L.pc);
   uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R), *Rpd = (double *)(base + L.Rp);
   float *Rf = (float *)(base + L.fr), *Rpf = Rf + a.R.Dz;
@@ -647,7 +760,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
 
   Box b = box_of(a.R, r);
   b.region = min(max(b.region, 0), a.n_regions - 1);   // memory safety; the host validates region
-  const int V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  const int Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
   const double has_p = a.R.has_p ? (double)a.R.has_p[r] : 0.0;
@@ -656,7 +769,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double bgr = a.bg[r];
   const double gain = call.gain;
 
-  stage_values(b, a.sb, a.F, vals, vflag, prow, lane);
+  stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
   Frame f;
   kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
   rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
@@ -667,56 +780,77 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   for (int p = lane; p < P; p += WAVE) {
     const double r2 = U1[p] * U1[p] + U2[p] * U2[p];
     pflag[p] = r2 <= 9.0 ? PF_FG2 : 0;
-    const float br = template_at(det, co, Dx, p);
-    bref[p] = br;
-    bpx[p] = (float)(bgr * (double)br);
-    p2f[p] = (float)p2[p];
-    pp2f[p] = 0.f;
+    bref[p] = template_at(det, co, Dx, p);
+    pp2[p] = 0.0;
   }
   frames_fg(Rd, Dz, fflag, lane);
   for (int j = lane; j < Dz; j += WAVE) {
     Rf[j] = (float)Rd[j];
     Rpf[j] = 0.f;
+    Rpd[j] = 0.0;
   }
   if (partner) {
     // the overlapping partner's profile on this box's grid, fresh geometry (xpg/integrate_kernels.py:393-400)
     Frame fp;
     kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
+    wave_sync();
     rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], fp.zeta, Dz, scr, Rpd, lane);
     double *PU1 = scr, *PU2 = scr + P, *PJ = scr + 2 * P;
     const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), PU1, PU2, PJ, lane);
     profile_pixels(a.prof, b.region, PU1, PU2, PJ, seffp, P, pp2, lane);
-    for (int p = lane; p < P; p += WAVE) pp2f[p] = (float)pp2[p];
     for (int j = lane; j < Dz; j += WAVE) Rpf[j] = (float)Rpd[j];
   }
   wave_sync();
   neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, prow, lane);
 
+  // Lanes run over pixels, each lane over the frames of its pixel (per-pixel terms in registers, per-frame terms
+  // broadcast); sums in float64.
+
   // ---- fit mask: valid & ~nbm & (p > 0.01 max p | pp > 0.01 max(max pp, 1e-12)), float64 products ----
-  double pmax = -INFINITY, ppmax = -INFINITY;
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    pmax = fmax(pmax, p2[it.p] * Rd[it.j]);
-    ppmax = fmax(ppmax, partner ? (pp2[it.p] * Rpd[it.j]) * has_p : 0.0);
+  // p = p2 R and pp = (pp2 Rp) has_p are products of non-negative factors, so their maxima over the voxels are the
+  // products of the factors' maxima (rounding is monotonic).
+  double p2m = 0.0, pp2m = 0.0, Rm = 0.0, Rpm = 0.0;
+  for (int p = lane; p < P; p += WAVE) {
+    p2m = fmax(p2m, p2[p]);
+    pp2m = fmax(pp2m, pp2[p]);
   }
-  pmax = wmax_d(pmax);
-  ppmax = wmax_d(ppmax);
+  for (int j = lane; j < Dz; j += WAVE) {
+    Rm = fmax(Rm, Rd[j]);
+    Rpm = fmax(Rpm, Rpd[j]);
+  }
+  const double pmax = wmax_d(p2m) * wmax_d(Rm);
+  const double ppmax = partner ? (wmax_d(pp2m) * wmax_d(Rpm)) * has_p : 0.0;
   const double pthr = 0.01 * pmax, ppthr = 0.01 * fmax(ppmax, 1e-12);
-  csum s_net = cz(), s_pw = cz();
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    const float c = vals[it.v];
-    const double pd = p2[it.p] * Rd[it.j];
-    const double ppd = partner ? (pp2[it.p] * Rpd[it.j]) * has_p : 0.0;
-    uint8_t fl = vflag[it.v];
-    if ((pflag[it.p] & PF_FG2) && fflag[it.j]) fl |= VF_FG;
-    if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
-      fl |= VF_USE;
-      cadd(s_net, c - bpx[it.p]);
-      cadd(s_pw, (float)pd + (float)ppd);
+  double s_c = 0.0, s_b = 0.0, s_pw = 0.0;
+  for (int p0 = 0; p0 < P; p0 += WAVE) {
+    const int p = p0 + lane;
+    if (p < P) {
+      const double p2p = p2[p], pp2p = pp2[p];
+      const bool fg2 = pflag[p] & PF_FG2;
+      double cs = 0.0, pw = 0.0;
+      int cnt = 0;
+      for (int j = 0; j < Dz; ++j) {
+        const int v = j * P + p;
+        const float c = vals[v];
+        const double pd = p2p * Rd[j];
+        const double ppd = partner ? (pp2p * Rpd[j]) * has_p : 0.0;
+        uint8_t fl = vflag[v];
+        if (fg2 && fflag[j]) fl |= VF_FG;
+        if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
+          fl |= VF_USE;
+          cs += (double)c;
+          pw += pd + ppd;
+          ++cnt;
+        }
+        vflag[v] = fl;
+      }
+      s_c += cs;
+      s_b += (double)cnt * (bgr * (double)bref[p]);
+      s_pw += pw;
     }
-    vflag[it.v] = fl;
   }
   wave_sync();
-  double I = creduce(s_net) / fmax(creduce(s_pw), 1e-6);
+  double I = (wsum_d(s_c) - wsum_d(s_b)) / fmax(wsum_d(s_pw), 1e-6);
   double Ip = I * has_p;
 
   // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
@@ -726,28 +860,35 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     Iw = I;
     Ipw = Ip;
     const float Ii = (float)fmax(I, 0.0), Ipi = (float)fmax(Ip, 0.0);
-    csum s11 = cz(), s12 = cz(), s22 = cz(), sb1 = cz(), sb2 = cz();
-    for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-      if (!(vflag[it.v] & VF_USE)) continue;
-      const float b0 = bpx[it.p];
-      const float net = vals[it.v] - b0;
-      const float pf = p2f[it.p] * Rf[it.j];
-      const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
-      const float mu = (b0 + Ii * pf) + Ipi * ppf;
-      const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
-      cadd(s11, (pf * pf) * iv);
-      cadd(sb1, (pf * net) * iv);
-      if (partner) {
-        cadd(s12, (pf * ppf) * iv);
-        cadd(s22, (ppf * ppf) * iv);
-        cadd(sb2, (ppf * net) * iv);
+    double s11 = 0.0, s12 = 0.0, s22 = 0.0, sb1 = 0.0, sb2 = 0.0;
+    for (int p0 = 0; p0 < P; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const float b0 = (float)(bgr * (double)bref[p]);
+        const float p2f = (float)p2[p], pp2f = (float)pp2[p];
+        for (int j = 0; j < Dz; ++j) {
+          const int v = j * P + p;
+          if (!(vflag[v] & VF_USE)) continue;
+          const float net = vals[v] - b0;
+          const float pf = p2f * Rf[j];
+          const float ppf = (pp2f * Rpf[j]) * hpf;
+          const float mu = (b0 + Ii * pf) + Ipi * ppf;
+          const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
+          s11 += (double)((pf * pf) * iv);
+          sb1 += (double)((pf * net) * iv);
+          if (partner) {
+            s12 += (double)((pf * ppf) * iv);
+            s22 += (double)((ppf * ppf) * iv);
+            sb2 += (double)((ppf * net) * iv);
+          }
+        }
       }
     }
-    a11 = creduce(s11);
-    const double b1 = creduce(sb1);
-    a12 = partner ? creduce(s12) : 0.0;
-    a22 = (partner ? creduce(s22) : 0.0) + (1.0 - has_p) * 1.0;
-    const double b2 = partner ? creduce(sb2) : 0.0;
+    a11 = wsum_d(s11);
+    const double b1 = wsum_d(sb1);
+    a12 = partner ? wsum_d(s12) : 0.0;
+    a22 = (partner ? wsum_d(s22) : 0.0) + (1.0 - has_p) * 1.0;
+    const double b2 = partner ? wsum_d(sb2) : 0.0;
     det2 = fmax(a11 * a22 - a12 * a12, 1e-30);
     const double I_old = I;
     I = (a22 * b1 - a12 * b2) / det2;
@@ -761,49 +902,63 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   // ---- tail (xpg/integrate_kernels.py:405-429): weights of the last pass (intensities Iw, Ipw) ----
   const float Ii = (float)fmax(Iw, 0.0), Ipi = (float)fmax(Ipw, 0.0);
   const float Ipos = (float)fmax(Ip, 0.0);
-  csum sC = cz(), s110 = cz(), sC0 = cz(), swp = cz(), sres = cz();
+  double sC = 0.0, s110 = 0.0, sC0 = 0.0, swp = 0.0, sres = 0.0;
   int nuse = 0;
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    if (!(vflag[it.v] & VF_USE)) continue;
-    const float b0 = bpx[it.p];
-    const float net = vals[it.v] - b0;
-    const float pf = p2f[it.p] * Rf[it.j];
-    const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
-    const float mu = (b0 + Ii * pf) + Ipi * ppf;
-    const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
-    const float iv0 = 1.0f / (gainf * fmaxf(b0, (float)VAR_FLOOR));
-    const float pb = pf * bref[it.p];
-    cadd(sC, pb * iv);
-    cadd(s110, (pf * pf) * iv0);
-    cadd(sC0, pb * iv0);
-    cadd(swp, pf);
-    cadd(sres, net - Ipos * ppf);
-    nuse += 1;
+  for (int p0 = 0; p0 < P; p0 += WAVE) {
+    const int p = p0 + lane;
+    if (p < P) {
+      const float b0 = (float)(bgr * (double)bref[p]), br = bref[p];
+      const float p2f = (float)p2[p], pp2f = (float)pp2[p];
+      const float iv0 = 1.0f / (gainf * fmaxf(b0, (float)VAR_FLOOR));
+      for (int j = 0; j < Dz; ++j) {
+        const int v = j * P + p;
+        if (!(vflag[v] & VF_USE)) continue;
+        const float net = vals[v] - b0;
+        const float pf = p2f * Rf[j];
+        const float ppf = (pp2f * Rpf[j]) * hpf;
+        const float mu = (b0 + Ii * pf) + Ipi * ppf;
+        const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
+        const float pb = pf * br;
+        sC += (double)(pb * iv);
+        s110 += (double)((pf * pf) * iv0);
+        sC0 += (double)(pb * iv0);
+        swp += (double)pf;
+        sres += (double)(net - Ipos * ppf);
+        nuse += 1;
+      }
+    }
   }
-  const double C = creduce(sC), A110 = creduce(s110), C0 = creduce(sC0), Swp = creduce(swp), Sres = creduce(sres);
+  const double C = wsum_d(sC), A110 = wsum_d(s110), C0 = wsum_d(sC0), Swp = wsum_d(swp), Sres = wsum_d(sres);
   nuse = wsum_i(nuse);
   const double nu = fmax((double)nuse, 1.0);
   const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
-  csum sdd = cz(), sdp = cz(), snn = cz();
-  csum s_pv = cz(), s_pt = cz(), s_pfg = cz();
-  for (VoxIter it(lane, P, V); it.ok(); it.next()) {
-    const uint8_t fl = vflag[it.v];
-    const float pf = p2f[it.p] * Rf[it.j];
-    const float c = vals[it.v];
-    cadd(s_pt, pf);
-    if (c >= 0.f && !(fl & VF_NBM)) cadd(s_pv, pf);
-    if (fl & VF_FG) cadd(s_pfg, pf);
-    if (!(fl & VF_USE)) continue;
-    const float net = c - bpx[it.p];
-    const float ppf = (pp2f[it.p] * Rpf[it.j]) * hpf;
-    const float dp = pf - pm;
-    const float dn = (net - Ipos * ppf) - nm;
-    cadd(sdp, dp * dn);
-    cadd(sdd, dp * dp);
-    cadd(snn, dn * dn);
+  double sdd = 0.0, sdp = 0.0, snn = 0.0, s_pv = 0.0, s_pt = 0.0, s_pfg = 0.0;
+  for (int p0 = 0; p0 < P; p0 += WAVE) {
+    const int p = p0 + lane;
+    if (p < P) {
+      const float b0 = (float)(bgr * (double)bref[p]);
+      const float p2f = (float)p2[p], pp2f = (float)pp2[p];
+      for (int j = 0; j < Dz; ++j) {
+        const int v = j * P + p;
+        const uint8_t fl = vflag[v];
+        const float pf = p2f * Rf[j];
+        const float c = vals[v];
+        s_pt += (double)pf;
+        if (c >= 0.f && !(fl & VF_NBM)) s_pv += (double)pf;
+        if (fl & VF_FG) s_pfg += (double)pf;
+        if (!(fl & VF_USE)) continue;
+        const float net = c - b0;
+        const float ppf = (pp2f * Rpf[j]) * hpf;
+        const float dp = pf - pm;
+        const float dn = (net - Ipos * ppf) - nm;
+        sdp += (double)(dp * dn);
+        sdd += (double)(dp * dp);
+        snn += (double)(dn * dn);
+      }
+    }
   }
-  const double Sdp = creduce(sdp), Sdd = creduce(sdd), Snn = creduce(snn);
-  const double Spv = creduce(s_pv), Spt = creduce(s_pt), Spfg = creduce(s_pfg);
+  const double Sdp = wsum_d(sdp), Sdd = wsum_d(sdd), Snn = wsum_d(snn);
+  const double Spv = wsum_d(s_pv), Spt = wsum_d(s_pt), Spfg = wsum_d(s_pfg);
   if (lane == 0) {
     const double var_b = gain * fmax(bgr, VAR_FLOOR) / fmax(a.nbg[r], 1.0);
     const double a11i = a22 / det2;
