@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one copy-like kernel (out = -in, float32) over MIB MiB, outside the timed region: "
+                         "the reference of known traffic for hardware byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: every box here has one GPU, so --gpus must be 1 (multi-GPU sharding is "
@@ -131,6 +134,13 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        src = torch.ones(args.calibrate_copy * 2 ** 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize(dev)
+        log(f"copy calibration: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
+        del src, dst
     spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
                 stress=synth.spec_stress)[args.workload]()
     if args.frames:
@@ -156,21 +166,20 @@ def main():
     for i in range(args.warmup):
         r = sweep.run()
         log(f"warmup {i + 1}/{args.warmup}: {r['t_gpu'] * 1e3:.1f} ms")
-    launches = {}
     events = []
-    stream = torch.cuda.current_stream(dev)
 
     class Timer:
-        def __init__(self, kind, b, gi):
-            self.kind, self.gi = kind, gi
+        """HIP events around one launch, recorded on the stream the launch goes to."""
+        def __init__(self, kind, b, gi, st):
+            self.kind, self.gi, self.st = kind, gi, st
 
         def __enter__(self):
             self.e0 = torch.cuda.Event(enable_timing=True)
             self.e1 = torch.cuda.Event(enable_timing=True)
-            self.e0.record(stream)
+            self.e0.record(self.st)
 
         def __exit__(self, *a):
-            self.e1.record(stream)
+            self.e1.record(self.st)
             events.append((self.kind, self.gi, self.e0, self.e1))
             return False
 

@@ -60,6 +60,7 @@ typedef struct xpg_widths {
   double sd, sm;
   float rcw[XPG_RC_K];
   int32_t n_strong;        /* strong reflections the widths were fitted on (0: carried over) */
+  uint32_t active;         /* bit j: fitted mixture weight j is positive (the next fit starts from this set; 0: cold) */
   double sd_fit, sm_fit;   /* this block's own estimates before shrinkage (radians; 0 when not fitted) */
 } xpg_widths;
 

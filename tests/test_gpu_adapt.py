@@ -11,7 +11,7 @@ from oracle import adapt as oa
 pytestmark = pytest.mark.gpu
 
 
-def _run_device(b, sd, sm, dev):
+def _run_device(b, sd, sm, dev, active=0):
     from xpg_amd import _lib as L
     from xpg_amd.integrate_kernels import rc_default_weights
     n = len(b["isum"])
@@ -24,6 +24,7 @@ def _run_device(b, sd, sm, dev):
     w = L.Widths()
     w.sd, w.sm = sd, sm
     w.rcw[:] = [float(v) for v in rc_default_weights()]
+    w.active = active
     sz = C.sizeof(L.Widths)
     buf = torch.frombuffer(bytearray(bytes(w) * 3), dtype=torch.uint8).to(dev)
     wsb = int(L.lib().xpg_adapt_workspace_bytes(n))
@@ -57,3 +58,10 @@ def test_adapt_block_matches_oracle(seed, n, sm_true):
     assert abs(wn.sd - sd_new) <= 1e-12 * sd_new and abs(w2.sd_fit - sd_fit) <= 1e-12 * sd_fit
     assert w2.sd == sd                                      # this block's pass 2 keeps its own sigma_D
     np.testing.assert_allclose(np.array(w2.rcw[:]), w, atol=2e-6)
+    assert w2.active == wn.active and w2.active != 0
+    # a fit started from the positive set of the previous one (the next block's warm start) gives the same weights
+    w2w, _ = _run_device(b, sd, sm, torch.device("cuda:0"), active=w2.active)
+    np.testing.assert_allclose(np.array(w2w.rcw[:]), np.array(w2.rcw[:]), atol=1e-6)
+    # and so does a start from an unrelated set
+    w2x, _ = _run_device(b, sd, sm, torch.device("cuda:0"), active=0b101010101010101010101)
+    np.testing.assert_allclose(np.array(w2x.rcw[:]), np.array(w2.rcw[:]), atol=1e-6)

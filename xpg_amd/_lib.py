@@ -39,7 +39,7 @@ class Call(C.Structure):
 
 class Widths(C.Structure):
     _fields_ = [("sd", C.c_double), ("sm", C.c_double), ("rcw", C.c_float * RC_K), ("n_strong", C.c_int32),
-                ("sd_fit", C.c_double), ("sm_fit", C.c_double)]
+                ("active", C.c_uint32), ("sd_fit", C.c_double), ("sm_fit", C.c_double)]
 
 
 class Refl(C.Structure):

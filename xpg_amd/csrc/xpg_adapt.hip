@@ -24,8 +24,8 @@ namespace {
 
 constexpr int K = XPG_RC_K;               // 21 mixture components
 constexpr int NF = 65;                     // frames of a per-frame profile row (max(DZ_BUCKETS))
-constexpr int NG = 256;                    // workgroups of the Gram accumulation
-constexpr int NCH = 32;                    // row ranges of the grid costs
+constexpr int NG = 1024;                   // workgroups of the Gram accumulation
+constexpr int NCH = 20;                    // row ranges of the grid costs (61 x 20 x 4 wavefronts: one round)
 constexpr int NE = K * (K + 1) / 2 + K;    // unique Gram entries + right-hand side
 constexpr int NGRID = 61;
 constexpr double PI = 3.141592653589793;
@@ -327,7 +327,7 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
 constexpr int GW = 4;                      // wavefronts per workgroup
 __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
   __shared__ double A[GW][NF + 1][K + 1];  // boundary erfs E[b][j], then in place A[k][j] = B_kj sqrt(tot), A[k][K] = y
-  __shared__ double red[GW][NE];
+  double (*red)[NE] = (double (*)[NE])&A[0][0][0];   // after the rows: the wavefronts' partial sums
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double (*Aw)[K + 1] = A[wv];
   double *Af = &A[wv][0][0];                // the same storage, flat
@@ -363,29 +363,20 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
       const double tot = fmax(w.tot[r], 1e-9), st = sqrt(tot);
       const int iz0 = a.iz0[r];
       // boundary b saturates every component to +1 when (bd - phi) - 3.5 s > ERF_SAT sw, to -1 when
-      // (bd - phi) + 3.5 s < -ERF_SAT sw (|RC_C| <= 3.5)
-      uint64_t pos_lo, neg_lo;
-      bool pos64, neg64, pos65, neg65;
-      {
-        const double bd = bound_deg(a, iz0, lane) * (PI / 180.0) - phi;
-        pos_lo = __ballot(bd - 3.5 * s > ERF_SAT * sw);
-        neg_lo = __ballot(bd + 3.5 * s < -ERF_SAT * sw);
-        const double b64 = bound_deg(a, iz0, 64) * (PI / 180.0) - phi, b65 = bound_deg(a, iz0, 65) * (PI / 180.0) - phi;
-        pos64 = b64 - 3.5 * s > ERF_SAT * sw;
-        neg64 = b64 + 3.5 * s < -ERF_SAT * sw;
-        pos65 = b65 - 3.5 * s > ERF_SAT * sw;
-        neg65 = b65 + 3.5 * s < -ERF_SAT * sw;
-      }
-      auto pos = [&](int b) { return b < 64 ? ((pos_lo >> b) & 1) != 0 : (b == 64 ? pos64 : pos65); };
-      auto neg = [&](int b) { return b < 64 ? ((neg_lo >> b) & 1) != 0 : (b == 64 ? neg64 : neg65); };
-      int kl = NF, kh = -1;
-      for (int k = 0; k < NF; ++k) {       // uniform scalar loop
-        const bool dead = (pos(k) && pos(k + 1)) || (neg(k) && neg(k + 1));
-        if (!dead) {
-          kl = min(kl, k);
-          kh = k;
-        }
-      }
+      // (bd - phi) + 3.5 s < -ERF_SAT sw (|RC_C| <= 3.5); frame k is dead (B_kj = 0 for all j) when both its
+      // boundaries saturate to the same sign
+      const double bdl = bound_deg(a, iz0, lane) * (PI / 180.0) - phi;
+      const uint64_t pos_lo = __ballot(bdl - 3.5 * s > ERF_SAT * sw);
+      const uint64_t neg_lo = __ballot(bdl + 3.5 * s < -ERF_SAT * sw);
+      const double b64 = bound_deg(a, iz0, 64) * (PI / 180.0) - phi, b65 = bound_deg(a, iz0, 65) * (PI / 180.0) - phi;
+      const bool pos64 = b64 - 3.5 * s > ERF_SAT * sw, neg64 = b64 + 3.5 * s < -ERF_SAT * sw;
+      const bool pos65 = b65 - 3.5 * s > ERF_SAT * sw, neg65 = b65 + 3.5 * s < -ERF_SAT * sw;
+      uint64_t dead = (pos_lo & (pos_lo >> 1)) | (neg_lo & (neg_lo >> 1));      // frames 0 .. 62
+      if ((((pos_lo >> 63) & 1ull) && pos64) || (((neg_lo >> 63) & 1ull) && neg64)) dead |= 1ull << 63;
+      const bool live64 = !((pos64 && pos65) || (neg64 && neg65));
+      const uint64_t live = ~dead;
+      const int kl = live ? __ffsll((long long)live) - 1 : (live64 ? 64 : NF);
+      const int kh = live64 ? 64 : (live ? 63 - __clzll((long long)live) : -1);
       if (kh < kl) continue;
       const int nb = kh - kl + 2;          // boundaries kl .. kh + 1
       for (int u = lane; u < nb * K; u += 64) {
@@ -433,6 +424,7 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
+  __syncthreads();
   for (int i = 0; i < 4; ++i)
     if (lane + 64 * i < NE) red[wv][lane + 64 * i] = acc[i];
   __syncthreads();
@@ -476,9 +468,30 @@ __global__ void __launch_bounds__(1024) k_ad_gsum(Ws w) {
 // holds row i (a) and column i (ct) of L, and z = L^-1 h[P].  Adding a variable appends one row (a forward
 // substitution); removing variables refactors by re-appending the remaining ones in order.  Every index into a
 // per-lane array is static; broadcasts are readlanes.
-typedef double dvec32 __attribute__((ext_vector_type(32)));   // per-lane vectors kept in registers
+// compile-time loop: f(IC<0>{}), ..., f(IC<N - 1>{}) in order (indices are constants, so per-lane arrays stay in
+// registers)
+template <int I>
+struct IC {
+  static constexpr int value = I;
+};
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (N > 0) {
+    static_for<N - 1>(f);
+    f(IC<N - 1>{});
+  }
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for_down(F &&f) {     // f(IC<N - 1>{}), ..., f(IC<0>{})
+  if constexpr (N > 0) {
+    f(IC<N - 1>{});
+    static_for_down<N - 1>(f);
+  }
+}
+
 struct Nnls {
-  dvec32 a, ct;
+  double a[K], ct[K];
+  double invd;     // lane i < m: 1 / L_ii
   double zv;       // lane i < m: z_i
   int ordv;        // lane i < m: variable at position i
   int posv;        // lane j: position of variable j, or -1
@@ -486,54 +499,58 @@ struct Nnls {
 };
 
 __device__ __forceinline__ void nnls_reset(Nnls &S) {
-  S.a = 0.0;
-  S.ct = 0.0;
+  static_for<K>([&](auto k) { S.a[k.value] = S.ct[k.value] = 0.0; });
+  S.invd = 0.0;
   S.zv = 0.0;
   S.ordv = -1;
   S.posv = -1;
   S.m = 0;
 }
 
-// append variable t (uniform); false (state unchanged) when G[P+t, P+t] is not numerically positive definite
+// append variable t (uniform); false (state unchanged) when G[P+t, P+t] is not numerically positive definite.
+// The substitutions multiply by the stored reciprocal diagonal and use fused multiply-adds: each step of a triangular
+// solve is a readlane, a multiply and one fma.
 __device__ __forceinline__ bool nnls_append(Nnls &S, const double *Gs, const double *hs, int t) {
   const int lane = threadIdx.x & 63;
   const int m = S.m;
   double b = lane < m ? Gs[S.ordv * K + t] : 0.0;
   // L l = G[P, t], column-oriented: after step k lane k holds l_k
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
     if (k < m) {
-      const double lk = rld(b, k) / rld(S.a[k], k);
+      const double lk = rld(b, k) * rld(S.invd, k);
       if (lane == k) b = lk;
-      else if (lane > k) b -= S.a[k] * lk;
+      else if (lane > k) b = fma(-S.a[k], lk, b);
     }
-  }
+  });
   double d2 = Gs[t * K + t], zm = hs[t];
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
     if (k < m) {
       const double lk = rld(b, k);
-      d2 -= lk * lk;
-      zm -= lk * rld(S.zv, k);
+      d2 = fma(-lk, lk, d2);
+      zm = fma(-lk, rld(S.zv, k), zm);
     }
-  }
+  });
   if (!(d2 > 0)) return false;
   const double d = sqrt(d2);
-  zm = zm / d;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
+  const double id = 1.0 / d;
+  zm = zm * id;
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    // unconditional stores of selected values keep the per-lane arrays in registers
     if (k < m) {
       const double lk = rld(b, k);
-      if (lane == m) S.a[k] = lk;
+      S.a[k] = lane == m ? lk : S.a[k];
     } else if (k == m) {
-      if (lane == m) S.a[k] = d;
-      if (lane < m) S.ct[k] = b;
-      else if (lane == m) S.ct[k] = d;
+      S.a[k] = lane == m ? d : S.a[k];
+      S.ct[k] = lane < m ? b : (lane == m ? d : S.ct[k]);
     }
-  }
+  });
   if (lane == m) {
     S.zv = zm;
     S.ordv = t;
+    S.invd = id;
   }
   if (lane == t) S.posv = m;
   S.m = m + 1;
@@ -545,14 +562,14 @@ __device__ __forceinline__ double nnls_solve(const Nnls &S) {
   const int lane = threadIdx.x & 63;
   const int m = S.m;
   double c = lane < m ? S.zv : 0.0;
-#pragma unroll
-  for (int k = K - 1; k >= 0; --k) {
+  static_for_down<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
     if (k < m) {
-      const double sk = rld(c, k) / rld(S.ct[k], k);
+      const double sk = rld(c, k) * rld(S.invd, k);
       if (lane == k) c = sk;
-      else if (lane < k) c -= S.ct[k] * sk;
+      else if (lane < k) c = fma(-S.ct[k], sk, c);
     }
-  }
+  });
   const double sv = __shfl(c, S.posv < 0 ? 0 : S.posv, 64);
   return S.posv < 0 ? 0.0 : sv;
 }
@@ -582,21 +599,46 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
   __syncthreads();
   double grow[K];
   const int il = lane < K ? lane : 0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) grow[k] = lane < K ? Gs[il * K + k] : 0.0;
+  static_for<K>([&](auto k) { grow[k.value] = lane < K ? Gs[il * K + k.value] : 0.0; });
   const double hl = lane < K ? hs[il] : 0.0;
   double gmax = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) gmax = fmax(gmax, fabs(grow[k]));
+  static_for<K>([&](auto k) { gmax = fmax(gmax, fabs(grow[k.value])); });
   for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o, 64));
   const double tol = 10.0 * 2.220446049250313e-16 * gmax * K;
   Nnls S;
   nnls_reset(S);
   double x = 0.0;                          // lane j: x_j
+  int n_outer = 0, n_inner = 0;
+  // Warm start from the previous fit's positive set: append it, drop the variables whose unconstrained solution is not
+  // positive until all are, and continue from that point (x_P the minimiser on P, x >= 0: the invariant of the
+  // outer loop).  The minimiser is unique for a positive definite G, so only the iteration count depends on it.
+  {
+    const uint32_t warm = in->active & ((1u << K) - 1u);
+    for (int j = 0; j < K; ++j)
+      if ((warm >> j) & 1u) nnls_append(S, Gs, hs, j);
+    for (int pass = 0; pass < K && S.m > 0; ++pass) {
+      ++n_inner;
+      const double sj = nnls_solve(S);
+      const bool act = S.posv >= 0;
+      const bool bad = act && !(sj > tol);
+      if (!__ballot(bad)) {
+        x = act ? sj : 0.0;
+        break;
+      }
+      const int mo = S.m;
+      const int ord_old = S.ordv;
+      const uint64_t keep = __ballot(lane < K && act && !bad);
+      nnls_reset(S);
+      for (int i = 0; i < mo; ++i) {
+        const int v = __builtin_amdgcn_readlane(ord_old, i);
+        if ((keep >> v) & 1ull) nnls_append(S, Gs, hs, v);
+      }
+    }
+  }
   for (int outer = 0; outer < 3 * K; ++outer) {
+    ++n_outer;
     double g = hl;
-#pragma unroll
-    for (int k = 0; k < K; ++k) g -= grow[k] * rld(x, k);
+    static_for<K>([&](auto k) { g = fma(-grow[k.value], rld(x, k.value), g); });
     // the first variable outside P with the largest gradient above tol
     const bool cand = lane < K && S.posv < 0 && g > tol;
     double gb = cand ? g : -INFINITY;
@@ -606,6 +648,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     const int t = __ffsll((long long)hit) - 1;
     if (!nnls_append(S, Gs, hs, t)) break;   // the same t would be chosen again with x unchanged
     for (int inner = 0; inner < 3 * K; ++inner) {
+      ++n_inner;
       const double sj = nnls_solve(S);
       const bool act = S.posv >= 0;
       if (!__ballot(act && sj <= 0)) {
@@ -651,7 +694,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
     out2->rcw[lane] = wj;
     outn->rcw[lane] = wj;
   }
+  const uint32_t act_out = (uint32_t)__ballot(lane < K && xs > 0.0);
   if (lane == 0) {
+    w.scal[5] = (double)n_outer;           // iteration counts of the last solve (diagnostics)
+    w.scal[6] = (double)n_inner;
+    out2->active = act_out;
+    outn->active = act_out;
     out2->sd = in->sd;
     out2->sm = w.scal[1];
     out2->n_strong = n;

@@ -37,47 +37,43 @@ static int fail(int code, const char *fmt, ...) {
   } while (0)
 
 // ---------------------------------------------------------------------------------------------------------------------
-// LDS layout of one wavefront (pass 1 and pass 2 share it)
+// LDS layout of one wavefront, per pass
 //   vals  float[V]    shoebox values                 vflag u8[V]   per-voxel flags
-//   scr   doubles     max(21 Dz, 3 P): rocking-curve terms, then the partner's u1, u2, J in pass 2
-//   U1,U2 double[P]   u1, u2                          Jd    double[P] pixel area
-//   pa    8P bytes    pass 1: bref, thr (float)    /  pass 2: p2 (double)
-//   pb    8P bytes    pass 1: e1v, e2v (float)     /  pass 2: pp2 (double)
-//   pc    8P bytes    pass 1: x, y (float)         /  pass 2: bref, b*bref (float)
-//   pd    8P bytes                                     pass 2: p2, pp2 (float)
+//   scr   double[21 Dz]  rocking-curve terms (and the recorded-frame fractions of pass 1)
+//   U1,U2 double[P]   u1, u2                          Jd    double[P] pixel area (pass 2: then the partner's)
+//   pa    8P bytes    pass 1: bref, thr (float)    /  pass 2: p2 (double)   (staging: detector offsets, mask bits)
+//   pb    8P bytes                                     pass 2: pp2 (double)
+//   pc    4P bytes                                     pass 2: bref (float)
 //   prow  u8[P]       row of each pixel               pflag u8[P]
-//   R,Rp  double[Dz]  rocking-curve fractions         fr    24 Dz bytes: per-frame values and flags
+//   R     double[Dz]  rocking-curve fractions (staging: store frames)    Rp double[Dz] pass 2: the partner's
+//   fr    pass 1: zz, e3 (double[Dz] each), flags u8[Dz] / pass 2: R, Rp (float[Dz] each), flags u8[Dz]
 //   co    double[Dx + Dy + Dz] box coordinates        tab   double[Dx + Dy + Dz] neighbour terms
 // ---------------------------------------------------------------------------------------------------------------------
 
 __host__ __device__ static inline size_t al8(size_t x) { return (x + 7) & ~(size_t)7; }
 
 struct Layout {
-  size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, pd, prow, pflag, R, Rp, fr, co, tab, bytes;
+  size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, prow, pflag, R, Rp, fr, co, tab, bytes;
 };
 
-__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx) {
+__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int pass) {
   const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
   Layout L;
   size_t o = 0;
   L.vals = o; o = al8(o + 4 * V);
   L.vflag = o; o = al8(o + V);
-  L.scr = o; {
-    size_t a = 21 * (size_t)Dz, b = 3 * P;
-    o = al8(o + 8 * (a > b ? a : b));
-  }
+  L.scr = o; o += 8 * 21 * (size_t)Dz;
   L.U1 = o; o += 8 * P;
   L.U2 = o; o += 8 * P;
   L.Jd = o; o += 8 * P;
   L.pa = o; o += 8 * P;
-  L.pb = o; o += 8 * P;
-  L.pc = o; o += 8 * P;
-  L.pd = o; o += 8 * P;
+  L.pb = o; if (pass == 2) o += 8 * P;
+  L.pc = o; if (pass == 2) o = al8(o + 4 * P);
   L.prow = o; o = al8(o + P);
   L.pflag = o; o = al8(o + P);
   L.R = o; o += 8 * Dz;
-  L.Rp = o; o += 8 * Dz;
-  L.fr = o; o = al8(o + 24 * (size_t)Dz);
+  L.Rp = o; if (pass == 2) o += 8 * Dz;
+  L.fr = o; o = al8(o + (pass == 2 ? 9 : 17) * (size_t)Dz);
   L.co = o; o += 8 * C;
   L.tab = o; o += 8 * C;
   L.bytes = al8(o);
@@ -390,7 +386,7 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const int wid = threadIdx.x / WAVE;
   const int r = blockIdx.x * a.waves + wid;
   if (r >= a.R.n) return;
-  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx);
+  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx, 1);
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
   const int P = a.R.Dy * a.R.Dx;
   float *vals = (float *)(base + L.vals);
@@ -743,7 +739,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const int wid = threadIdx.x / WAVE;
   const int r = blockIdx.x * a.waves + wid;
   if (r >= a.R.n) return;
-  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx);
+  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx, 2);
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
   const int P = a.R.Dy * a.R.Dx;
   float *vals = (float *)(base + L.vals);
@@ -795,9 +791,9 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
     wave_sync();
     rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], fp.zeta, Dz, scr, Rpd, lane);
-    double *PU1 = scr, *PU2 = scr + P, *PJ = scr + 2 * P;
-    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), PU1, PU2, PJ, lane);
-    profile_pixels(a.prof, b.region, PU1, PU2, PJ, seffp, P, pp2, lane);
+    // U1, U2, Jd of this reflection are no longer needed (pflag is set): they take the partner's
+    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+    profile_pixels(a.prof, b.region, U1, U2, Jd, seffp, P, pp2, lane);
     for (int j = lane; j < Dz; j += WAVE) Rpf[j] = (float)Rpd[j];
   }
   wave_sync();
@@ -1009,8 +1005,8 @@ static int check_refl(const xpg_refl *R, bool need_partner_arrays) {
   return XPG_OK;
 }
 
-static int launch_shape(const xpg_refl *R, int *waves, int *wave_bytes, size_t *lds) {
-  const Layout L = make_layout(R->Dz, R->Dy, R->Dx);
+static int launch_shape(const xpg_refl *R, int pass, int *waves, int *wave_bytes, size_t *lds) {
+  const Layout L = make_layout(R->Dz, R->Dy, R->Dx, pass);
   const int lim = lds_limit();
   if ((long long)L.bytes > lim)
     return fail(XPG_ERR_TOO_LARGE, "shoebox %dx%dx%d needs %zu bytes of LDS > %d", R->Dz, R->Dy, R->Dx, L.bytes, lim);
@@ -1092,7 +1088,7 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   if (acc) a.acc = *acc;
   else memset(&a.acc, 0, sizeof(a.acc));
   size_t lds;
-  rc = launch_shape(refl, &a.waves, &a.wave_bytes, &lds);
+  rc = launch_shape(refl, 1, &a.waves, &a.wave_bytes, &lds);
   if (rc) return rc;
   rc = set_lds(k_pass1, lds);
   if (rc) return rc;
@@ -1164,7 +1160,7 @@ static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   a.nbg = nbg;
   a.out = *out;
   size_t lds;
-  rc = launch_shape(refl, &a.waves, &a.wave_bytes, &lds);
+  rc = launch_shape(refl, 2, &a.waves, &a.wave_bytes, &lds);
   if (rc) return rc;
   rc = set_lds(k_pass2, lds);
   if (rc) return rc;
@@ -1188,6 +1184,9 @@ int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_re
   return pass2_impl(det, call, refl, nullptr, frames, prof, n_regions, bg, nbg, irls_rule, out, stream);
 }
 
-int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) { return (int)make_layout(Dz, Dy, Dx).bytes; }
+int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) {
+  const size_t a = make_layout(Dz, Dy, Dx, 1).bytes, b = make_layout(Dz, Dy, Dx, 2).bytes;
+  return (int)(a > b ? a : b);
+}
 
 }  // extern "C"

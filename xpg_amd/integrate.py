@@ -391,7 +391,9 @@ class Sweep:
 
     The adaptive chain stays on the device: pass 1 of a block reads its widths record (sigma_D, sigma_M, rocking-curve
     weights) from device memory, xpg_adapt_block writes the records for the block's pass 2 and for the next block, so
-    a whole sweep is a sequence of launches on one stream with no host round trip."""
+    a whole sweep is a sequence of launches with no host round trip.  The chain pass 1 -> fits -> normalisation of
+    consecutive blocks is serial and runs on the caller's stream; each block's pass 2 depends only on its own chain
+    and runs on a second stream, overlapping the next blocks' chains."""
 
     def __init__(self, images, exp_blocks, model, hp, opts):
         from . import _lib as L
@@ -464,9 +466,13 @@ class Sweep:
         self.widths.copy_(raw.repeat(2 * nb + 2).to(self.device))
 
     def run(self, timer=None):
-        """Device execution of the whole sweep; ``timer(kind, b, gi)`` may return a context that brackets launches."""
+        """Device execution of the whole sweep; ``timer(kind, b, gi, stream)`` may return a context that brackets
+        launches on ``stream``."""
         from . import _lib as L
         o, sp, device, model = self.o, self.sp, self.device, self.model
+        main = torch.cuda.current_stream(device)
+        if not hasattr(self, "side"):
+            self.side = torch.cuda.Stream(device)      # default priority
         N = sp.n
         f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
         o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
@@ -493,7 +499,7 @@ class Sweep:
                 view = sp.refl_view(start, m, (B, bx, bx))
                 out = {k: v[start:start + m] for k, v in o1.items()}
                 out["fz"] = self.fz[start:start + m]
-                with (timer("pass1", b, gi) if timer else nullctx):
+                with (timer("pass1", b, gi, main) if timer else nullctx):
                     _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS))
             s0 = groups[0][2]
             n_b = sum(m for *_, m in groups)
@@ -514,12 +520,15 @@ class Sweep:
                 continue
             call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
             call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
-            for gi, (B, bx, _), start, m in groups:
-                view = sp.refl_view(start, m, (B, bx, bx))
-                out = {k: v[start:start + m] for k, v in o2.items()}
-                with (timer("pass2", b, gi) if timer else nullctx):
-                    _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
-                                  o1["nbg"][start:start + m], out)
+            self.side.wait_stream(main)
+            with torch.cuda.stream(self.side):
+                for gi, (B, bx, _), start, m in groups:
+                    view = sp.refl_view(start, m, (B, bx, bx))
+                    out = {k: v[start:start + m] for k, v in o2.items()}
+                    with (timer("pass2", b, gi, self.side) if timer else nullctx):
+                        _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
+                                      o1["nbg"][start:start + m], out)
+        main.wait_stream(self.side)
         torch.cuda.synchronize(device)
         return dict(o1=o1, o2=o2, prof_by_block=prof_by_block, t_gpu=time.time() - t_dev0)
 
