@@ -22,6 +22,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
+FORMS = dict(
+    plain="plain: one stream; per phi block one pass-1 launch per shoebox group (voxels read from the frame store), "
+          "the adaptive fits, normalisation, one pass-2 launch per group",
+    batched="batched: widths-independent work (shoebox gather, Kabsch frame, eps, J, neighbour masks) ahead on a "
+            "second stream, the serial pass-1/fits chain on one stream, pass 2 overlapped on a third; timed as eager "
+            "launches and as replays of the captured HIP graph (batched_forms)")
 
 
 def log(msg):
@@ -108,6 +114,58 @@ def cpu_baseline(sweep, hp, spec, n_sample, seconds):
     return base, parity
 
 
+def summarize_stats(stats_csv, command, out_path):
+    """Markdown summary of a `rocprofv3 --kernel-trace --stats --output-format csv` run (its *_kernel_stats.csv)."""
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = [f"# Kernel time: `{command}`", "", "| kernel | calls | total ms | average us | min us | max us | % |",
+             "|---|---|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"].replace("(anonymous namespace)::", "").split("(")[0]
+        lines.append(f"| {name} | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                     f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | "
+                     f"{float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+    Path(out_path).write_text("\n".join(lines) + "\n")
+
+
+def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
+    """HBM bytes per dispatch of `kernel` from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs (KiB per dispatch), each
+    scaled by what the same counter reads for the copy-like calibration kernel of known size (bench.py
+    --calibrate-copy MIB: MIB MiB read, MIB MiB written).  Writes profiles/traffic_<kernel>_<workload>.json and .md."""
+    import csv
+    import statistics
+
+    def per_kernel(path, counter):
+        vals = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            vals.setdefault(r["Kernel_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
+            vals[r["Kernel_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+        return {k: list(v.values()) for k, v in vals.items()}
+
+    fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    calib = next(k for k in fetch if "neg" in k.lower())
+    nbytes = copy_mib * 2 ** 20
+    f_fac = statistics.mean(fetch[calib]) * 1024 / nbytes
+    w_fac = statistics.mean(write[calib]) * 1024 / nbytes
+    kname = next(k for k in fetch if k.startswith(kernel + "("))
+    f_b = statistics.mean(fetch[kname]) * 1024 / f_fac
+    w_b = statistics.mean(write[kname]) * 1024 / w_fac
+    rec = dict(kernel=kernel, workload=workload, bytes_per_launch=round(f_b + w_b), fetch_bytes=round(f_b),
+               write_bytes=round(w_b), dispatches=len(fetch[kname]), copy_calibration=dict(
+                   mib=copy_mib, fetch_factor=round(f_fac, 4), write_factor=round(w_fac, 4)), command=command)
+    base = ROOT / "profiles" / f"traffic_{kernel[2:] if kernel.startswith('k_') else kernel}_{workload}"
+    base.with_suffix(".json").write_text(json.dumps(rec, indent=1) + "\n")
+    base.with_suffix(".md").write_text(
+        f"# HBM traffic of {kernel} ({workload})\n\nCommand (each counter in its own run): `{command}`\n\n"
+        f"- FETCH_SIZE per dispatch: {f_b / 1e6:.2f} MB (raw / copy factor {f_fac:.4f})\n"
+        f"- WRITE_SIZE per dispatch: {w_b / 1e6:.2f} MB (raw / copy factor {w_fac:.4f})\n"
+        f"- dispatches: {len(fetch[kname])}\n- calibration: one copy-like kernel over {copy_mib} MiB read and "
+        f"written (`--calibrate-copy {copy_mib}`), counters read {f_fac:.4f} x and {w_fac:.4f} x the known bytes\n")
+    return rec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,10 +176,24 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
+                    help="batched (the reported value) and plain (reported beside it), or one of them")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one copy-like kernel (out = -in, float32) over MIB MiB, outside the timed region: "
                          "the reference of known traffic for hardware byte counters")
+    ap.add_argument("--summarize-stats", nargs=3, metavar=("STATS_CSV", "COMMAND", "OUT_MD"),
+                    help="write a markdown summary of a rocprofv3 kernel-stats csv and exit")
+    ap.add_argument("--summarize-pmc", nargs=6, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_MIB", "KERNEL", "WORKLOAD",
+                                                          "COMMAND"),
+                    help="write profiles/traffic_<kernel>_<workload>.json/.md from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(*args.summarize_stats)
+        return
+    if args.summarize_pmc:
+        f, w, mib, kern, wl, cmd = args.summarize_pmc
+        print(json.dumps(summarize_pmc(f, w, int(mib), kern, wl, cmd)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one MI355X: every box here has one GPU, so --gpus must be 1 (multi-GPU sharding is "
               "described in DESIGN.md, not run)", file=sys.stderr)
@@ -163,9 +235,6 @@ def main():
     log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups over {len(edges) - 1} blocks "
         f"({time.time() - t:.1f}s)")
 
-    for i in range(args.warmup):
-        r = sweep.run()
-        log(f"warmup {i + 1}/{args.warmup}: {r['t_gpu'] * 1e3:.1f} ms")
     events = []
 
     class Timer:
@@ -183,24 +252,80 @@ def main():
             events.append((self.kind, self.gi, self.e0, self.e1))
             return False
 
-    torch.cuda.synchronize(dev)
-    t0 = time.perf_counter()
-    last = None
-    for i in range(args.steps):
-        last = sweep.run(timer=Timer)
-    torch.cuda.synchronize(dev)
-    el = time.perf_counter() - t0
-    ms = el / args.steps * 1e3
-    integrated = int(torch.isfinite(last["o2"]["iprf"]).sum().item())
-    value = integrated / (el / args.steps)
-    log(f"timed {args.steps} steps: {ms:.1f} ms/step, {integrated} profile-fitted reflections/step")
+    def time_form(form, steps, warmup, timer):
+        sweep.o["form"] = form
+        for i in range(warmup):
+            r = sweep.run()
+            log(f"{form} warmup {i + 1}/{warmup}: {r['t_gpu'] * 1e3:.1f} ms")
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        last = None
+        for i in range(steps):
+            last = sweep.run(timer=timer)
+        torch.cuda.synchronize(dev)
+        el = time.perf_counter() - t0
+        integrated = int(torch.isfinite(last["o2"]["iprf"]).sum().item())
+        log(f"{form}: timed {steps} steps: {el / steps * 1e3:.1f} ms/step, {integrated} profile-fitted "
+            f"reflections/step")
+        return el / steps, integrated, last
+
+    plain = None
+    graph_info = None
+    if args.form in ("both", "plain"):
+        t_plain, n_plain, last_plain = time_form("plain", args.steps, args.warmup, None)
+        plain = dict(value=round(n_plain / t_plain, 1), ms_per_step=round(t_plain * 1e3, 3))
+    form = "plain" if args.form == "plain" else "batched"
+    if form == "plain":
+        t_step, integrated, last = t_plain, n_plain, last_plain
+        events.clear()
+        time_form("plain", 1, 0, Timer)
+    else:
+        # one eager batched run with HIP events around every launch (per-kernel durations for the roofline); the
+        # batched form timed with eager launches on its three streams and as replays of one captured HIP graph: the
+        # faster of the two is the reported value, the other is reported beside it
+        _, _, eager = time_form("batched", 1, args.warmup, Timer)
+        t_eager, n_eager, last_eager = time_form("batched", args.steps, 0, None)
+        sweep.capture()
+        for i in range(args.warmup):
+            sweep.replay()
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            last_graph = sweep.replay()
+        torch.cuda.synchronize(dev)
+        t_graph = (time.perf_counter() - t0) / args.steps
+        n_graph = int(torch.isfinite(last_graph["o2"]["iprf"]).sum().item())
+        log(f"batched (graph): timed {args.steps} replays: {t_graph * 1e3:.1f} ms/step, {n_graph} profile-fitted "
+            f"reflections/step")
+        graph_info = dict(eager_ms_per_step=round(t_eager * 1e3, 3), graph_ms_per_step=round(t_graph * 1e3, 3))
+        if t_graph < t_eager:
+            t_step, integrated, last = t_graph, n_graph, last_graph
+            graph_info["reported"] = "graph replay"
+        else:
+            t_step, integrated, last = t_eager, n_eager, last_eager
+            graph_info["reported"] = "eager launches"
+        ref = last_plain if plain is not None else eager
+        a, b_ = ref["o2"]["iprf"], last["o2"]["iprf"]
+        same = bool(torch.equal(torch.isfinite(a), torch.isfinite(b_)) and
+                    torch.equal(a[torch.isfinite(a)], b_[torch.isfinite(b_)]))
+        if plain is not None:
+            plain["identical_results"] = same
+        for other in (last_eager, last_graph):
+            if not torch.equal(torch.nan_to_num(other["o2"]["iprf"], nan=-1.0), torch.nan_to_num(b_, nan=-1.0)):
+                graph_info["identical_results"] = False
+        graph_info.setdefault("identical_results", True)
+    ms = t_step * 1e3
+    value = integrated / t_step
 
     # dominant kernel and its roofline (HIP events on the launch stream, algorithmic bytes per launch)
     per = {}
     K_nb = sweep.sp.K
+    n_timed = 1
     for kind, gi, e0, e1 in events:
         b, (B, bx, s), start, m = sweep.sp.groups[gi]
         V = B * bx * bx
+        if kind == "prepare":
+            continue
         d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
         d["ms"] += e0.elapsed_time(e1)
         d["bytes"] += m * per_refl_bytes(V, B, K_nb, kind)
@@ -216,8 +341,8 @@ def main():
         traffic = json.loads(tf.read_text()).get("bytes_per_launch")
     roof = dict(bound="hbm", kernel=f"k_{dom}", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, avg_launch_ms=round(avg_ms, 4),
-                alg_bytes_per_launch=round(avg_bytes), launches_per_step=dd["n"] // args.steps,
-                kernels={k: dict(ms_per_step=round(v["ms"] / args.steps, 3),
+                alg_bytes_per_launch=round(avg_bytes), launches_per_step=dd["n"] // n_timed,
+                kernels={k: dict(ms_per_step=round(v["ms"] / n_timed, 3),
                                  GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)) for k, v in per.items()})
 
     cpu, parity = (None, None)
@@ -229,12 +354,13 @@ def main():
     out = dict(metric="profile-fitted reflections/s on 1 MI355X", value=round(value, 1), unit="reflections/s",
                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 3), higher_is_better=True,
                scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
-               config=dict(workload=spec.name, form="plain: one pass-1 and one pass-2 launch per shoebox group per "
-                                                    "phi block, extraction fused, frames resident in HBM",
+               config=dict(workload=spec.name, form=FORMS[form],
                            frames=spec.nframes, frame_shape=[spec.ny, spec.nx], reflections=N,
                            integrated=integrated, blocks=len(edges) - 1, box=[spec.box_z, spec.box_xy, spec.box_xy]
                            if spec.box_xy else "reference rules", parallelism="single GPU"),
-               roofline=roof, cpu_baseline=cpu, parity=parity)
+               roofline=roof, cpu_baseline=cpu, parity=parity,
+               plain=plain, vs_plain=round(value / plain["value"], 3) if plain and form == "batched" else None,
+               batched_forms=graph_info if form == "batched" else None)
     print(json.dumps(out), flush=True)
 
 

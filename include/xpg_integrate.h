@@ -185,6 +185,31 @@ int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int
                     void *stream);
 int64_t xpg_adapt_workspace_bytes(int32_t n);
 
+/* Widths-independent per-reflection data of a chunk (device pointers), written by xpg_prepare ahead of the adaptive
+ * chain and read by xpg_pass1_prepared / xpg_pass2_prepared. */
+typedef struct xpg_prepared {
+  float *sb;       /* (n, Dz, Dy, Dx) value-coded shoeboxes, as xpg_extract writes them */
+  double *geo;     /* (n, 3, Dy, Dx): eps1, eps2 and the pixel area J at every pixel centre */
+  double *zeta;    /* (n,) zeta of the Kabsch frame */
+  uint8_t *nbm;    /* (n, Dz, Dy, Dx) bit 0: inside a neighbour ellipsoid; bit 1: the same with column 0 skipped */
+  double *geo_p;   /* (n, 3, Dy, Dx): the overlapping partner's eps1, eps2, J on this box (rows with has_p only) */
+  double *zeta_p;  /* (n,) the partner's zeta (rows with has_p only) */
+} xpg_prepared;
+
+/* Shoebox gather (host gather xpg/integrate.py:704-803 with _decode_bins xpg/shoebox.py:264-268), the widths-independent
+ * geometry of geometry_terms (Kabsch frame, eps, J: xpg/integrate_kernels.py:219-272) and the neighbour masks
+ * (_neighbour_mask, :186-213) of a chunk, into prep. */
+int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                const xpg_prepared *prep, void *stream);
+
+/* xpg_pass1 / xpg_pass2 reading the prepared data of xpg_prepare (results identical to xpg_pass1_frames /
+ * xpg_pass2_frames on the same frames). */
+int xpg_pass1_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
+                       const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream);
+int xpg_pass2_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
+                       const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+                       const xpg_pass2_out *out, void *stream);
+
 /* Bytes of LDS one wavefront of pass 1 / pass 2 uses for a (Dz, Dy, Dx) shoebox (launch planning, autotuning). */
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx);
 

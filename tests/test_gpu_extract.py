@@ -71,3 +71,33 @@ def test_fused_extraction_gives_identical_results():
     b2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], frames=store)
     for k in K.P2_F64:
         assert torch.equal(a2[k], b2[k]), k
+
+
+def test_prepared_passes_give_identical_results():
+    """xpg_prepare (gather, Kabsch frame, eps, J, neighbour masks ahead) + the prepared passes == the fused passes,
+    bit for bit, with partners and neighbours."""
+    from chunks import make_chunk
+    from xpg_amd import integrate_kernels as K
+    from xpg_amd.frames import DeviceFrames
+    dev = torch.device("cuda:0")
+    c = make_chunk(n=96, seed=23)
+    assert np.asarray(c["rc"]["has_p"]).any() and np.isfinite(np.asarray(c["nb"]["x"])).any()
+    nfr, ny, nx = 420, 1024, 1024
+    store_np = np.random.default_rng(2).poisson(3.0, (nfr, ny, nx)).astype(np.uint32)
+    frames = DeviceFrames(torch.as_tensor(store_np.view(np.int32), device=dev), first=1)
+    store = frames.store()
+    det = K.device_detector(c["det"], dev)
+    call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"], c["rcw"])
+    refl = K.DeviceRefl(c["rc"], c["nb"], (7, 11, 11), dev)
+    print("prepare n=96 7x11x11", flush=True)
+    ps, bufs = K.run_prepare(det, call, refl, store, dev)
+    assert torch.equal(bufs["sb"], K.run_extract(store, refl, dev))
+    a = K.run_pass1(det, call, refl, prep=ps)
+    b = K.run_pass1(det, call, refl, frames=store)
+    for k in K.P1_F64 + K.P1_I32 + ("fz", "img2", "J", "u1", "u2", "all_valid2"):
+        assert torch.equal(a[k], b[k]), k
+    prof = torch.as_tensor(c["prof"], device=dev)
+    a2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], prep=ps)
+    b2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], frames=store)
+    for k in K.P2_F64:
+        assert torch.equal(a2[k], b2[k]), k

@@ -62,3 +62,23 @@ def test_integrate_matches_oracle_sweep(which):
     # repeated runs are bit-identical (fixed-point profile grids, fixed reduction orders)
     res2 = I.integrate(frames, blocks, edges, model, dmin, verbose=False)
     assert np.array_equal(res2.refl["iprf"], r["iprf"], equal_nan=True)
+    # the plain form (one stream, voxels read from the frame store) gives the same results as the batched default
+    hp2 = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False)
+    sw = I.Sweep(frames, blocks, model, hp2, dict(frames=(1, spec.nframes), mask=None, exclude_frames=(),
+                                                  learn_profiles=True, adaptive=True, bkg_template=None,
+                                                  moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
+                                                  sm=np.radians(model.sigma_m), form="plain"))
+    res3 = sw.finish(sw.run())
+    for k in ("iprf", "vprf", "isum", "bg"):
+        assert np.array_equal(res3.refl[k], r[k], equal_nan=True), k
+    # the batched form captured in a HIP graph: every replay equals the eager result, with eager runs and
+    # synchronisations between the replays
+    sw.o["form"] = "batched"
+    sw.capture()
+    for rep in range(3):
+        rr = sw.finish(sw.replay())
+        for k in ("iprf", "vprf", "isum", "bg", "corr"):
+            assert np.array_equal(rr.refl[k], r[k], equal_nan=True), (rep, k)
+        assert rr.block_params == res.block_params, rep
+        sw.run()
+        torch.cuda.synchronize()

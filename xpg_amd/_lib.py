@@ -75,6 +75,11 @@ class Frames(C.Structure):
                 ("dtype", C.c_int32)]
 
 
+class Prepared(C.Structure):
+    _fields_ = [("sb", C.c_void_p), ("geo", C.c_void_p), ("zeta", C.c_void_p), ("nbm", C.c_void_p),
+                ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p)]
+
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -90,6 +95,11 @@ _SIGS = {
                    C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_pass2_frames": ([P(Detector), P(Call), P(Refl), P(Frames), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                           C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
+    "xpg_prepare": ([P(Detector), P(Call), P(Refl), P(Frames), P(Prepared), C.c_void_p], C.c_int),
+    "xpg_pass1_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), P(Pass1Out), P(ProfileAcc), C.c_void_p],
+                           C.c_int),
+    "xpg_pass2_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                            C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
     "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 12 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
                                                                                          C.c_void_p], C.c_int),

@@ -18,6 +18,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "xpg_erf.h"
 #include "xpg_integrate.h"
 
 namespace {
@@ -134,7 +135,7 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
 
 // erf(x), with |x| > ERF_SAT taken as saturated (float64 erf is +-1 to the last bit there)
 constexpr double ERF_SAT = 6.5;
-__device__ __forceinline__ double erf_sat(double x) { return fabs(x) > ERF_SAT ? copysign(1.0, x) : erf(x); }
+__device__ __forceinline__ double erf_sat(double x) { return fabs(x) > ERF_SAT ? copysign(1.0, x) : xpg::erf_bf(x); }
 
 __global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
   __shared__ double red[256];
@@ -151,17 +152,18 @@ __global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
     if (!(tot > 0)) continue;
     const double phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
     const double s = smv / zeta * sqrt(2.0);
+    const double inv_s = 1.0 / s, inv_tot = 1.0 / tot;
     const float *fz = a.fz + (int64_t)r * NF;
     const int iz0 = a.iz0[r];
     // frame k spans [lo, hi] = sorted(phi0 + osc (iz0 + k), phi0 + osc (iz0 + k + 1)); the row total of R telescopes
-    const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) / s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) / s);
-    const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9);
+    const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) * inv_s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) * inv_s);
+    const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9), inv_den = 1.0 / den;
     double cst = 0.0, Eprev = e0;
     for (int k = 0; k < NF; ++k) {
-      const double E = k == NF - 1 ? e1 : erf_sat((bound_deg(a, iz0, k + 1) - phi) / s);
+      const double E = k == NF - 1 ? e1 : erf_sat((bound_deg(a, iz0, k + 1) - phi) * inv_s);
       const double R = a.osc >= 0 ? 0.5 * (E - Eprev) : 0.5 * (Eprev - E);
-      const double d = (double)fz[k] / tot - R / den;
-      cst += tot * (d * d);
+      const double d = (double)fz[k] * inv_tot - R * inv_den;
+      cst = fma(tot, d * d, cst);
       Eprev = E;
     }
     part += cst;

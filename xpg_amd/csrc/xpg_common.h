@@ -7,6 +7,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "xpg_erf.h"
 #include "xpg_integrate.h"
 
 namespace xpg {
@@ -242,13 +243,13 @@ __device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double h
   if (!use_rcw_of(c)) {
     double a = (hi - phi) / (s * sqrt(2.0));
     double b = (lo - phi) / (s * sqrt(2.0));
-    return 0.5 * (erf(a) - erf(b));
+    return 0.5 * (erf_bf(a) - erf_bf(b));
   }
   double sw = (s * 0.25) * sqrt(2.0);
   double cj = (double)RC_C32[j];
   double a = ((hi - phi) - cj * s) / sw;
   double b = ((lo - phi) - cj * s) / sw;
-  return ((double)rcw_of(c, j) * 0.5) * (erf(a) - erf(b));
+  return ((double)rcw_of(c, j) * 0.5) * (erf_bf(a) - erf_bf(b));
 }
 
 __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb, int j, double &lo, double &hi) {
@@ -289,10 +290,10 @@ __device__ __forceinline__ double pix_x(int ix0, int i, int s) {
   return ((double)ix0 + (double)i) * (double)s + ((double)s - 1.0) * 0.5 + PIX0;
 }
 
-// Writes u1 = eps1/seff, u2 = eps2/seff (float64) into U1, U2 (P each) and J into Jd (float64, P); returns seff.
-// Small boxes (a side < 3): J by central differences at the pixel edges.
-__device__ double box_geometry(const xpg_detector &d, const Frame &f, int ix0, int iy0, int s, int Dy, int Dx,
-                               double sd, double *U1, double *U2, double *Jd, int lane) {
+// The widths-independent half: eps1, eps2 (float64) of every pixel centre into U1, U2 (P each) and the pixel area J
+// into Jd (float64, P).  Small boxes (a side < 3): J by central differences at the pixel edges.
+__device__ void box_eps_J(const xpg_detector &d, const Frame &f, int ix0, int iy0, int s, int Dy, int Dx, double *U1,
+                          double *U2, double *Jd, int lane) {
   const int P = Dy * Dx;
   for (int p = lane; p < P; p += WAVE) {
     int iy = p / Dx, ix = p - iy * Dx;
@@ -343,6 +344,11 @@ __device__ double box_geometry(const xpg_detector &d, const Frame &f, int ix0, i
     }
   }
   wave_sync();
+}
+
+// The sigma_D-dependent half: seff from the area of the centre pixel, then u = eps / seff in place; returns seff.
+__device__ double box_u(int Dy, int Dx, double sd, double *U1, double *U2, const double *Jd, int lane) {
+  const int P = Dy * Dx;
   double psz = sqrt(Jd[(Dy / 2) * Dx + Dx / 2]);
   double t = psz / sd;
   double seff = sd * sqrt(1.0 + (t * t) / 12.0);
@@ -353,6 +359,14 @@ __device__ double box_geometry(const xpg_detector &d, const Frame &f, int ix0, i
   wave_sync();
   return seff;
 }
+
+// u1 = eps1/seff, u2 = eps2/seff into U1, U2 and J into Jd; returns seff
+__device__ __forceinline__ double box_geometry(const xpg_detector &d, const Frame &f, int ix0, int iy0, int s, int Dy,
+                                               int Dx, double sd, double *U1, double *U2, double *Jd, int lane) {
+  box_eps_J(d, f, ix0, iy0, s, Dy, Dx, U1, U2, Jd, lane);
+  return box_u(Dy, Dx, sd, U1, U2, Jd, lane);
+}
+
 
 // ---- bilinear profile grid coordinates (xpg/integrate_kernels.py:154-166) ---------------------------------------
 

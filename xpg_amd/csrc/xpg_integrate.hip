@@ -167,19 +167,6 @@ __device__ float voxel_value(const xpg_frames &F, int iz0, int fb, int j, int sy
   return sum;
 }
 
-__global__ void k_extract(xpg_frames F, xpg_refl R, float *sb) {
-  const int P = R.Dy * R.Dx, V = R.Dz * P;
-  const int64_t total = (int64_t)R.n * V;
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-    int r = (int)(g / V);
-    int v = (int)(g - (int64_t)r * V);
-    int j = v / P, p = v - j * P;
-    int iy = p / R.Dx, ix = p - iy * R.Dx;
-    int s = R.pxs ? R.pxs[r] : 1;
-    sb[g] = voxel_value(F, R.iz0[r], R.fb[r], j, R.iy0[r] + iy, R.ix0[r] + ix, s);
-  }
-}
-
 __global__ void k_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int B, int bxbx) {
   const int64_t total = n_rows * B * (int64_t)bxbx;
   for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
@@ -250,7 +237,15 @@ __device__ __forceinline__ void stage_values(const Box &b, const float *sb, cons
   wave_sync();
   if (sb) {
     const float *src = sb + (int64_t)b.r * b.V;
-    for (int v = lane; v < b.V; v += WAVE) vals[v] = src[v];
+    constexpr int NB = 16;
+    for (int v0 = lane; v0 < b.V; v0 += NB * WAVE) {
+      float t[NB];
+#pragma unroll
+      for (int u = 0; u < NB; ++u) t[u] = v0 + u * WAVE < b.V ? src[v0 + u * WAVE] : 0.f;
+#pragma unroll
+      for (int u = 0; u < NB; ++u)
+        if (v0 + u * WAVE < b.V) vals[v0 + u * WAVE] = t[u];
+    }
   } else if (b.fb == 1 && b.s == 1) {
     int32_t *poff = (int32_t *)tmp;                 // P: y nx + x, or -1 off the detector
     uint8_t *pm = tmp + 4 * (size_t)b.P;            // P: mask bits
@@ -273,7 +268,7 @@ __device__ __forceinline__ void stage_values(const Box &b, const float *sb, cons
     }
     wave_sync();
     const uint32_t *src = (const uint32_t *)F.data;
-    constexpr int NB = 8;
+    constexpr int NB = 16;
     VoxIter it(lane, b.P, b.V);
     while (it.ok()) {
       uint32_t raw[NB];
@@ -313,6 +308,68 @@ __device__ __forceinline__ void stage_values(const Box &b, const float *sb, cons
   wave_sync();
 }
 
+// shoeboxes of a chunk from the frame store: one wavefront per reflection stages its box in LDS (the same reads as
+// the passes' fused path) and writes it out contiguously
+__host__ __device__ static inline size_t extract_wave_bytes(int Dz, int Dy, int Dx) {
+  const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P;
+  return al8(al8(4 * V) + al8(V) + al8(P) + 8 * P + 4 * (size_t)Dz);
+}
+
+__global__ void __launch_bounds__(256) k_extract(xpg_frames F, xpg_refl R, float *sb, int wave_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wid = threadIdx.x / WAVE;
+  const int r = blockIdx.x * (blockDim.x / WAVE) + wid;
+  if (r >= R.n) return;
+  const Box b = box_of(R, r);
+  const size_t P = (size_t)b.P, V = (size_t)b.V;
+  unsigned char *base = smem + (size_t)wid * wave_bytes;
+  float *vals = (float *)base;
+  uint8_t *vflag = base + al8(4 * V);
+  uint8_t *prow = vflag + al8(V);
+  unsigned char *tmp = prow + al8(P);
+  unsigned char *ftmp = tmp + 8 * P;
+  stage_values(b, nullptr, F, vals, vflag, prow, tmp, ftmp, lane);
+  float *dst = sb + (int64_t)r * b.V;
+  for (int v = lane; v < b.V; v += WAVE) dst[v] = vals[v];
+}
+
+// prepared data of reflection r: shoebox values, neighbour-mask bit `bit` as VF_NBM, eps1 / eps2 / J into U1, U2, Jd;
+// returns zeta
+__device__ __forceinline__ double stage_prepared(const Box &b, const xpg_prepared &pp, int bit, float *vals,
+                                                 uint8_t *vflag, uint8_t *prow, double *U1, double *U2, double *Jd,
+                                                 int lane) {
+  for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
+  const float *src = pp.sb + (int64_t)b.r * b.V;
+  const uint8_t *nsrc = pp.nbm + (int64_t)b.r * b.V;
+  constexpr int NB = 16;
+  for (int v0 = lane; v0 < b.V; v0 += NB * WAVE) {
+    float t[NB];
+    uint8_t m[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const bool in = v0 + u * WAVE < b.V;
+      t[u] = in ? src[v0 + u * WAVE] : 0.f;
+      m[u] = in ? nsrc[v0 + u * WAVE] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+      if (v0 + u * WAVE < b.V) {
+        vals[v0 + u * WAVE] = t[u];
+        vflag[v0 + u * WAVE] = ((m[u] >> bit) & 1) ? 1 : 0;     // VF_NBM
+      }
+  }
+  const double *g = pp.geo + (int64_t)b.r * 3 * b.P;
+  for (int p = lane; p < b.P; p += WAVE) {
+    U1[p] = g[p];
+    U2[p] = g[b.P + p];
+    Jd[p] = g[2 * b.P + p];
+  }
+  const double zeta = pp.zeta[b.r];
+  wave_sync();
+  return zeta;
+}
+
 // box coordinates: co[0..Dx) pixel-centre x, co[Dx..Dx+Dy) y, then the Dz frame centres iz0 + fb (j + 1/2)
 __device__ __forceinline__ void box_coords(const Box &b, double *co, int lane) {
   for (int i = lane; i < b.Dx + b.Dy + b.Dz; i += WAVE) {
@@ -343,6 +400,82 @@ __device__ __forceinline__ double frames_fg(const double *Rd, int Dz, uint8_t *f
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
+// prepare: the widths-independent part of both passes, ahead of the adaptive chain
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct PrepLayout {
+  size_t vals, vflag, prow, tmp, ftmp, U1, U2, Jd, co, tab, bytes;
+};
+__host__ __device__ static inline PrepLayout prep_layout(int Dz, int Dy, int Dx) {
+  const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
+  PrepLayout L;
+  size_t o = 0;
+  L.vals = o; o = al8(o + 4 * V);
+  L.vflag = o; o = al8(o + V);
+  L.prow = o; o = al8(o + P);
+  L.tmp = o; o += 8 * P;
+  L.ftmp = o; o = al8(o + 4 * (size_t)Dz);
+  L.U1 = o; o += 8 * P;
+  L.U2 = o; o += 8 * P;
+  L.Jd = o; o += 8 * P;
+  L.co = o; o += 8 * C;
+  L.tab = o; o += 8 * C;
+  L.bytes = al8(o);
+  return L;
+}
+
+__global__ void __launch_bounds__(256) k_prepare(xpg_detector det, xpg_call call, xpg_refl R, xpg_frames F,
+                                                 xpg_prepared pp, int wave_bytes) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wid = threadIdx.x / WAVE;
+  const int r = blockIdx.x * (blockDim.x / WAVE) + wid;
+  if (r >= R.n) return;
+  const Box b = box_of(R, r);
+  const PrepLayout L = prep_layout(b.Dz, b.Dy, b.Dx);
+  unsigned char *base = smem + (size_t)wid * wave_bytes;
+  float *vals = (float *)(base + L.vals);
+  uint8_t *vflag = base + L.vflag, *prow = base + L.prow;
+  double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
+  double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
+  const int P = b.P, V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
+  stage_values(b, nullptr, F, vals, vflag, prow, base + L.tmp, base + L.ftmp, lane);
+  Frame f;
+  kabsch(R.S + 3 * (int64_t)r, call.s0, call.axis, f);
+  box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+  box_coords(b, co, lane);
+  wave_sync();
+  neighbour_mask(R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, 1, tab, prow, lane);
+  neighbour_mask(R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 1, vflag, 2, tab, prow, lane);
+  float *sdst = pp.sb + (int64_t)r * V;
+  uint8_t *ndst = pp.nbm + (int64_t)r * V;
+  for (int v = lane; v < V; v += WAVE) {
+    sdst[v] = vals[v];
+    ndst[v] = vflag[v];
+  }
+  double *g = pp.geo + (int64_t)r * 3 * P;
+  for (int p = lane; p < P; p += WAVE) {
+    g[p] = U1[p];
+    g[P + p] = U2[p];
+    g[2 * P + p] = Jd[p];
+  }
+  if (lane == 0) pp.zeta[r] = f.zeta;
+  if (R.has_p && (double)R.has_p[r] > 0.5) {
+    // the overlapping partner's frame and eps, J on this box's pixels (pass 2 models its profile here)
+    Frame fp;
+    kabsch(R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
+    box_eps_J(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+    double *gp = pp.geo_p + (int64_t)r * 3 * P;
+    for (int p = lane; p < P; p += WAVE) {
+      gp[p] = U1[p];
+      gp[P + p] = U2[p];
+      gp[2 * P + p] = Jd[p];
+    }
+    if (lane == 0) pp.zeta_p[r] = fp.zeta;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
 // pass 1 (xpg/integrate_kernels.py:287-365)
 // ---------------------------------------------------------------------------------------------------------------------
 
@@ -352,6 +485,8 @@ struct P1Args {
   xpg_refl R;
   xpg_frames F;
   const float *sb;
+  xpg_prepared prep;
+  int use_prep;
   xpg_pass1_out out;
   xpg_profile_acc acc;
   int use_acc;
@@ -380,6 +515,9 @@ __device__ __forceinline__ void splat4(const xpg_profile_acc &acc, int region, d
   splat(acc.den, i00 + G + 1, w11);
 }
 
+// PREP: the prepared-data form (xpg_pass1_prepared), compiled separately so that it carries none of the fused
+// form's register pressure
+template <bool PREP>
 __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -405,11 +543,18 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
 
-  stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
-  Frame f;
-  kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
-  rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
-  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+  double zeta;
+  if constexpr (PREP) {
+    zeta = stage_prepared(b, a.prep, 0, vals, vflag, prow, U1, U2, Jd, lane);
+  } else {
+    stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
+    Frame f;
+    kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
+    zeta = f.zeta;
+    box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+  }
+  rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
+  const double seff = box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   wave_sync();
   for (int p = lane; p < P; p += WAVE) {
@@ -422,21 +567,25 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   for (int j = lane; j < Dz; j += WAVE) {
     double lo, hi;
     frame_bounds(call, b.iz0, b.fb, j, lo, hi);
-    e3d[j] = f.zeta * (0.5 * (lo + hi) - b.phi);
+    e3d[j] = zeta * (0.5 * (lo + hi) - b.phi);
     zzd[j] = co[Dx + Dy + j];
   }
   wave_sync();
-  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, prow, lane);
+  if constexpr (!PREP) neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, VF_NBM, tab, prow, lane);
 
   // Lanes run over pixels (p = lane, lane + 64, ...) and each lane over the frames of its pixel: per-pixel terms stay
   // in registers, per-frame terms are broadcasts.  Sums are float64; sums of counts and of template values are exact.
 
-  // per-voxel classes: fg = fg2 & fgz; fgv = fg & valid & ~nbm; background candidates (xpg/integrate_kernels.py:291-305)
+  // per-voxel classes: fg = fg2 & fgz; fgv = fg & valid & ~nbm; background candidates (xpg/integrate_kernels.py:291-305),
+  // with the unclipped background sums of the first round
+  double sc0 = 0.0, sT0 = 0.0;
   for (int p0 = 0; p0 < P; p0 += WAVE) {
     const int p = p0 + lane;
     if (p < P) {
       const uint8_t pfl = pflag[p];
       const bool fg2 = pfl & PF_FG2, bgr = pfl & PF_BGR;
+      double cs = 0.0;
+      int cnt = 0;
       for (int j = 0; j < Dz; ++j) {
         const int v = j * P + p;
         const float c = vals[v];
@@ -445,21 +594,37 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         const bool fg = fg2 && fflag[j];
         if (fg) fl |= VF_FG;
         if (fg && valid && !nbm) fl |= VF_FGV;
-        if (!fg && valid && !nbm && bgr) fl |= VF_BGM;
+        if (!fg && valid && !nbm && bgr) {
+          fl |= VF_BGM;
+          cs += (double)c;
+          ++cnt;
+        }
         vflag[v] = fl;
       }
+      sc0 += cs;
+      sT0 += (double)cnt * (double)bref[p];
     }
   }
   wave_sync();
 
   // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
-  double bk = 0.0, nbg = 0.0;
-  for (int round = 0; round < 5; ++round) {
+  // A round that leaves b unchanged (bit for bit) sets the same thresholds again, so every later round would repeat
+  // it: the loop stops there.
+  double Sc = wsum_d(sc0);
+  double nbg = wsum_d(sT0);
+  double bk = Sc / fmax(nbg, 1e-6);
+  for (int round = 1; round < 5; ++round) {
+    for (int p = lane; p < P; p += WAVE) {
+      const double bp = bk * (double)bref[p];
+      const double t = (bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0;
+      thr[p] = __double2float_rd(t);     // c <= t  <=>  c <= rd(t) for float32 c
+    }
+    wave_sync();
     double sc = 0.0, sT = 0.0;
     for (int p0 = 0; p0 < P; p0 += WAVE) {
       const int p = p0 + lane;
       if (p < P) {
-        const float th = round == 0 ? INFINITY : thr[p];
+        const float th = thr[p];
         double cs = 0.0;
         int cnt = 0;
         for (int j = 0; j < Dz; ++j) {
@@ -476,17 +641,11 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         sT += (double)cnt * (double)bref[p];
       }
     }
-    const double Sc = wsum_d(sc);
+    Sc = wsum_d(sc);
     nbg = wsum_d(sT);
+    const double bk_prev = bk;
     bk = Sc / fmax(nbg, 1e-6);
-    if (round < 4) {
-      for (int p = lane; p < P; p += WAVE) {
-        const double bp = bk * (double)bref[p];
-        const double t = (bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0;
-        thr[p] = __double2float_rd(t);     // c <= t  <=>  c <= rd(t) for float32 c
-      }
-      wave_sync();
-    }
+    if (bk == bk_prev) break;           // the next threshold equals this one: uniform over the wavefront
   }
 
   // ---- sums over the foreground ----
@@ -606,7 +765,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const bool strong = a.use_acc && (isum > 0.0) && (isum / sqrt(fmax(vsum, 1e-6)) > 10.0) && (n_over == 0) &&
                       (frac_rec > 0.95) && (n_fgnb == 0) && (n_fgm == 0);
   const double seff2 = seff * seff;
-  for (int p = lane; p < P; p += WAVE) {
+  const bool pix_out = a.out.img2 || a.out.all_valid2 || a.out.J || a.out.u1 || a.out.u2;
+  for (int p = lane; p < P && (strong || pix_out); p += WAVE) {
     double img = 0.0;
     bool okv = true;
     const double bpx = bk * (double)bref[p];
@@ -642,7 +802,7 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     o.m_m[r] = m_m;
     o.frac_rec[r] = frac_rec;
     o.nfg[r] = (double)n_fg;
-    o.zeta[r] = f.zeta;
+    o.zeta[r] = zeta;
     o.seff[r] = seff;
     o.n_over[r] = n_over;
     o.n_fg_masked[r] = n_fgm;
@@ -710,6 +870,8 @@ struct P2Args {
   xpg_refl R;
   xpg_frames F;
   const float *sb;
+  xpg_prepared prep;
+  int use_prep;
   const float *prof;
   int n_regions;
   const double *bg, *nbg;
@@ -733,6 +895,7 @@ __device__ __forceinline__ void profile_pixels(const float *prof, int region, co
   }
 }
 
+template <bool PREP>
 __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
@@ -765,11 +928,18 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double bgr = a.bg[r];
   const double gain = call.gain;
 
-  stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
-  Frame f;
-  kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
-  rocking_curve(call, b.iz0, b.fb, b.phi, f.zeta, Dz, scr, Rd, lane);
-  const double seff = box_geometry(det, f, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+  double zeta;
+  if constexpr (PREP) {
+    zeta = stage_prepared(b, a.prep, partner ? 1 : 0, vals, vflag, prow, U1, U2, Jd, lane);
+  } else {
+    stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
+    Frame f;
+    kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
+    zeta = f.zeta;
+    box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+  }
+  rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
+  const double seff = box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   profile_pixels(a.prof, b.region, U1, U2, Jd, seff, P, p2, lane);
   wave_sync();
@@ -786,18 +956,32 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     Rpd[j] = 0.0;
   }
   if (partner) {
-    // the overlapping partner's profile on this box's grid, fresh geometry (xpg/integrate_kernels.py:393-400)
-    Frame fp;
-    kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
+    // the overlapping partner's profile on this box's grid, fresh geometry (xpg/integrate_kernels.py:393-400); U1,
+    // U2, Jd of this reflection are no longer needed (pflag is set): they take the partner's
+    double zeta_p;
     wave_sync();
-    rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], fp.zeta, Dz, scr, Rpd, lane);
-    // U1, U2, Jd of this reflection are no longer needed (pflag is set): they take the partner's
-    const double seffp = box_geometry(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+    if constexpr (PREP) {
+      const double *gp = a.prep.geo_p + (int64_t)r * 3 * P;
+      for (int p = lane; p < P; p += WAVE) {
+        U1[p] = gp[p];
+        U2[p] = gp[P + p];
+        Jd[p] = gp[2 * P + p];
+      }
+      zeta_p = a.prep.zeta_p[r];
+    } else {
+      Frame fp;
+      kabsch(a.R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
+      zeta_p = fp.zeta;
+      box_eps_J(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+    }
+    rocking_curve(call, b.iz0, b.fb, a.R.pphi[r], zeta_p, Dz, scr, Rpd, lane);
+    const double seffp = box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
     profile_pixels(a.prof, b.region, U1, U2, Jd, seffp, P, pp2, lane);
     for (int j = lane; j < Dz; j += WAVE) Rpf[j] = (float)Rpd[j];
   }
   wave_sync();
-  neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, prow, lane);
+  if constexpr (!PREP)
+    neighbour_mask(a.R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, partner ? 1 : 0, vflag, VF_NBM, tab, prow, lane);
 
   // Lanes run over pixels, each lane over the frames of its pixel (per-pixel terms in registers, per-frame terms
   // broadcast); sums in float64.
@@ -869,7 +1053,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
           const float pf = p2f * Rf[j];
           const float ppf = (pp2f * Rpf[j]) * hpf;
           const float mu = (b0 + Ii * pf) + Ipi * ppf;
-          const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
+          const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
           s11 += (double)((pf * pf) * iv);
           sb1 += (double)((pf * net) * iv);
           if (partner) {
@@ -905,7 +1089,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]), br = bref[p];
       const float p2f = (float)p2[p], pp2f = (float)pp2[p];
-      const float iv0 = 1.0f / (gainf * fmaxf(b0, (float)VAR_FLOOR));
+      const float iv0 = __builtin_amdgcn_rcpf(gainf * fmaxf(b0, (float)VAR_FLOOR));
       for (int j = 0; j < Dz; ++j) {
         const int v = j * P + p;
         if (!(vflag[v] & VF_USE)) continue;
@@ -913,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         const float pf = p2f * Rf[j];
         const float ppf = (pp2f * Rpf[j]) * hpf;
         const float mu = (b0 + Ii * pf) + Ipi * ppf;
-        const float iv = 1.0f / (gainf * fmaxf(mu, (float)VAR_FLOOR));
+        const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
         const float pb = pf * br;
         sC += (double)(pb * iv);
         s110 += (double)((pf * pf) * iv0);
@@ -1045,9 +1229,16 @@ int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void 
   if (frames->nx <= 0 || frames->ny <= 0 || frames->frame_stride < (int64_t)frames->nx * frames->ny ||
       (frames->dtype != 0 && frames->dtype != 1))
     return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
-  const int64_t total = (int64_t)refl->n * refl->Dz * refl->Dy * refl->Dx;
-  if (total == 0) return XPG_OK;
-  hipLaunchKernelGGL(k_extract, dim3(grid_for(total)), dim3(256), 0, (hipStream_t)stream, *frames, *refl, sb);
+  if (refl->n == 0) return XPG_OK;
+  const size_t wb = extract_wave_bytes(refl->Dz, refl->Dy, refl->Dx);
+  const int lim = lds_limit();
+  if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
+  int w = 4;
+  while (w > 1 && (long long)w * wb > lim) w >>= 1;
+  rc = set_lds(k_extract, (size_t)w * wb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_extract, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *frames,
+                     *refl, sb, (int)wb);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1061,8 +1252,46 @@ int xpg_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int32_t B, i
   return XPG_OK;
 }
 
+static int check_frames(const xpg_frames *frames) {
+  if (!frames || !frames->data) return fail(XPG_ERR_ARG, "frames is NULL");
+  if (frames->nx <= 0 || frames->ny <= 0 || frames->frame_stride < (int64_t)frames->nx * frames->ny ||
+      (frames->dtype != 0 && frames->dtype != 1))
+    return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
+  return XPG_OK;
+}
+
+static int check_prep(const xpg_prepared *prep) {
+  if (!prep || !prep->sb || !prep->geo || !prep->zeta || !prep->nbm || !prep->geo_p || !prep->zeta_p)
+    return fail(XPG_ERR_ARG, "incomplete xpg_prepared");
+  return XPG_OK;
+}
+
+int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                const xpg_prepared *prep, void *stream) {
+  int rc = check_refl(refl, false);
+  if (rc) return rc;
+  if (!det || !call) return fail(XPG_ERR_ARG, "det / call is NULL");
+  rc = check_frames(frames);
+  if (rc) return rc;
+  rc = check_prep(prep);
+  if (rc) return rc;
+  if (refl->n == 0) return XPG_OK;
+  const size_t wb = prep_layout(refl->Dz, refl->Dy, refl->Dx).bytes;
+  const int lim = lds_limit();
+  if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
+  int w = 4;
+  while (w > 1 && (long long)w * wb > lim) w >>= 1;
+  rc = set_lds(k_prepare, (size_t)w * wb);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_prepare, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *det,
+                     *call, *refl, *frames, *prep, (int)wb);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
+}
+
 static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
-                      const xpg_frames *frames, const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
+                      const xpg_frames *frames, const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream,
+                      const xpg_prepared *prep = nullptr) {
   int rc = check_refl(refl, false);
   if (rc) return rc;
   if (!det || !call || !out) return fail(XPG_ERR_ARG, "det / call / out is NULL");
@@ -1083,6 +1312,9 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   a.R = *refl;
   a.F = frames ? *frames : kNoFrames;
   a.sb = sb;
+  a.use_prep = prep != nullptr;
+  if (prep) a.prep = *prep;
+  else memset(&a.prep, 0, sizeof(a.prep));
   a.out = *out;
   a.use_acc = acc != nullptr;
   if (acc) a.acc = *acc;
@@ -1090,10 +1322,11 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   size_t lds;
   rc = launch_shape(refl, 1, &a.waves, &a.wave_bytes, &lds);
   if (rc) return rc;
-  rc = set_lds(k_pass1, lds);
+  rc = prep ? set_lds(k_pass1<true>, lds) : set_lds(k_pass1<false>, lds);
   if (rc) return rc;
   const int blocks = (refl->n + a.waves - 1) / a.waves;
-  hipLaunchKernelGGL(k_pass1, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  if (prep) hipLaunchKernelGGL(k_pass1<true>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  else hipLaunchKernelGGL(k_pass1<false>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1135,7 +1368,8 @@ int xpg_normalise_profiles(const xpg_profile_acc *acc, const float *gauss, float
 
 static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
                       const xpg_frames *frames, const float *prof, int32_t n_regions, const double *bg,
-                      const double *nbg, int32_t irls_rule, const xpg_pass2_out *out, void *stream) {
+                      const double *nbg, int32_t irls_rule, const xpg_pass2_out *out, void *stream,
+                      const xpg_prepared *prep = nullptr) {
   int rc = check_refl(refl, true);
   if (rc) return rc;
   if (!det || !call || !out || !prof || !bg || !nbg) return fail(XPG_ERR_ARG, "NULL argument");
@@ -1154,6 +1388,9 @@ static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   a.R = *refl;
   a.F = frames ? *frames : kNoFrames;
   a.sb = sb;
+  a.use_prep = prep != nullptr;
+  if (prep) a.prep = *prep;
+  else memset(&a.prep, 0, sizeof(a.prep));
   a.prof = prof;
   a.n_regions = n_regions;
   a.bg = bg;
@@ -1162,10 +1399,11 @@ static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   size_t lds;
   rc = launch_shape(refl, 2, &a.waves, &a.wave_bytes, &lds);
   if (rc) return rc;
-  rc = set_lds(k_pass2, lds);
+  rc = prep ? set_lds(k_pass2<true>, lds) : set_lds(k_pass2<false>, lds);
   if (rc) return rc;
   const int blocks = (refl->n + a.waves - 1) / a.waves;
-  hipLaunchKernelGGL(k_pass2, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  if (prep) hipLaunchKernelGGL(k_pass2<true>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  else hipLaunchKernelGGL(k_pass2<false>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1182,6 +1420,21 @@ int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_re
                      const xpg_pass2_out *out, void *stream) {
   if (!frames || !frames->data) return fail(XPG_ERR_ARG, "frames is NULL");
   return pass2_impl(det, call, refl, nullptr, frames, prof, n_regions, bg, nbg, irls_rule, out, stream);
+}
+
+int xpg_pass1_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
+                       const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
+  const int rc = check_prep(prep);
+  if (rc) return rc;
+  return pass1_impl(det, call, refl, prep->sb, nullptr, out, acc, stream, prep);
+}
+
+int xpg_pass2_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
+                       const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
+                       const xpg_pass2_out *out, void *stream) {
+  const int rc = check_prep(prep);
+  if (rc) return rc;
+  return pass2_impl(det, call, refl, prep->sb, nullptr, prof, n_regions, bg, nbg, irls_rule, out, stream, prep);
 }
 
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) {

@@ -391,9 +391,17 @@ class Sweep:
 
     The adaptive chain stays on the device: pass 1 of a block reads its widths record (sigma_D, sigma_M, rocking-curve
     weights) from device memory, xpg_adapt_block writes the records for the block's pass 2 and for the next block, so
-    a whole sweep is a sequence of launches with no host round trip.  The chain pass 1 -> fits -> normalisation of
-    consecutive blocks is serial and runs on the caller's stream; each block's pass 2 depends only on its own chain
-    and runs on a second stream, overlapping the next blocks' chains."""
+    a whole sweep is a sequence of launches with no host round trip.
+
+    Two execution forms (``opts["form"]``):
+      plain    one stream; per block: pass 1 reading its voxels from the frame store, fits, normalisation, pass 2
+               (also from the frame store)
+      batched  the widths-independent work of every block (shoebox gather, Kabsch frame, eps and J, neighbour masks:
+               xpg_prepare) runs ahead on a third stream into contiguous buffers; the serial chain pass 1 -> fits ->
+               normalisation of consecutive blocks runs on the caller's stream reading them;
+               each block's pass 2 depends only on its own chain and runs on a second stream, overlapping the next
+               blocks' chains.
+    Both give identical results (the same kernels on the same values)."""
 
     def __init__(self, images, exp_blocks, model, hp, opts):
         from . import _lib as L
@@ -456,34 +464,99 @@ class Sweep:
         return self.widths.data_ptr() + slot * self.wsz
 
     def _init_widths(self):
+        """Every widths slot to the run's starting widths (slot 2i: pass 1 of the i-th block, 2i + 1: its pass 2),
+        from a device copy made once (device-to-device: capturable in a graph)."""
         from . import _lib as L
-        w = L.Widths()
-        w.sd, w.sm = float(self.o["sd"]), float(self.o["sm"])
-        w.rcw[:] = [float(v) for v in rc_default_weights()]
-        raw = torch.frombuffer(bytearray(bytes(w)), dtype=torch.uint8)
-        nb = len(self.order)
-        # slot 2i: pass 1 of the i-th block, slot 2i + 1: its pass 2 (every slot starts at the run's widths)
-        self.widths.copy_(raw.repeat(2 * nb + 2).to(self.device))
+        if not hasattr(self, "widths0"):
+            w = L.Widths()
+            w.sd, w.sm = float(self.o["sd"]), float(self.o["sm"])
+            w.rcw[:] = [float(v) for v in rc_default_weights()]
+            raw = torch.frombuffer(bytearray(bytes(w)), dtype=torch.uint8)
+            self.widths0 = raw.repeat(2 * len(self.order) + 2).to(self.device)
+            self.gauss_t = torch.as_tensor(gaussian_profile(), device=self.device)
+            self.gauss_rep = self.gauss_t[None].repeat(self.o["nreg"], 1, 1).contiguous()
+        self.widths.copy_(self.widths0)
 
     def run(self, timer=None):
         """Device execution of the whole sweep; ``timer(kind, b, gi, stream)`` may return a context that brackets
         launches on ``stream``."""
+        t_dev0 = time.time()
+        out = self.enqueue(timer)
+        torch.cuda.synchronize(self.device)
+        out["t_gpu"] = time.time() - t_dev0
+        return out
+
+    def capture(self):
+        """Capture the batched form of the whole sweep in a HIP graph (after one eager run that allocates and
+        uploads everything the launches need); ``replay()`` then reruns it.  Returns the graph's output tensors."""
+        self.o["form"] = "batched"
+        self.run()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.graph_out = self.enqueue(None)
+        return self.graph_out
+
+    def replay(self):
+        t_dev0 = time.time()
+        self.graph.replay()
+        torch.cuda.synchronize(self.device)
+        return dict(self.graph_out, t_gpu=time.time() - t_dev0)
+
+    def enqueue(self, timer=None):
+        """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
+        and no host-to-device copy."""
         from . import _lib as L
         o, sp, device, model = self.o, self.sp, self.device, self.model
         main = torch.cuda.current_stream(device)
-        if not hasattr(self, "side"):
-            self.side = torch.cuda.Stream(device)      # default priority
+        batched = o.get("form", "batched") == "batched"
+        if batched and not hasattr(self, "side"):
+            self.side = torch.cuda.Stream(device)      # default priorities
+            self.xs = torch.cuda.Stream(device)
+            self.prep = {}
+            hasp = self.hp["has_partner"][sp.perm]
+            for gi, (b, (B, bx, _), start, m) in enumerate(sp.groups):
+                # rows with an overlapping partner come first in a group: their partner geometry has rows [0, m_p)
+                m_p = int(hasp[start:start + m].sum())
+                if not hasp[start:start + m_p].all():
+                    raise XpgError("partner rows are not first in their shoebox group")
+                bufs = dict(sb=torch.empty((m, B, bx, bx), dtype=torch.float32, device=device),
+                            geo=torch.empty((m, 3, bx, bx), dtype=torch.float64, device=device),
+                            zeta=torch.empty(m, dtype=torch.float64, device=device),
+                            nbm=torch.empty((m, B, bx, bx), dtype=torch.uint8, device=device),
+                            geo_p=torch.empty((max(m_p, 1), 3, bx, bx), dtype=torch.float64, device=device),
+                            zeta_p=torch.empty(max(m_p, 1), dtype=torch.float64, device=device))
+                ps = L.Prepared()
+                for k, t in bufs.items():
+                    setattr(ps, k, L.ptr(t))
+                self.prep[gi] = (ps, bufs)
+        side = self.side if batched else main
         N = sp.n
         f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
         o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
         o2 = {k: f64() for k in K.P2_F64}
         prof_by_block = {}
-        gauss = gaussian_profile()
         nreg = o["nreg"]
         self._init_widths()
-        t_dev0 = time.time()
         nullctx = _Null()
         st = L.stream_ptr(device)
+        ready = {}
+        if batched:
+            # every block's widths-independent work (shoebox gather, Kabsch frame, eps and J, neighbour masks) ahead,
+            # in block order, on the preparation stream
+            self.xs.wait_stream(main)
+            with torch.cuda.stream(self.xs):
+                for b in self.order:
+                    g = self.exp_blocks[b].geom
+                    det = self.det_for(g)
+                    callp = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
+                    for gi, (B, bx, _), start, m in self.by_block[b]:
+                        with (timer("prepare", b, gi, self.xs) if timer else nullctx):
+                            L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp),
+                                                        C.byref(sp.refl_view(start, m, (B, bx, bx)).s),
+                                                        C.byref(self.store), C.byref(self.prep[gi][0]),
+                                                        L.stream_ptr(device)))
+                    ready[b] = torch.cuda.Event()
+                    ready[b].record(self.xs)
         for i, b in enumerate(self.order):
             groups = self.by_block[b]
             g = self.exp_blocks[b].geom
@@ -495,12 +568,15 @@ class Sweep:
             acc = None
             if o["learn_profiles"]:
                 num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
+            if batched:
+                main.wait_event(ready[b])
             for gi, (B, bx, _), start, m in groups:
                 view = sp.refl_view(start, m, (B, bx, bx))
                 out = {k: v[start:start + m] for k, v in o1.items()}
                 out["fz"] = self.fz[start:start + m]
                 with (timer("pass1", b, gi, main) if timer else nullctx):
-                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS))
+                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS),
+                                  prep=self.prep[gi][0] if batched else None)
             s0 = groups[0][2]
             n_b = sum(m for *_, m in groups)
             if o["adaptive"]:
@@ -512,25 +588,26 @@ class Sweep:
                     float(g.qx), C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
                     C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes, st))
             if o["learn_profiles"]:
-                prof, _, _ = K.normalise_profiles(num_fx, den_fx, device)
+                prof, _, _ = K.normalise_profiles(num_fx, den_fx, device, gauss=self.gauss_t)
             else:
-                prof = torch.as_tensor(np.repeat(gauss[None], nreg, 0), device=device)
+                prof = self.gauss_rep
             prof_by_block[b] = prof
             if o["moments_only"]:
                 continue
             call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
             call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
-            self.side.wait_stream(main)
-            with torch.cuda.stream(self.side):
+            if batched:
+                side.wait_stream(main)
+            with torch.cuda.stream(side):
                 for gi, (B, bx, _), start, m in groups:
                     view = sp.refl_view(start, m, (B, bx, bx))
                     out = {k: v[start:start + m] for k, v in o2.items()}
-                    with (timer("pass2", b, gi, self.side) if timer else nullctx):
+                    with (timer("pass2", b, gi, side) if timer else nullctx):
                         _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
-                                      o1["nbg"][start:start + m], out)
-        main.wait_stream(self.side)
-        torch.cuda.synchronize(device)
-        return dict(o1=o1, o2=o2, prof_by_block=prof_by_block, t_gpu=time.time() - t_dev0)
+                                      o1["nbg"][start:start + m], out, prep=self.prep[gi][0] if batched else None)
+        if batched:
+            main.wait_stream(side)
+        return dict(o1=o1, o2=o2, prof_by_block=prof_by_block)
 
     def block_params(self):
         """Per-block widths (xpg/integrate.py:427-429, :458-460) from the device records of the last run."""
@@ -582,7 +659,9 @@ class _Null:
         return False
 
 
-def _launch_pass1(det, call, view, store, out, acc, fz_stride=0):
+def _launch_pass1(det, call, view, store, out, acc, fz_stride=0, prep=None):
+    """Pass 1 of a table view, voxels read from the frame store, or from the view's prepared data ``prep``
+    (xpg_prepare)."""
     from . import _lib as L
     o = L.Pass1Out()
     o.fz_stride = int(fz_stride)
@@ -591,18 +670,29 @@ def _launch_pass1(det, call, view, store, out, acc, fz_stride=0):
     for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m", "frac_rec", "nfg", "zeta", "seff"):
         if k not in out:
             raise XpgError(f"missing output {k}")
-    L.check(L.lib().xpg_pass1_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), C.byref(o),
-                                     C.byref(acc) if acc is not None else None, L.stream_ptr(det.device)))
+    a = C.byref(acc) if acc is not None else None
+    if prep is not None:
+        L.check(L.lib().xpg_pass1_prepared(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(prep), C.byref(o),
+                                           a, L.stream_ptr(det.device)))
+    else:
+        L.check(L.lib().xpg_pass1_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), C.byref(o),
+                                         a, L.stream_ptr(det.device)))
 
 
-def _launch_pass2(det, call, view, store, prof, bg, nbg, out):
+def _launch_pass2(det, call, view, store, prof, bg, nbg, out, prep=None):
+    """Pass 2 of a table view, voxels read from the frame store, or from the view's prepared data ``prep``."""
     from . import _lib as L
     o = L.Pass2Out()
     for k in K.P2_F64:
         setattr(o, k, L.ptr(out[k]))
-    L.check(L.lib().xpg_pass2_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), L.ptr(prof),
-                                     int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
-                                     L.stream_ptr(det.device)))
+    if prep is not None:
+        L.check(L.lib().xpg_pass2_prepared(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(prep), L.ptr(prof),
+                                           int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
+                                           L.stream_ptr(det.device)))
+    else:
+        L.check(L.lib().xpg_pass2_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), L.ptr(prof),
+                                         int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
+                                         L.stream_ptr(det.device)))
 
 
 # ---------------------------------------------------------------------------------------------------------------------

@@ -182,8 +182,26 @@ def new_profile_acc(n_regions, device, G=PROFILE_GRID):
     return num, den, acc
 
 
-def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=True):
-    """One pass-1 launch; ``frames`` (a FrameStore view) instead of ``sb`` extracts from the frame store in-kernel."""
+def run_prepare(det, call, refl, frames, device):
+    """xpg_prepare of a chunk: (ctypes xpg_prepared, dict of its device buffers)."""
+    n, Dz, Dy, Dx = refl.n, refl.s.Dz, refl.s.Dy, refl.s.Dx
+    bufs = dict(sb=torch.empty((n, Dz, Dy, Dx), dtype=torch.float32, device=device),
+                geo=torch.empty((n, 3, Dy, Dx), dtype=torch.float64, device=device),
+                zeta=torch.empty(n, dtype=torch.float64, device=device),
+                nbm=torch.empty((n, Dz, Dy, Dx), dtype=torch.uint8, device=device),
+                geo_p=torch.empty((n, 3, Dy, Dx), dtype=torch.float64, device=device),
+                zeta_p=torch.empty(n, dtype=torch.float64, device=device))
+    ps = L.Prepared()
+    for k, t in bufs.items():
+        setattr(ps, k, L.ptr(t))
+    L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call), C.byref(refl.s), C.byref(frames), C.byref(ps),
+                                L.stream_ptr(device)))
+    return ps, bufs
+
+
+def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=True, prep=None):
+    """One pass-1 launch; ``frames`` (a FrameStore view) instead of ``sb`` extracts from the frame store in-kernel,
+    ``prep`` (run_prepare) reads prepared data."""
     n, Dz, Dy, Dx = refl.n, refl.s.Dz, refl.s.Dy, refl.s.Dx
     device = det.device
     out = {k: torch.empty(n, dtype=torch.float64, device=device) for k in P1_F64}
@@ -199,7 +217,10 @@ def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=Tru
             setattr(o, k, L.ptr(out.get(k)))
     st = L.stream_ptr(device)
     accp = C.byref(acc) if acc is not None else None
-    if frames is not None:
+    if prep is not None:
+        L.check(L.lib().xpg_pass1_prepared(C.byref(det.s), C.byref(call), C.byref(refl.s), C.byref(prep), C.byref(o),
+                                           accp, st))
+    elif frames is not None:
         L.check(L.lib().xpg_pass1_frames(C.byref(det.s), C.byref(call), C.byref(refl.s), C.byref(frames), C.byref(o),
                                          accp, st))
     else:
@@ -209,7 +230,7 @@ def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=Tru
     return out
 
 
-def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0):
+def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0, prep=None):
     n = refl.n
     device = det.device
     out = {k: torch.empty(n, dtype=torch.float64, device=device) for k in P2_F64}
@@ -220,7 +241,11 @@ def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0)
     bg = _dev(bg, torch.float64, device)
     nbg = _dev(nbg, torch.float64, device)
     st = L.stream_ptr(device)
-    if frames is not None:
+    if prep is not None:
+        L.check(L.lib().xpg_pass2_prepared(C.byref(det.s), C.byref(call), C.byref(refl.s), C.byref(prep),
+                                           L.ptr(prof), int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), int(irls_rule),
+                                           C.byref(o), st))
+    elif frames is not None:
         L.check(L.lib().xpg_pass2_frames(C.byref(det.s), C.byref(call), C.byref(refl.s), C.byref(frames), L.ptr(prof),
                                          int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), int(irls_rule), C.byref(o), st))
     else:
@@ -229,11 +254,13 @@ def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0)
     return out
 
 
-def normalise_profiles(num_fx, den_fx, device, G=PROFILE_GRID):
-    """Fixed-point grids -> float32 profiles (xpg/integrate.py:546-556), on the device; also the float64 grids."""
+def normalise_profiles(num_fx, den_fx, device, G=PROFILE_GRID, gauss=None):
+    """Fixed-point grids -> float32 profiles (xpg/integrate.py:546-556), on the device; also the float64 grids.
+    ``gauss``: the (G, G) float32 Gaussian prior already on the device (no host copy: capturable in a graph)."""
     R = int(num_fx.shape[0])
     acc = L.ProfileAcc(L.ptr(num_fx), L.ptr(den_fx), R)
-    gauss = torch.as_tensor(gaussian_profile(G), device=device)
+    if gauss is None:
+        gauss = torch.as_tensor(gaussian_profile(G), device=device)
     prof = torch.empty((R, G, G), dtype=torch.float32, device=device)
     num64 = torch.empty((R, G, G), dtype=torch.float64, device=device)
     den64 = torch.empty((R, G, G), dtype=torch.float64, device=device)
