@@ -101,12 +101,30 @@ def cpu_baseline(sweep, hp, spec, n_sample, seconds):
     r2 = K.run_pass2(detd, call, view, torch.as_tensor(prof, device=dev), r1["bg"], r1["nbg"], frames=sweep.store)
     torch.cuda.synchronize(dev)
     hI = r2["iprf"].cpu().numpy()
-    dI = np.abs(hI - o2["iprf"]) / np.sqrt(np.maximum(o2["vprf"], 1e-30))
-    ok_ = np.isfinite(dI)
+    hV = r2["vprf"].cpu().numpy()
+    sig = np.sqrt(np.maximum(o2["vprf"], 1e-30))
+    dI = np.abs(hI - o2["iprf"]) / sig
+    ok_ = np.isfinite(dI) & (o2["vprf"] > 0)
+    # per decile of the oracle's intensity: |dI|/sigma and the error of I/sigma
+    deciles = []
+    if ok_.sum() >= 10:
+        I0 = o2["iprf"][ok_]
+        zs = o2["iprf"][ok_] / sig[ok_] - hI[ok_] / np.sqrt(np.maximum(hV[ok_], 1e-30))
+        edges = np.quantile(I0, np.linspace(0, 1, 11))
+        idx = np.clip(np.searchsorted(edges, I0, side="right") - 1, 0, 9)
+        for d in range(10):
+            sel = idx == d
+            if sel.any():
+                deciles.append(dict(I_lo=float(edges[d]), I_hi=float(edges[d + 1]), n=int(sel.sum()),
+                                    max_dI_over_sigma=float(dI[ok_][sel].max()),
+                                    median_dI_over_sigma=float(np.median(dI[ok_][sel])),
+                                    max_d_I_over_sigma=float(np.abs(zs[sel]).max())))
     parity = dict(sample=int(m), max_dI_over_sigma=float(np.max(dI[ok_])) if ok_.any() else None,
                   median_dI_over_sigma=float(np.median(dI[ok_])) if ok_.any() else None,
                   n_over_equal=bool(np.array_equal(r1["n_over"].cpu().numpy(), o1["n_over"])),
-                  n_fg_nb_equal=bool(np.array_equal(r1["n_fg_nb"].cpu().numpy(), o1["n_fg_nb"])))
+                  n_fg_nb_equal=bool(np.array_equal(r1["n_fg_nb"].cpu().numpy(), o1["n_fg_nb"])),
+                  n_fg_masked_equal=bool(np.array_equal(r1["n_fg_masked"].cpu().numpy(), o1["n_fg_masked"])),
+                  by_intensity_decile=deciles)
     base = dict(value=round(rate, 2), unit="reflections/s", cores=1, kind="port",
                 sample=f"{m} reflections ({B}x{bx}x{bx} shoeboxes, block {b_mid}) x {reps} repetitions, float64 "
                        f"oracle pass1 + profile accumulation + normalisation + pass2 on shoeboxes extracted on the "
@@ -225,22 +243,24 @@ def main():
     edges = synth.block_edges(spec)
     model = IntegrationModel(spec.sigma_d, spec.sigma_m, box_xy=spec.box_xy, box_z=spec.box_z)
     t = time.time()
-    hp = _integrate(frames, [exp] * (len(edges) - 1), edges, model, spots["dmin"], plan_only=True, verbose=False)
+    hp = _integrate(frames, [exp] * (len(edges) - 1), edges, model, spots["dmin"], plan_only=True, verbose=False,
+                    _box_draw=synth.box_draw_for(spec))
     from xpg_amd.integrate import DZ_BUCKETS  # noqa: F401
     opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
                 bkg_template=None, moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
                 sm=np.radians(model.sigma_m))
     sweep = Sweep(frames, [exp] * (len(edges) - 1), model, hp, opts)
     N = sweep.sp.n
-    log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups over {len(edges) - 1} blocks "
+    log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups ({len(sweep.sp.bins)} size-bin launches) "
+        f"over {len(edges) - 1} blocks "
         f"({time.time() - t:.1f}s)")
 
     events = []
 
     class Timer:
         """HIP events around one launch, recorded on the stream the launch goes to."""
-        def __init__(self, kind, b, gi, st):
-            self.kind, self.gi, self.st = kind, gi, st
+        def __init__(self, kind, b, rows, st):
+            self.kind, self.gi, self.st = kind, rows, st
 
         def __enter__(self):
             self.e0 = torch.cuda.Event(enable_timing=True)
@@ -321,14 +341,14 @@ def main():
     per = {}
     K_nb = sweep.sp.K
     n_timed = 1
-    for kind, gi, e0, e1 in events:
-        b, (B, bx, s), start, m = sweep.sp.groups[gi]
-        V = B * bx * bx
+    for kind, (start, m), e0, e1 in events:
         if kind == "prepare":
             continue
+        Bs = sweep.sp.rows_dz[start:start + m]
+        bxs = sweep.sp.rows_bx[start:start + m]
         d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
         d["ms"] += e0.elapsed_time(e1)
-        d["bytes"] += m * per_refl_bytes(V, B, K_nb, kind)
+        d["bytes"] += float(np.sum(per_refl_bytes(Bs * bxs * bxs, Bs, K_nb, kind)))
         d["n"] += 1
     dom = max(per, key=lambda k: per[k]["ms"])
     dd = per[dom]
@@ -357,7 +377,8 @@ def main():
                config=dict(workload=spec.name, form=FORMS[form],
                            frames=spec.nframes, frame_shape=[spec.ny, spec.nx], reflections=N,
                            integrated=integrated, blocks=len(edges) - 1, box=[spec.box_z, spec.box_xy, spec.box_xy]
-                           if spec.box_xy else "reference rules", parallelism="single GPU"),
+                           if spec.box_xy else (f"drawn: side log-U{list(spec.box_draw[0])}, depth log-U"
+                                                f"{list(spec.box_draw[1])}, odd" if spec.box_draw else "reference rules"), parallelism="single GPU"),
                roofline=roof, cpu_baseline=cpu, parity=parity,
                plain=plain, vs_plain=round(value / plain["value"], 3) if plain and form == "batched" else None,
                batched_forms=graph_info if form == "batched" else None)

@@ -74,6 +74,14 @@ typedef struct xpg_refl {
   const double *pS, *pphi;
   int32_t K;
   const float *nbx, *nby, *nbz, *nbsx, *nbsz;
+  /* Mixed shapes in one launch (all NULL: every row is Dz x Dy x Dx).  rdz, rdy, rdx: each row's box; Dz, Dy, Dx then
+   * hold their maxima (they size the LDS).  voff: offset of a row's voxels in sb and nbm (elements); poff: offset of
+   * its pixels (a row's geo / geo_p block starts at element 3 poff, its per-pixel pass-1 outputs at poff).
+   * wave_lds: LDS bytes per wavefront of the launch, at least xpg_lds_bytes of every row's shape (0: sized from
+   * Dz x Dy x Dx); a row whose layout exceeds it is skipped, its outputs left as they were. */
+  const int32_t *rdz, *rdy, *rdx;
+  const int64_t *voff, *poff;
+  int32_t wave_lds;
 } xpg_refl;
 
 /* pass 1 outputs (xpg/integrate_kernels.py:362-365).  Per-reflection float64 / int32 (n); fz (n, Dz); per-pixel
@@ -212,6 +220,10 @@ int xpg_pass2_prepared(const xpg_detector *det, const xpg_call *call, const xpg_
 
 /* Bytes of LDS one wavefront of pass 1 / pass 2 uses for a (Dz, Dy, Dx) shoebox (launch planning, autotuning). */
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx);
+
+/* LDS bytes a workgroup may use on the current device.  Shoeboxes whose xpg_lds_bytes exceeds it are integrated from
+ * gathered values (xpg_prepare / xpg_extract), read in place; the *_frames entry points reject them. */
+int xpg_lds_limit(void);
 
 #ifdef __cplusplus
 }

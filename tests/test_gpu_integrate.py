@@ -28,8 +28,11 @@ def _coded(frames):
     return {i: ox.frame_codes(raw[i], flag_value=frames.flag_value, overload=frames.overload) for i in range(len(raw))}
 
 
-@pytest.mark.parametrize("which", ["small", "medium"])
+@pytest.mark.parametrize("which", ["small", "medium", "uneven"])
 def test_integrate_matches_oracle_sweep(which):
+    """small: one fixed shoebox shape; medium: the reference's box rules (several depths and sides, phi-binned wide
+    reflections); uneven: drawn odd sides 5-21 and depths 3-15 with sigma_M per detector region, so the batched
+    form's size-bin launches mix shapes that the plain form launches one shape at a time."""
     from xpg_amd import integrate as I, synth
     if which == "small":
         spec = synth.spec_small()
@@ -37,10 +40,13 @@ def test_integrate_matches_oracle_sweep(which):
         spec = synth.SynthSpec(name="medium", nx=1536, ny=1536, nframes=40, distance=120.0,
                                cell=(45.0, 50.0, 60.0, 90.0, 90.0, 90.0), space_group="P 21 21 21", bg_centre=3.0,
                                bg_edge=1.0, dead_frac=0.01, block_frames=20, seed=77, mean_i0=5e4)
+        if which == "uneven":
+            spec = synth.replace(spec, name="uneven", sigma_m_range=(0.05, 0.8), box_draw=((5, 21), (3, 15)), seed=78)
+    draw = synth.box_draw_for(spec)
     print(f"integrate {spec.name}", flush=True)
     frames, exp, edges, model, blocks, dmin, det = _dataset(spec)
-    res = I.integrate(frames, blocks, edges, model, dmin, verbose=False)
-    hp = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False)
+    res = I.integrate(frames, blocks, edges, model, dmin, verbose=False, _box_draw=draw)
+    hp = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False, _box_draw=draw)
     coded = _coded(frames)
     out, bp = osw.integrate_sweep(coded, list(range(spec.nframes)), hp, blocks, model, det)
     r = res.refl
@@ -60,14 +66,17 @@ def test_integrate_matches_oracle_sweep(which):
         if rec["n"] >= 100:
             assert abs(res.block_params[b]["sigma_m"] - rec["sigma_m"]) < 1e-9 * rec["sigma_m"]
     # repeated runs are bit-identical (fixed-point profile grids, fixed reduction orders)
-    res2 = I.integrate(frames, blocks, edges, model, dmin, verbose=False)
+    res2 = I.integrate(frames, blocks, edges, model, dmin, verbose=False, _box_draw=draw)
     assert np.array_equal(res2.refl["iprf"], r["iprf"], equal_nan=True)
-    # the plain form (one stream, voxels read from the frame store) gives the same results as the batched default
-    hp2 = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False)
+    # the plain form (one stream, one launch per shoebox shape, voxels read from the frame store) gives the same
+    # results as the batched default (size-bin launches over prepared data)
+    hp2 = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False, _box_draw=draw)
     sw = I.Sweep(frames, blocks, model, hp2, dict(frames=(1, spec.nframes), mask=None, exclude_frames=(),
                                                   learn_profiles=True, adaptive=True, bkg_template=None,
                                                   moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
                                                   sm=np.radians(model.sigma_m), form="plain"))
+    if which != "small":
+        assert len(sw.sp.bins) < len(sw.sp.groups)
     res3 = sw.finish(sw.run())
     for k in ("iprf", "vprf", "isum", "bg"):
         assert np.array_equal(res3.refl[k], r[k], equal_nan=True), k

@@ -90,10 +90,12 @@ def test_kernels_match_oracle_on_golden_chunks(name):
     (104, (17, 21, 21), dict(intensity=(10.0, 1e6))),
     (105, (33, 9, 9), dict(sm_deg=0.4)),
     (106, (5, 9, 9), dict(intensity=(1.0, 2 ** 20 * 20), background=0.5)),
+    (107, (33, 31, 31), dict(n=12, sd_deg=0.06, sm_deg=0.5)),    # too large for LDS: values read in place
 ])
 def test_kernels_match_oracle_synthetic(seed, shape, kw):
     from chunks import make_chunk
-    c = make_chunk(n=96, Dz=shape[0], Dy=shape[1], Dx=shape[2], seed=seed, **kw)
+    kw = dict(kw)
+    c = make_chunk(n=kw.pop("n", 96), Dz=shape[0], Dy=shape[1], Dx=shape[2], seed=seed, **kw)
     h1, h2 = _run(c, torch.device("cuda:0"))
     o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
     _check_pass1(h1, o1, seed)

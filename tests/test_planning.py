@@ -108,3 +108,59 @@ def test_plan_only_full_workload_shapes():
                       verbose=False)
     assert (hp["bxs"] == 11).all() and (hp["bucket"] == 7).all()
     assert len(hp["block_order"]) == 2 and hp["refl"]["x"].size > 20000
+
+
+def test_drawn_boxes_and_size_bins():
+    """Drawn per-reflection boxes (odd side log-U[5, 21], depth log-U[3, 15]) and the sweep plan's size bins: every
+    block's rows are covered by consecutive bin launches, each bin's largest depth x largest side fits the bin's LDS
+    bound, and the per-row voxel / pixel offsets pack each bin's rows back to back."""
+    import torch
+    from xpg_amd import _lib as L
+    spec = synth.replace(synth.spec_uneven(), nframes=100)
+    exp = synth.make_experiment(spec)
+    edges = synth.block_edges(spec)
+    img = SimpleNamespace(first=1, last=100, nx=spec.nx, ny=spec.ny)
+    model = I.IntegrationModel(spec.sigma_d, spec.sigma_m)
+    hp = I._integrate(img, [exp] * (len(edges) - 1), edges, model, synth.edge_resolution(exp.geom), plan_only=True,
+                      verbose=False, _box_draw=synth.box_draw_for(spec))
+    dz, bx = hp["dz"], hp["bxs"]
+    assert (dz % 2 == 1).all() and (bx % 2 == 1).all()
+    assert dz.min() == 3 and dz.max() == 15 and bx.min() == 5 and bx.max() == 21
+    assert (hp["fb"] == 1).all() and (hp["pxs"] == 1).all() and np.array_equal(hp["bucket"], dz)
+    # log-uniform: E[side^2] = (21^2 - 5^2) / (2 ln(21 / 5)) ~ 145 and E[depth] = 12 / ln 5 ~ 7.46 before rounding
+    assert abs(np.mean(bx.astype(float) ** 2) / 145.0 - 1) < 0.05 and abs(np.mean(dz) / 7.46 - 1) < 0.05
+    # the same reflection gets the same box whatever else is predicted
+    d2, s2 = synth.draw_boxes(spec, {"hkl": hp["refl"]["hkl"][::-1]})
+    assert np.array_equal(d2[::-1], dz) and np.array_equal(s2[::-1], bx)
+    # partials: boxes reaching into the next or previous 50-frame block
+    blk_lo = np.asarray(edges)[hp["refl"]["block"]]
+    part = (hp["iz0"] < blk_lo) | (hp["iz0"] + dz > blk_lo + spec.block_frames)
+    assert 0.05 < part.mean() < 0.3
+    sp = I.SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"], hp["iy0"],
+                     hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"], hp["block_order"],
+                     torch.device("cpu"))
+    assert sorted(sp.perm.tolist()) == list(range(hp["refl"]["x"].size))
+    voff, poff = sp.t["voff"].numpy(), sp.t["poff"].numpy()
+    pos = 0
+    for bi, (b, lb, start, m, (mB, mbx), wl) in enumerate(sp.bins):
+        assert start == pos and m > 0
+        pos += m
+        rows = sp.perm[start:start + m]
+        assert (hp["refl"]["block"][rows] == b).all()
+        assert mB == dz[rows].max() and mbx == bx[rows].max()
+        need = [L.lib().xpg_lds_bytes(int(B), int(s), int(s)) for B, s in set(zip(dz[rows], bx[rows]))]
+        assert wl == max(need) and (lb == len(I.LDS_BINS) or wl <= I.LDS_BINS[lb])
+        assert lb == 0 or min(need) > I.LDS_BINS[lb - 1]
+        V = (dz[rows] * bx[rows] ** 2).astype(np.int64)
+        assert np.array_equal(voff[start:start + m], np.cumsum(V) - V)
+        assert np.array_equal(poff[start:start + m], np.cumsum(bx[rows] ** 2) - bx[rows] ** 2)
+        assert sp.bin_sizes[bi] == (int(V.sum()), int((bx[rows] ** 2).sum()))
+    assert pos == sp.n
+    # shape groups nest in bins, with partner rows first in each group
+    for b, (B, s, _), start, m in sp.groups:
+        rows = sp.perm[start:start + m]
+        assert (dz[rows] == B).all() and (bx[rows] == s).all()
+        hpr = hp["has_partner"][rows]
+        assert not (~hpr[:-1] & hpr[1:]).any()
+        assert any(s2 <= start and start + m <= s2 + m2 for _, _, s2, m2, _, _ in sp.bins)
+    assert len(sp.bins) < len(sp.groups)

@@ -48,7 +48,8 @@ class Refl(C.Structure):
                 ("iz0", C.c_void_p), ("fb", C.c_void_p), ("pxs", C.c_void_p), ("region", C.c_void_p),
                 ("has_p", C.c_void_p), ("pS", C.c_void_p), ("pphi", C.c_void_p), ("K", C.c_int32),
                 ("nbx", C.c_void_p), ("nby", C.c_void_p), ("nbz", C.c_void_p), ("nbsx", C.c_void_p),
-                ("nbsz", C.c_void_p)]
+                ("nbsz", C.c_void_p), ("rdz", C.c_void_p), ("rdy", C.c_void_p), ("rdx", C.c_void_p),
+                ("voff", C.c_void_p), ("poff", C.c_void_p), ("wave_lds", C.c_int32)]
 
 
 class Pass1Out(C.Structure):
@@ -101,6 +102,7 @@ _SIGS = {
     "xpg_pass2_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                             C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
+    "xpg_lds_limit": ([], C.c_int),
     "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 12 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
                                                                                          C.c_void_p], C.c_int),
     "xpg_adapt_workspace_bytes": ([C.c_int32], C.c_int64),

@@ -56,11 +56,12 @@ struct Layout {
   size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, prow, pflag, R, Rp, fr, co, tab, bytes;
 };
 
-__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int pass) {
+// vals_lds = false: the shoebox values are read in place from a device buffer (boxes too large for LDS)
+__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int pass, bool vals_lds = true) {
   const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
   Layout L;
   size_t o = 0;
-  L.vals = o; o = al8(o + 4 * V);
+  L.vals = o; if (vals_lds) o = al8(o + 4 * V);
   L.vflag = o; o = al8(o + V);
   L.scr = o; o += 8 * 21 * (size_t)Dz;
   L.U1 = o; o += 8 * P;
@@ -186,24 +187,36 @@ __global__ void k_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, in
 
 struct Box {
   int r, Dz, Dy, Dx, P, V, ix0, iy0, iz0, fb, s, region;
+  int64_t voff, poff;       // offsets of the row's voxels and pixels in the gathered / prepared buffers
   double phi;
 };
+
+// a value the whole wavefront holds alike (one row per wavefront), kept in scalar registers
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v), hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+__device__ __forceinline__ float uni_f(float v) { return __builtin_bit_cast(float, uni(__builtin_bit_cast(int, v))); }
+__device__ __forceinline__ double uni_d(double v) { return __builtin_bit_cast(double, uni64(__builtin_bit_cast(int64_t, v))); }
 
 __device__ __forceinline__ Box box_of(const xpg_refl &R, int r) {
   Box b;
   b.r = r;
-  b.Dz = R.Dz;
-  b.Dy = R.Dy;
-  b.Dx = R.Dx;
-  b.P = R.Dy * R.Dx;
-  b.V = R.Dz * b.P;
-  b.ix0 = R.ix0[r];
-  b.iy0 = R.iy0[r];
-  b.iz0 = R.iz0[r];
-  b.fb = R.fb[r];
-  b.s = R.pxs ? R.pxs[r] : 1;
-  b.region = R.region ? R.region[r] : 0;
-  b.phi = R.phi[r];
+  b.Dz = uni(R.rdz ? R.rdz[r] : R.Dz);
+  b.Dy = uni(R.rdy ? R.rdy[r] : R.Dy);
+  b.Dx = uni(R.rdx ? R.rdx[r] : R.Dx);
+  b.P = b.Dy * b.Dx;
+  b.V = b.Dz * b.P;
+  b.voff = uni64(R.voff ? R.voff[r] : (int64_t)r * b.V);
+  b.poff = uni64(R.poff ? R.poff[r] : (int64_t)r * b.P);
+  b.ix0 = uni(R.ix0[r]);
+  b.iy0 = uni(R.iy0[r]);
+  b.iz0 = uni(R.iz0[r]);
+  b.fb = uni(R.fb[r]);
+  b.s = uni(R.pxs ? R.pxs[r] : 1);
+  b.region = uni(R.region ? R.region[r] : 0);
+  b.phi = uni_d(R.phi[r]);
   return b;
 }
 
@@ -236,9 +249,9 @@ __device__ __forceinline__ void stage_values(const Box &b, const float *sb, cons
   for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
   wave_sync();
   if (sb) {
-    const float *src = sb + (int64_t)b.r * b.V;
+    const float *src = sb + b.voff;
     constexpr int NB = 16;
-    for (int v0 = lane; v0 < b.V; v0 += NB * WAVE) {
+    for (int v0 = lane; v0 < b.V && vals != src; v0 += NB * WAVE) {
       float t[NB];
 #pragma unroll
       for (int u = 0; u < NB; ++u) t[u] = v0 + u * WAVE < b.V ? src[v0 + u * WAVE] : 0.f;
@@ -318,11 +331,12 @@ __host__ __device__ static inline size_t extract_wave_bytes(int Dz, int Dy, int 
 __global__ void __launch_bounds__(256) k_extract(xpg_frames F, xpg_refl R, float *sb, int wave_bytes) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wid = threadIdx.x / WAVE;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);   // wave-uniform: row data in SGPRs
   const int r = blockIdx.x * (blockDim.x / WAVE) + wid;
   if (r >= R.n) return;
   const Box b = box_of(R, r);
   const size_t P = (size_t)b.P, V = (size_t)b.V;
+  if (extract_wave_bytes(b.Dz, b.Dy, b.Dx) > (size_t)wave_bytes) return;
   unsigned char *base = smem + (size_t)wid * wave_bytes;
   float *vals = (float *)base;
   uint8_t *vflag = base + al8(4 * V);
@@ -330,7 +344,7 @@ __global__ void __launch_bounds__(256) k_extract(xpg_frames F, xpg_refl R, float
   unsigned char *tmp = prow + al8(P);
   unsigned char *ftmp = tmp + 8 * P;
   stage_values(b, nullptr, F, vals, vflag, prow, tmp, ftmp, lane);
-  float *dst = sb + (int64_t)r * b.V;
+  float *dst = sb + b.voff;
   for (int v = lane; v < b.V; v += WAVE) dst[v] = vals[v];
 }
 
@@ -340,8 +354,8 @@ __device__ __forceinline__ double stage_prepared(const Box &b, const xpg_prepare
                                                  uint8_t *vflag, uint8_t *prow, double *U1, double *U2, double *Jd,
                                                  int lane) {
   for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
-  const float *src = pp.sb + (int64_t)b.r * b.V;
-  const uint8_t *nsrc = pp.nbm + (int64_t)b.r * b.V;
+  const float *src = pp.sb + b.voff;
+  const uint8_t *nsrc = pp.nbm + b.voff;
   constexpr int NB = 16;
   for (int v0 = lane; v0 < b.V; v0 += NB * WAVE) {
     float t[NB];
@@ -355,11 +369,11 @@ __device__ __forceinline__ double stage_prepared(const Box &b, const xpg_prepare
 #pragma unroll
     for (int u = 0; u < NB; ++u)
       if (v0 + u * WAVE < b.V) {
-        vals[v0 + u * WAVE] = t[u];
+        if (vals != src) vals[v0 + u * WAVE] = t[u];
         vflag[v0 + u * WAVE] = ((m[u] >> bit) & 1) ? 1 : 0;     // VF_NBM
       }
   }
-  const double *g = pp.geo + (int64_t)b.r * 3 * b.P;
+  const double *g = pp.geo + 3 * b.poff;
   for (int p = lane; p < b.P; p += WAVE) {
     U1[p] = g[p];
     U2[p] = g[b.P + p];
@@ -406,12 +420,13 @@ __device__ __forceinline__ double frames_fg(const double *Rd, int Dz, uint8_t *f
 struct PrepLayout {
   size_t vals, vflag, prow, tmp, ftmp, U1, U2, Jd, co, tab, bytes;
 };
-__host__ __device__ static inline PrepLayout prep_layout(int Dz, int Dy, int Dx) {
+// big: values and mask bits are written in place into the output buffers (boxes too large for LDS)
+__host__ __device__ static inline PrepLayout prep_layout(int Dz, int Dy, int Dx, bool big = false) {
   const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
   PrepLayout L;
   size_t o = 0;
-  L.vals = o; o = al8(o + 4 * V);
-  L.vflag = o; o = al8(o + V);
+  L.vals = o; if (!big) o = al8(o + 4 * V);
+  L.vflag = o; if (!big) o = al8(o + V);
   L.prow = o; o = al8(o + P);
   L.tmp = o; o += 8 * P;
   L.ftmp = o; o = al8(o + 4 * (size_t)Dz);
@@ -425,17 +440,19 @@ __host__ __device__ static inline PrepLayout prep_layout(int Dz, int Dy, int Dx)
 }
 
 __global__ void __launch_bounds__(256) k_prepare(xpg_detector det, xpg_call call, xpg_refl R, xpg_frames F,
-                                                 xpg_prepared pp, int wave_bytes) {
+                                                 xpg_prepared pp, int wave_bytes, int big) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wid = threadIdx.x / WAVE;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);   // wave-uniform: row data in SGPRs
   const int r = blockIdx.x * (blockDim.x / WAVE) + wid;
   if (r >= R.n) return;
   const Box b = box_of(R, r);
-  const PrepLayout L = prep_layout(b.Dz, b.Dy, b.Dx);
+  const PrepLayout L = prep_layout(b.Dz, b.Dy, b.Dx, big != 0);
+  if (L.bytes > (size_t)wave_bytes) return;
   unsigned char *base = smem + (size_t)wid * wave_bytes;
-  float *vals = (float *)(base + L.vals);
-  uint8_t *vflag = base + L.vflag, *prow = base + L.prow;
+  float *vals = big ? pp.sb + b.voff : (float *)(base + L.vals);
+  uint8_t *vflag = big ? pp.nbm + b.voff : base + L.vflag;
+  uint8_t *prow = base + L.prow;
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
   const int P = b.P, V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
@@ -447,25 +464,25 @@ __global__ void __launch_bounds__(256) k_prepare(xpg_detector det, xpg_call call
   wave_sync();
   neighbour_mask(R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 0, vflag, 1, tab, prow, lane);
   neighbour_mask(R, r, co, co + Dx, co + Dx + Dy, Dz, Dy, Dx, 1, vflag, 2, tab, prow, lane);
-  float *sdst = pp.sb + (int64_t)r * V;
-  uint8_t *ndst = pp.nbm + (int64_t)r * V;
-  for (int v = lane; v < V; v += WAVE) {
+  float *sdst = pp.sb + b.voff;
+  uint8_t *ndst = pp.nbm + b.voff;
+  for (int v = lane; v < V && !big; v += WAVE) {
     sdst[v] = vals[v];
     ndst[v] = vflag[v];
   }
-  double *g = pp.geo + (int64_t)r * 3 * P;
+  double *g = pp.geo + 3 * b.poff;
   for (int p = lane; p < P; p += WAVE) {
     g[p] = U1[p];
     g[P + p] = U2[p];
     g[2 * P + p] = Jd[p];
   }
   if (lane == 0) pp.zeta[r] = f.zeta;
-  if (R.has_p && (double)R.has_p[r] > 0.5) {
+  if (R.has_p && (double)uni_f(R.has_p[r]) > 0.5) {
     // the overlapping partner's frame and eps, J on this box's pixels (pass 2 models its profile here)
     Frame fp;
     kabsch(R.pS + 3 * (int64_t)r, call.s0, call.axis, fp);
     box_eps_J(det, fp, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
-    double *gp = pp.geo_p + (int64_t)r * 3 * P;
+    double *gp = pp.geo_p + 3 * b.poff;
     for (int p = lane; p < P; p += WAVE) {
       gp[p] = U1[p];
       gp[P + p] = U2[p];
@@ -487,6 +504,7 @@ struct P1Args {
   const float *sb;
   xpg_prepared prep;
   int use_prep;
+  int vals_global;          // shoebox values read in place (sb / prep.sb), not staged in LDS
   xpg_pass1_out out;
   xpg_profile_acc acc;
   int use_acc;
@@ -516,29 +534,31 @@ __device__ __forceinline__ void splat4(const xpg_profile_acc &acc, int region, d
 }
 
 // PREP: the prepared-data form (xpg_pass1_prepared), compiled separately so that it carries none of the fused
-// form's register pressure
-template <bool PREP>
+// form's register pressure; VG: shoebox values read in place from global memory (boxes too large for LDS), compiled
+// separately so that the LDS form's value accesses stay LDS instructions
+template <bool PREP, bool VG>
 __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wid = threadIdx.x / WAVE;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);   // wave-uniform: row data in SGPRs
   const int r = blockIdx.x * a.waves + wid;
   if (r >= a.R.n) return;
-  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx, 1);
+  const Box b = box_of(a.R, r);
+  const Layout L = make_layout(b.Dz, b.Dy, b.Dx, 1, !VG);
+  if (L.bytes > (size_t)a.wave_bytes) return;
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
-  const int P = a.R.Dy * a.R.Dx;
-  float *vals = (float *)(base + L.vals);
+  const int P = b.P;
+  float *vals = VG ? const_cast<float *>((PREP ? a.prep.sb : a.sb) + b.voff) : (float *)(base + L.vals);
   uint8_t *vflag = base + L.vflag;
   double *scr = (double *)(base + L.scr);
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   float *bref = (float *)(base + L.pa), *thr = bref + P;
   uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R);
-  double *zzd = (double *)(base + L.fr), *e3d = zzd + a.R.Dz;
-  uint8_t *fflag = (uint8_t *)(e3d + a.R.Dz);
+  double *zzd = (double *)(base + L.fr), *e3d = zzd + b.Dz;
+  uint8_t *fflag = (uint8_t *)(e3d + b.Dz);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
 
-  const Box b = box_of(a.R, r);
   const int Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
@@ -776,7 +796,7 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
       if (c >= 0.f && !(vflag[j * P + p] & VF_NBM)) img += (double)c - bpx;
       else okv = false;
     }
-    const int64_t o = (int64_t)r * P + p;
+    const int64_t o = b.poff + p;
     if (a.out.img2) a.out.img2[o] = (float)img;
     if (a.out.all_valid2) a.out.all_valid2[o] = okv ? 1 : 0;
     if (a.out.J) a.out.J[o] = (float)Jd[p];
@@ -872,6 +892,7 @@ struct P2Args {
   const float *sb;
   xpg_prepared prep;
   int use_prep;
+  int vals_global;          // shoebox values read in place (sb / prep.sb), not staged in LDS
   const float *prof;
   int n_regions;
   const double *bg, *nbg;
@@ -895,17 +916,19 @@ __device__ __forceinline__ void profile_pixels(const float *prof, int region, co
   }
 }
 
-template <bool PREP>
+template <bool PREP, bool VG>
 __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
-  const int wid = threadIdx.x / WAVE;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);   // wave-uniform: row data in SGPRs
   const int r = blockIdx.x * a.waves + wid;
   if (r >= a.R.n) return;
-  const Layout L = make_layout(a.R.Dz, a.R.Dy, a.R.Dx, 2);
+  Box b = box_of(a.R, r);
+  const Layout L = make_layout(b.Dz, b.Dy, b.Dx, 2, !VG);
+  if (L.bytes > (size_t)a.wave_bytes) return;
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
-  const int P = a.R.Dy * a.R.Dx;
-  float *vals = (float *)(base + L.vals);
+  const int P = b.P;
+  float *vals = VG ? const_cast<float *>((PREP ? a.prep.sb : a.sb) + b.voff) : (float *)(base + L.vals);
   uint8_t *vflag = base + L.vflag;
   double *scr = (double *)(base + L.scr);
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
@@ -913,16 +936,15 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   float *bref = (float *)(base + L.pc);
   uint8_t *prow = base + L.prow, *pflag = base + L.pflag;
   double *Rd = (double *)(base + L.R), *Rpd = (double *)(base + L.Rp);
-  float *Rf = (float *)(base + L.fr), *Rpf = Rf + a.R.Dz;
-  uint8_t *fflag = (uint8_t *)(Rpf + a.R.Dz);
+  float *Rf = (float *)(base + L.fr), *Rpf = Rf + b.Dz;
+  uint8_t *fflag = (uint8_t *)(Rpf + b.Dz);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
 
-  Box b = box_of(a.R, r);
   b.region = min(max(b.region, 0), a.n_regions - 1);   // memory safety; the host validates region
   const int Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
-  const double has_p = a.R.has_p ? (double)a.R.has_p[r] : 0.0;
+  const double has_p = a.R.has_p ? (double)uni_f(a.R.has_p[r]) : 0.0;
   const bool partner = has_p > 0.5;
   const float hpf = (float)has_p;
   const double bgr = a.bg[r];
@@ -961,7 +983,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     double zeta_p;
     wave_sync();
     if constexpr (PREP) {
-      const double *gp = a.prep.geo_p + (int64_t)r * 3 * P;
+      const double *gp = a.prep.geo_p + 3 * b.poff;
       for (int p = lane; p < P; p += WAVE) {
         U1[p] = gp[p];
         U2[p] = gp[P + p];
@@ -1189,11 +1211,20 @@ static int check_refl(const xpg_refl *R, bool need_partner_arrays) {
   return XPG_OK;
 }
 
-static int launch_shape(const xpg_refl *R, int pass, int *waves, int *wave_bytes, size_t *lds) {
-  const Layout L = make_layout(R->Dz, R->Dy, R->Dx, pass);
+// in_place_ok: the shoebox values are in a device buffer the kernel may read in place when they do not fit in LDS
+static int launch_shape(const xpg_refl *R, int pass, bool in_place_ok, int *waves, int *wave_bytes, size_t *lds,
+                        int *vals_global) {
+  Layout L = make_layout(R->Dz, R->Dy, R->Dx, pass);
   const int lim = lds_limit();
+  *vals_global = 0;
+  if (R->wave_lds > 0 && R->wave_lds <= lim && (size_t)R->wave_lds < L.bytes) L.bytes = al8((size_t)R->wave_lds);
+  if ((long long)L.bytes > lim && in_place_ok) {
+    L = make_layout(R->Dz, R->Dy, R->Dx, pass, false);
+    *vals_global = 1;
+  }
   if ((long long)L.bytes > lim)
-    return fail(XPG_ERR_TOO_LARGE, "shoebox %dx%dx%d needs %zu bytes of LDS > %d", R->Dz, R->Dy, R->Dx, L.bytes, lim);
+    return fail(XPG_ERR_TOO_LARGE, "shoebox %dx%dx%d needs %zu bytes of LDS > %d%s", R->Dz, R->Dy, R->Dx, L.bytes,
+                lim, in_place_ok ? "" : " (gather it first: xpg_prepare or xpg_extract)");
   int w = 4;
   while (w > 1 && (long long)w * L.bytes > lim) w >>= 1;
   // keep several workgroups per CU: at most 32 KB per workgroup when the shoebox is small
@@ -1230,8 +1261,9 @@ int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void 
       (frames->dtype != 0 && frames->dtype != 1))
     return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
   if (refl->n == 0) return XPG_OK;
-  const size_t wb = extract_wave_bytes(refl->Dz, refl->Dy, refl->Dx);
   const int lim = lds_limit();
+  size_t wb = extract_wave_bytes(refl->Dz, refl->Dy, refl->Dx);
+  if (refl->wave_lds > 0 && refl->wave_lds <= lim && (size_t)refl->wave_lds < wb) wb = al8((size_t)refl->wave_lds);
   if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
   int w = 4;
   while (w > 1 && (long long)w * wb > lim) w >>= 1;
@@ -1276,15 +1308,21 @@ int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *r
   rc = check_prep(prep);
   if (rc) return rc;
   if (refl->n == 0) return XPG_OK;
-  const size_t wb = prep_layout(refl->Dz, refl->Dy, refl->Dx).bytes;
   const int lim = lds_limit();
+  int big = 0;
+  size_t wb = prep_layout(refl->Dz, refl->Dy, refl->Dx).bytes;
+  if (refl->wave_lds > 0 && refl->wave_lds <= lim && (size_t)refl->wave_lds < wb) wb = al8((size_t)refl->wave_lds);
+  if ((long long)wb > lim) {
+    big = 1;
+    wb = prep_layout(refl->Dz, refl->Dy, refl->Dx, true).bytes;
+  }
   if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
   int w = 4;
   while (w > 1 && (long long)w * wb > lim) w >>= 1;
   rc = set_lds(k_prepare, (size_t)w * wb);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prepare, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *det,
-                     *call, *refl, *frames, *prep, (int)wb);
+                     *call, *refl, *frames, *prep, (int)wb, big);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1320,13 +1358,14 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   if (acc) a.acc = *acc;
   else memset(&a.acc, 0, sizeof(a.acc));
   size_t lds;
-  rc = launch_shape(refl, 1, &a.waves, &a.wave_bytes, &lds);
+  rc = launch_shape(refl, 1, sb != nullptr, &a.waves, &a.wave_bytes, &lds, &a.vals_global);
   if (rc) return rc;
-  rc = prep ? set_lds(k_pass1<true>, lds) : set_lds(k_pass1<false>, lds);
+  void (*kern)(P1Args) = prep ? (a.vals_global ? k_pass1<true, true> : k_pass1<true, false>)
+                               : (a.vals_global ? k_pass1<false, true> : k_pass1<false, false>);
+  rc = set_lds(kern, lds);
   if (rc) return rc;
   const int blocks = (refl->n + a.waves - 1) / a.waves;
-  if (prep) hipLaunchKernelGGL(k_pass1<true>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
-  else hipLaunchKernelGGL(k_pass1<false>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1397,13 +1436,14 @@ static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   a.nbg = nbg;
   a.out = *out;
   size_t lds;
-  rc = launch_shape(refl, 2, &a.waves, &a.wave_bytes, &lds);
+  rc = launch_shape(refl, 2, sb != nullptr, &a.waves, &a.wave_bytes, &lds, &a.vals_global);
   if (rc) return rc;
-  rc = prep ? set_lds(k_pass2<true>, lds) : set_lds(k_pass2<false>, lds);
+  void (*kern)(P2Args) = prep ? (a.vals_global ? k_pass2<true, true> : k_pass2<true, false>)
+                               : (a.vals_global ? k_pass2<false, true> : k_pass2<false, false>);
+  rc = set_lds(kern, lds);
   if (rc) return rc;
   const int blocks = (refl->n + a.waves - 1) / a.waves;
-  if (prep) hipLaunchKernelGGL(k_pass2<true>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
-  else hipLaunchKernelGGL(k_pass2<false>, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * a.waves), lds, (hipStream_t)stream, a);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
 }
@@ -1436,6 +1476,8 @@ int xpg_pass2_prepared(const xpg_detector *det, const xpg_call *call, const xpg_
   if (rc) return rc;
   return pass2_impl(det, call, refl, prep->sb, nullptr, prof, n_regions, bg, nbg, irls_rule, out, stream, prep);
 }
+
+int xpg_lds_limit(void) { return lds_limit(); }
 
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) {
   const size_t a = make_layout(Dz, Dy, Dx, 1).bytes, b = make_layout(Dz, Dy, Dx, 2).bytes;

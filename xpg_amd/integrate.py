@@ -230,24 +230,55 @@ P1_KEEP = ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "frac_rec", "
            "n_fg_nb", "nfg", "zeta", "seff")
 
 
+# LDS bytes per wavefront that separate the size bins of the batched form (a launch covers one bin of one block, with
+# mixed shoebox shapes; its wavefronts get the bin's largest slice)
+LDS_BINS = (8192, 12288, 16384, 24576, 32768, 49152, 65536, 98304, 163840)
+
+
 class SweepPlan:
-    """Everything the device needs for one integrate() call: the reflection table in processing order (blocks, then
-    shoebox-shape groups with overlapping-partner rows first, xpg/integrate.py:372-383), uploaded once."""
+    """Everything the device needs for one integrate() call: the reflection table in processing order, uploaded once.
+    Rows are ordered by phi block, then size bin (LDS bytes of the shoebox shape), then shape, with the rows that
+    have an overlapping partner first in each shape group (xpg/integrate.py:372-383).  ``groups`` are the
+    (block, shape) ranges (one launch each in the plain form), ``bins`` the (block, bin) ranges (one launch each in the
+    batched form, mixed shapes)."""
 
     def __init__(self, refl, iz0, fb, bucket, bxs, pxs, ix0, iy0, region, has_partner, pj, nb, blocks_refl, order,
                  device):
+        from . import _lib as L
         self.groups = []                 # (block, (B, bx, sp), start, n)
+        self.bins = []                   # (block, bin, start, n, (max B, max bx), LDS bytes per wavefront)
         perm = []
+        lds = {}
+
+        def lds_of(B, bx):
+            if (B, bx) not in lds:
+                lds[(B, bx)] = int(L.lib().xpg_lds_bytes(B, bx, bx))
+            return lds[(B, bx)]
+
+        bin_of = lambda k: int(np.searchsorted(LDS_BINS, lds_of(k[0], k[1])))
+
         for b in order:
             rows = blocks_refl[b]
             keys = sorted({(int(bucket[r]), int(bxs[r]), int(pxs[r])) for r in rows},
-                          key=lambda k: (DZ_BUCKETS.index(k[0]) if k[0] in DZ_BUCKETS else k[0], k[1], k[2]))
+                          key=lambda k: (bin_of(k), DZ_BUCKETS.index(k[0]) if k[0] in DZ_BUCKETS else k[0], k[1], k[2]))
+            cur = None
             for gk in keys:
                 sel = rows[(bucket[rows] == gk[0]) & (bxs[rows] == gk[1]) & (pxs[rows] == gk[2])]
                 sel = sel[np.argsort(~has_partner[sel], kind="stable")]
-                if sel.size:
-                    self.groups.append((b, gk, len(perm), int(sel.size)))
-                    perm.extend(sel.tolist())
+                if not sel.size:
+                    continue
+                bi = bin_of(gk)
+                if cur is None or cur[1] != bi:
+                    if cur is not None:
+                        self.bins.append(tuple(cur))
+                    cur = [b, bi, len(perm), 0, (0, 0), 0]
+                cur[3] += int(sel.size)
+                cur[4] = (max(cur[4][0], gk[0]), max(cur[4][1], gk[1]))
+                cur[5] = max(cur[5], lds_of(gk[0], gk[1]))
+                self.groups.append((b, gk, len(perm), int(sel.size)))
+                perm.extend(sel.tolist())
+            if cur is not None:
+                self.bins.append(tuple(cur))
         self.perm = np.asarray(perm, np.int64)
         p = self.perm
         T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
@@ -258,12 +289,28 @@ class SweepPlan:
                       pphi=T(np.radians(refl["phi"][pj[p]]), torch.float64))
         for k in ("x", "y", "z", "sx", "sz"):
             self.t["nb" + k] = T(nb[k][p], torch.float32)
+        # per-row shapes, and each row's voxel / pixel offset inside its bin's buffers
+        dzr, bxr = bucket[p].astype(np.int64), bxs[p].astype(np.int64)
+        Vr, Pr = dzr * bxr * bxr, bxr * bxr
+        voff = np.zeros(p.size, np.int64)
+        poff = np.zeros(p.size, np.int64)
+        self.bin_sizes = []
+        for b, bi, start, n, mx, wl in self.bins:
+            sl = slice(start, start + n)
+            voff[sl] = np.cumsum(Vr[sl]) - Vr[sl]
+            poff[sl] = np.cumsum(Pr[sl]) - Pr[sl]
+            self.bin_sizes.append((int(Vr[sl].sum()), int(Pr[sl].sum())))
+        self.rows_dz, self.rows_bx = dzr, bxr
+        self.t.update(rdz=T(dzr, torch.int32), rdx=T(bxr, torch.int32), voff=T(voff, torch.int64),
+                      poff=T(poff, torch.int64))
         self.K = int(nb["x"].shape[1])
         self.n = int(p.size)
         self.device = device
 
-    def refl_view(self, start, n, shape):
-        """ctypes table of rows [start, start + n) (pointer offsets into the uploaded table)."""
+    def refl_view(self, start, n, shape, mixed=False, wave_lds=0):
+        """ctypes table of rows [start, start + n) (pointer offsets into the uploaded table); ``mixed``: with the
+        per-row shapes and bin offsets (``shape`` then gives the maxima, ``wave_lds`` the LDS bytes per wavefront its
+        largest row needs)."""
         import ctypes as C
         from . import _lib as L
         s = L.Refl()
@@ -277,6 +324,12 @@ class SweepPlan:
         s.K = self.K
         for k in ("x", "y", "z", "sx", "sz"):
             setattr(s, "nb" + k, C.c_void_p(self.t["nb" + k].data_ptr() + 4 * self.K * start))
+        if mixed:
+            s.rdz = C.c_void_p(self.t["rdz"].data_ptr() + 4 * start)
+            s.rdy = s.rdx = C.c_void_p(self.t["rdx"].data_ptr() + 4 * start)
+            s.voff = C.c_void_p(self.t["voff"].data_ptr() + 8 * start)
+            s.poff = C.c_void_p(self.t["poff"].data_ptr() + 8 * start)
+            s.wave_lds = int(wave_lds)
         v = K.DeviceRefl.__new__(K.DeviceRefl)
         v.s, v.n, v.t = s, n, {}
         return v
@@ -285,7 +338,7 @@ class SweepPlan:
 def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, dmax=1e9, frames=None, mask=None,
                minimum_zeta=0.05, verbose=True, log=print, n_regions_xy=3, learn_profiles=True, exclude_frames=(),
                adaptive=True, deconvolve_overlaps=True, extra_lattices=None, bkg_template=None, foreign_spots=None,
-               n_foreign=4, moments_only=False, plan_only=False, timing=None):
+               n_foreign=4, moments_only=False, plan_only=False, timing=None, _box_draw=None):
     """Integrate a sweep (xpg/integrate.py:209-999).  ``images``: a DeviceFrames store, or an ImageSet-like object
     (first, last, nx, ny, iter_frames) whose frames are uploaded once.  ``plan_only`` returns the host plan without
     device work (planning tests); ``timing`` (dict) receives host / device times."""
@@ -311,11 +364,25 @@ def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, d
         log(f"  INTEGRATE: {n0} predicted reflections, box {box}x{box} px, sigma_D {model.sigma_d:.4f} "
             f"sigma_M {model.sigma_m:.4f} deg; {int(pl['wide'].sum())} wide in phi (> {max(DZ_BUCKETS)} frames): "
             f"integrated in bins of 2-{BIN_MAX} frames, {int(pl['trunc'].sum())} of them still truncated")
+    if _box_draw is not None:
+        keep = np.ones(n0, bool)
+        pl["coverage"] = np.ones(n0, np.float32)
     refl["coverage"] = pl["coverage"]
     refl = {k: v[keep] for k, v in refl.items()}
     iz0, dz, bucket, fb = (pl[k][keep] for k in ("iz0", "dz", "bucket", "fb"))
     n = refl["x"].size
-    bxs, pxs, ix0, iy0 = plan_sides(refl, g0, model, box, adaptive)
+    if _box_draw is None:
+        bxs, pxs, ix0, iy0 = plan_sides(refl, g0, model, box, adaptive)
+    else:
+        # a synthetic workload's drawn shapes (odd depth and side per reflection; no phi-binning, no super-pixels),
+        # placed as the fixed-size boxes are
+        dz, bxs = (np.asarray(a, int) for a in _box_draw(refl))
+        fb = np.ones(n, int)
+        pxs = np.ones(n, int)
+        iz0 = np.floor(refl["z"]).astype(int) - dz // 2
+        bucket = dz.copy()
+        ix0 = np.floor(refl["x"] - PIX0 + 0.5).astype(int) - bxs // 2
+        iy0 = np.floor(refl["y"] - PIX0 + 0.5).astype(int) - bxs // 2
     # neighbours for overlap masking (xpg/integrate.py:288-302)
     Xlen = np.linalg.norm(g0.pixel_to_lab(refl["x"], refl["y"]), axis=1)
     sx_px = np.maximum(Xlen * sd / g0.qx, 0.5) * N_FG
@@ -417,6 +484,13 @@ class Sweep:
         self.by_block = {}
         for gi, (b, gk, start, m) in enumerate(self.sp.groups):
             self.by_block.setdefault(b, []).append((gi, gk, start, m))
+        self.bins_by_block = {}
+        self.bin_of_group = {}
+        for bi, (b, lb, start, m, mx, wl) in enumerate(self.sp.bins):
+            self.bins_by_block.setdefault(b, []).append((bi, start, m, (mx[0], mx[1], mx[1]), wl))
+        for gi, (b, gk, start, m) in enumerate(self.sp.groups):
+            self.bin_of_group[gi] = next(bi for bi, (b2, lb, s2, m2, mx, wl) in enumerate(self.sp.bins)
+                                         if b2 == b and s2 <= start < s2 + m2)
         self.order = [b for b in hp["block_order"] if b in self.by_block]
         self.dets = {}
         self.t_plan = 0.0
@@ -450,15 +524,50 @@ class Sweep:
         return self.dets[key][1]
 
     def launches(self):
-        """(kernel, block, group index, n reflections, (Dz, Dy, Dx)) of every launch, in order."""
+        """(kernel, block, first row, n rows, (Dz, Dy, Dx) or its maxima) of every pass launch of the current form,
+        in order."""
         out = []
+        batched = self.o.get("form", "batched") == "batched"
         for b in self.order:
-            for gi, (B, bx, _), start, m in self.by_block[b]:
-                out.append(("pass1", b, gi, m, (B, bx, bx)))
-            if not self.o["moments_only"]:
-                for gi, (B, bx, _), start, m in self.by_block[b]:
-                    out.append(("pass2", b, gi, m, (B, bx, bx)))
+            units = ([(start, m, shape) for bi, start, m, shape, wl in self.bins_by_block[b]] if batched else
+                     [(start, m, (B, bx, bx)) for gi, (B, bx, _), start, m in self.by_block[b]])
+            for kind in ("pass1",) + (() if self.o["moments_only"] else ("pass2",)):
+                out += [(kind, b, start, m, shape) for start, m, shape in units]
         return out
+
+    def _prep_for(self, bi):
+        """Prepared-data buffers of bin launch bi (allocated on first use): values and neighbour masks at each row's
+        voxel offset, eps1 / eps2 / J (and the partner's) at three times its pixel offset, zeta per row."""
+        from . import _lib as L
+        if bi not in self.prep:
+            b, lb, start, m, mx, wl = self.sp.bins[bi]
+            nV, nP = self.sp.bin_sizes[bi]
+            hasp = bool(self.hp["has_partner"][self.sp.perm[start:start + m]].any())
+            dev = self.device
+            bufs = dict(sb=torch.empty(nV, dtype=torch.float32, device=dev),
+                        geo=torch.empty(3 * nP, dtype=torch.float64, device=dev),
+                        zeta=torch.empty(m, dtype=torch.float64, device=dev),
+                        nbm=torch.empty(nV, dtype=torch.uint8, device=dev),
+                        geo_p=torch.empty(3 * nP if hasp else 1, dtype=torch.float64, device=dev),
+                        zeta_p=torch.empty(m if hasp else 1, dtype=torch.float64, device=dev))
+            ps = L.Prepared()
+            for k, t in bufs.items():
+                setattr(ps, k, L.ptr(t))
+            self.prep[bi] = (ps, bufs)
+        return self.prep[bi][0]
+
+    def _prep_group(self, gi):
+        """The prepared data of shape group gi inside its bin's buffers (per-row arrays offset to the group's rows)."""
+        from . import _lib as L
+        bi = self.bin_of_group[gi]
+        ps = self._prep_for(bi)
+        d = self.sp.groups[gi][2] - self.sp.bins[bi][2]
+        g = L.Prepared()
+        for k in ("sb", "geo", "nbm", "geo_p"):
+            setattr(g, k, getattr(ps, k))
+        g.zeta = ps.zeta + 8 * d
+        g.zeta_p = ps.zeta_p + 8 * d if self.prep[bi][1]["zeta_p"].numel() > 1 else ps.zeta_p
+        return g
 
     def _widths_ptr(self, slot):
         return self.widths.data_ptr() + slot * self.wsz
@@ -509,26 +618,24 @@ class Sweep:
         o, sp, device, model = self.o, self.sp, self.device, self.model
         main = torch.cuda.current_stream(device)
         batched = o.get("form", "batched") == "batched"
+        if not hasattr(self, "prep"):
+            # prepared-data buffers per bin launch, allocated on first use: every bin in the batched form; in the plain
+            # form the bins of the shape groups too large for the fused kernels' LDS (gathered first, on the same
+            # stream)
+            lim = L.lib().xpg_lds_limit()
+            self.big = {gi for gi, (b, (B, bx, _), start, m) in enumerate(sp.groups)
+                        if L.lib().xpg_lds_bytes(B, bx, bx) > lim}
+            self.prep = {}
+            self.prep_g = {}
+        if batched:
+            for bi in range(len(sp.bins)):
+                self._prep_for(bi)
+        for gi in self.big:
+            if gi not in self.prep_g:
+                self.prep_g[gi] = self._prep_group(gi)
         if batched and not hasattr(self, "side"):
             self.side = torch.cuda.Stream(device)      # default priorities
             self.xs = torch.cuda.Stream(device)
-            self.prep = {}
-            hasp = self.hp["has_partner"][sp.perm]
-            for gi, (b, (B, bx, _), start, m) in enumerate(sp.groups):
-                # rows with an overlapping partner come first in a group: their partner geometry has rows [0, m_p)
-                m_p = int(hasp[start:start + m].sum())
-                if not hasp[start:start + m_p].all():
-                    raise XpgError("partner rows are not first in their shoebox group")
-                bufs = dict(sb=torch.empty((m, B, bx, bx), dtype=torch.float32, device=device),
-                            geo=torch.empty((m, 3, bx, bx), dtype=torch.float64, device=device),
-                            zeta=torch.empty(m, dtype=torch.float64, device=device),
-                            nbm=torch.empty((m, B, bx, bx), dtype=torch.uint8, device=device),
-                            geo_p=torch.empty((max(m_p, 1), 3, bx, bx), dtype=torch.float64, device=device),
-                            zeta_p=torch.empty(max(m_p, 1), dtype=torch.float64, device=device))
-                ps = L.Prepared()
-                for k, t in bufs.items():
-                    setattr(ps, k, L.ptr(t))
-                self.prep[gi] = (ps, bufs)
         side = self.side if batched else main
         N = sp.n
         f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
@@ -549,11 +656,11 @@ class Sweep:
                     g = self.exp_blocks[b].geom
                     det = self.det_for(g)
                     callp = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
-                    for gi, (B, bx, _), start, m in self.by_block[b]:
-                        with (timer("prepare", b, gi, self.xs) if timer else nullctx):
+                    for bi, start, m, shape, wl in self.bins_by_block[b]:
+                        with (timer("prepare", b, (start, m), self.xs) if timer else nullctx):
                             L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp),
-                                                        C.byref(sp.refl_view(start, m, (B, bx, bx)).s),
-                                                        C.byref(self.store), C.byref(self.prep[gi][0]),
+                                                        C.byref(sp.refl_view(start, m, shape, True, wl).s),
+                                                        C.byref(self.store), C.byref(self.prep[bi][0]),
                                                         L.stream_ptr(device)))
                     ready[b] = torch.cuda.Event()
                     ready[b].record(self.xs)
@@ -568,15 +675,29 @@ class Sweep:
             acc = None
             if o["learn_profiles"]:
                 num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
+            # launch units: (first row, rows, shape or shape maxima, mixed shapes, prepared data or None)
             if batched:
                 main.wait_event(ready[b])
-            for gi, (B, bx, _), start, m in groups:
-                view = sp.refl_view(start, m, (B, bx, bx))
+                units = [(start, m, shape, wl, self.prep[bi][0]) for bi, start, m, shape, wl in self.bins_by_block[b]]
+            else:
+                units = []
+                for gi, (B, bx, _), start, m in groups:
+                    big = gi in self.big
+                    units.append((start, m, (B, bx, bx), -1 if big else None, self.prep_g[gi] if big else None))
+                    if big:
+                        L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call1),
+                                                    C.byref(sp.refl_view(start, m, (B, bx, bx), mixed=True).s),
+                                                    C.byref(self.store), C.byref(self.prep_g[gi]), st))
+            # mixed: None for a view of one shape with its own row layout, else the LDS bytes per wavefront of a view
+            # with per-row shapes and bin offsets (-1: sized from the shape)
+            view_of = lambda start, m, shape, mixed: sp.refl_view(start, m, shape, mixed is not None,
+                                                                  max(mixed or 0, 0))
+            for start, m, shape, mixed, pp in units:
+                view = view_of(start, m, shape, mixed)
                 out = {k: v[start:start + m] for k, v in o1.items()}
                 out["fz"] = self.fz[start:start + m]
-                with (timer("pass1", b, gi, main) if timer else nullctx):
-                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS),
-                                  prep=self.prep[gi][0] if batched else None)
+                with (timer("pass1", b, (start, m), main) if timer else nullctx):
+                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS), prep=pp)
             s0 = groups[0][2]
             n_b = sum(m for *_, m in groups)
             if o["adaptive"]:
@@ -599,12 +720,12 @@ class Sweep:
             if batched:
                 side.wait_stream(main)
             with torch.cuda.stream(side):
-                for gi, (B, bx, _), start, m in groups:
-                    view = sp.refl_view(start, m, (B, bx, bx))
+                for start, m, shape, mixed, pp in units:
+                    view = view_of(start, m, shape, mixed)
                     out = {k: v[start:start + m] for k, v in o2.items()}
-                    with (timer("pass2", b, gi, side) if timer else nullctx):
+                    with (timer("pass2", b, (start, m), side) if timer else nullctx):
                         _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
-                                      o1["nbg"][start:start + m], out, prep=self.prep[gi][0] if batched else None)
+                                      o1["nbg"][start:start + m], out, prep=pp)
         if batched:
             main.wait_stream(side)
         return dict(o1=o1, o2=o2, prof_by_block=prof_by_block)

@@ -63,6 +63,7 @@ class SynthSpec:
     peak_range: tuple | None = None   # (lo, hi): spot peak counts log-uniform (overrides Wilson intensities)
     box_xy: int = 0
     box_z: int = 0
+    box_draw: tuple | None = None     # ((side lo, hi), (depth lo, hi)): per-reflection boxes log-uniform, odd
     block_frames: int = 50
     seed: int = 1
     dmin: float | None = None
@@ -95,7 +96,33 @@ def spec_uneven():
     """The full-size stream with sigma_M per detector region uniform in 0.05-0.8 deg (uneven shoebox sizes, wide
     reflections binned in phi, partials over frame blocks); boxes from the reference's own rules."""
     return replace(spec_full(), name="uneven_4148x4362x1800", sigma_m_range=(0.05, 0.8), box_xy=0, box_z=0,
-                   seed=1003)
+                   box_draw=((5, 21), (3, 15)), seed=1003)
+
+
+def _odd_log_uniform(u, lo, hi):
+    """Odd integers in [lo, hi] (lo, hi odd) from uniforms u: log-uniform over [lo, hi], rounded to the nearest odd
+    (so E[v] before rounding is (hi - lo) / ln(hi / lo))."""
+    v = np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * u)
+    return np.clip(2 * np.floor((v - 1.0) / 2.0 + 0.5) + 1, lo, hi).astype(int)
+
+
+def draw_boxes(spec: SynthSpec, refl):
+    """Per-reflection shoebox (depth, side) of a workload with drawn boxes: side and depth independent, log-uniform
+    over spec.box_draw's ranges and odd, keyed on the Miller index (the same reflection gets the same box in every
+    run, whatever else is predicted)."""
+    (s_lo, s_hi), (d_lo, d_hi) = spec.box_draw
+    h = np.asarray(refl["hkl"]).astype(np.int64)
+    key = ((h[:, 0] + 512) * 1048576 + (h[:, 1] + 512) * 1024 + (h[:, 2] + 512)).astype(np.uint64)
+    side = _odd_log_uniform(_hash_uniform(key, spec.seed + 11), s_lo, s_hi)
+    depth = _odd_log_uniform(_hash_uniform(key, spec.seed + 12), d_lo, d_hi)
+    return depth, side
+
+
+def box_draw_for(spec: SynthSpec):
+    """The per-reflection box override of spec (a callable of the prediction table), or None."""
+    if spec.box_draw is None:
+        return None
+    return lambda refl: draw_boxes(spec, refl)
 
 
 def spec_stress():
