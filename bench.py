@@ -35,14 +35,48 @@ def log(msg):
 
 
 def per_refl_bytes(V, Dz, K, kind):
-    """Algorithmic HBM bytes of one reflection in one launch: its shoebox voxels read once from the frame store as
-    uint32 (4 V), its table row (S, pS: 48; phi, pphi: 16; ix0, iy0, iz0, fb, pxs, region, has_p: 28; neighbour
-    columns 20 K), and its outputs (pass 1: 14 float64 + 3 int32 + fz 4 Dz; pass 2: 9 float64; pass 2 also reads
-    bg, nbg: 16)."""
-    table = 48 + 16 + 28 + 20 * K
-    if kind == "pass1":
-        return 4 * V + table + 14 * 8 + 3 * 4 + 4 * Dz
-    return 4 * V + table + 16 + 9 * 8
+    """Algorithmic HBM bytes of one reflection in one pass launch: half of the per-reflection figure 8 V + 160 B of
+    the step's byte count below (each pass reads the shoebox's V voxels once as 4-byte values, plus 80 B of
+    parameters in and results out)."""
+    return 4 * V + 80
+
+
+def step_bytes(hp, nframes, ny, nx, device):
+    """Algorithmic bytes of one whole step.  B_alg = 4 nx ny N_frames + sum over reflections (8 V + 160): every frame
+    pixel read once as uint32 plus two passes over each shoebox.  This design reads only shoebox voxels, so its bound
+    is B_min = 4 U + sum (4 V + 160), U = the exact union of shoebox voxels (frame x pixel) on the detector and in
+    the frame range, counted on the device with a bitmap per window of frames."""
+    import numpy as np
+    import torch
+    dz, fb, pxs, bx = (np.asarray(hp[k], np.int64) for k in ("dz", "fb", "pxs", "bxs"))
+    nz, side = dz * fb, bx * pxs                        # frames and detector pixels a box spans
+    z0 = np.asarray(hp["iz0"], np.int64)
+    y0 = np.asarray(hp["iy0"], np.int64) * pxs
+    x0 = np.asarray(hp["ix0"], np.int64) * pxs
+    V = dz * bx * bx
+    W = 100
+    U = 0
+    for w0 in range(0, nframes, W):
+        w1 = min(w0 + W, nframes)
+        bm = torch.zeros((w1 - w0) * ny * nx, dtype=torch.bool, device=device)
+        hit = (z0 < w1) & (z0 + nz > w0)
+        for a, b_ in sorted({(int(p), int(q)) for p, q in zip(nz[hit], side[hit])}):
+            sel = np.nonzero(hit & (nz == a) & (side == b_))[0]
+            for c0 in range(0, sel.size, 20000):
+                rows = sel[c0:c0 + 20000]
+                T = lambda v: torch.as_tensor(v[rows], device=device)
+                z = T(z0)[:, None, None, None] + torch.arange(a, device=device)[None, :, None, None]
+                y = T(y0)[:, None, None, None] + torch.arange(b_, device=device)[None, None, :, None]
+                x = T(x0)[:, None, None, None] + torch.arange(b_, device=device)[None, None, None, :]
+                ok = (z >= w0) & (z < w1) & (y >= 0) & (y < ny) & (x >= 0) & (x < nx)
+                idx = ((z - w0) * ny + y) * nx + x
+                bm[idx[ok]] = True
+        U += int(bm.sum().item())
+        del bm
+    n = V.size
+    b_alg = 4 * nx * ny * nframes + int(np.sum(8 * V)) + 160 * n
+    b_min = 4 * U + int(np.sum(4 * V)) + 160 * n
+    return dict(B_alg=b_alg, B_min=b_min, union_voxels=U, sum_V=int(V.sum()), reflections=n)
 
 
 def cpu_baseline(sweep, hp, spec, n_sample, seconds):
@@ -163,15 +197,18 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
         return {k: list(v.values()) for k, v in vals.items()}
 
     fetch, write = per_kernel(fetch_csv, "FETCH_SIZE"), per_kernel(write_csv, "WRITE_SIZE")
+    import re
     calib = next(k for k in fetch if "neg" in k.lower())
     nbytes = copy_mib * 2 ** 20
     f_fac = statistics.mean(fetch[calib]) * 1024 / nbytes
     w_fac = statistics.mean(write[calib]) * 1024 / nbytes
-    kname = next(k for k in fetch if k.startswith(kernel + "("))
-    f_b = statistics.mean(fetch[kname]) * 1024 / f_fac
-    w_b = statistics.mean(write[kname]) * 1024 / w_fac
+    pat = re.compile(r"\b" + re.escape(kernel) + r"\b")
+    fk = [v for k, vs in fetch.items() if pat.search(k) for v in vs]
+    wk = [v for k, vs in write.items() if pat.search(k) for v in vs]
+    f_b = statistics.mean(fk) * 1024 / f_fac
+    w_b = statistics.mean(wk) * 1024 / w_fac
     rec = dict(kernel=kernel, workload=workload, bytes_per_launch=round(f_b + w_b), fetch_bytes=round(f_b),
-               write_bytes=round(w_b), dispatches=len(fetch[kname]), copy_calibration=dict(
+               write_bytes=round(w_b), dispatches=len(fk), copy_calibration=dict(
                    mib=copy_mib, fetch_factor=round(f_fac, 4), write_factor=round(w_fac, 4)), command=command)
     base = ROOT / "profiles" / f"traffic_{kernel[2:] if kernel.startswith('k_') else kernel}_{workload}"
     base.with_suffix(".json").write_text(json.dumps(rec, indent=1) + "\n")
@@ -179,7 +216,7 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
         f"# HBM traffic of {kernel} ({workload})\n\nCommand (each counter in its own run): `{command}`\n\n"
         f"- FETCH_SIZE per dispatch: {f_b / 1e6:.2f} MB (raw / copy factor {f_fac:.4f})\n"
         f"- WRITE_SIZE per dispatch: {w_b / 1e6:.2f} MB (raw / copy factor {w_fac:.4f})\n"
-        f"- dispatches: {len(fetch[kname])}\n- calibration: one copy-like kernel over {copy_mib} MiB read and "
+        f"- dispatches: {len(fk)}\n- calibration: one copy-like kernel over {copy_mib} MiB read and "
         f"written (`--calibrate-copy {copy_mib}`), counters read {f_fac:.4f} x and {w_fac:.4f} x the known bytes\n")
     return rec
 
@@ -364,6 +401,16 @@ def main():
                 alg_bytes_per_launch=round(avg_bytes), launches_per_step=dd["n"] // n_timed,
                 kernels={k: dict(ms_per_step=round(v["ms"] / n_timed, 3),
                                  GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)) for k, v in per.items()})
+
+    # the whole step against the HBM roofline: B_alg (every frame pixel once + two passes per shoebox) and the bound of
+    # a design that reads only shoebox voxels, B_min (their exact union once + two passes)
+    sb_ = step_bytes(hp, spec.nframes, spec.ny, spec.nx, dev)
+    roof["step"] = dict(B_alg_GB=round(sb_["B_alg"] / 1e9, 3), B_min_GB=round(sb_["B_min"] / 1e9, 3),
+                        union_voxels=sb_["union_voxels"], sum_V=sb_["sum_V"],
+                        frac_B_alg=round(sb_["B_alg"] / t_step / (PEAK_HBM_GBS * 1e9), 4),
+                        frac_B_min=round(sb_["B_min"] / t_step / (PEAK_HBM_GBS * 1e9), 4),
+                        t_roof_B_min_ms=round(sb_["B_min"] / (PEAK_HBM_GBS * 1e9) * 1e3, 3))
+    log(f"step bytes {roof['step']}")
 
     cpu, parity = (None, None)
     if not args.no_cpu:
