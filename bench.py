@@ -221,12 +221,83 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
     return rec
 
 
+def bench_serve(args, dev):
+    """Serving workload: datasets arriving as a Poisson process at args.serve_load of the measured capacity, frames
+    generated on the device per 10-frame batch, continuous batching under a 200 GB HBM budget; p50 / p99 per-batch
+    latency and aggregate reflections/s."""
+    import concurrent.futures as cf
+    import multiprocessing as mp
+
+    import numpy as np
+    import torch
+    from xpg_amd import serve
+
+    torch.cuda.set_device(dev)
+    reqs = serve.serving_requests(n=args.serve_datasets, frame_scale=args.serve_frame_scale)
+    t = time.time()
+    # host plans (predictions, boxes, neighbours, block order, the generator's spot table) are made when a request
+    # is submitted, before its frames arrive: worker processes, outside the served window
+    with cf.ProcessPoolExecutor(max_workers=min(16, os.cpu_count() or 1), mp_context=mp.get_context("spawn")) as ex:
+        plans = list(ex.map(serve.plan_request, reqs))
+    n_refl = sum(p.hp["refl"]["x"].size for p in plans)
+    frames = sum(p.req.spec.nframes for p in plans)
+    fbytes = sum(4 * p.req.spec.nx * p.req.spec.ny * p.req.spec.nframes for p in plans)
+    log(f"planned {len(plans)} datasets, {frames} frames ({fbytes / 1e9:.1f} GB), {n_refl} reflections in "
+        f"{time.time() - t:.1f}s")
+    server = serve.Server(dev, hbm_budget=200e9, slots=args.serve_slots)
+    small = min(plans, key=lambda p: p.req.spec.nframes * p.req.spec.nx * p.req.spec.ny)
+    server.run([small])                                          # warm-up
+    for p in plans:
+        p.req.arrival = 0.0
+    cap = server.run(plans)                                      # capacity: closed loop, every request at t = 0
+    log(f"capacity run: {cap['wall_s']:.2f}s, {cap['reflections_per_s']:.0f} reflections/s, p50 {cap['p50_ms']} ms "
+        f"p99 {cap['p99_ms']} ms; host {cap['host_s']}")
+    rate = args.serve_load * len(plans) / cap["wall_s"]
+    serve.poisson_arrivals([p.req for p in plans], rate)
+    rep = server.run(plans)
+    log(f"served: {rep['wall_s']:.2f}s, {rep['reflections_per_s']:.0f} reflections/s, p50 {rep['p50_ms']} ms "
+        f"p99 {rep['p99_ms']} ms; host {rep['host_s']}")
+    b_alg = 0
+    for p in plans:
+        V = p.hp["dz"] * p.hp["bxs"] ** 2
+        b_alg += 4 * p.req.spec.nx * p.req.spec.ny * p.req.spec.nframes + int(np.sum(8 * V)) + 160 * V.size
+    per = rep.pop("per_dataset")
+    out = dict(metric="profile-fitted reflections/s and p50/p99 per-batch latency, serving on 1 MI355X",
+               value=rep["reflections_per_s"], unit="reflections/s", n_gpus=1, steps=1, warmup=1,
+               ms_per_step=round(rep["wall_s"] * 1e3, 3), higher_is_better=True, scaling="weak", vs_baseline=None,
+               dtype="f32", data="synthetic (frames generated on the device per batch)",
+               config=dict(workload=f"serve: {len(plans)} datasets, frame counts U{{50..3600}} x "
+                                    f"{args.serve_frame_scale}, 10-frame batches, 200 GB HBM budget, "
+                                    f"{args.serve_slots} slots",
+                           frames=frames, frame_gb=round(fbytes / 1e9, 2), reflections=int(n_refl),
+                           parallelism="single GPU"),
+               host_s=rep["host_s"],
+               latency=dict(p50_ms=rep["p50_ms"], p99_ms=rep["p99_ms"], max_ms=rep["max_ms"],
+                            batches=rep["batches_with_results"],
+                            definition="batch frames resident on the device -> every reflection the batch completes "
+                                       "integrated (HIP events on the dataset's stream)"),
+               capacity=dict(wall_s=cap["wall_s"], reflections_per_s=cap["reflections_per_s"],
+                             datasets_per_s=round(len(plans) / cap["wall_s"], 3), p50_ms=cap["p50_ms"],
+                             p99_ms=cap["p99_ms"]),
+               offered=dict(load=args.serve_load, arrivals_per_s=round(rate, 3)),
+               roofline=dict(bound="hbm", B_alg_GB=round(b_alg / 1e9, 3), peak=PEAK_HBM_GBS, unit="GB/s",
+                             achieved=round(b_alg / rep["wall_s"] / 1e9, 1),
+                             frac=round(b_alg / rep["wall_s"] / (PEAK_HBM_GBS * 1e9), 4)),
+               per_dataset=per)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve"])
+    ap.add_argument("--serve-datasets", type=int, default=32)
+    ap.add_argument("--serve-frame-scale", type=float, default=0.1,
+                    help="serving workload: frame counts U{50..3600} times this factor (at least 20 frames)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=0.8, help="offered load as a fraction of measured capacity")
     ap.add_argument("--frames", type=int, default=0, help="override the number of frames (smaller runs)")
     ap.add_argument("--cpu-sample", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -260,6 +331,9 @@ def main():
     from xpg_amd.integrate import IntegrationModel, Sweep, _integrate
 
     dev = torch.device("cuda:0")
+    if args.workload == "serve":
+        bench_serve(args, dev)
+        return
     torch.cuda.set_device(dev)
     if args.calibrate_copy:
         src = torch.ones(args.calibrate_copy * 2 ** 18, dtype=torch.float32, device=dev)

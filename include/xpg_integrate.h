@@ -121,6 +121,8 @@ typedef struct xpg_frames {
   uint32_t min_valid, flag_value, overload;
   int32_t use_flag_value;
   int32_t dtype;
+  int32_t ring;                /* > 0: a ring of `ring` frame slots, frame frame0 + t in slot t % ring (the caller keeps
+                                * every frame a launch reads resident); 0: frame frame0 + t in slot t */
 } xpg_frames;
 
 /* ---- entry points ------------------------------------------------------------------------------------------- */

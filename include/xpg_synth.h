@@ -35,6 +35,8 @@ typedef struct xpg_synth_spots {
   int32_t nx, ny, f0, nf, n, half;
   const double *x, *y, *z, *counts, *sig_r, *sig_t, *sig_z, *rdx, *rdy;
   uint32_t flag_value, sat_value;
+  const int32_t *sid;   /* each spot's index in its dataset's spot table, keying its random draws (NULL: 0 .. n-1),
+                         * so frames do not depend on which frames a launch renders */
 } xpg_synth_spots;
 
 int xpg_synth_spots_add(const xpg_synth_spots *p, uint32_t *frames, void *stream);

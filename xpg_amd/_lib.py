@@ -73,7 +73,7 @@ class Frames(C.Structure):
                 ("frame0", C.c_int32), ("n_frames", C.c_int32), ("available", C.c_void_p),
                 ("src_mask", C.c_void_p), ("ceil_mask", C.c_void_p), ("min_valid", C.c_uint32),
                 ("flag_value", C.c_uint32), ("overload", C.c_uint32), ("use_flag_value", C.c_int32),
-                ("dtype", C.c_int32)]
+                ("dtype", C.c_int32), ("ring", C.c_int32)]
 
 
 class Prepared(C.Structure):

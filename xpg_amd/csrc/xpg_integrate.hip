@@ -135,10 +135,13 @@ __device__ __forceinline__ float pixel_value(const xpg_frames &F, int64_t t, int
   return sum;
 }
 
+// frame fi recorded and readable: t = its slot in the store
 __device__ __forceinline__ bool frame_read(const xpg_frames &F, int fi, int64_t &t) {
   t = (int64_t)fi - F.frame0;
   if (t < 0 || t >= F.n_frames) return false;
-  return F.available == nullptr || F.available[t] != 0;
+  if (F.available != nullptr && F.available[t] == 0) return false;
+  if (F.ring > 0) t %= F.ring;
+  return true;
 }
 
 // voxel (j, sy, sx): unbinned frames are read or unrecorded (-3); phi-bins of fb frames are summed with the
@@ -1258,8 +1261,8 @@ int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void 
   if (rc) return rc;
   if (!frames || !frames->data || !sb) return fail(XPG_ERR_ARG, "frames / sb is NULL");
   if (frames->nx <= 0 || frames->ny <= 0 || frames->frame_stride < (int64_t)frames->nx * frames->ny ||
-      (frames->dtype != 0 && frames->dtype != 1))
-    return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
+      (frames->dtype != 0 && frames->dtype != 1) || frames->ring < 0)
+    return fail(XPG_ERR_ARG, "bad frame store shape, dtype or ring");
   if (refl->n == 0) return XPG_OK;
   const int lim = lds_limit();
   size_t wb = extract_wave_bytes(refl->Dz, refl->Dy, refl->Dx);
@@ -1287,8 +1290,8 @@ int xpg_decode_bins(float *arr, const int16_t *cnt, int64_t n_rows, int32_t B, i
 static int check_frames(const xpg_frames *frames) {
   if (!frames || !frames->data) return fail(XPG_ERR_ARG, "frames is NULL");
   if (frames->nx <= 0 || frames->ny <= 0 || frames->frame_stride < (int64_t)frames->nx * frames->ny ||
-      (frames->dtype != 0 && frames->dtype != 1))
-    return fail(XPG_ERR_ARG, "bad frame store shape or dtype");
+      (frames->dtype != 0 && frames->dtype != 1) || frames->ring < 0)
+    return fail(XPG_ERR_ARG, "bad frame store shape, dtype or ring");
   return XPG_OK;
 }
 

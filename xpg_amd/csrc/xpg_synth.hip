@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(256) k_spots(xpg_synth_spots p, uint32_t *fram
     const int64_t o = (int64_t)(fi - p.f0) * npix + (int64_t)iy * p.nx + ix;
     const uint32_t cur = frames[o];
     if (cur == p.flag_value || cur == p.sat_value) continue;
-    const uint32_t k = poisson(mu, key3(p.seed ^ 0x1234567ull, (uint64_t)r, ((uint64_t)fi << 32) | (uint32_t)pp));
+    const uint64_t id = p.sid ? (uint64_t)p.sid[r] : (uint64_t)r;
+    const uint32_t k = poisson(mu, key3(p.seed ^ 0x1234567ull, id, ((uint64_t)fi << 32) | (uint32_t)pp));
     if (k) atomicAdd(frames + o, k);
   }
 }

@@ -259,7 +259,8 @@ class SweepPlan:
 
         for b in order:
             rows = blocks_refl[b]
-            keys = sorted({(int(bucket[r]), int(bxs[r]), int(pxs[r])) for r in rows},
+            ck = np.unique((bucket[rows].astype(np.int64) * 4096 + bxs[rows]) * 4096 + pxs[rows])
+            keys = sorted(((int(k // 4096 ** 2), int(k // 4096 % 4096), int(k % 4096)) for k in ck),
                           key=lambda k: (bin_of(k), DZ_BUCKETS.index(k[0]) if k[0] in DZ_BUCKETS else k[0], k[1], k[2]))
             cur = None
             for gk in keys:
@@ -281,14 +282,14 @@ class SweepPlan:
                 self.bins.append(tuple(cur))
         self.perm = np.asarray(perm, np.int64)
         p = self.perm
-        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
-        self.t = dict(S=T(refl["S"][p], torch.float64), phi=T(np.radians(refl["phi"][p]), torch.float64),
-                      ix0=T(ix0[p], torch.int32), iy0=T(iy0[p], torch.int32), iz0=T(iz0[p], torch.int32),
-                      fb=T(fb[p], torch.int32), pxs=T(pxs[p], torch.int32), region=T(region[p], torch.int32),
-                      has_p=T(has_partner[p], torch.float32), pS=T(refl["S"][pj[p]], torch.float64),
-                      pphi=T(np.radians(refl["phi"][pj[p]]), torch.float64))
+        T = lambda a, dt: np.ascontiguousarray(a, dtype=dt)
+        self.h = dict(S=T(refl["S"][p], np.float64), phi=T(np.radians(refl["phi"][p]), np.float64),
+                      ix0=T(ix0[p], np.int32), iy0=T(iy0[p], np.int32), iz0=T(iz0[p], np.int32),
+                      fb=T(fb[p], np.int32), pxs=T(pxs[p], np.int32), region=T(region[p], np.int32),
+                      has_p=T(has_partner[p], np.float32), pS=T(refl["S"][pj[p]], np.float64),
+                      pphi=T(np.radians(refl["phi"][pj[p]]), np.float64))
         for k in ("x", "y", "z", "sx", "sz"):
-            self.t["nb" + k] = T(nb[k][p], torch.float32)
+            self.h["nb" + k] = T(nb[k][p], np.float32)
         # per-row shapes, and each row's voxel / pixel offset inside its bin's buffers
         dzr, bxr = bucket[p].astype(np.int64), bxs[p].astype(np.int64)
         Vr, Pr = dzr * bxr * bxr, bxr * bxr
@@ -300,12 +301,55 @@ class SweepPlan:
             voff[sl] = np.cumsum(Vr[sl]) - Vr[sl]
             poff[sl] = np.cumsum(Pr[sl]) - Pr[sl]
             self.bin_sizes.append((int(Vr[sl].sum()), int(Pr[sl].sum())))
+        # each bin's voxel / pixel / row offset inside its block (bins of a block back to back), and the largest block
+        self.bin_arena = []
+        acc = {}
+        for bi, (b, lb, start, n, mx, wl) in enumerate(self.bins):
+            v0, p0, r0 = acc.get(b, (0, 0, 0))
+            self.bin_arena.append((v0, p0, r0))
+            nV, nP = self.bin_sizes[bi]
+            acc[b] = (v0 + nV, p0 + nP, r0 + n)
+        self.block_arena = tuple(max((t[k] for t in acc.values()), default=0) for k in range(3))
         self.rows_dz, self.rows_bx = dzr, bxr
-        self.t.update(rdz=T(dzr, torch.int32), rdx=T(bxr, torch.int32), voff=T(voff, torch.int64),
-                      poff=T(poff, torch.int64))
+        self.h.update(rdz=T(dzr, np.int32), rdx=T(bxr, np.int32), voff=T(voff, np.int64), poff=T(poff, np.int64))
         self.K = int(nb["x"].shape[1])
         self.n = int(p.size)
-        self.device = device
+        self.device = None
+        self.t = {}
+        if device is not None:
+            self.t = {k: torch.as_tensor(v, device=device) for k, v in self.h.items()}
+            self.device = device
+
+    def add_fit_inputs(self, refl, exp_blocks):
+        """Per-row inputs of the device adaptive fits, in table order: phi, zeta, |X| and |cos| of the detector
+        normal angle of each reflection (host arrays; uploaded by to())."""
+        perm = self.perm
+        Xlen = np.empty(perm.size)
+        cosa = np.empty(perm.size)
+        starts = {}
+        for b, gk, start, m in self.groups:
+            s0, n0 = starts.get(b, (start, 0))
+            starts[b] = (min(s0, start), n0 + m)
+        for b, (s0, n_b) in starts.items():
+            g = exp_blocks[b].geom
+            rows = perm[s0:s0 + n_b]
+            Xv = g.pixel_to_lab(refl["x"][rows], refl["y"][rows])
+            Xlen[s0:s0 + n_b] = np.linalg.norm(Xv, axis=1)
+            cosa[s0:s0 + n_b] = np.abs(Xv @ g.ed3) / Xlen[s0:s0 + n_b]
+        self.h.update(ad_phi=np.ascontiguousarray(refl["phi"][perm], np.float64),
+                      ad_zeta_k=np.ascontiguousarray(refl["zeta_k"][perm], np.float64), ad_Xlen=Xlen, ad_cosa=cosa)
+        if self.device is not None:
+            for k in ("ad_phi", "ad_zeta_k", "ad_Xlen", "ad_cosa"):
+                self.t[k] = torch.as_tensor(self.h[k], device=self.device)
+        return self
+
+    def to(self, device):
+        """A copy of the plan with its table uploaded to ``device`` (the host arrays are shared)."""
+        import copy
+        c = copy.copy(self)
+        c.t = {k: torch.as_tensor(v, device=device) for k, v in self.h.items()}
+        c.device = device
+        return c
 
     def refl_view(self, start, n, shape, mixed=False, wave_lds=0):
         """ctypes table of rows [start, start + n) (pointer offsets into the uploaded table); ``mixed``: with the
@@ -451,6 +495,14 @@ def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, d
     return res
 
 
+def sweep_plan(hp, exp_blocks):
+    """The host part of a sweep's device table (SweepPlan with the fits' inputs), from a host plan of
+    _integrate(plan_only=True); a Sweep uploads it."""
+    return SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"], hp["iy0"],
+                     hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"], hp["block_order"],
+                     None).add_fit_inputs(hp["refl"], exp_blocks)
+
+
 class Sweep:
     """A planned sweep on the device: the reflection table uploaded once (SweepPlan), the frame store view, and the
     per-block loop of pass 1 -> adaptive widths -> profile normalisation -> pass 2 (xpg/integrate.py:463-574), which
@@ -478,9 +530,10 @@ class Sweep:
         src = images if isinstance(images, DeviceFrames) else from_imageset(images, opts["frames"], device)
         self.frames_src = src
         self.store = src.store(mask=opts["mask"], exclude_frames=opts["exclude_frames"], frames=opts["frames"])
-        self.sp = SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"],
-                            hp["iy0"], hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"],
-                            hp["block_order"], device)
+        sp = hp.get("sweep_plan")          # built ahead on the host (sweep_plan())
+        if sp is None:
+            sp = sweep_plan(hp, exp_blocks)
+        self.sp = sp.to(device)
         self.by_block = {}
         for gi, (b, gk, start, m) in enumerate(self.sp.groups):
             self.by_block.setdefault(b, []).append((gi, gk, start, m))
@@ -494,21 +547,8 @@ class Sweep:
         self.order = [b for b in hp["block_order"] if b in self.by_block]
         self.dets = {}
         self.t_plan = 0.0
-        # per-row inputs of the device adaptive fits, in table order
-        refl, perm = hp["refl"], self.sp.perm
-        Xlen = np.empty(perm.size)
-        cosa = np.empty(perm.size)
-        for b in self.order:
-            g = exp_blocks[b].geom
-            s0 = self.by_block[b][0][2]
-            n_b = sum(m for *_, m in self.by_block[b])
-            rows = perm[s0:s0 + n_b]
-            Xv = g.pixel_to_lab(refl["x"][rows], refl["y"][rows])
-            Xlen[s0:s0 + n_b] = np.linalg.norm(Xv, axis=1)
-            cosa[s0:s0 + n_b] = np.abs(Xv @ g.ed3) / Xlen[s0:s0 + n_b]
-        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)
-        self.ad = dict(phi=T(refl["phi"][perm], torch.float64), zeta_k=T(refl["zeta_k"][perm], torch.float64),
-                       Xlen=T(Xlen, torch.float64), cosa=T(cosa, torch.float64))
+        t = self.sp.t
+        self.ad = dict(phi=t["ad_phi"], zeta_k=t["ad_zeta_k"], Xlen=t["ad_Xlen"], cosa=t["ad_cosa"])
         nmax = max([sum(m for *_, m in self.by_block[b]) for b in self.order] or [1])
         self.ws_bytes = int(L.lib().xpg_adapt_workspace_bytes(int(nmax)))
         self.ws = torch.empty(self.ws_bytes, dtype=torch.uint8, device=device)
@@ -611,13 +651,9 @@ class Sweep:
         torch.cuda.synchronize(self.device)
         return dict(self.graph_out, t_gpu=time.time() - t_dev0)
 
-    def enqueue(self, timer=None):
-        """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
-        and no host-to-device copy."""
+    def _setup(self, batched):
         from . import _lib as L
-        o, sp, device, model = self.o, self.sp, self.device, self.model
-        main = torch.cuda.current_stream(device)
-        batched = o.get("form", "batched") == "batched"
+        sp = self.sp
         if not hasattr(self, "prep"):
             # prepared-data buffers per bin launch, allocated on first use: every bin in the batched form; in the plain
             # form the bins of the shape groups too large for the fused kernels' LDS (gathered first, on the same
@@ -633,16 +669,29 @@ class Sweep:
         for gi in self.big:
             if gi not in self.prep_g:
                 self.prep_g[gi] = self._prep_group(gi)
+
+    def alloc_outputs(self):
+        """Per-reflection output tensors of one run (NaN until written)."""
+        N, device = self.sp.n, self.device
+        f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
+        o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
+        o2 = {k: f64() for k in K.P2_F64}
+        return o1, o2
+
+    def enqueue(self, timer=None):
+        """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
+        and no host-to-device copy."""
+        from . import _lib as L
+        o, sp, device, model = self.o, self.sp, self.device, self.model
+        main = torch.cuda.current_stream(device)
+        batched = o.get("form", "batched") == "batched"
+        self._setup(batched)
         if batched and not hasattr(self, "side"):
             self.side = torch.cuda.Stream(device)      # default priorities
             self.xs = torch.cuda.Stream(device)
         side = self.side if batched else main
-        N = sp.n
-        f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
-        o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
-        o2 = {k: f64() for k in K.P2_F64}
+        o1, o2 = self.alloc_outputs()
         prof_by_block = {}
-        nreg = o["nreg"]
         self._init_widths()
         nullctx = _Null()
         st = L.stream_ptr(device)
@@ -665,70 +714,129 @@ class Sweep:
                     ready[b] = torch.cuda.Event()
                     ready[b].record(self.xs)
         for i, b in enumerate(self.order):
-            groups = self.by_block[b]
             g = self.exp_blocks[b].geom
             det = self.det_for(g)
-            frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
-            osc = float(np.radians(g.osc))
-            call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
-            call1.widths = C.c_void_p(self._widths_ptr(2 * i))
-            acc = None
-            if o["learn_profiles"]:
-                num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
-            # launch units: (first row, rows, shape or shape maxima, mixed shapes, prepared data or None)
+            # launch units: (first row, rows, shape or shape maxima, mixed, prepared data or None); mixed: None for a
+            # view of one shape with its own row layout, else the LDS bytes per wavefront of a view with per-row
+            # shapes and bin offsets (-1: sized from the shape)
             if batched:
                 main.wait_event(ready[b])
                 units = [(start, m, shape, wl, self.prep[bi][0]) for bi, start, m, shape, wl in self.bins_by_block[b]]
             else:
                 units = []
-                for gi, (B, bx, _), start, m in groups:
+                for gi, (B, bx, _), start, m in self.by_block[b]:
                     big = gi in self.big
                     units.append((start, m, (B, bx, bx), -1 if big else None, self.prep_g[gi] if big else None))
                     if big:
-                        L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call1),
+                        call0 = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
+                        L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call0),
                                                     C.byref(sp.refl_view(start, m, (B, bx, bx), mixed=True).s),
                                                     C.byref(self.store), C.byref(self.prep_g[gi]), st))
-            # mixed: None for a view of one shape with its own row layout, else the LDS bytes per wavefront of a view
-            # with per-row shapes and bin offsets (-1: sized from the shape)
-            view_of = lambda start, m, shape, mixed: sp.refl_view(start, m, shape, mixed is not None,
-                                                                  max(mixed or 0, 0))
-            for start, m, shape, mixed, pp in units:
-                view = view_of(start, m, shape, mixed)
-                out = {k: v[start:start + m] for k, v in o1.items()}
-                out["fz"] = self.fz[start:start + m]
-                with (timer("pass1", b, (start, m), main) if timer else nullctx):
-                    _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS), prep=pp)
-            s0 = groups[0][2]
-            n_b = sum(m for *_, m in groups)
-            if o["adaptive"]:
-                a = lambda t: C.c_void_p(t.data_ptr() + t.element_size() * s0 * (t.numel() // N))
-                L.check(L.lib().xpg_adapt_block(
-                    n_b, a(o1["isum"]), a(o1["vsum"]), a(o1["n_over"]), a(o1["frac_rec"]), a(o1["m_d"]), a(self.fz),
-                    a(self.ad["phi"]), a(self.ad["zeta_k"]), a(sp.t["iz0"]), a(sp.t["fb"]), a(self.ad["Xlen"]),
-                    a(self.ad["cosa"]), float(o["sd"]), float(o["sm"]), float(g.z_to_phi(0.0)), float(g.osc),
-                    float(g.qx), C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
-                    C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes, st))
-            if o["learn_profiles"]:
-                prof, _, _ = K.normalise_profiles(num_fx, den_fx, device, gauss=self.gauss_t)
-            else:
-                prof = self.gauss_rep
-            prof_by_block[b] = prof
-            if o["moments_only"]:
-                continue
-            call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
-            call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
-            if batched:
-                side.wait_stream(main)
-            with torch.cuda.stream(side):
-                for start, m, shape, mixed, pp in units:
-                    view = view_of(start, m, shape, mixed)
-                    out = {k: v[start:start + m] for k, v in o2.items()}
-                    with (timer("pass2", b, (start, m), side) if timer else nullctx):
-                        _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
-                                      o1["nbg"][start:start + m], out, prep=pp)
+            self._block_chain(i, b, units, o1, o2, prof_by_block, main, side, timer, cross=batched)
         if batched:
             main.wait_stream(side)
         return dict(o1=o1, o2=o2, prof_by_block=prof_by_block)
+
+    def _block_chain(self, i, b, units, o1, o2, prof_by_block, main, side, timer=None, cross=False):
+        """The i-th block (table block b): pass 1 of every launch unit, the adaptive fits and the profile
+        normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``)."""
+        from . import _lib as L
+        o, sp, device, model = self.o, self.sp, self.device, self.model
+        nullctx = _Null()
+        st = L.stream_ptr(device)
+        N = sp.n
+        g = self.exp_blocks[b].geom
+        det = self.det_for(g)
+        frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
+        osc = float(np.radians(g.osc))
+        call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+        call1.widths = C.c_void_p(self._widths_ptr(2 * i))
+        acc = None
+        nreg = o["nreg"]
+        if o["learn_profiles"]:
+            num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
+        view_of = lambda start, m, shape, mixed: sp.refl_view(start, m, shape, mixed is not None, max(mixed or 0, 0))
+        for start, m, shape, mixed, pp in units:
+            view = view_of(start, m, shape, mixed)
+            out = {k: v[start:start + m] for k, v in o1.items()}
+            out["fz"] = self.fz[start:start + m]
+            with (timer("pass1", b, (start, m), main) if timer else nullctx):
+                _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS), prep=pp)
+        groups = self.by_block[b]
+        s0 = groups[0][2]
+        n_b = sum(m for *_, m in groups)
+        if o["adaptive"]:
+            a = lambda t: C.c_void_p(t.data_ptr() + t.element_size() * s0 * (t.numel() // N))
+            L.check(L.lib().xpg_adapt_block(
+                n_b, a(o1["isum"]), a(o1["vsum"]), a(o1["n_over"]), a(o1["frac_rec"]), a(o1["m_d"]), a(self.fz),
+                a(self.ad["phi"]), a(self.ad["zeta_k"]), a(sp.t["iz0"]), a(sp.t["fb"]), a(self.ad["Xlen"]),
+                a(self.ad["cosa"]), float(o["sd"]), float(o["sm"]), float(g.z_to_phi(0.0)), float(g.osc),
+                float(g.qx), C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
+                C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes, st))
+        if o["learn_profiles"]:
+            prof, _, _ = K.normalise_profiles(num_fx, den_fx, device, gauss=self.gauss_t)
+        else:
+            prof = self.gauss_rep
+        prof_by_block[b] = prof
+        if o["moments_only"]:
+            return
+        call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+        call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
+        if cross:
+            side.wait_stream(main)
+        with torch.cuda.stream(side):
+            for start, m, shape, mixed, pp in units:
+                view = view_of(start, m, shape, mixed)
+                out = {k: v[start:start + m] for k, v in o2.items()}
+                with (timer("pass2", b, (start, m), side) if timer else nullctx):
+                    _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
+                                  o1["nbg"][start:start + m], out, prep=pp)
+
+    def arena(self):
+        """Prepared-data buffers for one block at a time (the largest block's bins back to back): the single-stream
+        block-by-block form (enqueue_block)."""
+        from . import _lib as L
+        nV, nP, nR = self.sp.block_arena
+        dev = self.device
+        bufs = dict(sb=torch.empty(max(nV, 1), dtype=torch.float32, device=dev),
+                    geo=torch.empty(max(3 * nP, 1), dtype=torch.float64, device=dev),
+                    zeta=torch.empty(max(nR, 1), dtype=torch.float64, device=dev),
+                    nbm=torch.empty(max(nV, 1), dtype=torch.uint8, device=dev),
+                    geo_p=torch.empty(max(3 * nP, 1), dtype=torch.float64, device=dev),
+                    zeta_p=torch.empty(max(nR, 1), dtype=torch.float64, device=dev))
+        return bufs
+
+    def arena_bytes(self):
+        nV, nP, nR = self.sp.block_arena
+        return 5 * nV + 48 * nP + 16 * nR
+
+    def enqueue_block(self, i, store, arena, o1, o2, prof_by_block):
+        """The i-th block of the order on the current stream only, reading frames through ``store`` (any xpg_frames
+        view holding the block's frames, e.g. a ring) and its prepared data from ``arena`` (arena()): every bin's
+        shoebox gather and geometry, pass 1, the adaptive fits, normalisation and pass 2, in stream order.  Gives the
+        results of enqueue() (the same kernels on the same values)."""
+        from . import _lib as L
+        o, sp, model = self.o, self.sp, self.model
+        b = self.order[i]
+        g = self.exp_blocks[b].geom
+        det = self.det_for(g)
+        st = L.stream_ptr(self.device)
+        callp = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
+        units = []
+        for bi, start, m, shape, wl in self.bins_by_block[b]:
+            v0, p0, r0 = sp.bin_arena[bi]
+            pp = L.Prepared()
+            pp.sb = arena["sb"].data_ptr() + 4 * v0
+            pp.nbm = arena["nbm"].data_ptr() + v0
+            pp.geo = arena["geo"].data_ptr() + 24 * p0
+            pp.geo_p = arena["geo_p"].data_ptr() + 24 * p0
+            pp.zeta = arena["zeta"].data_ptr() + 8 * r0
+            pp.zeta_p = arena["zeta_p"].data_ptr() + 8 * r0
+            L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp), C.byref(sp.refl_view(start, m, shape, True, wl).s),
+                                        C.byref(store), C.byref(pp), st))
+            units.append((start, m, shape, wl, pp))
+        main = torch.cuda.current_stream(self.device)
+        self._block_chain(i, b, units, o1, o2, prof_by_block, main, main)
 
     def block_params(self):
         """Per-block widths (xpg/integrate.py:427-429, :458-460) from the device records of the last run."""

@@ -35,7 +35,7 @@ class SynthSpots(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("nx", C.c_int32), ("ny", C.c_int32), ("f0", C.c_int32), ("nf", C.c_int32),
                 ("n", C.c_int32), ("half", C.c_int32)] + \
                [(k, C.c_void_p) for k in ("x", "y", "z", "counts", "sig_r", "sig_t", "sig_z", "rdx", "rdy")] + \
-               [("flag_value", C.c_uint32), ("sat_value", C.c_uint32)]
+               [("flag_value", C.c_uint32), ("sat_value", C.c_uint32), ("sid", C.c_void_p)]
 
 
 @dataclass
@@ -215,8 +215,59 @@ def _lib_synth():
     return L.lib()
 
 
-def render(spec: SynthSpec, spots: dict, out: torch.Tensor, f0: int, nf: int):
-    """Frames [f0, f0 + nf) (0-based) of the dataset into ``out`` (nf, ny, nx) int32/uint32 device tensor."""
+def spot_half(spots):
+    """Half-width (pixels) of the square each spot is rendered over, from the spread of the whole table."""
+    return int(np.clip(np.ceil(4.0 * np.quantile(spots["sig_r"], 0.99)), 6, 40)) if len(spots["sig_r"]) else 6
+
+
+def _select(lo_sorted, order, hi, span, f0, nf):
+    a = np.searchsorted(lo_sorted, f0 - span, side="left")
+    b = np.searchsorted(lo_sorted, f0 + nf, side="left")
+    cand = order[a:b]
+    return np.sort(cand[hi[cand] >= f0])
+
+
+def batch_spot_index(spots, nframes, batch):
+    """Spots that reach each batch of ``batch`` frames, as one index array and per-batch offsets (host)."""
+    z, sz = spots["z"], spots["sig_z"]
+    lo, hi = z - 5 * sz, z + 5 * sz
+    order = np.argsort(lo, kind="stable")
+    span = float(np.max(hi - lo)) if z.size else 0.0
+    parts = [_select(lo[order], order, hi, span, f0, min(batch, nframes - f0)) for f0 in range(0, nframes, batch)]
+    off = np.zeros(len(parts) + 1, np.int64)
+    off[1:] = np.cumsum([p_.size for p_ in parts])
+    return (np.concatenate(parts) if parts else np.zeros(0, np.int64)).astype(np.int64), off
+
+
+class DeviceSpots:
+    """A dataset's spot table on the device, with the host index to pick the spots that reach a range of frames."""
+    KEYS = ("x", "y", "z", "counts", "sig_r", "sig_t", "sig_z", "rdx", "rdy")
+
+    def __init__(self, spots, device):
+        z, sz = spots["z"], spots["sig_z"]
+        self.lo, self.hi = z - 5 * sz, z + 5 * sz
+        self.order = np.argsort(self.lo, kind="stable")
+        self.lo_sorted = self.lo[self.order]
+        self.span = float(np.max(self.hi - self.lo)) if z.size else 0.0
+        self.t = {k: torch.as_tensor(np.ascontiguousarray(spots[k]), dtype=torch.float64, device=device)
+                  for k in self.KEYS}
+        self.half = spot_half(spots)
+        self.n = int(z.size)
+
+    def select(self, f0, nf):
+        """Table indices of the spots whose +-5 sigma frame range meets frames [f0, f0 + nf)."""
+        return _select(self.lo_sorted, self.order, self.hi, self.span, f0, nf)
+
+    def nbytes(self):
+        return 8 * len(self.KEYS) * self.n
+
+
+def render(spec: SynthSpec, spots, out: torch.Tensor, f0: int, nf: int, sync=True, sel=None):
+    """Frames [f0, f0 + nf) (0-based) of the dataset into ``out`` (nf, ny, nx) int32/uint32 device tensor, on the
+    current stream.  ``spots``: the spot table (dict) or its DeviceSpots.  Every spot keys its random draws by its
+    table index, so a frame's values do not depend on the range a call renders.  ``sync`` = False leaves the launches
+    in flight (temporaries are allocated on the current stream).  ``sel``: the spots reaching these frames as an int64
+    device tensor of table indices (batch_spot_index), else selected here."""
     h = _lib_synth()
     dev = out.device
     bg = SynthBg(spec.seed, spec.nx, spec.ny, f0, nf, spec.orgx, spec.orgy, spec.bg_centre, spec.bg_edge,
@@ -224,27 +275,31 @@ def render(spec: SynthSpec, spots: dict, out: torch.Tensor, f0: int, nf: int):
     st = L.stream_ptr(dev)
     if h.xpg_synth_background(C.byref(bg), L.ptr(out), st) != 0:
         raise L.XpgError("synthetic background launch failed")
-    z, sz = spots["z"], spots["sig_z"]
-    sel = np.flatnonzero((z + 5 * sz >= f0) & (z - 5 * sz < f0 + nf))
-    if sel.size:
-        T = lambda a: torch.as_tensor(np.ascontiguousarray(a[sel]), dtype=torch.float64, device=dev)
-        arrs = {k: T(spots[k]) for k in ("x", "y", "z", "counts", "sig_r", "sig_t", "sig_z", "rdx", "rdy")}
-        half = int(np.clip(np.ceil(4.0 * np.quantile(spots["sig_r"], 0.99)), 6, 40))
-        sp = SynthSpots(spec.seed, spec.nx, spec.ny, f0, nf, int(sel.size), half,
-                        *[L.ptr(arrs[k]) for k in ("x", "y", "z", "counts", "sig_r", "sig_t", "sig_z", "rdx", "rdy")],
-                        FLAG, SAT)
+    ds = spots if isinstance(spots, DeviceSpots) else DeviceSpots(spots, dev)
+    if sel is None:
+        sel_h = ds.select(f0, nf)
+        idx = torch.from_numpy(sel_h.astype(np.int64)).pin_memory().to(dev, non_blocking=True) if sel_h.size else None
+    else:
+        idx = sel if sel.numel() else None
+    if idx is not None:
+        arrs = {k: ds.t[k].index_select(0, idx) for k in DeviceSpots.KEYS}
+        sid = idx.to(torch.int32)
+        sp = SynthSpots(spec.seed, spec.nx, spec.ny, f0, nf, int(idx.numel()), ds.half,
+                        *[L.ptr(arrs[k]) for k in DeviceSpots.KEYS], FLAG, SAT, L.ptr(sid))
         if h.xpg_synth_spots_add(C.byref(sp), L.ptr(out), st) != 0:
             raise L.XpgError("synthetic spot launch failed")
-        torch.cuda.current_stream(dev).synchronize()      # keeps arrs alive until the kernel has read them
+    if sync:
+        torch.cuda.current_stream(dev).synchronize()
 
 
 def generate(spec: SynthSpec, device, batch=50, log=None):
     """The whole dataset on the device: (DeviceFrames, spot table)."""
     spots = spot_table(spec)
     data = torch.empty((spec.nframes, spec.ny, spec.nx), dtype=torch.int32, device=device)
+    ds = DeviceSpots(spots, device)
     for f0 in range(0, spec.nframes, batch):
         nf = min(batch, spec.nframes - f0)
-        render(spec, spots, data[f0:f0 + nf], f0, nf)
+        render(spec, ds, data[f0:f0 + nf], f0, nf, sync=False)
         if log and (f0 // batch) % 6 == 0:
             log(f"  synth {spec.name}: frames {f0 + 1}-{f0 + nf}")
     torch.cuda.synchronize(device)
