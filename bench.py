@@ -304,6 +304,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="batched (the reported value) and plain (reported beside it), or one of them")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the batched form's launch-parameter candidates (xpg_amd/autotune.py) on this workload "
+                         "first and run with the fastest")
+    ap.add_argument("--autotune-save", default="", help="write the autotuner's choice to this file (tuned.json form)")
+    ap.add_argument("--prep-ahead", type=int, default=-1,
+                    help="batched form: prepare blocks this many ahead of the chain (-1: all up front)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one copy-like kernel (out = -in, float32) over MIB MiB, outside the timed region: "
                          "the reference of known traffic for hardware byte counters")
@@ -359,7 +365,16 @@ def main():
     from xpg_amd.integrate import DZ_BUCKETS  # noqa: F401
     opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
                 bkg_template=None, moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
-                sm=np.radians(model.sigma_m))
+                sm=np.radians(model.sigma_m), prep_ahead=None if args.prep_ahead < 0 else args.prep_ahead)
+    tuned = None
+    if args.autotune:
+        from xpg_amd import autotune
+        best, table = autotune.tune(frames, [exp] * (len(edges) - 1), model, hp, opts, log=log)
+        opts.update(lds_bins=tuple(best["lds_bins"]), wg_waves=best["wg_waves"])
+        tuned = dict(best=best, candidates=len(table), worst_ms=max(r["ms"] for r in table),
+                     all_identical=all(r["identical"] for r in table))
+        if args.autotune_save:
+            autotune.save(best, table, spec.name, args.autotune_save)
     sweep = Sweep(frames, [exp] * (len(edges) - 1), model, hp, opts)
     N = sweep.sp.n
     log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups ({len(sweep.sp.bins)} size-bin launches) "
@@ -502,7 +517,8 @@ def main():
                                                 f"{list(spec.box_draw[1])}, odd" if spec.box_draw else "reference rules"), parallelism="single GPU"),
                roofline=roof, cpu_baseline=cpu, parity=parity,
                plain=plain, vs_plain=round(value / plain["value"], 3) if plain and form == "batched" else None,
-               batched_forms=graph_info if form == "batched" else None)
+               batched_forms=graph_info if form == "batched" else None,
+               launch=dict(lds_bins=list(sweep.sp.lds_bins), wg_waves=sweep.sp.wg_waves, autotune=tuned))
     print(json.dumps(out), flush=True)
 
 

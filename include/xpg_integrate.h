@@ -78,10 +78,12 @@ typedef struct xpg_refl {
    * hold their maxima (they size the LDS).  voff: offset of a row's voxels in sb and nbm (elements); poff: offset of
    * its pixels (a row's geo / geo_p block starts at element 3 poff, its per-pixel pass-1 outputs at poff).
    * wave_lds: LDS bytes per wavefront of the launch, at least xpg_lds_bytes of every row's shape (0: sized from
-   * Dz x Dy x Dx); a row whose layout exceeds it is skipped, its outputs left as they were. */
+   * Dz x Dy x Dx); a row whose layout exceeds it is skipped, its outputs left as they were.
+   * wg_waves: at most this many wavefronts (rows) per workgroup (0: as many as fit, up to 4). */
   const int32_t *rdz, *rdy, *rdx;
   const int64_t *voff, *poff;
   int32_t wave_lds;
+  int32_t wg_waves;
 } xpg_refl;
 
 /* pass 1 outputs (xpg/integrate_kernels.py:362-365).  Per-reflection float64 / int32 (n); fz (n, Dz); per-pixel

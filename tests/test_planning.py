@@ -149,8 +149,8 @@ def test_drawn_boxes_and_size_bins():
         assert (hp["refl"]["block"][rows] == b).all()
         assert mB == dz[rows].max() and mbx == bx[rows].max()
         need = [L.lib().xpg_lds_bytes(int(B), int(s), int(s)) for B, s in set(zip(dz[rows], bx[rows]))]
-        assert wl == max(need) and (lb == len(I.LDS_BINS) or wl <= I.LDS_BINS[lb])
-        assert lb == 0 or min(need) > I.LDS_BINS[lb - 1]
+        assert wl == max(need) and (lb == len(sp.lds_bins) or wl <= sp.lds_bins[lb])
+        assert lb == 0 or min(need) > sp.lds_bins[lb - 1]
         V = (dz[rows] * bx[rows] ** 2).astype(np.int64)
         assert np.array_equal(voff[start:start + m], np.cumsum(V) - V)
         assert np.array_equal(poff[start:start + m], np.cumsum(bx[rows] ** 2) - bx[rows] ** 2)
@@ -164,3 +164,26 @@ def test_drawn_boxes_and_size_bins():
         assert not (~hpr[:-1] & hpr[1:]).any()
         assert any(s2 <= start and start + m <= s2 + m2 for _, _, s2, m2, _, _ in sp.bins)
     assert len(sp.bins) < len(sp.groups)
+
+
+def test_row_order_does_not_depend_on_the_bins():
+    """The autotuner's bin ladders only cut the same row order into different launches (so the adaptive fits reduce
+    the rows in the same order and every candidate gives the same results)."""
+    import torch
+    from xpg_amd import autotune
+    spec = synth.replace(synth.spec_uneven(), nframes=60)
+    exp = synth.make_experiment(spec)
+    edges = synth.block_edges(spec)
+    img = SimpleNamespace(first=1, last=60, nx=spec.nx, ny=spec.ny)
+    hp = I._integrate(img, [exp] * (len(edges) - 1), edges, I.IntegrationModel(spec.sigma_d, spec.sigma_m),
+                      synth.edge_resolution(exp.geom), plan_only=True, verbose=False,
+                      _box_draw=synth.box_draw_for(spec))
+    perms = []
+    for r in (1.25, 1.5, 2.0, None):
+        sp = I.SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"], hp["iy0"],
+                         hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"], hp["block_order"],
+                         torch.device("cpu"), lds_bins=autotune.ladder(r))
+        perms.append(sp.perm)
+        assert len(sp.bins) <= len(sp.groups)
+    for p in perms[1:]:
+        assert np.array_equal(p, perms[0])

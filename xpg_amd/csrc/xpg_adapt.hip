@@ -9,7 +9,7 @@
 //                                selection of the two middle ranks
 //   k_ad_gram    (NG x 256)      Gram matrix A^T A and A^T y of the rocking-curve basis over a fixed row range
 //                                (rc_basis_np, :437-443)
-//   k_ad_gsum    (1 x 1024)      fixed-order reduction of the Gram partials
+//   k_ad_gsum    (NE x 256)      fixed-order reduction of the Gram partials, one workgroup per entry
 //   k_ad_solve   (1 x 64)        non-negative least squares on the Gram form (_nnls_gram, :1095-1104) in one
 //                                wavefront, regularisation towards the Gaussian weights, the widths records
 //
@@ -41,6 +41,7 @@ struct Ws {
   uint8_t *st;        // n: strong-row flag
   uint64_t *key;      // n: order-preserving bits of the sigma_D term
   double *tot;        // n: fz total
+  int32_t *nzl;       // n: 1 + the last frame with fz != 0 (0: none)
   int32_t *cnt;       // nprep: strong rows per k_ad_prep workgroup
   double *cost;       // NGRID x NCH partial grid costs
   double *gram;       // NG x NE partial Gram entries
@@ -62,6 +63,7 @@ __host__ __device__ inline Ws carve(void *base, int32_t n) {
   w.st = (uint8_t *)take(m);
   w.key = (uint64_t *)take(8 * m);
   w.tot = (double *)take(8 * m);
+  w.nzl = (int32_t *)take(4 * m);
   w.cnt = (int32_t *)take(4 * (size_t)w.nprep);
   w.cost = (double *)take(8 * NGRID * NCH);
   w.gram = (double *)take(8 * (size_t)NG * NE);
@@ -114,8 +116,13 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
     st = (a.isum[r] > 0) && (a.isum[r] / sqrt(fmax(a.vsum[r], 1e-6)) > 15.0) && (a.n_over[r] == 0) &&
          (a.frac_rec[r] > 0.98) && (a.fb[r] == 1);
     double s = 0.0, md = 0.0;
+    int last = 0;
     if (st) {
-      for (int k = 0; k < NF; ++k) s += (double)a.fz[(int64_t)r * NF + k];
+      for (int k = 0; k < NF; ++k) {
+        const float f = a.fz[(int64_t)r * NF + k];
+        s += (double)f;
+        if (f != 0.f) last = k + 1;
+      }
       const double t2 = 1.0 - (9.0 / 2) * exp(-9.0 / 2) / (1 - exp(-9.0 / 2));
       const double q = a.qx / a.Xlen[r];
       md = a.m_d[r] / t2 - (q * q) * (1 + a.cosa[r] * a.cosa[r]) / 24.0;
@@ -123,6 +130,7 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
     const uint64_t b = __builtin_bit_cast(uint64_t, md);
     w.key[r] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);      // unsigned order = numeric order
     w.tot[r] = s;
+    w.nzl[r] = last;
     w.st[r] = st;
   }
   const uint64_t bal = __ballot(st);
@@ -158,9 +166,24 @@ __global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
     // frame k spans [lo, hi] = sorted(phi0 + osc (iz0 + k), phi0 + osc (iz0 + k + 1)); the row total of R telescopes
     const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) * inv_s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) * inv_s);
     const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9), inv_den = 1.0 / den;
+    // the boundaries move monotonically away from phi: once one is past the saturation point every later erf is the
+    // same +-1 (erf_sat), so it is not evaluated again
     double cst = 0.0, Eprev = e0;
+    bool sat = false;
+    const int kend = w.nzl[r];
     for (int k = 0; k < NF; ++k) {
-      const double E = k == NF - 1 ? e1 : erf_sat((bound_deg(a, iz0, k + 1) - phi) * inv_s);
+      // past the last non-zero fz with the rocking curve saturated every term is tot * (+0)^2: cst is final
+      if (sat && k >= kend) break;
+      double E;
+      if (k == NF - 1) {
+        E = e1;
+      } else if (sat) {
+        E = Eprev;
+      } else {
+        const double x = (bound_deg(a, iz0, k + 1) - phi) * inv_s;
+        sat = fabs(x) > ERF_SAT && (x > 0) == (a.osc >= 0);
+        E = erf_sat(x);
+      }
       const double R = a.osc >= 0 ? 0.5 * (E - Eprev) : 0.5 * (Eprev - E);
       const double d = (double)fz[k] * inv_tot - R * inv_den;
       cst = fma(tot, d * d, cst);
@@ -220,9 +243,14 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
   __shared__ double s_cost[NGRID];
   __shared__ uint32_t hist[2][256];
   __shared__ uint64_t s_pre[2];
+  __shared__ uint64_t s_mm[2];
   __shared__ uint32_t s_k[2];
   const int t = threadIdx.x;
-  if (t == 0) s_n = 0;
+  if (t == 0) {
+    s_n = 0;
+    s_mm[0] = ~0ull;
+    s_mm[1] = 0;
+  }
   for (int i = t; i < NGRID * NCH; i += 1024) s_part[i] = w.cost[i];
   __syncthreads();
   int cn = 0;
@@ -271,7 +299,40 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
       }
     }
   }
-  for (int shift = 56; shift >= 0; shift -= 8) {
+  // the bytes every strong key shares (from the smallest and largest key) need no round: the selection starts at the
+  // first byte where they differ, with that common prefix as both ranks' prefix
+  uint64_t kmin = ~0ull, kmax = 0;
+  if (in_regs) {
+#pragma unroll
+    for (int i = 0; i < KR; ++i)
+      if ((kv >> i) & 1u) {
+        kmin = kr[i] < kmin ? kr[i] : kmin;
+        kmax = kr[i] > kmax ? kr[i] : kmax;
+      }
+  } else {
+    for (int r = t; r < a.n; r += 1024)
+      if (w.st[r]) {
+        const uint64_t key = w.key[r];
+        kmin = key < kmin ? key : kmin;
+        kmax = key > kmax ? key : kmax;
+      }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a0 = __shfl_xor(kmin, o, 64), a1 = __shfl_xor(kmax, o, 64);
+    kmin = a0 < kmin ? a0 : kmin;
+    kmax = a1 > kmax ? a1 : kmax;
+  }
+  if ((t & 63) == 0) {
+    atomicMin((unsigned long long *)&s_mm[0], (unsigned long long)kmin);
+    atomicMax((unsigned long long *)&s_mm[1], (unsigned long long)kmax);
+  }
+  __syncthreads();
+  const uint64_t diff = s_mm[0] ^ s_mm[1];
+  const int top = diff ? (63 - __clzll((long long)diff)) / 8 * 8 : 0;   // byte of the highest differing bit
+  if (t == 0) s_pre[0] = s_pre[1] = top == 56 ? 0ull : (s_mm[0] & (~0ull << (top + 8)));
+  const bool two = s_k[0] != s_k[1];
+  __syncthreads();
+  for (int shift = top; shift >= 0; shift -= 8) {
     for (int i = t; i < 512; i += 1024) (&hist[0][0])[i] = 0;
     __syncthreads();
     const uint64_t hm = shift == 56 ? 0ull : (~0ull << (shift + 8));
@@ -283,7 +344,7 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
         const uint32_t bin = (uint32_t)(key >> shift) & 255u;
         const bool live = (kv >> i) & 1u;
         hist_add(hist[0], bin, live && (key & hm) == p0);
-        hist_add(hist[1], bin, live && (key & hm) == p1);
+        if (two) hist_add(hist[1], bin, live && (key & hm) == p1);
       }
     } else {
       for (int r0 = 0; r0 < a.n; r0 += 1024) {
@@ -292,17 +353,21 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
         const uint64_t key = live ? w.key[r] : 0;
         const uint32_t bin = (uint32_t)(key >> shift) & 255u;
         hist_add(hist[0], bin, live && (key & hm) == p0);
-        hist_add(hist[1], bin, live && (key & hm) == p1);
+        if (two) hist_add(hist[1], bin, live && (key & hm) == p1);
       }
     }
     __syncthreads();
-    if (t < 128) {
+    if (t < (two ? 128 : 64)) {
       const int q = t >> 6;
       uint32_t k = s_k[q];
       const int bin = select_bin(hist[q], k);
       if ((t & 63) == 0) {
         s_pre[q] |= (uint64_t)bin << shift;
         s_k[q] = k;
+        if (!two) {
+          s_pre[1] = s_pre[0];
+          s_k[1] = k;
+        }
       }
     }
     __syncthreads();
@@ -441,27 +506,16 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
 
 // Gram totals: entry e of each of four quarters of the NG partials summed in order (loads 16 at a time), then the
 // quarters in order
-__global__ void __launch_bounds__(1024) k_ad_gsum(Ws w) {
-  __shared__ double part4[4][NE];
-  const int t = threadIdx.x;
+__global__ void __launch_bounds__(256) k_ad_gsum(Ws w) {
+  __shared__ double red[256];
   if ((int)w.scal[0] < 100) return;
-  if (t < 4 * 256) {
-    const int e = t & 255, qt = t >> 8;
-    if (e < NE) {
-      const double *src = w.gram + (int64_t)qt * (NG / 4) * NE + e;
-      double s = 0.0;
-      for (int g0 = 0; g0 < NG / 4; g0 += 16) {
-        double v[16];
+  const int e = blockIdx.x, t = threadIdx.x;
+  // entry e: thread t sums the partials t, t + 256, ... in order, then a fixed tree over the threads
+  double s = 0.0;
 #pragma unroll
-        for (int i = 0; i < 16; ++i) v[i] = src[(int64_t)(g0 + i) * NE];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) s += v[i];
-      }
-      part4[qt][e] = s;
-    }
-  }
-  __syncthreads();
-  if (t < NE) w.gtot[t] = ((part4[0][t] + part4[1][t]) + part4[2][t]) + part4[3][t];
+  for (int i = 0; i < NG / 256; ++i) s += w.gram[(int64_t)(t + 256 * i) * NE + e];
+  s = block_sum256(s, red);
+  if (t == 0) w.gtot[e] = s;
 }
 
 // Lawson-Hanson active-set NNLS in Gram form (min w^T G w - 2 w^T h, w >= 0) on one wavefront.
@@ -760,7 +814,7 @@ int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int
   hipLaunchKernelGGL(k_ad_grid, dim3(NGRID, NCH), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(1024), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_gram, dim3(NG), dim3(64 * GW), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_gsum, dim3(1), dim3(1024), 0, st, w);
+  hipLaunchKernelGGL(k_ad_gsum, dim3(NE), dim3(256), 0, st, w);
   hipLaunchKernelGGL(k_ad_solve, dim3(1), dim3(64), 0, st, a, w, in, out_pass2, out_next);
   return hipGetLastError() == hipSuccess ? XPG_OK : XPG_ERR_HIP;
 }

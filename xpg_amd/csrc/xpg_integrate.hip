@@ -1214,6 +1214,23 @@ static int check_refl(const xpg_refl *R, bool need_partner_arrays) {
   return XPG_OK;
 }
 
+// wavefronts per workgroup for a per-wavefront LDS slice of `bytes`: the count that keeps the most wavefronts of the
+// launch resident on a CU (its LDS holds floor(lim / (w bytes)) workgroups of w; at most 16 by registers), the larger
+// on a tie; cap > 0 limits it
+static int pack_waves(size_t bytes, int lim, int cap) {
+  int best_w = 1, best = 0;
+  for (int w = 4; w >= 1; w >>= 1) {
+    if ((cap > 0 && w > cap) || (long long)w * (long long)bytes > lim) continue;
+    const int resident = (int)((long long)lim / ((long long)w * (long long)bytes)) * w;
+    const int waves = resident < 16 ? resident : 16;
+    if (waves > best) {
+      best = waves;
+      best_w = w;
+    }
+  }
+  return best_w;
+}
+
 // in_place_ok: the shoebox values are in a device buffer the kernel may read in place when they do not fit in LDS
 static int launch_shape(const xpg_refl *R, int pass, bool in_place_ok, int *waves, int *wave_bytes, size_t *lds,
                         int *vals_global) {
@@ -1228,9 +1245,7 @@ static int launch_shape(const xpg_refl *R, int pass, bool in_place_ok, int *wave
   if ((long long)L.bytes > lim)
     return fail(XPG_ERR_TOO_LARGE, "shoebox %dx%dx%d needs %zu bytes of LDS > %d%s", R->Dz, R->Dy, R->Dx, L.bytes,
                 lim, in_place_ok ? "" : " (gather it first: xpg_prepare or xpg_extract)");
-  int w = 4;
-  while (w > 1 && (long long)w * L.bytes > lim) w >>= 1;
-  // keep several workgroups per CU: at most 32 KB per workgroup when the shoebox is small
+  const int w = pack_waves(L.bytes, lim, R->wg_waves);
   *waves = w;
   *wave_bytes = (int)L.bytes;
   *lds = (size_t)w * L.bytes;
@@ -1268,8 +1283,7 @@ int xpg_extract(const xpg_frames *frames, const xpg_refl *refl, float *sb, void 
   size_t wb = extract_wave_bytes(refl->Dz, refl->Dy, refl->Dx);
   if (refl->wave_lds > 0 && refl->wave_lds <= lim && (size_t)refl->wave_lds < wb) wb = al8((size_t)refl->wave_lds);
   if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
-  int w = 4;
-  while (w > 1 && (long long)w * wb > lim) w >>= 1;
+  const int w = pack_waves(wb, lim, refl->wg_waves);
   rc = set_lds(k_extract, (size_t)w * wb);
   if (rc) return rc;
   hipLaunchKernelGGL(k_extract, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *frames,
@@ -1320,8 +1334,7 @@ int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *r
     wb = prep_layout(refl->Dz, refl->Dy, refl->Dx, true).bytes;
   }
   if ((long long)wb > lim) return fail(XPG_ERR_TOO_LARGE, "shoebox needs %zu bytes of LDS > %d", wb, lim);
-  int w = 4;
-  while (w > 1 && (long long)w * wb > lim) w >>= 1;
+  const int w = pack_waves(wb, lim, refl->wg_waves);
   rc = set_lds(k_prepare, (size_t)w * wb);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prepare, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *det,

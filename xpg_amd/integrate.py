@@ -237,14 +237,17 @@ LDS_BINS = (8192, 12288, 16384, 24576, 32768, 49152, 65536, 98304, 163840)
 
 class SweepPlan:
     """Everything the device needs for one integrate() call: the reflection table in processing order, uploaded once.
-    Rows are ordered by phi block, then size bin (LDS bytes of the shoebox shape), then shape, with the rows that
-    have an overlapping partner first in each shape group (xpg/integrate.py:372-383).  ``groups`` are the
+    Rows are ordered by phi block, then shoebox shape in order of its LDS bytes (so size bins are contiguous), with
+    the rows that have an overlapping partner first in each shape group (xpg/integrate.py:372-383).  ``groups`` are the
     (block, shape) ranges (one launch each in the plain form), ``bins`` the (block, bin) ranges (one launch each in the
     batched form, mixed shapes)."""
 
     def __init__(self, refl, iz0, fb, bucket, bxs, pxs, ix0, iy0, region, has_partner, pj, nb, blocks_refl, order,
-                 device):
+                 device, lds_bins=None):
         from . import _lib as L
+        lds_bins = tuple(lds_bins or LDS_BINS)
+        self.lds_bins = lds_bins
+        self.wg_waves = 0
         self.groups = []                 # (block, (B, bx, sp), start, n)
         self.bins = []                   # (block, bin, start, n, (max B, max bx), LDS bytes per wavefront)
         perm = []
@@ -255,13 +258,16 @@ class SweepPlan:
                 lds[(B, bx)] = int(L.lib().xpg_lds_bytes(B, bx, bx))
             return lds[(B, bx)]
 
-        bin_of = lambda k: int(np.searchsorted(LDS_BINS, lds_of(k[0], k[1])))
+        bin_of = lambda k: int(np.searchsorted(lds_bins, lds_of(k[0], k[1])))
 
         for b in order:
             rows = blocks_refl[b]
             ck = np.unique((bucket[rows].astype(np.int64) * 4096 + bxs[rows]) * 4096 + pxs[rows])
+            # shapes in order of their LDS need (then depth, side, super-pixel): any bin ladder cuts this order into
+            # contiguous launches, so the row order -- and with it the order the fits reduce rows in -- does not
+            # depend on the bins
             keys = sorted(((int(k // 4096 ** 2), int(k // 4096 % 4096), int(k % 4096)) for k in ck),
-                          key=lambda k: (bin_of(k), DZ_BUCKETS.index(k[0]) if k[0] in DZ_BUCKETS else k[0], k[1], k[2]))
+                          key=lambda k: (lds_of(k[0], k[1]), k[0], k[1], k[2]))
             cur = None
             for gk in keys:
                 sel = rows[(bucket[rows] == gk[0]) & (bxs[rows] == gk[1]) & (pxs[rows] == gk[2])]
@@ -366,6 +372,7 @@ class SweepPlan:
         for k in ("ix0", "iy0", "iz0", "fb", "pxs", "region", "has_p"):
             setattr(s, k, C.c_void_p(self.t[k].data_ptr() + 4 * start))
         s.K = self.K
+        s.wg_waves = int(self.wg_waves)
         for k in ("x", "y", "z", "sx", "sz"):
             setattr(s, "nb" + k, C.c_void_p(self.t["nb" + k].data_ptr() + 4 * self.K * start))
         if mixed:
@@ -495,12 +502,28 @@ def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, d
     return res
 
 
-def sweep_plan(hp, exp_blocks):
+def sweep_plan(hp, exp_blocks, lds_bins=None, wg_waves=None):
     """The host part of a sweep's device table (SweepPlan with the fits' inputs), from a host plan of
-    _integrate(plan_only=True); a Sweep uploads it."""
-    return SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"], hp["iy0"],
-                     hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"], hp["block_order"],
-                     None).add_fit_inputs(hp["refl"], exp_blocks)
+    _integrate(plan_only=True); a Sweep uploads it.  lds_bins / wg_waves: launch parameters (default: the tuned
+    ones, xpg_amd/tuned.json, else LDS_BINS and as many wavefronts per workgroup as fit)."""
+    tb, tw = tuned_launch()
+    sp = SweepPlan(hp["refl"], hp["iz0"], hp["fb"], hp["bucket"], hp["bxs"], hp["pxs"], hp["ix0"], hp["iy0"],
+                   hp["region"], hp["has_partner"], hp["pj"], hp["nb"], hp["blocks_refl"], hp["block_order"],
+                   None, lds_bins=lds_bins or tb)
+    sp.wg_waves = tw if wg_waves is None else int(wg_waves)
+    return sp.add_fit_inputs(hp["refl"], exp_blocks)
+
+
+def tuned_launch():
+    """(LDS bin bounds, wavefronts per workgroup) chosen by the autotuner (xpg_amd/autotune.py) for this device
+    type, from xpg_amd/tuned.json; defaults when absent."""
+    import json
+    from pathlib import Path
+    f = Path(__file__).with_name("tuned.json")
+    if f.exists():
+        rec = json.loads(f.read_text())
+        return tuple(rec.get("lds_bins") or LDS_BINS), int(rec.get("wg_waves", 0))
+    return LDS_BINS, 0
 
 
 class Sweep:
@@ -531,8 +554,8 @@ class Sweep:
         self.frames_src = src
         self.store = src.store(mask=opts["mask"], exclude_frames=opts["exclude_frames"], frames=opts["frames"])
         sp = hp.get("sweep_plan")          # built ahead on the host (sweep_plan())
-        if sp is None:
-            sp = sweep_plan(hp, exp_blocks)
+        if sp is None or "lds_bins" in opts or "wg_waves" in opts:
+            sp = sweep_plan(hp, exp_blocks, opts.get("lds_bins"), opts.get("wg_waves"))
         self.sp = sp.to(device)
         self.by_block = {}
         for gi, (b, gk, start, m) in enumerate(self.sp.groups):
@@ -696,24 +719,35 @@ class Sweep:
         nullctx = _Null()
         st = L.stream_ptr(device)
         ready = {}
-        if batched:
-            # every block's widths-independent work (shoebox gather, Kabsch frame, eps and J, neighbour masks) ahead,
-            # in block order, on the preparation stream
-            self.xs.wait_stream(main)
+        # the widths-independent work of a block (shoebox gather, Kabsch frame, eps and J, neighbour masks) on the
+        # preparation stream: every block up front, or (opts "prep_ahead" = D) D blocks ahead of the chain, each after
+        # the previous block's pass 1, so that its output is still in the last-level cache when its pass 1 reads it
+        ahead = o.get("prep_ahead")
+
+        def prepare(b):
             with torch.cuda.stream(self.xs):
-                for b in self.order:
-                    g = self.exp_blocks[b].geom
-                    det = self.det_for(g)
-                    callp = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
-                    for bi, start, m, shape, wl in self.bins_by_block[b]:
-                        with (timer("prepare", b, (start, m), self.xs) if timer else nullctx):
-                            L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp),
-                                                        C.byref(sp.refl_view(start, m, shape, True, wl).s),
-                                                        C.byref(self.store), C.byref(self.prep[bi][0]),
-                                                        L.stream_ptr(device)))
-                    ready[b] = torch.cuda.Event()
-                    ready[b].record(self.xs)
+                g = self.exp_blocks[b].geom
+                det = self.det_for(g)
+                callp = K.make_call(g.axis, g.s0, 0.0, 1.0, o["sd"], o["sm"], model.gain, rc_default_weights())
+                for bi, start, m, shape, wl in self.bins_by_block[b]:
+                    with (timer("prepare", b, (start, m), self.xs) if timer else nullctx):
+                        L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp),
+                                                    C.byref(sp.refl_view(start, m, shape, True, wl).s),
+                                                    C.byref(self.store), C.byref(self.prep[bi][0]),
+                                                    L.stream_ptr(device)))
+                ready[b] = torch.cuda.Event()
+                ready[b].record(self.xs)
+
+        p1_done = {}
+        if batched:
+            self.xs.wait_stream(main)
+            for b in (self.order if ahead is None else self.order[:ahead + 1]):
+                prepare(b)
         for i, b in enumerate(self.order):
+            if batched and ahead is not None and i + ahead + 1 < len(self.order):
+                if i >= 1:
+                    self.xs.wait_event(p1_done[self.order[i - 1]])
+                prepare(self.order[i + ahead + 1])
             g = self.exp_blocks[b].geom
             det = self.det_for(g)
             # launch units: (first row, rows, shape or shape maxima, mixed, prepared data or None); mixed: None for a
@@ -732,12 +766,14 @@ class Sweep:
                         L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call0),
                                                     C.byref(sp.refl_view(start, m, (B, bx, bx), mixed=True).s),
                                                     C.byref(self.store), C.byref(self.prep_g[gi]), st))
-            self._block_chain(i, b, units, o1, o2, prof_by_block, main, side, timer, cross=batched)
+            self._block_chain(i, b, units, o1, o2, prof_by_block, main, side, timer, cross=batched,
+                              after_pass1=lambda b=b: p1_done.__setitem__(b, _record(main)))
         if batched:
             main.wait_stream(side)
         return dict(o1=o1, o2=o2, prof_by_block=prof_by_block)
 
-    def _block_chain(self, i, b, units, o1, o2, prof_by_block, main, side, timer=None, cross=False):
+    def _block_chain(self, i, b, units, o1, o2, prof_by_block, main, side, timer=None, cross=False,
+                     after_pass1=None):
         """The i-th block (table block b): pass 1 of every launch unit, the adaptive fits and the profile
         normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``)."""
         from . import _lib as L
@@ -762,6 +798,8 @@ class Sweep:
             out["fz"] = self.fz[start:start + m]
             with (timer("pass1", b, (start, m), main) if timer else nullctx):
                 _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS), prep=pp)
+        if after_pass1 is not None:
+            after_pass1()
         groups = self.by_block[b]
         s0 = groups[0][2]
         n_b = sum(m for *_, m in groups)
@@ -878,6 +916,12 @@ class Sweep:
         res = IntegrationResult(refl, self.model, profiles, tm)
         res.block_params = self.block_params() if self.o["adaptive"] else {}
         return res
+
+
+def _record(stream):
+    e = torch.cuda.Event()
+    e.record(stream)
+    return e
 
 
 class _Null:
