@@ -277,7 +277,11 @@ __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, do
     if (nt == 1) {
       out = scratch[j];
     } else {
-      for (int q = 0; q < nt; ++q) out = out + scratch[j * nt + q];
+      double v[RCK];               // every load in flight at once; the sum keeps component order
+#pragma unroll
+      for (int q = 0; q < RCK; ++q) v[q] = scratch[j * RCK + q];
+#pragma unroll
+      for (int q = 0; q < RCK; ++q) out = out + v[q];
     }
     R[j] = out;
   }

@@ -752,20 +752,28 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   if (fz_out)
     for (int j = Dz + lane; j < fzs; j += WAVE) fz_out[j] = 0.f;
   if (Dz <= 16) {
-    for (int j = 0; j < Dz; ++j) {               // lanes over the pixels of frame j, then a wave reduction
-      double part = 0.0;
-      bool rec = false;
-      for (int p = lane; p < P; p += WAVE) {
+    // every frame at once: g lanes per frame (the largest of 16, 8, 4 with g Dz <= 64) stride over its pixels, then
+    // a sum within each group of g lanes
+    const int g = Dz <= 4 ? 16 : (Dz <= 8 ? 8 : 4);
+    const int j = lane / g, q = lane - j * g;
+    double part = 0.0;
+    bool rec = false;
+    if (j < Dz) {
+      for (int p = q; p < P; p += g) {
         const float c = vals[j * P + p];
         rec |= c > -2.5f;
         if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) part += (double)c - bk * (double)bref[p];
       }
-      const double fzj = wsum_d(part);
-      const bool anyrec = wany(rec);
-      if (lane == 0) {
-        if (fz_out) fz_out[j] = (float)fzj;
-        scr[j] = anyrec ? Rd[j] : 0.0;
-      }
+    }
+    part += dppd<DPP_Q1>(part);
+    part += dppd<DPP_Q2>(part);
+    if (g >= 8) part += __shfl_xor(part, 4, WAVE);
+    if (g >= 16) part += __shfl_xor(part, 8, WAVE);
+    const uint64_t rb = __ballot(rec);
+    if (q == 0 && j < Dz) {
+      const bool anyrec = ((rb >> (j * g)) & ((1ull << g) - 1)) != 0;
+      if (fz_out) fz_out[j] = (float)part;
+      scr[j] = anyrec ? Rd[j] : 0.0;
     }
   } else {
     for (int j = lane; j < Dz; j += WAVE) {      // one lane per frame

@@ -545,7 +545,8 @@ class Sweep:
                blocks' chains.
     Both give identical results (the same kernels on the same values)."""
 
-    def __init__(self, images, exp_blocks, model, hp, opts):
+    def __init__(self, images, exp_blocks, model, hp, opts, uploaded=None):
+        """``uploaded``: the device table (``sp``) and detectors (``dets``) already uploaded by the caller."""
         from . import _lib as L
         self.images, self.exp_blocks, self.model, self.hp, self.o = images, exp_blocks, model, hp, opts
         device = _kernel_device(images)
@@ -553,10 +554,13 @@ class Sweep:
         src = images if isinstance(images, DeviceFrames) else from_imageset(images, opts["frames"], device)
         self.frames_src = src
         self.store = src.store(mask=opts["mask"], exclude_frames=opts["exclude_frames"], frames=opts["frames"])
-        sp = hp.get("sweep_plan")          # built ahead on the host (sweep_plan())
-        if sp is None or "lds_bins" in opts or "wg_waves" in opts:
-            sp = sweep_plan(hp, exp_blocks, opts.get("lds_bins"), opts.get("wg_waves"))
-        self.sp = sp.to(device)
+        if uploaded is not None:
+            self.sp = uploaded["sp"]
+        else:
+            sp = hp.get("sweep_plan")          # built ahead on the host (sweep_plan())
+            if sp is None or "lds_bins" in opts or "wg_waves" in opts:
+                sp = sweep_plan(hp, exp_blocks, opts.get("lds_bins"), opts.get("wg_waves"))
+            self.sp = sp.to(device)
         self.by_block = {}
         for gi, (b, gk, start, m) in enumerate(self.sp.groups):
             self.by_block.setdefault(b, []).append((gi, gk, start, m))
@@ -568,7 +572,9 @@ class Sweep:
             self.bin_of_group[gi] = next(bi for bi, (b2, lb, s2, m2, mx, wl) in enumerate(self.sp.bins)
                                          if b2 == b and s2 <= start < s2 + m2)
         self.order = [b for b in hp["block_order"] if b in self.by_block]
-        self.dets = {}
+        self.dets = dict(uploaded["dets"]) if uploaded is not None else {}
+        self._views = {}
+        self._call_cache = {}
         self.t_plan = 0.0
         t = self.sp.t
         self.ad = dict(phi=t["ad_phi"], zeta_k=t["ad_zeta_k"], Xlen=t["ad_Xlen"], cosa=t["ad_cosa"])
@@ -713,8 +719,7 @@ class Sweep:
             self.side = torch.cuda.Stream(device)      # default priorities
             self.xs = torch.cuda.Stream(device)
         side = self.side if batched else main
-        o1, o2 = self.alloc_outputs()
-        prof_by_block = {}
+        rb = RunBufs(self)
         self._init_widths()
         nullctx = _Null()
         st = L.stream_ptr(device)
@@ -766,69 +771,110 @@ class Sweep:
                         L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call0),
                                                     C.byref(sp.refl_view(start, m, (B, bx, bx), mixed=True).s),
                                                     C.byref(self.store), C.byref(self.prep_g[gi]), st))
-            self._block_chain(i, b, units, o1, o2, prof_by_block, main, side, timer, cross=batched,
+            self._block_chain(i, b, units, rb, main, side, timer, cross=batched,
                               after_pass1=lambda b=b: p1_done.__setitem__(b, _record(main)))
         if batched:
             main.wait_stream(side)
-        return dict(o1=o1, o2=o2, prof_by_block=prof_by_block)
+        return rb.result()
 
-    def _block_chain(self, i, b, units, o1, o2, prof_by_block, main, side, timer=None, cross=False,
-                     after_pass1=None):
+    def _view(self, start, m, shape, mixed):
+        """Cached ctypes table view: mixed None for one shape with its own row layout, else the LDS bytes per
+        wavefront of a view with per-row shapes and bin offsets (-1: sized from the shape)."""
+        key = (start, m, shape, mixed)
+        v = self._views.get(key)
+        if v is None:
+            v = self._views[key] = self.sp.refl_view(start, m, shape, mixed is not None, max(mixed or 0, 0))
+        return v
+
+    def _calls(self, i, b):
+        """Cached (pass-1 call, pass-2 call, adaptive-fit arguments without the outputs) of the i-th block."""
+        c = self._call_cache.get(i)
+        if c is None:
+            from . import _lib as L
+            o, model = self.o, self.model
+            g = self.exp_blocks[b].geom
+            frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
+            osc = float(np.radians(g.osc))
+            call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+            call1.widths = C.c_void_p(self._widths_ptr(2 * i))
+            call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
+            call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
+            groups = self.by_block[b]
+            s0, n_b = groups[0][2], sum(m for *_, m in groups)
+            N = self.sp.n
+            a = lambda t: C.c_void_p(t.data_ptr() + t.element_size() * s0 * (t.numel() // N))
+            fit = (s0, n_b, a(self.fz), a(self.ad["phi"]), a(self.ad["zeta_k"]), a(self.sp.t["iz0"]),
+                   a(self.sp.t["fb"]), a(self.ad["Xlen"]), a(self.ad["cosa"]),
+                   (float(o["sd"]), float(o["sm"]), float(g.z_to_phi(0.0)), float(g.osc), float(g.qx)),
+                   (C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
+                    C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes))
+            c = self._call_cache[i] = (call1, call2, fit)
+        return c
+
+    def _block_chain(self, i, b, units, rb, main, side, timer=None, cross=False, after_pass1=None):
         """The i-th block (table block b): pass 1 of every launch unit, the adaptive fits and the profile
-        normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``)."""
+        normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``); outputs into ``rb``
+        (RunBufs)."""
         from . import _lib as L
-        o, sp, device, model = self.o, self.sp, self.device, self.model
+        o, device = self.o, self.device
         nullctx = _Null()
+        lib = L.lib()
         st = L.stream_ptr(device)
-        N = sp.n
-        g = self.exp_blocks[b].geom
-        det = self.det_for(g)
-        frames_phi0 = float(np.radians(g.z_to_phi(0.0)))
-        osc = float(np.radians(g.osc))
-        call1 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
-        call1.widths = C.c_void_p(self._widths_ptr(2 * i))
+        det = self.det_for(self.exp_blocks[b].geom)
+        call1, call2, fit = self._calls(i, b)
         acc = None
-        nreg = o["nreg"]
-        if o["learn_profiles"]:
-            num_fx, den_fx, acc = K.new_profile_acc(nreg, device)
-        view_of = lambda start, m, shape, mixed: sp.refl_view(start, m, shape, mixed is not None, max(mixed or 0, 0))
+        if rb.learn:
+            G2 = rb.acc[i]
+            acc = L.ProfileAcc(G2[0].data_ptr(), G2[1].data_ptr(), o["nreg"])
+        fz_base, fz_row = self.fz.data_ptr(), 4 * self.fz.shape[1]
         for start, m, shape, mixed, pp in units:
-            view = view_of(start, m, shape, mixed)
-            out = {k: v[start:start + m] for k, v in o1.items()}
-            out["fz"] = self.fz[start:start + m]
+            view = self._view(start, m, shape, mixed)
+            out = L.Pass1Out()
+            for k in K.P1_F64 + K.P1_I32:
+                setattr(out, k, rb.p1(k, start))
+            out.fz = fz_base + fz_row * start
+            out.fz_stride = self.fz.shape[1]
             with (timer("pass1", b, (start, m), main) if timer else nullctx):
-                _launch_pass1(det, call1, view, self.store, out, acc, fz_stride=max(DZ_BUCKETS), prep=pp)
+                a_ = C.byref(acc) if acc is not None else None
+                if pp is not None:
+                    L.check(lib.xpg_pass1_prepared(C.byref(det.s), C.byref(call1), C.byref(view.s), C.byref(pp),
+                                                   C.byref(out), a_, st))
+                else:
+                    L.check(lib.xpg_pass1_frames(C.byref(det.s), C.byref(call1), C.byref(view.s),
+                                                 C.byref(self.store), C.byref(out), a_, st))
         if after_pass1 is not None:
             after_pass1()
-        groups = self.by_block[b]
-        s0 = groups[0][2]
-        n_b = sum(m for *_, m in groups)
         if o["adaptive"]:
-            a = lambda t: C.c_void_p(t.data_ptr() + t.element_size() * s0 * (t.numel() // N))
-            L.check(L.lib().xpg_adapt_block(
-                n_b, a(o1["isum"]), a(o1["vsum"]), a(o1["n_over"]), a(o1["frac_rec"]), a(o1["m_d"]), a(self.fz),
-                a(self.ad["phi"]), a(self.ad["zeta_k"]), a(sp.t["iz0"]), a(sp.t["fb"]), a(self.ad["Xlen"]),
-                a(self.ad["cosa"]), float(o["sd"]), float(o["sm"]), float(g.z_to_phi(0.0)), float(g.osc),
-                float(g.qx), C.c_void_p(self._widths_ptr(2 * i)), C.c_void_p(self._widths_ptr(2 * i + 1)),
-                C.c_void_p(self._widths_ptr(2 * i + 2)), L.ptr(self.ws), self.ws_bytes, st))
-        if o["learn_profiles"]:
-            prof, _, _ = K.normalise_profiles(num_fx, den_fx, device, gauss=self.gauss_t)
+            s0, n_b, fz, phi, zeta_k, iz0, fb, Xlen, cosa, scal, tail = fit
+            L.check(lib.xpg_adapt_block(
+                n_b, rb.p1("isum", s0), rb.p1("vsum", s0), rb.p1("n_over", s0), rb.p1("frac_rec", s0),
+                rb.p1("m_d", s0), fz, phi, zeta_k, iz0, fb, Xlen, cosa, *scal, *tail, st))
+        if rb.learn:
+            prof = rb.prof[i]
+            L.check(lib.xpg_normalise_profiles(C.byref(acc), L.ptr(self.gauss_t), L.ptr(prof), L.ptr(rb.f64[0]),
+                                               L.ptr(rb.f64[1]), st))
         else:
             prof = self.gauss_rep
-        prof_by_block[b] = prof
+        rb.prof_by_block[b] = prof
         if o["moments_only"]:
             return
-        call2 = K.make_call(g.axis, g.s0, frames_phi0, osc, o["sd"], o["sm"], model.gain, rc_default_weights())
-        call2.widths = C.c_void_p(self._widths_ptr(2 * i + 1 if o["adaptive"] else 2 * i))
         if cross:
             side.wait_stream(main)
-        with torch.cuda.stream(side):
-            for start, m, shape, mixed, pp in units:
-                view = view_of(start, m, shape, mixed)
-                out = {k: v[start:start + m] for k, v in o2.items()}
-                with (timer("pass2", b, (start, m), side) if timer else nullctx):
-                    _launch_pass2(det, call2, view, self.store, prof, o1["bg"][start:start + m],
-                                  o1["nbg"][start:start + m], out, prep=pp)
+        sst = L.stream_ptr(device) if side is main else side.cuda_stream
+        for start, m, shape, mixed, pp in units:
+            view = self._view(start, m, shape, mixed)
+            out = L.Pass2Out()
+            for k in K.P2_F64:
+                setattr(out, k, rb.p2(k, start))
+            bg, nbg = rb.p1("bg", start), rb.p1("nbg", start)
+            with (timer("pass2", b, (start, m), side) if timer else nullctx):
+                if pp is not None:
+                    L.check(lib.xpg_pass2_prepared(C.byref(det.s), C.byref(call2), C.byref(view.s), C.byref(pp),
+                                                   L.ptr(prof), int(prof.shape[0]), bg, nbg, 0, C.byref(out), sst))
+                else:
+                    L.check(lib.xpg_pass2_frames(C.byref(det.s), C.byref(call2), C.byref(view.s),
+                                                 C.byref(self.store), L.ptr(prof), int(prof.shape[0]), bg, nbg, 0,
+                                                 C.byref(out), sst))
 
     def arena(self):
         """Prepared-data buffers for one block at a time (the largest block's bins back to back): the single-stream
@@ -848,9 +894,10 @@ class Sweep:
         nV, nP, nR = self.sp.block_arena
         return 5 * nV + 48 * nP + 16 * nR
 
-    def enqueue_block(self, i, store, arena, o1, o2, prof_by_block):
-        """The i-th block of the order on the current stream only, reading frames through ``store`` (any xpg_frames
-        view holding the block's frames, e.g. a ring) and its prepared data from ``arena`` (arena()): every bin's
+    def enqueue_block(self, i, store, arena, rb):
+        """The i-th block of the order on the current stream only, outputs into ``rb`` (RunBufs), reading frames
+        through ``store`` (any xpg_frames view holding the block's frames, e.g. a ring) and its prepared data from
+        ``arena`` (arena()): every bin's
         shoebox gather and geometry, pass 1, the adaptive fits, normalisation and pass 2, in stream order.  Gives the
         results of enqueue() (the same kernels on the same values)."""
         from . import _lib as L
@@ -874,7 +921,7 @@ class Sweep:
                                         C.byref(store), C.byref(pp), st))
             units.append((start, m, shape, wl, pp))
         main = torch.cuda.current_stream(self.device)
-        self._block_chain(i, b, units, o1, o2, prof_by_block, main, main)
+        self._block_chain(i, b, units, rb, main, main)
 
     def block_params(self):
         """Per-block widths (xpg/integrate.py:427-429, :458-460) from the device records of the last run."""
@@ -918,6 +965,35 @@ class Sweep:
         return res
 
 
+class RunBufs:
+    """Device buffers of one run of a sweep: per-reflection outputs (table order), the fixed-point profile grids and
+    the normalised profiles of every block, with the raw pointers the launches take (no per-launch tensor views)."""
+
+    def __init__(self, sweep):
+        self.o1, self.o2 = sweep.alloc_outputs()
+        dev, G = sweep.device, K.PROFILE_GRID
+        nb, nreg = max(len(sweep.order), 1), sweep.o["nreg"]
+        self.learn = bool(sweep.o["learn_profiles"])
+        if self.learn:
+            self.acc = torch.zeros((nb, 2, nreg, G, G), dtype=torch.int64, device=dev)
+            self.prof = torch.empty((nb, nreg, G, G), dtype=torch.float32, device=dev)
+            self.f64 = torch.empty((2, nreg, G, G), dtype=torch.float64, device=dev)
+        self.ptr1 = {k: (v.data_ptr(), v.element_size()) for k, v in self.o1.items()}
+        self.ptr2 = {k: (v.data_ptr(), v.element_size()) for k, v in self.o2.items()}
+        self.prof_by_block = {}
+
+    def p1(self, k, start):
+        base, es = self.ptr1[k]
+        return base + es * start
+
+    def p2(self, k, start):
+        base, es = self.ptr2[k]
+        return base + es * start
+
+    def result(self):
+        return dict(o1=self.o1, o2=self.o2, prof_by_block=self.prof_by_block)
+
+
 def _record(stream):
     e = torch.cuda.Event()
     e.record(stream)
@@ -930,42 +1006,6 @@ class _Null:
 
     def __exit__(self, *a):
         return False
-
-
-def _launch_pass1(det, call, view, store, out, acc, fz_stride=0, prep=None):
-    """Pass 1 of a table view, voxels read from the frame store, or from the view's prepared data ``prep``
-    (xpg_prepare)."""
-    from . import _lib as L
-    o = L.Pass1Out()
-    o.fz_stride = int(fz_stride)
-    for k in K.P1_F64 + K.P1_I32 + ("fz",):
-        setattr(o, k, L.ptr(out[k]) if k in out else None)
-    for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m", "frac_rec", "nfg", "zeta", "seff"):
-        if k not in out:
-            raise XpgError(f"missing output {k}")
-    a = C.byref(acc) if acc is not None else None
-    if prep is not None:
-        L.check(L.lib().xpg_pass1_prepared(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(prep), C.byref(o),
-                                           a, L.stream_ptr(det.device)))
-    else:
-        L.check(L.lib().xpg_pass1_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), C.byref(o),
-                                         a, L.stream_ptr(det.device)))
-
-
-def _launch_pass2(det, call, view, store, prof, bg, nbg, out, prep=None):
-    """Pass 2 of a table view, voxels read from the frame store, or from the view's prepared data ``prep``."""
-    from . import _lib as L
-    o = L.Pass2Out()
-    for k in K.P2_F64:
-        setattr(o, k, L.ptr(out[k]))
-    if prep is not None:
-        L.check(L.lib().xpg_pass2_prepared(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(prep), L.ptr(prof),
-                                           int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
-                                           L.stream_ptr(det.device)))
-    else:
-        L.check(L.lib().xpg_pass2_frames(C.byref(det.s), C.byref(call), C.byref(view.s), C.byref(store), L.ptr(prof),
-                                         int(prof.shape[0]), L.ptr(bg), L.ptr(nbg), 0, C.byref(o),
-                                         L.stream_ptr(det.device)))
 
 
 # ---------------------------------------------------------------------------------------------------------------------

@@ -34,7 +34,7 @@ import torch
 from . import _lib as L
 from . import synth
 from .frames import DeviceFrames
-from .integrate import IntegrationModel, Sweep, _integrate, sweep_plan
+from .integrate import IntegrationModel, RunBufs, Sweep, _integrate, sweep_plan
 
 
 @dataclass
@@ -110,10 +110,8 @@ class _Active:
     ring: torch.Tensor
     dspots: synth.DeviceSpots
     arena: dict
-    o1: dict
-    o2: dict
+    rb: RunBufs
     nbytes: int
-    prof: dict = field(default_factory=dict)
     next_batch: int = 0
     next_block: int = 0
     ev_res: list = field(default_factory=list)
@@ -162,24 +160,40 @@ class Server:
     def _admit(self, plan: Plan):
         spec = plan.req.spec
         st = self._stream()
+        if not hasattr(self, "_up"):
+            self._up = torch.cuda.Stream(self.device)       # uploads: an otherwise idle stream, so that the host
+        with torch.cuda.stream(self._up):                    # copies do not wait for the slot stream's earlier work
+            sweep_t = self._upload(plan)
+        st.wait_stream(self._up)
         with torch.cuda.stream(st):
             ring = torch.empty((plan.ring, spec.ny, spec.nx), dtype=torch.int32, device=self.device)
             frames = DeviceFrames(ring, first=1, overload=synth.OVERLOAD, flag_value=synth.FLAG)
             opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
                         bkg_template=None, moments_only=False, nreg=9, sd=np.radians(plan.model.sigma_d),
                         sm=np.radians(plan.model.sigma_m))
-            sweep = Sweep(frames, [plan.exp] * (len(plan.edges) - 1), plan.model, plan.hp, opts)
+            sweep = Sweep(frames, [plan.exp] * (len(plan.edges) - 1), plan.model, plan.hp, opts, uploaded=sweep_t)
             sweep.store.ring = plan.ring
             sweep.store.n_frames = 0          # frames arrived so far (every frame of the dataset is read)
             sweep.store.available = None
             if sweep.order != plan.order:
                 raise L.XpgError("block order of the sweep differs from the plan")
             arena = sweep.arena()
-            o1, o2 = sweep.alloc_outputs()
-            dspots = synth.DeviceSpots(plan.spots, self.device)
-            spot_idx = torch.as_tensor(plan.spot_idx, device=self.device)
+            rb = RunBufs(sweep)
+            dspots, spot_idx = sweep_t["spots"], sweep_t["spot_idx"]
             sweep._init_widths()
-        return _Active(plan, st, sweep, ring, dspots, arena, o1, o2, dataset_bytes(plan, sweep), spot_idx=spot_idx)
+        return _Active(plan, st, sweep, ring, dspots, arena, rb, dataset_bytes(plan, sweep), spot_idx=spot_idx)
+
+    def _upload(self, plan: Plan):
+        """Everything a dataset uploads once (reflection table, detector, spot table and per-batch spot index), on the
+        current stream."""
+        sp = plan.hp["sweep_plan"].to(self.device)
+        dets = {}
+        from .integrate import detector_consts
+        from . import integrate_kernels as K
+        g = plan.exp.geom
+        dets[id(g)] = (g, K.DeviceDetector(detector_consts(g, None), self.device))
+        return dict(sp=sp, dets=dets, spots=synth.DeviceSpots(plan.spots, self.device),
+                    spot_idx=torch.as_tensor(plan.spot_idx, device=self.device))
 
     def _step(self, a: _Active):
         """Arrival of a dataset's next batch (written into the ring on its stream) and every block it makes ready."""
@@ -199,7 +213,7 @@ class Server:
             t1 = time.perf_counter()
             self.host["render"] += t1 - t0
             while a.next_block < len(a.plan.order) and a.plan.ready_batch[a.next_block] <= k:
-                a.sweep.enqueue_block(a.next_block, a.sweep.store, a.arena, a.o1, a.o2, a.prof)
+                a.sweep.enqueue_block(a.next_block, a.sweep.store, a.arena, a.rb)
                 e = torch.cuda.Event(enable_timing=True)
                 e.record(a.stream)
                 a.ev_blk.append(e)
@@ -216,7 +230,7 @@ class Server:
 
     def results(self, a: _Active):
         """Host results in prediction order (after the dataset's stream has drained)."""
-        res = a.sweep.finish(dict(o1=a.o1, o2=a.o2, prof_by_block=a.prof, t_gpu=float("nan")))
+        res = a.sweep.finish(dict(a.rb.result(), t_gpu=float("nan")))
         return res
 
     def latencies(self, a: _Active):
@@ -273,13 +287,12 @@ class Server:
                     progressed = True
             for a in list(active):
                 if self._finished(a) and (not a.ev_blk or a.ev_blk[-1].query()) and a.ev_res[-1].query():
-                    tr = time.perf_counter()
-                    a.stream.synchronize()
+                    tr = time.perf_counter()      # the dataset's stream has drained (its last events completed)
                     t_end = time.perf_counter() - t0
                     ls = self.latencies(a)
                     lat += list(ls.values())
                     rid = a.plan.req.rid
-                    n_r = int(np.isfinite(a.o2["iprf"].cpu().numpy()).sum())
+                    n_r = int(np.isfinite(a.rb.o2["iprf"].cpu().numpy()).sum())
                     n_refl += n_r
                     per[rid].update(finished=round(t_end, 4), integrated=n_r, batches=len(a.ev_res),
                                     p50_ms=round(float(np.percentile(list(ls.values()), 50)), 3) if ls else None)

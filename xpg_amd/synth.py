@@ -244,19 +244,22 @@ class DeviceSpots:
     KEYS = ("x", "y", "z", "counts", "sig_r", "sig_t", "sig_z", "rdx", "rdy")
 
     def __init__(self, spots, device):
-        z, sz = spots["z"], spots["sig_z"]
-        self.lo, self.hi = z - 5 * sz, z + 5 * sz
-        self.order = np.argsort(self.lo, kind="stable")
-        self.lo_sorted = self.lo[self.order]
-        self.span = float(np.max(self.hi - self.lo)) if z.size else 0.0
+        self.spots = spots
         self.t = {k: torch.as_tensor(np.ascontiguousarray(spots[k]), dtype=torch.float64, device=device)
                   for k in self.KEYS}
         self.half = spot_half(spots)
-        self.n = int(z.size)
+        self.n = int(len(spots["z"]))
+        self._index = None
 
     def select(self, f0, nf):
         """Table indices of the spots whose +-5 sigma frame range meets frames [f0, f0 + nf)."""
-        return _select(self.lo_sorted, self.order, self.hi, self.span, f0, nf)
+        if self._index is None:
+            z, sz = self.spots["z"], self.spots["sig_z"]
+            lo, hi = z - 5 * sz, z + 5 * sz
+            order = np.argsort(lo, kind="stable")
+            self._index = (lo[order], order, hi, float(np.max(hi - lo)) if z.size else 0.0)
+        lo_sorted, order, hi, span = self._index
+        return _select(lo_sorted, order, hi, span, f0, nf)
 
     def nbytes(self):
         return 8 * len(self.KEYS) * self.n
