@@ -1,8 +1,8 @@
 /* Seeded synthetic rotation data generated on the device (workload generator for the benchmarks and the serving
  * loop).  No counterpart in the reference's integration path: it plays the role of xpg/simulate.py (spots as 3D
  * Gaussians with an erf rocking curve on a Poisson background) at full detector size, directly into the device frame
- * store.  Counter-based random numbers keyed by (seed, frame, pixel) and (seed, reflection, frame, pixel) make every
- * frame reproducible independently of batching and launch order.
+ * store, in float32 arithmetic.  Counter-based random numbers keyed by (seed, frame, pixel) and (seed, reflection,
+ * frame, pixel) make every frame reproducible independently of batching and launch order.
  */
 #ifndef XPG_SYNTH_H
 #define XPG_SYNTH_H

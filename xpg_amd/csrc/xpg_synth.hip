@@ -7,99 +7,107 @@
 
 namespace {
 
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-  x += 0x9E3779B97F4A7C15ull;
-  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-  return x ^ (x >> 31);
+// 32-bit counter-based hashing (a low-bias integer finaliser); a draw is keyed by (seed, stream, index) only
+__device__ __forceinline__ uint32_t fmix(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
 }
-
-// uniform in (0, 1) from a key
-__device__ __forceinline__ double uni(uint64_t key) {
-  return ((double)(mix64(key) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+__device__ __forceinline__ uint32_t hash3(uint32_t seed, uint32_t a, uint32_t b) {
+  return fmix(fmix(fmix(seed ^ 0x9e3779b9u) ^ a) + 0x85ebca6bu * b);
 }
+// uniform in (0, 1)
+__device__ __forceinline__ float unif(uint32_t h) { return ((float)(h >> 8) + 0.5f) * (1.0f / 16777216.0f); }
 
-__device__ __forceinline__ uint64_t key3(uint64_t seed, uint64_t a, uint64_t b) {
-  return mix64(mix64(seed ^ 0x5851F42D4C957F2Dull) ^ mix64(a * 0x2545F4914F6CDD1Dull + b));
-}
-
-// Poisson draw: inversion for small means, rounded normal approximation (Box-Muller) above 30
-__device__ uint32_t poisson(double mu, uint64_t key) {
-  if (mu <= 0.0) return 0;
-  double u = uni(key);
-  if (mu < 30.0) {
-    double p = exp(-mu), cdf = p;
+// Poisson draw of mean mu given e^-mu: inversion below 30, rounded normal approximation (Box-Muller) above
+__device__ __forceinline__ uint32_t poisson(float mu, float emu, uint32_t h) {
+  if (mu <= 0.f) return 0;
+  const float u = unif(h);
+  if (mu < 30.f) {
+    float p = emu, cdf = p;
     uint32_t k = 0;
     while (u > cdf && k < 200) {
       ++k;
-      p *= mu / (double)k;
+      p *= mu / (float)k;
       cdf += p;
     }
     return k;
   }
-  double u2 = uni(key ^ 0xD6E8FEB86659FD93ull);
-  double z = sqrt(-2.0 * log(u)) * cos(6.283185307179586 * u2);
-  double v = floor(mu + sqrt(mu) * z + 0.5);
-  return v < 0.0 ? 0u : (uint32_t)fmin(v, 4.0e9);
+  const float u2 = unif(fmix(h ^ 0xd6e8feb8u));
+  const float z = sqrtf(-2.f * logf(u)) * cosf(6.2831853f * u2);
+  const float v = floorf(mu + sqrtf(mu) * z + 0.5f);
+  return v < 0.f ? 0u : (uint32_t)fminf(v, 4.0e9f);
 }
 
-__global__ void k_background(xpg_synth_bg p, uint32_t *frames) {
-  const int64_t npx = (int64_t)p.nx * p.ny;
-  const int64_t total = npx * p.n;
-  const double rmax = sqrt(fmax(p.cx * p.cx, (p.nx - p.cx) * (p.nx - p.cx)) + fmax(p.cy * p.cy, (p.ny - p.cy) * (p.ny - p.cy)));
-  for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t t = g / npx, px = g - t * npx;
-    const int iy = (int)(px / p.nx), ix = (int)(px - (int64_t)iy * p.nx);
-    const double us = uni(key3(p.seed, 0xABCDull, (uint64_t)px));      // static per pixel
-    uint32_t v;
-    if (us < p.dead_frac) {
-      v = p.flag_value;
-    } else if (us < p.dead_frac + p.sat_frac) {
-      v = p.sat_value;
-    } else {
-      const double dx = ix + 1.0 - p.cx, dy = iy + 1.0 - p.cy;
-      const double lam = p.bg_centre + (p.bg_edge - p.bg_centre) * sqrt(dx * dx + dy * dy) / rmax;
-      v = poisson(lam, key3(p.seed, (uint64_t)(p.f0 + t) + 1, (uint64_t)px));
+// one thread per pixel, over the frames of the call: the pixel's static flag, mean and e^-mean are computed once
+__global__ void __launch_bounds__(256) k_background(xpg_synth_bg p, uint32_t *frames) {
+  const uint32_t npx = (uint32_t)p.nx * (uint32_t)p.ny;
+  const uint32_t seed = (uint32_t)p.seed ^ (uint32_t)(p.seed >> 32);
+  const float cx = (float)p.cx, cy = (float)p.cy;
+  const float rmax = sqrtf(fmaxf(cx * cx, (p.nx - cx) * (p.nx - cx)) + fmaxf(cy * cy, (p.ny - cy) * (p.ny - cy)));
+  for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npx; px += gridDim.x * blockDim.x) {
+    const uint32_t iy = px / (uint32_t)p.nx, ix = px - iy * (uint32_t)p.nx;
+    const float us = unif(hash3(seed, 0xabcdu, px));                  // static per pixel
+    const bool dead = us < (float)p.dead_frac, sat = !dead && us < (float)(p.dead_frac + p.sat_frac);
+    const float dx = ix + 1.f - cx, dy = iy + 1.f - cy;
+    const float lam = (float)p.bg_centre + (float)(p.bg_edge - p.bg_centre) * sqrtf(dx * dx + dy * dy) / rmax;
+    const float elam = expf(-lam);
+    for (int t = 0; t < p.n; ++t) {
+      uint32_t v;
+      if (dead) v = p.flag_value;
+      else if (sat) v = p.sat_value;
+      else v = poisson(lam, elam, hash3(seed, (uint32_t)(p.f0 + t) + 1u, px));
+      frames[(size_t)t * npx + px] = v;
     }
-    frames[g] = v;
   }
 }
 
 __global__ void __launch_bounds__(256) k_spots(xpg_synth_spots p, uint32_t *frames) {
   const int r = blockIdx.x;
   if (r >= p.n) return;
-  const double x = p.x[r], y = p.y[r], z = p.z[r], cnt = p.counts[r];
-  const double sr = p.sig_r[r], st = p.sig_t[r], sz = p.sig_z[r], rdx = p.rdx[r], rdy = p.rdy[r];
-  int za = (int)floor(z - 5.0 * sz), zb = (int)floor(z + 5.0 * sz);
+  const float x = (float)p.x[r], y = (float)p.y[r], z = (float)p.z[r], cnt = (float)p.counts[r];
+  const float sr = (float)p.sig_r[r], st = (float)p.sig_t[r], sz = (float)p.sig_z[r];
+  const float rdx = (float)p.rdx[r], rdy = (float)p.rdy[r];
+  int za = (int)floorf(z - 5.f * sz), zb = (int)floorf(z + 5.f * sz);
   za = max(za, p.f0);
   zb = min(zb, p.f0 + p.nf - 1);
   if (zb < za) return;
-  const int ix0 = (int)floor(x - 0.5), iy0 = (int)floor(y - 0.5);
+  const int ix0 = (int)floorf(x - 0.5f), iy0 = (int)floorf(y - 0.5f);
   const int w = 2 * p.half + 1, npx = w * w;
-  const int64_t npix = (int64_t)p.nx * p.ny;
-  const double norm = 1.0 / (9.0 * 6.283185307179586 * sr * st);
-  const double s2z = 1.0 / (sqrt(2.0) * sz);
-  for (int q = threadIdx.x; q < (zb - za + 1) * npx; q += blockDim.x) {
-    const int fi = za + q / npx, pp = q % npx;
+  const size_t npix = (size_t)p.nx * p.ny;
+  const float norm = 1.f / (9.f * 6.2831853f * sr * st);
+  const float s2z = 1.f / (1.41421356f * sz);
+  const float isr = 1.f / sr, ist = 1.f / st;
+  const uint32_t seed = ((uint32_t)p.seed ^ (uint32_t)(p.seed >> 32)) ^ 0x1234567u;
+  const uint32_t id = p.sid ? (uint32_t)p.sid[r] : (uint32_t)r;
+  for (int pp = threadIdx.x; pp < npx; pp += blockDim.x) {
     const int iy = iy0 - p.half + pp / w, ix = ix0 - p.half + pp % w;
     if (ix < 0 || iy < 0 || ix >= p.nx || iy >= p.ny) continue;
-    const double fz = 0.5 * (erf((fi + 1.0 - z) * s2z) - erf((fi - z) * s2z));
-    if (fz < 1e-9) continue;
-    double g = 0.0;
+    float g = 0.f;                        // 3 x 3 sub-pixel samples of the elliptical Gaussian
     for (int a = -1; a <= 1; ++a)
       for (int b = -1; b <= 1; ++b) {
-        const double dxs = ix + 1.0 + b / 3.0 - x, dys = iy + 1.0 + a / 3.0 - y;
-        const double ur = dxs * rdx + dys * rdy, ut = -dxs * rdy + dys * rdx;
-        g += exp(-0.5 * ((ur / sr) * (ur / sr) + (ut / st) * (ut / st)));
+        const float dxs = ix + 1.f + b / 3.f - x, dys = iy + 1.f + a / 3.f - y;
+        const float ur = (dxs * rdx + dys * rdy) * isr, ut = (-dxs * rdy + dys * rdx) * ist;
+        g += expf(-0.5f * (ur * ur + ut * ut));
       }
-    const double mu = cnt * fz * g * norm;
-    if (mu < 1e-7) continue;
-    const int64_t o = (int64_t)(fi - p.f0) * npix + (int64_t)iy * p.nx + ix;
-    const uint32_t cur = frames[o];
-    if (cur == p.flag_value || cur == p.sat_value) continue;
-    const uint64_t id = p.sid ? (uint64_t)p.sid[r] : (uint64_t)r;
-    const uint32_t k = poisson(mu, key3(p.seed ^ 0x1234567ull, id, ((uint64_t)fi << 32) | (uint32_t)pp));
-    if (k) atomicAdd(frames + o, k);
+    const float gn = cnt * g * norm;
+    if (gn < 1e-7f) continue;
+    const size_t o0 = (size_t)iy * p.nx + ix;
+    float ehi = erff((za - z) * s2z);
+    for (int fi = za; fi <= zb; ++fi) {
+      const float elo = ehi;
+      ehi = erff((fi + 1.f - z) * s2z);
+      const float mu = gn * 0.5f * (ehi - elo);
+      if (mu < 1e-7f) continue;
+      const size_t o = (size_t)(fi - p.f0) * npix + o0;
+      const uint32_t cur = frames[o];
+      if (cur == p.flag_value || cur == p.sat_value) continue;
+      const uint32_t k = poisson(mu, expf(-mu), hash3(seed ^ id, (uint32_t)fi, (uint32_t)pp));
+      if (k) atomicAdd(frames + o, k);
+    }
   }
 }
 
@@ -109,10 +117,11 @@ extern "C" {
 
 int xpg_synth_background(const xpg_synth_bg *p, uint32_t *frames, void *stream) {
   if (!p || !frames || p->nx <= 0 || p->ny <= 0 || p->n < 0) return -1;
-  const int64_t total = (int64_t)p->nx * p->ny * p->n;
-  if (!total) return 0;
-  int64_t grid = (total + 255) / 256;
-  if (grid > (1 << 20)) grid = 1 << 20;
+  const int64_t npx = (int64_t)p->nx * p->ny;
+  if (!npx || !p->n) return 0;
+  if (npx >= (1ll << 32)) return -1;
+  int64_t grid = (npx + 255) / 256;
+  if (grid > (1 << 16)) grid = 1 << 16;
   hipLaunchKernelGGL(k_background, dim3((unsigned)grid), dim3(256), 0, (hipStream_t)stream, *p, frames);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
