@@ -308,6 +308,9 @@ def main():
                     help="time the batched form's launch-parameter candidates (xpg_amd/autotune.py) on this workload "
                          "first and run with the fastest")
     ap.add_argument("--autotune-save", default="", help="write the autotuner's choice to this file (tuned.json form)")
+    ap.add_argument("--defer-pass2", type=int, default=1,
+                    help="batched form: launch each block's pass 2 behind the next block's pass 1 (1) or right after "
+                         "its own chain (0)")
     ap.add_argument("--prep-ahead", type=int, default=-1,
                     help="batched form: prepare blocks this many ahead of the chain (-1: all up front)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -365,7 +368,8 @@ def main():
     from xpg_amd.integrate import DZ_BUCKETS  # noqa: F401
     opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
                 bkg_template=None, moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
-                sm=np.radians(model.sigma_m), prep_ahead=None if args.prep_ahead < 0 else args.prep_ahead)
+                sm=np.radians(model.sigma_m), prep_ahead=None if args.prep_ahead < 0 else args.prep_ahead,
+                defer_pass2=bool(args.defer_pass2))
     tuned = None
     if args.autotune:
         from xpg_amd import autotune

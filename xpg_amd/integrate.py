@@ -744,6 +744,10 @@ class Sweep:
                 ready[b].record(self.xs)
 
         p1_done = {}
+        # batched: a block's pass 2 is launched behind the next block's pass 1, so that it fills the device while the
+        # (small) adaptive-fit launches of that next block run, instead of sharing it with pass 1
+        defer = batched and o.get("defer_pass2", True)
+        held = []
         if batched:
             self.xs.wait_stream(main)
             for b in (self.order if ahead is None else self.order[:ahead + 1]):
@@ -771,8 +775,17 @@ class Sweep:
                         L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(call0),
                                                     C.byref(sp.refl_view(start, m, (B, bx, bx), mixed=True).s),
                                                     C.byref(self.store), C.byref(self.prep_g[gi]), st))
-            self._block_chain(i, b, units, rb, main, side, timer, cross=batched,
-                              after_pass1=lambda b=b: p1_done.__setitem__(b, _record(main)))
+            def after_pass1(b=b):
+                p1_done[b] = _record(main)
+                if held:                 # the previous block's pass 2, behind this block's pass 1 (see below)
+                    held.pop()()
+
+            f = self._block_chain(i, b, units, rb, main, side, timer, cross=batched, after_pass1=after_pass1,
+                                  defer=defer)
+            if f is not None:
+                held.append(f)
+        while held:
+            held.pop()()
         if batched:
             main.wait_stream(side)
         return rb.result()
@@ -811,10 +824,10 @@ class Sweep:
             c = self._call_cache[i] = (call1, call2, fit)
         return c
 
-    def _block_chain(self, i, b, units, rb, main, side, timer=None, cross=False, after_pass1=None):
+    def _block_chain(self, i, b, units, rb, main, side, timer=None, cross=False, after_pass1=None, defer=False):
         """The i-th block (table block b): pass 1 of every launch unit, the adaptive fits and the profile
         normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``); outputs into ``rb``
-        (RunBufs)."""
+        (RunBufs).  ``defer``: pass 2 is not launched but returned as a callable, for the caller to launch later."""
         from . import _lib as L
         o, device = self.o, self.device
         nullctx = _Null()
@@ -857,24 +870,32 @@ class Sweep:
             prof = self.gauss_rep
         rb.prof_by_block[b] = prof
         if o["moments_only"]:
-            return
-        if cross:
-            side.wait_stream(main)
-        sst = L.stream_ptr(device) if side is main else side.cuda_stream
-        for start, m, shape, mixed, pp in units:
-            view = self._view(start, m, shape, mixed)
-            out = L.Pass2Out()
-            for k in K.P2_F64:
-                setattr(out, k, rb.p2(k, start))
-            bg, nbg = rb.p1("bg", start), rb.p1("nbg", start)
-            with (timer("pass2", b, (start, m), side) if timer else nullctx):
-                if pp is not None:
-                    L.check(lib.xpg_pass2_prepared(C.byref(det.s), C.byref(call2), C.byref(view.s), C.byref(pp),
-                                                   L.ptr(prof), int(prof.shape[0]), bg, nbg, 0, C.byref(out), sst))
-                else:
-                    L.check(lib.xpg_pass2_frames(C.byref(det.s), C.byref(call2), C.byref(view.s),
-                                                 C.byref(self.store), L.ptr(prof), int(prof.shape[0]), bg, nbg, 0,
-                                                 C.byref(out), sst))
+            return None
+
+        def pass2():
+            if cross:
+                side.wait_stream(main)
+            sst = L.stream_ptr(device) if side is main else side.cuda_stream
+            for start, m, shape, mixed, pp in units:
+                view = self._view(start, m, shape, mixed)
+                out = L.Pass2Out()
+                for k in K.P2_F64:
+                    setattr(out, k, rb.p2(k, start))
+                bg, nbg = rb.p1("bg", start), rb.p1("nbg", start)
+                with (timer("pass2", b, (start, m), side) if timer else nullctx):
+                    if pp is not None:
+                        L.check(lib.xpg_pass2_prepared(C.byref(det.s), C.byref(call2), C.byref(view.s), C.byref(pp),
+                                                       L.ptr(prof), int(prof.shape[0]), bg, nbg, 0, C.byref(out),
+                                                       sst))
+                    else:
+                        L.check(lib.xpg_pass2_frames(C.byref(det.s), C.byref(call2), C.byref(view.s),
+                                                     C.byref(self.store), L.ptr(prof), int(prof.shape[0]), bg, nbg,
+                                                     0, C.byref(out), sst))
+
+        if defer:
+            return pass2
+        pass2()
+        return None
 
     def arena(self):
         """Prepared-data buffers for one block at a time (the largest block's bins back to back): the single-stream
