@@ -330,8 +330,8 @@ def main():
         print(json.dumps(summarize_pmc(f, w, int(mib), kern, wl, cmd)))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one MI355X: every box here has one GPU, so --gpus must be 1 (multi-GPU sharding is "
-              "described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py measures one MI355X: --gpus must be 1 (the sweep shards by phi range and serving by dataset "
+              "across GPUs; neither is run here)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

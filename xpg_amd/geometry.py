@@ -3,7 +3,7 @@
 Follows the conventions of xpg/geometry.py:1-14 (Kabsch 1988/2010): right-handed lab frame, S0 = beam / lambda,
 pixel (x, y) continuous with array element ix centred at x = ix + 1 (PIX0), lab position
 X = QX (x - ORGX) ED1 + QY (y - ORGY) ED2 + F ED3, frame i covering z in [i - 1, i), phi(z) = phi0 + osc (z - (first - 1)),
-A* columns = a*, b*, c* at phi = 0.  Only what the integrator reads is provided: a flat single panel with
+A* columns = a*, b*, c* at phi = 0.  Only what the integrator reads is provided (the Geometry fields of xpg/geometry.py:129-407 it uses): a flat single panel with
 optional sensor parallax; the integration kernels themselves also accept segmented detectors through
 ``detector_consts``.
 """
