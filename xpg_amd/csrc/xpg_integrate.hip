@@ -636,7 +636,55 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   double Sc = wsum_d(sc0);
   double nbg = wsum_d(sT0);
   double bk = Sc / fmax(nbg, 1e-6);
-  for (int round = 1; round < 5; ++round) {
+  // boxes of at most NPL pixels x DZM frames per lane: the background candidates' values stay in registers across
+  // the rounds (the same sums in the same order)
+  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
+  const bool regs = P <= NPL * WAVE && Dz <= DZM;
+  if (regs) {
+    float bgv[NK], brq[NPL];
+    uint32_t bgm = 0;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      brq[q] = p < P ? bref[p] : 0.f;
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        bgv[k] = 0.f;
+        if (p < P && j < Dz && (vflag[j * P + p] & VF_BGM)) {
+          bgm |= 1u << k;
+          bgv[k] = vals[j * P + p];
+        }
+      }
+    }
+    for (int round = 1; round < 5; ++round) {
+      double sc = 0.0, sT = 0.0;
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) {
+        if (lane + q * WAVE >= P) continue;
+        const double bp = bk * (double)brq[q];
+        const float th = __double2float_rd((bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0);
+        double cs = 0.0;
+        int cnt = 0;
+#pragma unroll
+        for (int j = 0; j < DZM; ++j) {
+          const int k = q * DZM + j;
+          if (((bgm >> k) & 1u) && bgv[k] <= th) {
+            cs += (double)bgv[k];
+            ++cnt;
+          }
+        }
+        sc += cs;
+        sT += (double)cnt * (double)brq[q];
+      }
+      Sc = wsum_d(sc);
+      nbg = wsum_d(sT);
+      const double bk_prev = bk;
+      bk = Sc / fmax(nbg, 1e-6);
+      if (bk == bk_prev) break;
+    }
+  }
+  for (int round = 1; round < 5 && !regs; ++round) {
     for (int p = lane; p < P; p += WAVE) {
       const double bp = bk * (double)bref[p];
       const double t = (bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0;
@@ -1068,13 +1116,58 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
 
   // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
   const float gainf = (float)gain;
+  // Rows without a partner whose voxels fit NPL pixels x DZM frames per lane keep their fit voxels' net counts and
+  // profile values in registers across the iterations (the same products, summed in the same order)
+  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
+  const bool regs = !partner && P <= NPL * WAVE && Dz <= DZM;
+  float netr[NK], pfr[NK], b0r[NPL];
+  uint32_t usem = 0, inm = 0, validm = 0, fgm = 0;   // bit k: in the box / fit voxel / valid & ~nbm / foreground
+  if (regs) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      const bool pin = p < P;
+      const float b0 = pin ? (float)(bgr * (double)bref[p]) : 0.f;
+      const float p2f = pin ? (float)p2[p] : 0.f;
+      b0r[q] = b0;
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        netr[k] = 0.f;
+        pfr[k] = 0.f;
+        if (pin && j < Dz) {
+          const int v = j * P + p;
+          const uint8_t fl = vflag[v];
+          const float c = vals[v];
+          inm |= 1u << k;
+          pfr[k] = p2f * Rf[j];
+          netr[k] = c - b0;
+          if (fl & VF_USE) usem |= 1u << k;
+          if (c >= 0.f && !(fl & VF_NBM)) validm |= 1u << k;
+          if (fl & VF_FG) fgm |= 1u << k;
+        }
+      }
+    }
+  }
   double a11 = 0.0, a12 = 0.0, a22 = 1.0, det2 = 1.0, Iw = I, Ipw = Ip;
   for (int it_ = 0; it_ < N_IRLS; ++it_) {
     Iw = I;
     Ipw = Ip;
     const float Ii = (float)fmax(I, 0.0), Ipi = (float)fmax(Ip, 0.0);
     double s11 = 0.0, s12 = 0.0, s22 = 0.0, sb1 = 0.0, sb2 = 0.0;
-    for (int p0 = 0; p0 < P; p0 += WAVE) {
+    if (regs) {
+#pragma unroll
+      for (int k = 0; k < NK; ++k) {
+        if ((usem >> k) & 1u) {
+          const float pf = pfr[k], net = netr[k];
+          const float mu = b0r[k / DZM] + Ii * pf;         // the partner term is +0 here
+          const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));
+          s11 += (double)((pf * pf) * iv);
+          sb1 += (double)((pf * net) * iv);
+        }
+      }
+    }
+    for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
       const int p = p0 + lane;
       if (p < P) {
         const float b0 = (float)(bgr * (double)bref[p]);
@@ -1117,7 +1210,31 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const float Ipos = (float)fmax(Ip, 0.0);
   double sC = 0.0, s110 = 0.0, sC0 = 0.0, swp = 0.0, sres = 0.0;
   int nuse = 0;
-  for (int p0 = 0; p0 < P; p0 += WAVE) {
+  if (regs) {                            // the same sums over the register-held fit voxels (partner terms +0)
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      const float br = p < P ? bref[p] : 0.f;
+      const float iv0 = __builtin_amdgcn_rcpf(gainf * fmaxf(b0r[q], (float)VAR_FLOOR));
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        if ((usem >> k) & 1u) {
+          const float pf = pfr[k], net = netr[k];
+          const float mu = b0r[q] + Ii * pf;
+          const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));
+          const float pb = pf * br;
+          sC += (double)(pb * iv);
+          s110 += (double)((pf * pf) * iv0);
+          sC0 += (double)(pb * iv0);
+          swp += (double)pf;
+          sres += (double)net;
+          nuse += 1;
+        }
+      }
+    }
+  }
+  for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
     const int p = p0 + lane;
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]), br = bref[p];
@@ -1146,7 +1263,23 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double nu = fmax((double)nuse, 1.0);
   const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
   double sdd = 0.0, sdp = 0.0, snn = 0.0, s_pv = 0.0, s_pt = 0.0, s_pfg = 0.0;
-  for (int p0 = 0; p0 < P; p0 += WAVE) {
+  if (regs) {
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      if (!((inm >> k) & 1u)) continue;
+      const float pf = pfr[k];
+      s_pt += (double)pf;
+      if ((validm >> k) & 1u) s_pv += (double)pf;
+      if ((fgm >> k) & 1u) s_pfg += (double)pf;
+      if (!((usem >> k) & 1u)) continue;
+      const float dp = pf - pm;
+      const float dn = netr[k] - nm;       // (net - Ipos pp) - nm with pp = +0
+      sdp += (double)(dp * dn);
+      sdd += (double)(dp * dp);
+      snn += (double)(dn * dn);
+    }
+  }
+  for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
     const int p = p0 + lane;
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]);
