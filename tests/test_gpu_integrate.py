@@ -91,3 +91,31 @@ def test_integrate_matches_oracle_sweep(which):
         assert rr.block_params == res.block_params, rep
         sw.run()
         torch.cuda.synchronize()
+
+
+def test_integrate_options_match_oracle_sweep():
+    """integrate() with the integrator's own pixel mask, excluded frames, fixed widths (adaptive=False) and the
+    Gaussian prior profiles (learn_profiles=False) against the oracle sweep given the same options."""
+    from xpg_amd import integrate as I, synth
+    spec = synth.replace(synth.spec_small(), seed=41)
+    print("integrate options", spec.name, flush=True)
+    frames, exp, edges, model, blocks, dmin, det = _dataset(spec)
+    rng = np.random.default_rng(5)
+    mask = rng.random((spec.ny, spec.nx)) > 0.02          # True = trusted
+    excl = (17, 18, 61)
+    kw = dict(mask=mask, exclude_frames=excl, adaptive=False, learn_profiles=False)
+    res = I.integrate(frames, blocks, edges, model, dmin, verbose=False, **kw)
+    hp = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False, **kw)
+    coded = _coded(frames)
+    read = [f for f in range(spec.nframes) if f + 1 not in excl]
+    out, _ = osw.integrate_sweep(coded, read, hp, blocks, model, det, adaptive=False, learn_profiles=False, mask=mask)
+    r = res.refl
+    ok = np.isfinite(out["iprf"])
+    assert ok.sum() > 0.8 * ok.size and np.array_equal(np.isfinite(r["iprf"]), ok)
+    assert np.array_equal(r["n_over"], out["n_over"])
+    dI = np.abs(r["iprf"][ok] - out["iprf"][ok]) / np.sqrt(out["vprf"][ok])
+    dS = np.abs(r["isum"][ok] - out["isum"][ok]) / np.sqrt(np.maximum(out["vsum"][ok], 1e-6))
+    print(f"options: {ok.sum()} reflections, max |dI|/sigma prf {dI.max():.2e} sum {dS.max():.2e}", flush=True)
+    assert dI.max() < 1e-3 and dS.max() < 1e-3
+    np.testing.assert_allclose(r["frac_rec"][ok], out["frac_rec"][ok], rtol=1e-9, atol=1e-12)
+    assert res.block_params == {}
