@@ -119,3 +119,34 @@ def test_integrate_options_match_oracle_sweep():
     assert dI.max() < 1e-3 and dS.max() < 1e-3
     np.testing.assert_allclose(r["frac_rec"][ok], out["frac_rec"][ok], rtol=1e-9, atol=1e-12)
     assert res.block_params == {}
+
+
+def test_integrate_lattices_foreign_spots_template_match_oracle():
+    """A second lattice integrated simultaneously (its reflections mask or share boxes with the first), foreign spots
+    masked in the boxes, and a background template, against the oracle sweep on the same plan."""
+    from xpg_amd import integrate as I, synth
+    from xpg_amd.geometry import detector_consts
+    spec = synth.replace(synth.spec_small(), seed=43)
+    print("integrate lattices/foreign/template", spec.name, flush=True)
+    frames, exp, edges, model, blocks, dmin, _ = _dataset(spec)
+    exp2 = synth.make_experiment(synth.replace(spec, seed=431))           # same cell, another orientation
+    rng = np.random.default_rng(9)
+    foreign = (rng.uniform(1, spec.nx, 300), rng.uniform(1, spec.ny, 300), rng.uniform(0, spec.nframes, 300))
+    yy, xx = np.mgrid[0:spec.ny, 0:spec.nx]
+    template = (0.6 + 0.4 * np.hypot(xx - spec.nx / 2, yy - spec.ny / 2) / (0.7 * spec.nx)).astype(np.float32)
+    lat2 = [exp2] * len(blocks)
+    kw = dict(extra_lattices=[lat2], foreign_spots=foreign, bkg_template=template)
+    res = I.integrate(frames, blocks, edges, model, dmin, verbose=False, **kw)
+    hp = I._integrate(frames, blocks, edges, model, dmin, plan_only=True, verbose=False, **kw)
+    coded = _coded(frames)
+    det = detector_consts(exp.geom, template)
+    out, _ = osw.integrate_sweep(coded, list(range(spec.nframes)), hp, list(blocks) + lat2, model, det)
+    r = res.refl
+    assert (r["lattice"] == 1).sum() > 0.3 * r["lattice"].size
+    ok = np.isfinite(out["iprf"])
+    assert ok.sum() > 0.8 * ok.size and np.array_equal(np.isfinite(r["iprf"]), ok)
+    assert np.array_equal(r["n_over"], out["n_over"])
+    dI = np.abs(r["iprf"][ok] - out["iprf"][ok]) / np.sqrt(out["vprf"][ok])
+    print(f"lattices: {ok.sum()} reflections, max |dI|/sigma prf {dI.max():.2e}", flush=True)
+    assert dI.max() < 1e-3
+    np.testing.assert_allclose(r["bg"][ok], out["bg"][ok], rtol=1e-6, atol=1e-9)
