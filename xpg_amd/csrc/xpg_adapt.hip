@@ -141,9 +141,9 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
 
 // ---- 2. grid of sigma_M candidates -------------------------------------------------------------------------------
 
-// erf(x), with |x| > ERF_SAT taken as saturated (float64 erf is +-1 to the last bit there)
-constexpr double ERF_SAT = 6.5;
-__device__ __forceinline__ double erf_sat(double x) { return fabs(x) > ERF_SAT ? copysign(1.0, x) : xpg::erf_bf(x); }
+// erf(x); beyond |x| = ERF_SAT it is +-1 exactly (float64 erf rounds to +-1 there, and erf_tab returns it)
+constexpr double ERF_SAT = 6.0;
+__device__ __forceinline__ double erf_sat(double x) { return xpg::erf_tab(x); }
 
 __global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
   __shared__ double red[256];

@@ -243,13 +243,13 @@ __device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double h
   if (!use_rcw_of(c)) {
     double a = (hi - phi) / (s * sqrt(2.0));
     double b = (lo - phi) / (s * sqrt(2.0));
-    return 0.5 * (erf_bf(a) - erf_bf(b));
+    return 0.5 * (erf_tab(a) - erf_tab(b));
   }
   double sw = (s * 0.25) * sqrt(2.0);
   double cj = (double)RC_C32[j];
   double a = ((hi - phi) - cj * s) / sw;
   double b = ((lo - phi) - cj * s) / sw;
-  return ((double)rcw_of(c, j) * 0.5) * (erf_bf(a) - erf_bf(b));
+  return ((double)rcw_of(c, j) * 0.5) * (erf_tab(a) - erf_tab(b));
 }
 
 __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb, int j, double &lo, double &hi) {
