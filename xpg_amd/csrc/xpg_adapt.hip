@@ -3,10 +3,11 @@
 // pass 1 -> fits -> pass 2 -> next block runs without a host round trip.
 //
 //   k_ad_prep    (n/256 x 256)   strong-row flags, fz totals and sigma_D terms per row; strong rows per workgroup
-//   k_ad_grid    (61 x NCH x 256) partial cost of each sigma_M of the 61-point log grid over a fixed row range
+//   k_ad_grid    (1 + NGW x 1024) workgroup 0: exact median of the sigma_D terms by radix selection of the two middle
+//                                ranks; the others: partial cost of each sigma_M of the 61-point log grid over fixed
+//                                row ranges, one wavefront per strong row and one lane per grid point
 //                                (_fit_rocking_width, :1130-1154)
-//   k_ad_finish  (1 x 1024)      argmin -> sigma_M fit and shrinkage; exact median of the sigma_D terms by radix
-//                                selection of the two middle ranks
+//   k_ad_finish  (1 x 64)        strong-row count, argmin over the grid -> sigma_M fit and shrinkage
 //   k_ad_gram    (NG x 256)      Gram matrix A^T A and A^T y of the rocking-curve basis over a fixed row range
 //                                (rc_basis_np, :437-443)
 //   k_ad_gsum    (NE x 256)      fixed-order reduction of the Gram partials, one workgroup per entry
@@ -26,7 +27,7 @@ namespace {
 constexpr int K = XPG_RC_K;               // 21 mixture components
 constexpr int NF = 65;                     // frames of a per-frame profile row (max(DZ_BUCKETS))
 constexpr int NG = 1024;                   // workgroups of the Gram accumulation
-constexpr int NCH = 20;                    // row ranges of the grid costs (61 x 20 x 4 wavefronts: one round)
+constexpr int NGW = 128;                   // workgroups of the grid costs (16 wavefronts, one strong-row range each)
 constexpr int NE = K * (K + 1) / 2 + K;    // unique Gram entries + right-hand side
 constexpr int NGRID = 61;
 constexpr double PI = 3.141592653589793;
@@ -43,7 +44,7 @@ struct Ws {
   double *tot;        // n: fz total
   int32_t *nzl;       // n: 1 + the last frame with fz != 0 (0: none)
   int32_t *cnt;       // nprep: strong rows per k_ad_prep workgroup
-  double *cost;       // NGRID x NCH partial grid costs
+  double *cost;       // NGW x 64 partial grid costs (lane c < NGRID: grid point c)
   double *gram;       // NG x NE partial Gram entries
   double *gtot;       // NE Gram totals
   double *scal;       // n_strong, sm_new, sm_fit, sd_new, sd_fit
@@ -65,7 +66,7 @@ __host__ __device__ inline Ws carve(void *base, int32_t n) {
   w.tot = (double *)take(8 * m);
   w.nzl = (int32_t *)take(4 * m);
   w.cnt = (int32_t *)take(4 * (size_t)w.nprep);
-  w.cost = (double *)take(8 * NGRID * NCH);
+  w.cost = (double *)take(8 * NGW * 64);
   w.gram = (double *)take(8 * (size_t)NG * NE);
   w.gtot = (double *)take(8 * (size_t)NE);
   w.scal = (double *)take(8 * 8);
@@ -145,56 +146,6 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
 constexpr double ERF_SAT = 6.0;
 __device__ __forceinline__ double erf_sat(double x) { return xpg::erf_tab(x); }
 
-__global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
-  __shared__ double red[256];
-  const int c = blockIdx.x, ch = blockIdx.y;
-  int r0, r1;
-  row_range(a.n, ch, NCH, r0, r1);
-  const double guess = a.sm_run * (180.0 / PI);
-  const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
-  const double smv = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
-  double part = 0.0;
-  for (int r = r0 + threadIdx.x; r < r1; r += 256) {
-    if (!w.st[r]) continue;
-    const double tot = w.tot[r];
-    if (!(tot > 0)) continue;
-    const double phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
-    const double s = smv / zeta * sqrt(2.0);
-    const double inv_s = 1.0 / s, inv_tot = 1.0 / tot;
-    const float *fz = a.fz + (int64_t)r * NF;
-    const int iz0 = a.iz0[r];
-    // frame k spans [lo, hi] = sorted(phi0 + osc (iz0 + k), phi0 + osc (iz0 + k + 1)); the row total of R telescopes
-    const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) * inv_s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) * inv_s);
-    const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9), inv_den = 1.0 / den;
-    // the boundaries move monotonically away from phi: once one is past the saturation point every later erf is the
-    // same +-1 (erf_sat), so it is not evaluated again
-    double cst = 0.0, Eprev = e0;
-    bool sat = false;
-    const int kend = w.nzl[r];
-    for (int k = 0; k < NF; ++k) {
-      // past the last non-zero fz with the rocking curve saturated every term is tot * (+0)^2: cst is final
-      if (sat && k >= kend) break;
-      double E;
-      if (k == NF - 1) {
-        E = e1;
-      } else if (sat) {
-        E = Eprev;
-      } else {
-        const double x = (bound_deg(a, iz0, k + 1) - phi) * inv_s;
-        sat = fabs(x) > ERF_SAT && (x > 0) == (a.osc >= 0);
-        E = erf_sat(x);
-      }
-      const double R = a.osc >= 0 ? 0.5 * (E - Eprev) : 0.5 * (Eprev - E);
-      const double d = (double)fz[k] * inv_tot - R * inv_den;
-      cst = fma(tot, d * d, cst);
-      Eprev = E;
-    }
-    part += cst;
-  }
-  part = block_sum256(part, red);
-  if (threadIdx.x == 0) w.cost[c * NCH + ch] = part;
-}
-
 // ---- 3. sigma_M from the grid, sigma_D from the median ----------------------------------------------------------
 
 // the bin of a 256-bin histogram holding rank k (one wavefront): returns the bin, k reduced to the rank inside it
@@ -237,47 +188,33 @@ __device__ __forceinline__ void hist_add(uint32_t *h, uint32_t bin, bool live) {
   }
 }
 
-__global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
+// Exact median of the strong rows' sigma_D terms (1024 threads): radix selection, 8 bits per round, of the two middle
+// ranks; sigma_D fit and shrinkage into scal[3], scal[4].  Needs only k_ad_prep's outputs.
+__device__ void median_select(const In &a, const Ws &w) {
   __shared__ int32_t s_n;
-  __shared__ double s_part[NGRID * NCH];
-  __shared__ double s_cost[NGRID];
   __shared__ uint32_t hist[2][256];
   __shared__ uint64_t s_pre[2];
   __shared__ uint64_t s_mm[2];
   __shared__ uint32_t s_k[2];
+  __shared__ uint64_t s_msk[2];            // bits of the key fixed so far (prefix s_pre)
+  __shared__ int32_t s_done[2], s_cn[2];   // rank q: at most 64 keys share its prefix; candidates gathered
+  __shared__ uint64_t s_cand[2][64];
   const int t = threadIdx.x;
   if (t == 0) {
     s_n = 0;
+    s_done[0] = s_done[1] = 0;
+    s_cn[0] = s_cn[1] = 0;
     s_mm[0] = ~0ull;
     s_mm[1] = 0;
   }
-  for (int i = t; i < NGRID * NCH; i += 1024) s_part[i] = w.cost[i];
   __syncthreads();
   int cn = 0;
   for (int b = t; b < w.nprep; b += 1024) cn += w.cnt[b];
   if (cn) atomicAdd(&s_n, cn);           // integer counts: order-free
-  if (t < NGRID) {
-    double s = 0.0;
-    for (int ch = 0; ch < NCH; ++ch) s += s_part[t * NCH + ch];
-    s_cost[t] = s;
-  }
   __syncthreads();
   const int n = s_n;
-  if (t == 0) w.scal[0] = (double)n;
   if (n < 100) return;
   if (t == 0) {
-    const double guess = a.sm_run * (180.0 / PI);
-    const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
-    double best = INFINITY, bsm = guess;
-    for (int c = 0; c < NGRID; ++c) {
-      if (s_cost[c] < best) {
-        best = s_cost[c];
-        bsm = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
-      }
-    }
-    const double sm_fit = bsm * (PI / 180.0);
-    w.scal[1] = exp(((double)n * log(sm_fit) + 300.0 * log(a.sm_run)) / ((double)n + 300.0));
-    w.scal[2] = sm_fit;
     s_pre[0] = s_pre[1] = 0;
     s_k[0] = (uint32_t)((n - 1) / 2);     // the two middle ranks (equal for odd n)
     s_k[1] = (uint32_t)(n / 2);
@@ -329,22 +266,28 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
   __syncthreads();
   const uint64_t diff = s_mm[0] ^ s_mm[1];
   const int top = diff ? (63 - __clzll((long long)diff)) / 8 * 8 : 0;   // byte of the highest differing bit
-  if (t == 0) s_pre[0] = s_pre[1] = top == 56 ? 0ull : (s_mm[0] & (~0ull << (top + 8)));
+  if (t == 0) {
+    s_pre[0] = s_pre[1] = top == 56 ? 0ull : (s_mm[0] & (~0ull << (top + 8)));
+    s_msk[0] = s_msk[1] = top == 56 ? 0ull : (~0ull << (top + 8));
+  }
   const bool two = s_k[0] != s_k[1];
   __syncthreads();
-  for (int shift = top; shift >= 0; shift -= 8) {
+  // a round fixes the next byte of each rank's key; once at most 64 keys share a rank's prefix, the rank is finished
+  // among those candidates instead (most blocks need two rounds, not seven)
+  for (int shift = top; shift >= 0 && !(s_done[0] && s_done[1]); shift -= 8) {
     for (int i = t; i < 512; i += 1024) (&hist[0][0])[i] = 0;
     __syncthreads();
     const uint64_t hm = shift == 56 ? 0ull : (~0ull << (shift + 8));
     const uint64_t p0 = s_pre[0], p1 = s_pre[1];
+    const bool a0 = !s_done[0], a1 = two && !s_done[1];
     if (in_regs) {
 #pragma unroll
       for (int i = 0; i < KR; ++i) {
         const uint64_t key = kr[i];
         const uint32_t bin = (uint32_t)(key >> shift) & 255u;
         const bool live = (kv >> i) & 1u;
-        hist_add(hist[0], bin, live && (key & hm) == p0);
-        if (two) hist_add(hist[1], bin, live && (key & hm) == p1);
+        if (a0) hist_add(hist[0], bin, live && (key & hm) == p0);
+        if (a1) hist_add(hist[1], bin, live && (key & hm) == p1);
       }
     } else {
       for (int r0 = 0; r0 < a.n; r0 += 1024) {
@@ -352,24 +295,70 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
         const bool live = r < a.n && w.st[r];
         const uint64_t key = live ? w.key[r] : 0;
         const uint32_t bin = (uint32_t)(key >> shift) & 255u;
-        hist_add(hist[0], bin, live && (key & hm) == p0);
-        if (two) hist_add(hist[1], bin, live && (key & hm) == p1);
+        if (a0) hist_add(hist[0], bin, live && (key & hm) == p0);
+        if (a1) hist_add(hist[1], bin, live && (key & hm) == p1);
       }
     }
     __syncthreads();
-    if (t < (two ? 128 : 64)) {
+    if (t < (two ? 128 : 64) && !s_done[t >> 6]) {
       const int q = t >> 6;
       uint32_t k = s_k[q];
       const int bin = select_bin(hist[q], k);
       if ((t & 63) == 0) {
+        const int32_t cnt = (int32_t)hist[q][bin];
         s_pre[q] |= (uint64_t)bin << shift;
+        s_msk[q] = ~0ull << shift;
         s_k[q] = k;
+        s_done[q] = cnt <= 64;
         if (!two) {
           s_pre[1] = s_pre[0];
+          s_msk[1] = s_msk[0];
           s_k[1] = k;
+          s_done[1] = s_done[0];
         }
       }
     }
+    __syncthreads();
+  }
+  // ranks finished early: gather the keys with the rank's prefix (at most 64; any order), then the candidate whose
+  // rank among them (ties broken by slot) is the remaining rank
+  const bool g0 = s_msk[0] != ~0ull, g1 = two && s_msk[1] != ~0ull;
+  if (g0 || g1) {
+    const uint64_t m0 = s_msk[0], m1 = s_msk[1], p0 = s_pre[0], p1 = s_pre[1];
+    auto gather = [&](uint64_t key) {
+      if (g0 && (key & m0) == p0) {
+        const int i = atomicAdd(&s_cn[0], 1);
+        if (i < 64) s_cand[0][i] = key;
+      }
+      if (g1 && (key & m1) == p1) {
+        const int i = atomicAdd(&s_cn[1], 1);
+        if (i < 64) s_cand[1][i] = key;
+      }
+    };
+    if (in_regs) {
+#pragma unroll
+      for (int i = 0; i < KR; ++i)
+        if ((kv >> i) & 1u) gather(kr[i]);
+    } else {
+      for (int r = t; r < a.n; r += 1024)
+        if (w.st[r]) gather(w.key[r]);
+    }
+    __syncthreads();
+    if (t < 128) {
+      const int q = t >> 6, l = t & 63;
+      if (q == 0 ? g0 : g1) {
+        const int cn = min(s_cn[q], 64);
+        const uint64_t c = l < cn ? s_cand[q][l] : ~0ull;
+        int rank = 0;
+        for (int j = 0; j < cn; ++j) {
+          const uint64_t o = s_cand[q][j];
+          rank += (o < c) || (o == c && j < l);
+        }
+        if (l < cn && rank == (int)s_k[q]) s_pre[q] = c;
+      }
+    }
+    __syncthreads();
+    if (t == 0 && !two) s_pre[1] = s_pre[0];
     __syncthreads();
   }
   if (t == 0) {
@@ -383,6 +372,141 @@ __global__ void __launch_bounds__(1024) k_ad_finish(In a, Ws w) {
     sd_fit = fmin(fmax(sd_fit, 0.5 * a.sd_run), 2.0 * a.sd_run);
     w.scal[3] = exp(((double)n * log(sd_fit) + 300.0 * log(a.sd_run)) / ((double)n + 300.0));
     w.scal[4] = sd_fit;
+  }
+}
+
+// strong-row count (scal[0]) and the sigma_M fit: the grid costs summed over the workgroups in order, the first
+// minimum, shrinkage towards the running value (scal[1], scal[2])
+__global__ void __launch_bounds__(64) k_ad_finish(In a, Ws w) {
+  const int lane = threadIdx.x;
+  int cn = 0;
+  for (int b = lane; b < w.nprep; b += 64) cn += w.cnt[b];
+  for (int o = 32; o > 0; o >>= 1) cn += __shfl_xor(cn, o, 64);
+  const int n = cn;
+  if (lane == 0) w.scal[0] = (double)n;
+  if (n < 100) return;
+  double cst = 0.0;
+  for (int g = 0; g < NGW; ++g) cst += w.cost[g * 64 + lane];
+  // first minimum over the grid points (lanes 0 .. NGRID - 1) below +inf; none: the running value
+  double best = lane < NGRID ? cst : INFINITY;
+  for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o, 64));
+  const uint64_t hit = __ballot(lane < NGRID && cst == best && cst < INFINITY);
+  const int c = hit ? __ffsll((long long)hit) - 1 : -1;
+  if (lane == 0) {
+    const double guess = a.sm_run * (180.0 / PI);
+    const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
+    const double bsm = c < 0 ? guess : guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
+    const double sm_fit = bsm * (PI / 180.0);
+    w.scal[1] = exp(((double)n * log(sm_fit) + 300.0 * log(a.sm_run)) / ((double)n + 300.0));
+    w.scal[2] = sm_fit;
+  }
+}
+
+// Grid costs: one wavefront per strong row (a fixed range of strong ranks per wavefront, in order), lane c for grid
+// point c.
+// The row's frame totals are one vector load (lane k: frame k < 64) broadcast by readlane; the row fields are uniform.
+__global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
+  if (blockIdx.x == 0) {                 // the median needs only k_ad_prep's outputs: it runs beside the grid
+    median_select(a, w);
+    return;
+  }
+  __shared__ double wpart[16][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g = blockIdx.x - 1;
+  const int c = lane < NGRID ? lane : NGRID - 1;      // lanes past the grid repeat its last point (not summed)
+  const double guess = a.sm_run * (180.0 / PI);
+  const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
+  const double smv = guess * exp(c == NGRID - 1 ? hi_l : (double)c * step + lo_l);
+  // The strong rows in row order; this wavefront takes strong ranks [q0, q1).  The k_ad_prep workgroup counts locate
+  // the first; windows of 64 rows (ballots of the strong flags) step through the rest.
+  int ns = 0;
+  for (int b0 = 0; b0 < w.nprep; b0 += 64) {
+    int cv = b0 + lane < w.nprep ? w.cnt[b0 + lane] : 0;
+    for (int o = 32; o > 0; o >>= 1) cv += __shfl_xor(cv, o, 64);
+    ns += cv;
+  }
+  int q0, q1;
+  row_range(ns, g * 16 + wv, NGW * 16, q0, q1);
+  int need = q1 - q0, skip = q0, base = 0;
+  for (int b0 = 0; need > 0 && b0 < w.nprep; b0 += 64) {
+    const int cv = b0 + lane < w.nprep ? w.cnt[b0 + lane] : 0;
+    int incl = cv;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += v;
+    }
+    const int all = __shfl(incl, 63, 64);
+    if (skip < all) {
+      const int L = __ffsll((long long)__ballot(incl > skip)) - 1;
+      skip -= __shfl(incl - cv, L, 64);
+      base = (b0 + L) * 256;
+      break;
+    }
+    skip -= all;
+  }
+  uint64_t m = 0;
+  base -= 64;
+  double part = 0.0;
+  while (need > 0) {
+    if (!m) {
+      base += 64;
+      if (base >= a.n) break;            // not reached: the counts are the flags' popcounts
+      m = __ballot(base + lane < a.n && w.st[base + lane]);
+      continue;
+    }
+    const int r = base + __ffsll((long long)m) - 1;
+    m &= m - 1;
+    if (skip > 0) {
+      --skip;
+      continue;
+    }
+    --need;
+    // r is uniform, and so are the row's fields
+    const double tot = w.tot[r], phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
+    const int iz0 = a.iz0[r], kend = w.nzl[r];
+    const float *fz = a.fz + (int64_t)r * NF;
+    const float fzl = fz[lane];          // frames 0 .. 63; frame 64 below
+    const float fz64 = fz[NF - 1];
+    if (!(tot > 0)) continue;
+    const double s = smv / zeta * sqrt(2.0);
+    const double inv_s = 1.0 / s, inv_tot = 1.0 / tot;
+    // frame k spans [lo, hi] = sorted(phi0 + osc (iz0 + k), phi0 + osc (iz0 + k + 1)); the row total of R telescopes
+    const double e0 = erf_sat((bound_deg(a, iz0, 0) - phi) * inv_s), e1 = erf_sat((bound_deg(a, iz0, NF) - phi) * inv_s);
+    const double den = fmax(a.osc >= 0 ? 0.5 * (e1 - e0) : 0.5 * (e0 - e1), 1e-9), inv_den = 1.0 / den;
+    // Frames in chunks of 8: the chunk's boundary erfs are evaluated together (independent table reads), then its
+    // terms are added in frame order.  The boundaries move monotonically away from phi: once boundary k0 is past the
+    // saturation point in that direction every later erf is the same +-1, so with k0 past the last non-zero fz every
+    // later term is tot * (+0)^2 and the lane's cost is final; the chunk loop stops when that holds for every lane.
+    double cst = 0.0, Eprev = e0;
+    for (int k0 = 0; k0 < NF; k0 += 8) {
+      const double xb = (bound_deg(a, iz0, k0) - phi) * inv_s;
+      const bool done = k0 >= kend && fabs(xb) > ERF_SAT && (xb > 0) == (a.osc >= 0);
+      if (!__ballot(!done)) break;
+      double E[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) E[u] = erf_sat((bound_deg(a, iz0, min(k0 + u + 1, NF)) - phi) * inv_s);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int k = k0 + u;
+        if (k < NF) {
+          const float fk = k < NF - 1
+                               ? __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, fzl), k))
+                               : fz64;
+          const double R = a.osc >= 0 ? 0.5 * (E[u] - Eprev) : 0.5 * (Eprev - E[u]);
+          const double d = (double)fk * inv_tot - R * inv_den;
+          cst = fma(tot, d * d, cst);
+          Eprev = E[u];
+        }
+      }
+    }
+    part += cst;
+  }
+  wpart[wv][lane] = part;
+  __syncthreads();
+  if (wv == 0) {
+    double t = 0.0;
+    for (int q = 0; q < 16; ++q) t += wpart[q][lane];
+    w.cost[g * 64 + lane] = t;
   }
 }
 
@@ -446,10 +570,20 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
       const int kh = live64 ? 64 : (live ? 63 - __clzll((long long)live) : -1);
       if (kh < kl) continue;
       const int nb = kh - kl + 2;          // boundaries kl .. kh + 1
-      for (int u = lane; u < nb * K; u += 64) {
-        const int b = u / K, j = u - b * K;
-        const double bd = bound_deg(a, iz0, kl + b) * (PI / 180.0);
-        Aw[b][j] = erf_sat(((bd - phi) - RC_C[j] * s) / sw);
+      for (int u0 = lane; u0 < nb * K; u0 += 4 * 64) {        // four erfs per lane at a time: the reads overlap
+        double e[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = min(u0 + 64 * i, nb * K - 1);
+          const int b = u / K, j = u - b * K;
+          const double bd = bound_deg(a, iz0, kl + b) * (PI / 180.0);
+          e[i] = erf_sat(((bd - phi) - RC_C[j] * s) / sw);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int u = u0 + 64 * i;
+          if (u < nb * K) Aw[u / K][u - (u / K) * K] = e[i];
+        }
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -613,6 +747,93 @@ __device__ __forceinline__ bool nnls_append(Nnls &S, const double *Gs, const dou
   return true;
 }
 
+// Factor an ordered set at once (lane i: ordl = the variable at position i, -1 past the m positions): right-looking
+// Cholesky with row i of the working matrix in lane i, so the dependent chain is m pivots instead of the m^2 / 2 steps
+// of m appends.  Fills S as the appends would (up to rounding).  Returns false, S reset, when a pivot is not
+// numerically positive; the caller then appends the variables one at a time, which skips such a variable.  Each
+// entry is computed by the same operations in the same order as by the appends (the same fma chains over the same
+// products), so the factor, z and the solution are bit-identical to appending the set in order.
+// X (K x K doubles of LDS) is scratch for the transposition.
+__device__ __forceinline__ bool nnls_factor(Nnls &S, const double *Gs, const double *hs, int ordl, int m, double *X,
+                                          int *pos) {
+  const int lane = threadIdx.x & 63;
+  const bool in = lane < m;
+  const int oi = in ? ordl : 0;
+  double a[K];
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    const int ok = __builtin_amdgcn_readlane(ordl, k < m ? k : 0);
+    a[k] = (in && k < m && ok >= 0) ? Gs[oi * K + (ok >= 0 ? ok : 0)] : 0.0;
+  });
+  double invd = 0.0;
+  bool good = true;
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < m && good) {
+      const double piv = rld(a[k], k);
+      if (!(piv > 0)) {
+        good = false;
+      } else {
+        const double d = sqrt(piv), id = 1.0 / d;
+        if (lane == k) {
+          a[k] = d;
+          invd = id;
+        } else if (lane > k) {
+          a[k] = a[k] * id;
+        }
+        static_for<K>([&](auto jc) {
+          constexpr int j = decltype(jc)::value;
+          if constexpr (j > k) {
+            if (j < m) {
+              const double ljk = rld(a[k], j);
+              if (lane >= j) a[j] = fma(-a[k], ljk, a[j]);
+            }
+          }
+        });
+      }
+    }
+  });
+  nnls_reset(S);
+  if (!good) return false;
+  // rows of L (zero above the diagonal and past m), their transpose into ct through LDS
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    S.a[k] = (in && k <= lane) ? a[k] : 0.0;
+  });
+  if (lane < K) {
+    static_for<K>([&](auto kc) { X[lane * K + kc.value] = S.a[kc.value]; });
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __syncthreads();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    S.ct[k] = (lane < K && lane <= k && k < m) ? X[k * K + lane] : 0.0;
+  });
+  // z = L^-1 h[P], column-oriented in position order (the appends' order of operations)
+  double z = in ? hs[oi] : 0.0;
+  static_for<K>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    if (k < m) {
+      const double zk = rld(z, k) * rld(invd, k);
+      if (lane == k) z = zk;
+      else if (lane > k) z = fma(-S.a[k], zk, z);
+    }
+  });
+  // positions of the variables
+  if (lane < K) pos[lane] = -1;
+  __syncthreads();
+  if (in) pos[oi] = lane;
+  __syncthreads();
+  S.zv = in ? z : 0.0;
+  S.invd = in ? invd : 0.0;
+  S.ordv = in ? ordl : -1;
+  S.posv = lane < K ? pos[lane] : -1;
+  S.m = m;
+  __syncthreads();
+  return true;
+}
+
 // L^T s = z; returns s_j on lane j (0 for inactive variables)
 __device__ __forceinline__ double nnls_solve(const Nnls &S) {
   const int lane = threadIdx.x & 63;
@@ -661,17 +882,50 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
   static_for<K>([&](auto k) { gmax = fmax(gmax, fabs(grow[k.value])); });
   for (int o = 32; o > 0; o >>= 1) gmax = fmax(gmax, __shfl_xor(gmax, o, 64));
   const double tol = 10.0 * 2.220446049250313e-16 * gmax * K;
+  __shared__ double Xs[K * K];
+  __shared__ int ords[64], poss[K];
   Nnls S;
   nnls_reset(S);
   double x = 0.0;                          // lane j: x_j
   int n_outer = 0, n_inner = 0;
+  // lane i: the i-th set bit of `bits` (variables in index order), -1 past them
+  auto ordered_bits = [&](uint32_t bits) {
+    ords[lane] = -1;
+    __syncthreads();
+    if (lane < K && ((bits >> lane) & 1u)) ords[__popc(bits & ((1u << lane) - 1u))] = lane;
+    __syncthreads();
+    const int o = ords[lane];
+    __syncthreads();
+    return o;
+  };
+  // lane i: the i-th kept variable of the order held in `ordv` (positions below mo, keep: bit per variable)
+  auto kept_order = [&](int ordv, int mo, uint64_t keepv) {
+    const bool kp = lane < mo && ordv >= 0 && ((keepv >> ordv) & 1ull);
+    const uint64_t kb = __ballot(kp);
+    ords[lane] = -1;
+    __syncthreads();
+    if (kp) ords[__popcll(kb & ((1ull << lane) - 1ull))] = ordv;
+    __syncthreads();
+    const int o = ords[lane];
+    __syncthreads();
+    return o;
+  };
+  // factor the ordered set at once; when a pivot is not positive, append one at a time as before (a variable that
+  // cannot be appended is left out, and with zero_failed its x is zeroed)
+  auto refactor = [&](int ordl, int mnew, bool zero_failed) {
+    if (nnls_factor(S, Gs, hs, ordl, mnew, Xs, poss)) return;
+    nnls_reset(S);
+    for (int i = 0; i < mnew; ++i) {
+      const int v = __builtin_amdgcn_readlane(ordl, i);
+      if (!nnls_append(S, Gs, hs, v) && zero_failed && lane == v) x = 0.0;
+    }
+  };
   // Warm start from the previous fit's positive set: append it, drop the variables whose unconstrained solution is not
   // positive until all are, and continue from that point (x_P the minimiser on P, x >= 0: the invariant of the
   // outer loop).  The minimiser is unique for a positive definite G, so only the iteration count depends on it.
   {
     const uint32_t warm = in->active & ((1u << K) - 1u);
-    for (int j = 0; j < K; ++j)
-      if ((warm >> j) & 1u) nnls_append(S, Gs, hs, j);
+    refactor(ordered_bits(warm), __popc(warm), false);
     for (int pass = 0; pass < K && S.m > 0; ++pass) {
       ++n_inner;
       const double sj = nnls_solve(S);
@@ -681,14 +935,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
         x = act ? sj : 0.0;
         break;
       }
-      const int mo = S.m;
-      const int ord_old = S.ordv;
       const uint64_t keep = __ballot(lane < K && act && !bad);
-      nnls_reset(S);
-      for (int i = 0; i < mo; ++i) {
-        const int v = __builtin_amdgcn_readlane(ord_old, i);
-        if ((keep >> v) & 1ull) nnls_append(S, Gs, hs, v);
-      }
+      const int mo = S.m;
+      refactor(kept_order(S.ordv, mo, keep), __popcll(keep), false);
     }
   }
   for (int outer = 0; outer < 3 * K; ++outer) {
@@ -717,18 +966,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
       const bool drop = act && x <= tol;
       if (drop) x = 0.0;
       if (!__ballot(drop)) continue;
-      // refactor: re-append the remaining variables in their previous order
+      // refactor: the remaining variables in their previous order
       const int mo = S.m;
-      const int ord_old = S.ordv;
       const bool keep_l = lane < K && S.posv >= 0 && !drop;
       const uint64_t keep = __ballot(keep_l);
-      nnls_reset(S);
-      for (int i = 0; i < mo; ++i) {
-        const int v = __builtin_amdgcn_readlane(ord_old, i);
-        if ((keep >> v) & 1ull) {
-          if (!nnls_append(S, Gs, hs, v) && lane == v) x = 0.0;
-        }
-      }
+      refactor(kept_order(S.ordv, mo, keep), __popcll(keep), true);
     }
   }
   double xs = x;
@@ -811,8 +1053,8 @@ int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int
   Ws w = carve(workspace, n);
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_ad_prep, dim3(w.nprep), dim3(256), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_grid, dim3(NGRID, NCH), dim3(256), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(1024), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_grid, dim3(1 + NGW), dim3(1024), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(64), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_gram, dim3(NG), dim3(64 * GW), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_gsum, dim3(NE), dim3(256), 0, st, w);
   hipLaunchKernelGGL(k_ad_solve, dim3(1), dim3(64), 0, st, a, w, in, out_pass2, out_next);

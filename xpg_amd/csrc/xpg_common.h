@@ -265,11 +265,20 @@ __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb,
 __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, double zeta, int Dz, double *scratch,
                               double *R, int lane) {
   const int nt = use_rcw_of(c) ? RCK : 1;
-  for (int t = lane; t < Dz * nt; t += WAVE) {
-    int j = t / nt, q = t - j * nt;
-    double lo, hi;
-    frame_bounds(c, iz0, fb, j, lo, hi);
-    scratch[t] = rc_term(c, lo, hi, phi, zeta, q);
+  constexpr int RB = 3;                      // terms per lane at a time: their erf table reads overlap
+  for (int t0 = lane; t0 < Dz * nt; t0 += RB * WAVE) {
+    double v[RB];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int t = min(t0 + i * WAVE, Dz * nt - 1);
+      const int j = t / nt, q = t - j * nt;
+      double lo, hi;
+      frame_bounds(c, iz0, fb, j, lo, hi);
+      v[i] = rc_term(c, lo, hi, phi, zeta, q);
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      if (t0 + i * WAVE < Dz * nt) scratch[t0 + i * WAVE] = v[i];
   }
   wave_sync();
   for (int j = lane; j < Dz; j += WAVE) {
