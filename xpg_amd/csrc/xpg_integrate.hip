@@ -598,37 +598,96 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
 
   // Lanes run over pixels (p = lane, lane + 64, ...) and each lane over the frames of its pixel: per-pixel terms stay
   // in registers, per-frame terms are broadcasts.  Sums are float64; sums of counts and of template values are exact.
+  //
+  // Boxes of at most NPL pixels x DZM frames per lane (regs) read their voxels from LDS once, into registers (value,
+  // neighbour bit), and keep the per-voxel classes as bit masks (bit k = q DZM + j for pixel lane + 64 q, frame j);
+  // the sums run over the same voxels in the same order as the LDS loops of larger boxes.
+  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
+  const bool regs = P <= NPL * WAVE && Dz <= DZM;
+  float cv[NK], brq[NPL];
+  uint32_t inm = 0, nbmm = 0, fgm = 0, fgvm = 0, bgmm = 0, fg2q = 0;   // in box / nbm / fg / fgv / bg / bit q: fg2
+  uint32_t fgz = 0;                                                   // bit j: foreground frame (uniform)
+  if (regs) {
+#pragma unroll
+    for (int j = 0; j < DZM; ++j)
+      if (j < Dz && fflag[j]) fgz |= 1u << j;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      const bool pin = p < P;
+      const uint8_t pfl = pin ? pflag[p] : 0;
+      brq[q] = pin ? bref[p] : 0.f;
+      if (pfl & PF_FG2) fg2q |= 1u << q;
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        cv[k] = 0.f;
+        if (pin && j < Dz) {
+          const int v = j * P + p;
+          const float c = vals[v];
+          const bool nbm = vflag[v] & VF_NBM;
+          cv[k] = c;
+          inm |= 1u << k;
+          if (nbm) nbmm |= 1u << k;
+          const bool fg = (pfl & PF_FG2) && ((fgz >> j) & 1u);
+          const bool vn = c >= 0.f && !nbm;
+          if (fg) fgm |= 1u << k;
+          if (fg && vn) fgvm |= 1u << k;
+          if (!fg && vn && (pfl & PF_BGR)) bgmm |= 1u << k;
+        }
+      }
+    }
+  }
 
   // per-voxel classes: fg = fg2 & fgz; fgv = fg & valid & ~nbm; background candidates (xpg/integrate_kernels.py:291-305),
   // with the unclipped background sums of the first round
   double sc0 = 0.0, sT0 = 0.0;
-  for (int p0 = 0; p0 < P; p0 += WAVE) {
-    const int p = p0 + lane;
-    if (p < P) {
-      const uint8_t pfl = pflag[p];
-      const bool fg2 = pfl & PF_FG2, bgr = pfl & PF_BGR;
+  if (regs) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      if (lane + q * WAVE >= P) continue;
       double cs = 0.0;
       int cnt = 0;
-      for (int j = 0; j < Dz; ++j) {
-        const int v = j * P + p;
-        const float c = vals[v];
-        uint8_t fl = vflag[v];
-        const bool valid = c >= 0.f, nbm = fl & VF_NBM;
-        const bool fg = fg2 && fflag[j];
-        if (fg) fl |= VF_FG;
-        if (fg && valid && !nbm) fl |= VF_FGV;
-        if (!fg && valid && !nbm && bgr) {
-          fl |= VF_BGM;
-          cs += (double)c;
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        if ((bgmm >> k) & 1u) {
+          cs += (double)cv[k];
           ++cnt;
         }
-        vflag[v] = fl;
       }
       sc0 += cs;
-      sT0 += (double)cnt * (double)bref[p];
+      sT0 += (double)cnt * (double)brq[q];
     }
+  } else {
+    for (int p0 = 0; p0 < P; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const uint8_t pfl = pflag[p];
+        const bool fg2 = pfl & PF_FG2, bgr = pfl & PF_BGR;
+        double cs = 0.0;
+        int cnt = 0;
+        for (int j = 0; j < Dz; ++j) {
+          const int v = j * P + p;
+          const float c = vals[v];
+          uint8_t fl = vflag[v];
+          const bool valid = c >= 0.f, nbm = fl & VF_NBM;
+          const bool fg = fg2 && fflag[j];
+          if (fg) fl |= VF_FG;
+          if (fg && valid && !nbm) fl |= VF_FGV;
+          if (!fg && valid && !nbm && bgr) {
+            fl |= VF_BGM;
+            cs += (double)c;
+            ++cnt;
+          }
+          vflag[v] = fl;
+        }
+        sc0 += cs;
+        sT0 += (double)cnt * (double)bref[p];
+      }
+    }
+    wave_sync();
   }
-  wave_sync();
 
   // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
   // A round that leaves b unchanged (bit for bit) sets the same thresholds again, so every later round would repeat
@@ -636,53 +695,31 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   double Sc = wsum_d(sc0);
   double nbg = wsum_d(sT0);
   double bk = Sc / fmax(nbg, 1e-6);
-  // boxes of at most NPL pixels x DZM frames per lane: the background candidates' values stay in registers across
-  // the rounds (the same sums in the same order)
-  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
-  const bool regs = P <= NPL * WAVE && Dz <= DZM;
-  if (regs) {
-    float bgv[NK], brq[NPL];
-    uint32_t bgm = 0;
+  for (int round = 1; round < 5 && regs; ++round) {
+    double sc = 0.0, sT = 0.0;
 #pragma unroll
     for (int q = 0; q < NPL; ++q) {
-      const int p = lane + q * WAVE;
-      brq[q] = p < P ? bref[p] : 0.f;
+      if (lane + q * WAVE >= P) continue;
+      const double bp = bk * (double)brq[q];
+      const float th = __double2float_rd((bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0);
+      double cs = 0.0;
+      int cnt = 0;
 #pragma unroll
       for (int j = 0; j < DZM; ++j) {
         const int k = q * DZM + j;
-        bgv[k] = 0.f;
-        if (p < P && j < Dz && (vflag[j * P + p] & VF_BGM)) {
-          bgm |= 1u << k;
-          bgv[k] = vals[j * P + p];
+        if (((bgmm >> k) & 1u) && cv[k] <= th) {
+          cs += (double)cv[k];
+          ++cnt;
         }
       }
+      sc += cs;
+      sT += (double)cnt * (double)brq[q];
     }
-    for (int round = 1; round < 5; ++round) {
-      double sc = 0.0, sT = 0.0;
-#pragma unroll
-      for (int q = 0; q < NPL; ++q) {
-        if (lane + q * WAVE >= P) continue;
-        const double bp = bk * (double)brq[q];
-        const float th = __double2float_rd((bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0);
-        double cs = 0.0;
-        int cnt = 0;
-#pragma unroll
-        for (int j = 0; j < DZM; ++j) {
-          const int k = q * DZM + j;
-          if (((bgm >> k) & 1u) && bgv[k] <= th) {
-            cs += (double)bgv[k];
-            ++cnt;
-          }
-        }
-        sc += cs;
-        sT += (double)cnt * (double)brq[q];
-      }
-      Sc = wsum_d(sc);
-      nbg = wsum_d(sT);
-      const double bk_prev = bk;
-      bk = Sc / fmax(nbg, 1e-6);
-      if (bk == bk_prev) break;
-    }
+    Sc = wsum_d(sc);
+    nbg = wsum_d(sT);
+    const double bk_prev = bk;
+    bk = Sc / fmax(nbg, 1e-6);
+    if (bk == bk_prev) break;
   }
   for (int round = 1; round < 5 && !regs; ++round) {
     for (int p = lane; p < P; p += WAVE) {
@@ -723,21 +760,39 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   double Sfgc = 0.0, sfb = 0.0, Swpos = 0.0, Swx = 0.0, Swy = 0.0, Swz = 0.0;
   double Sm1 = 0.0, Sm2 = 0.0, Smdd = 0.0, Sm3 = 0.0, Sm33 = 0.0;
   int n_fg = 0, n_over = 0, n_fgm = 0, n_fgnb = 0;
-  for (int p0 = 0; p0 < P; p0 += WAVE) {
-    const int p = p0 + lane;
-    if (p < P) {
-      const double bT = bk * (double)bref[p];
+  auto fg_pixel = [&](int p, double cs, double npos, double nzz, double nall, double n3, double n33, int cnt) {
+    const int iy = prow[p], ix = p - iy * Dx;
+    const double a1 = U1[p] * seff, a2 = U2[p] * seff;
+    n_fg += cnt;
+    Sfgc += cs;
+    sfb += (double)cnt * (double)bref[p];
+    Swpos += npos;
+    Swx += npos * co[ix];
+    Swy += npos * co[Dx + iy];
+    Swz += nzz;
+    Sm1 += nall * a1;
+    Sm2 += nall * a2;
+    Smdd += nall * (a1 * a1 + a2 * a2);
+    Sm3 += n3;
+    Sm33 += n33;
+  };
+  if (regs) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      if (p >= P) continue;
+      const double bT = bk * (double)brq[q];
       double cs = 0.0, npos = 0.0, nzz = 0.0, nall = 0.0, n3 = 0.0, n33 = 0.0;
       int cnt = 0;
-      for (int j = 0; j < Dz; ++j) {
-        const int v = j * P + p;
-        const uint8_t fl = vflag[v];
-        if (!(fl & VF_FG)) continue;
-        const float c = vals[v];
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        if (!((fgm >> k) & 1u)) continue;
+        const float c = cv[k];
         n_over += (c == -2.0f);
         n_fgm += (c < 0.f && c > -2.5f);
-        n_fgnb += (fl & VF_NBM) ? 1 : 0;
-        if (!(fl & VF_FGV)) continue;
+        n_fgnb += (nbmm >> k) & 1u;
+        if (!((fgvm >> k) & 1u)) continue;
         const double net = (double)c - bT;
         ++cnt;
         cs += (double)c;
@@ -750,20 +805,38 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         n3 += net * e3;
         n33 += net * (e3 * e3);
       }
-      const int iy = prow[p], ix = p - iy * Dx;
-      const double a1 = U1[p] * seff, a2 = U2[p] * seff;
-      n_fg += cnt;
-      Sfgc += cs;
-      sfb += (double)cnt * (double)bref[p];
-      Swpos += npos;
-      Swx += npos * co[ix];
-      Swy += npos * co[Dx + iy];
-      Swz += nzz;
-      Sm1 += nall * a1;
-      Sm2 += nall * a2;
-      Smdd += nall * (a1 * a1 + a2 * a2);
-      Sm3 += n3;
-      Sm33 += n33;
+      fg_pixel(p, cs, npos, nzz, nall, n3, n33, cnt);
+    }
+  } else {
+    for (int p0 = 0; p0 < P; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const double bT = bk * (double)bref[p];
+        double cs = 0.0, npos = 0.0, nzz = 0.0, nall = 0.0, n3 = 0.0, n33 = 0.0;
+        int cnt = 0;
+        for (int j = 0; j < Dz; ++j) {
+          const int v = j * P + p;
+          const uint8_t fl = vflag[v];
+          if (!(fl & VF_FG)) continue;
+          const float c = vals[v];
+          n_over += (c == -2.0f);
+          n_fgm += (c < 0.f && c > -2.5f);
+          n_fgnb += (fl & VF_NBM) ? 1 : 0;
+          if (!(fl & VF_FGV)) continue;
+          const double net = (double)c - bT;
+          ++cnt;
+          cs += (double)c;
+          if (net > 0.0) {
+            npos += net;
+            nzz += net * zzd[j];
+          }
+          const double e3 = e3d[j];
+          nall += net;
+          n3 += net * e3;
+          n33 += net * (e3 * e3);
+        }
+        fg_pixel(p, cs, npos, nzz, nall, n3, n33, cnt);
+      }
     }
   }
   Sfgc = wsum_d(Sfgc);
@@ -799,7 +872,36 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   float *fz_out = a.out.fz ? a.out.fz + (int64_t)r * fzs : nullptr;
   if (fz_out)
     for (int j = Dz + lane; j < fzs; j += WAVE) fz_out[j] = 0.f;
-  if (Dz <= 16) {
+  if (regs) {
+    // each lane's pixels per frame, then one sum over the wavefront per frame
+    double fzp[DZM];
+#pragma unroll
+    for (int j = 0; j < DZM; ++j) fzp[j] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      if (!((fg2q >> q) & 1u)) continue;
+      const double bpx = bk * (double)brq[q];
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        if (((inm >> k) & 1u) && cv[k] >= 0.f && !((nbmm >> k) & 1u)) fzp[j] += (double)cv[k] - bpx;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < DZM; ++j) {
+      if (j < Dz) {
+        const double t = wsum_d(fzp[j]);
+        bool rec = false;
+#pragma unroll
+        for (int q = 0; q < NPL; ++q) rec |= ((inm >> (q * DZM + j)) & 1u) && cv[q * DZM + j] > -2.5f;
+        const bool anyrec = __ballot(rec) != 0;
+        if (lane == 0) {
+          if (fz_out) fz_out[j] = (float)t;
+          scr[j] = anyrec ? Rd[j] : 0.0;
+        }
+      }
+    }
+  } else if (Dz <= 16) {
     // every frame at once: g lanes per frame (the largest of 16, 8, 4 with g Dz <= 64) stride over its pixels, then
     // a sum within each group of g lanes
     const int g = Dz <= 4 ? 16 : (Dz <= 8 ? 8 : 4);
@@ -845,16 +947,7 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
                       (frac_rec > 0.95) && (n_fgnb == 0) && (n_fgm == 0);
   const double seff2 = seff * seff;
   const bool pix_out = a.out.img2 || a.out.all_valid2 || a.out.J || a.out.u1 || a.out.u2;
-  for (int p = lane; p < P && (strong || pix_out); p += WAVE) {
-    double img = 0.0;
-    bool okv = true;
-    const double bpx = bk * (double)bref[p];
-    for (int j = 0; j < Dz; ++j) {
-      if (!fflag[j]) continue;
-      const float c = vals[j * P + p];
-      if (c >= 0.f && !(vflag[j * P + p] & VF_NBM)) img += (double)c - bpx;
-      else okv = false;
-    }
+  auto img_pixel = [&](int p, double img, bool okv) {
     const int64_t o = b.poff + p;
     if (a.out.img2) a.out.img2[o] = (float)img;
     if (a.out.all_valid2) a.out.all_valid2[o] = okv ? 1 : 0;
@@ -865,6 +958,36 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
       const double dens = ((double)(float)img / fmax(isum, 1e-6)) / fmax(Jd[p] / seff2, 1e-12);
       splat4(a.acc, b.region, U1[p], U2[p], dens);
     }
+  };
+  if (regs && (strong || pix_out)) {
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) {
+      const int p = lane + q * WAVE;
+      if (p >= P) continue;
+      double img = 0.0;
+      bool okv = true;
+      const double bpx = bk * (double)brq[q];
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int k = q * DZM + j;
+        if (!((inm >> k) & 1u) || !((fgz >> j) & 1u)) continue;
+        if (cv[k] >= 0.f && !((nbmm >> k) & 1u)) img += (double)cv[k] - bpx;
+        else okv = false;
+      }
+      img_pixel(p, img, okv);
+    }
+  }
+  for (int p = lane; p < P && !regs && (strong || pix_out); p += WAVE) {
+    double img = 0.0;
+    bool okv = true;
+    const double bpx = bk * (double)bref[p];
+    for (int j = 0; j < Dz; ++j) {
+      if (!fflag[j]) continue;
+      const float c = vals[j * P + p];
+      if (c >= 0.f && !(vflag[j * P + p] & VF_NBM)) img += (double)c - bpx;
+      else okv = false;
+    }
+    img_pixel(p, img, okv);
   }
 
   if (lane == 0) {
