@@ -3,14 +3,15 @@
 // pass 1 -> fits -> pass 2 -> next block runs without a host round trip.
 //
 //   k_ad_prep    (n/256 x 256)   strong-row flags, fz totals and sigma_D terms per row; strong rows per workgroup
-//   k_ad_grid    (1 + NGW x 1024) workgroup 0: exact median of the sigma_D terms by radix selection of the two middle
-//                                ranks; the others: partial cost of each sigma_M of the 61-point log grid over fixed
-//                                row ranges, one wavefront per strong row and one lane per grid point
-//                                (_fit_rocking_width, :1130-1154)
-//   k_ad_finish  (1 x 64)        strong-row count, argmin over the grid -> sigma_M fit and shrinkage
-//   k_ad_gram    (NG x 256)      Gram matrix A^T A and A^T y of the rocking-curve basis over a fixed row range
+//   k_ad_grid    (NGW x 256)     partial cost of each sigma_M of the 61-point log grid over fixed ranges of strong
+//                                rows, one wavefront per row and one lane per grid point (_fit_rocking_width,
+//                                :1130-1154)
+//   k_ad_finish  (1 x 256)       strong-row count, argmin over the grid -> sigma_M fit and shrinkage
+//   k_ad_gram    (NG x 64)       Gram matrix A^T A and A^T y of the rocking-curve basis over a fixed row range
 //                                (rc_basis_np, :437-443)
-//   k_ad_gsum    (NE x 256)      fixed-order reduction of the Gram partials, one workgroup per entry
+//   k_ad_gsum    (NE + 1 x 256)  fixed-order reduction of the Gram partials, one workgroup per entry; the last one
+//                                selects the exact median of the sigma_D terms (radix selection of the two middle
+//                                ranks among the strong keys k_ad_grid laid out)
 //   k_ad_solve   (1 x 64)        non-negative least squares on the Gram form (_nnls_gram, :1095-1104) in one
 //                                wavefront, regularisation towards the Gaussian weights, the widths records
 //
@@ -26,10 +27,11 @@ namespace {
 
 constexpr int K = XPG_RC_K;               // 21 mixture components
 constexpr int NF = 65;                     // frames of a per-frame profile row (max(DZ_BUCKETS))
-constexpr int NG = 1024;                   // workgroups of the Gram accumulation
-constexpr int NGW = 128;                   // workgroups of the grid costs (16 wavefronts, one strong-row range each)
+constexpr int NG = 4096;                   // workgroups of the Gram accumulation (one wavefront each)
+constexpr int NGW = 512;                   // workgroups of the grid costs (4 wavefronts, one strong-row range each)
 constexpr int NE = K * (K + 1) / 2 + K;    // unique Gram entries + right-hand side
 constexpr int NGRID = 61;
+static_assert(NG % 256 == 0 && NGW % 64 == 0, "the fixed-order reductions read whole chunks");
 constexpr double PI = 3.141592653589793;
 
 // rc_default_weights (float32 values of exp(-c^2 / (2 (1 - 0.25^2))) normalised)
@@ -48,6 +50,8 @@ struct Ws {
   double *gram;       // NG x NE partial Gram entries
   double *gtot;       // NE Gram totals
   double *scal;       // n_strong, sm_new, sm_fit, sd_new, sd_fit
+  uint64_t *kmm;      // nprep x 2: smallest and largest strong key per k_ad_prep workgroup
+  uint64_t *kc;       // n: the strong keys in row order (k_ad_grid)
   int32_t nprep;
 };
 
@@ -70,6 +74,8 @@ __host__ __device__ inline Ws carve(void *base, int32_t n) {
   w.gram = (double *)take(8 * (size_t)NG * NE);
   w.gtot = (double *)take(8 * (size_t)NE);
   w.scal = (double *)take(8 * 8);
+  w.kmm = (uint64_t *)take(16 * (size_t)w.nprep);
+  w.kc = (uint64_t *)take(8 * m);
   return w;
 }
 
@@ -111,8 +117,10 @@ __device__ __forceinline__ double block_sum256(double v, double *red) {
 // ---- 1. strong rows -------------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
   __shared__ int32_t wc[4];
+  __shared__ uint64_t wm[4][2];
   const int r = blockIdx.x * 256 + threadIdx.x;
   bool st = false;
+  uint64_t kk = 0;
   if (r < a.n) {
     st = (a.isum[r] > 0) && (a.isum[r] / sqrt(fmax(a.vsum[r], 1e-6)) > 15.0) && (a.n_over[r] == 0) &&
          (a.frac_rec[r] > 0.98) && (a.fb[r] == 1);
@@ -129,15 +137,60 @@ __global__ void __launch_bounds__(256) k_ad_prep(In a, Ws w) {
       md = a.m_d[r] / t2 - (q * q) * (1 + a.cosa[r] * a.cosa[r]) / 24.0;
     }
     const uint64_t b = __builtin_bit_cast(uint64_t, md);
-    w.key[r] = (b >> 63) ? ~b : (b | 0x8000000000000000ull);      // unsigned order = numeric order
+    kk = (b >> 63) ? ~b : (b | 0x8000000000000000ull);            // unsigned order = numeric order
+    w.key[r] = kk;
     w.tot[r] = s;
     w.nzl[r] = last;
     w.st[r] = st;
   }
   const uint64_t bal = __ballot(st);
-  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(bal);
+  uint64_t kmin = ~0ull, kmax = 0;
+  if (st) kmin = kmax = kk;
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a0 = __shfl_xor(kmin, o, 64), a1 = __shfl_xor(kmax, o, 64);
+    kmin = a0 < kmin ? a0 : kmin;
+    kmax = a1 > kmax ? a1 : kmax;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    wc[threadIdx.x >> 6] = __popcll(bal);
+    wm[threadIdx.x >> 6][0] = kmin;
+    wm[threadIdx.x >> 6][1] = kmax;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) w.cnt[blockIdx.x] = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+  if (threadIdx.x == 0) {
+    w.cnt[blockIdx.x] = (wc[0] + wc[1]) + (wc[2] + wc[3]);
+    uint64_t lo = wm[0][0], hi = wm[0][1];
+    for (int i = 1; i < 4; ++i) {
+      lo = wm[i][0] < lo ? wm[i][0] : lo;
+      hi = wm[i][1] > hi ? wm[i][1] : hi;
+    }
+    w.kmm[2 * blockIdx.x] = lo;
+    w.kmm[2 * blockIdx.x + 1] = hi;
+  }
+}
+
+// The sigma_D median runs in pieces across the chain, in small workgroups: k_ad_prep gives the strong keys' range,
+// k_ad_grid lays the strong keys out in row order, and one workgroup of k_ad_gsum selects the two middle ranks.
+
+// smallest and largest strong key (one wavefront; every lane returns them), the byte of their highest differing bit
+// and their common prefix above it
+__device__ __forceinline__ void key_range(const Ws &w, uint64_t &kmin, uint64_t &kmax, int &top, uint64_t &pre) {
+  const int lane = threadIdx.x & 63;
+  kmin = ~0ull;
+  kmax = 0;
+  for (int b = lane; b < w.nprep; b += 64) {
+    const uint64_t lo = w.kmm[2 * b], hi = w.kmm[2 * b + 1];
+    kmin = lo < kmin ? lo : kmin;
+    kmax = hi > kmax ? hi : kmax;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint64_t a0 = __shfl_xor(kmin, o, 64), a1 = __shfl_xor(kmax, o, 64);
+    kmin = a0 < kmin ? a0 : kmin;
+    kmax = a1 > kmax ? a1 : kmax;
+  }
+  const uint64_t diff = kmin ^ kmax;
+  top = diff ? (63 - __clzll((long long)diff)) / 8 * 8 : 0;
+  pre = top == 56 ? 0ull : (kmin & (~0ull << (top + 8)));
 }
 
 // ---- 2. grid of sigma_M candidates -------------------------------------------------------------------------------
@@ -188,183 +241,113 @@ __device__ __forceinline__ void hist_add(uint32_t *h, uint32_t bin, bool live) {
   }
 }
 
-// Exact median of the strong rows' sigma_D terms (1024 threads): radix selection, 8 bits per round, of the two middle
-// ranks; sigma_D fit and shrinkage into scal[3], scal[4].  Needs only k_ad_prep's outputs.
+// Exact median of the strong rows' sigma_D terms in one workgroup of MT = 256 threads: radix selection, 8 bits per
+// round from the first byte where the keys differ, of the two middle ranks among the n strong keys laid out by
+// k_ad_grid (held in registers, KR per thread, when n <= MT KR); once at most 64 keys share a rank's prefix, that rank
+// is finished among them directly.  sigma_D fit and shrinkage into scal[3], scal[4].
 __device__ void median_select(const In &a, const Ws &w) {
-  __shared__ int32_t s_n;
+  constexpr int MT = 256, KR = 24;
   __shared__ uint32_t hist[2][256];
-  __shared__ uint64_t s_pre[2];
-  __shared__ uint64_t s_mm[2];
-  __shared__ uint32_t s_k[2];
-  __shared__ uint64_t s_msk[2];            // bits of the key fixed so far (prefix s_pre)
-  __shared__ int32_t s_done[2], s_cn[2];   // rank q: at most 64 keys share its prefix; candidates gathered
+  __shared__ uint64_t s_pre[2], s_msk[2];  // rank q: the bits of its key fixed so far
+  __shared__ uint32_t s_k[2];              // rank q: its rank among the keys with that prefix
+  __shared__ int32_t s_done[2], s_cn[2];
   __shared__ uint64_t s_cand[2][64];
   const int t = threadIdx.x;
+  const int n = (int)w.scal[0];
+  const bool two = !(n & 1);               // the two middle ranks differ
+  uint64_t kmin, kmax, pre;
+  int top;
+  key_range(w, kmin, kmax, top, pre);
   if (t == 0) {
-    s_n = 0;
+    s_pre[0] = s_pre[1] = pre;
+    s_msk[0] = s_msk[1] = top == 56 ? 0ull : (~0ull << (top + 8));
+    s_k[0] = (uint32_t)((n - 1) / 2);
+    s_k[1] = (uint32_t)(n / 2);
     s_done[0] = s_done[1] = 0;
     s_cn[0] = s_cn[1] = 0;
-    s_mm[0] = ~0ull;
-    s_mm[1] = 0;
   }
-  __syncthreads();
-  int cn = 0;
-  for (int b = t; b < w.nprep; b += 1024) cn += w.cnt[b];
-  if (cn) atomicAdd(&s_n, cn);           // integer counts: order-free
-  __syncthreads();
-  const int n = s_n;
-  if (n < 100) return;
-  if (t == 0) {
-    s_pre[0] = s_pre[1] = 0;
-    s_k[0] = (uint32_t)((n - 1) / 2);     // the two middle ranks (equal for odd n)
-    s_k[1] = (uint32_t)(n / 2);
-  }
-  // radix selection, 8 bits per round, of both ranks among the strong rows' keys; the keys stay in registers
-  // (KR per thread) when the block has at most 1024 KR rows
-  constexpr int KR = 32;
-  const bool in_regs = a.n <= 1024 * KR;
+  const bool in_regs = n <= MT * KR;
   uint64_t kr[KR];
-  uint32_t kv = 0;                         // bit i: kr[i] is a strong row's key
-  if (in_regs) {
 #pragma unroll
-    for (int i = 0; i < KR; ++i) {
-      const int r = t + 1024 * i;
-      kr[i] = 0;
-      if (r < a.n && w.st[r]) {
-        kr[i] = w.key[r];
-        kv |= 1u << i;
+  for (int i = 0; i < KR; ++i) {
+    const int j = t + MT * i;
+    kr[i] = in_regs && j < n ? w.kc[j] : 0ull;
+  }
+  __syncthreads();
+  // one pass over the keys, all 64 lanes of every wavefront together (f takes a liveness flag)
+  auto for_keys = [&](auto &&f) {
+    if (in_regs) {
+#pragma unroll
+      for (int i = 0; i < KR; ++i) f(kr[i], t + MT * i < n);
+    } else {
+      for (int i0 = 0; i0 < n; i0 += MT) {
+        const int i = i0 + t;
+        const bool live = i < n;
+        f(live ? w.kc[i] : 0ull, live);
       }
     }
-  }
-  // the bytes every strong key shares (from the smallest and largest key) need no round: the selection starts at the
-  // first byte where they differ, with that common prefix as both ranks' prefix
-  uint64_t kmin = ~0ull, kmax = 0;
-  if (in_regs) {
-#pragma unroll
-    for (int i = 0; i < KR; ++i)
-      if ((kv >> i) & 1u) {
-        kmin = kr[i] < kmin ? kr[i] : kmin;
-        kmax = kr[i] > kmax ? kr[i] : kmax;
-      }
-  } else {
-    for (int r = t; r < a.n; r += 1024)
-      if (w.st[r]) {
-        const uint64_t key = w.key[r];
-        kmin = key < kmin ? key : kmin;
-        kmax = key > kmax ? key : kmax;
-      }
-  }
-  for (int o = 32; o > 0; o >>= 1) {
-    const uint64_t a0 = __shfl_xor(kmin, o, 64), a1 = __shfl_xor(kmax, o, 64);
-    kmin = a0 < kmin ? a0 : kmin;
-    kmax = a1 > kmax ? a1 : kmax;
-  }
-  if ((t & 63) == 0) {
-    atomicMin((unsigned long long *)&s_mm[0], (unsigned long long)kmin);
-    atomicMax((unsigned long long *)&s_mm[1], (unsigned long long)kmax);
-  }
-  __syncthreads();
-  const uint64_t diff = s_mm[0] ^ s_mm[1];
-  const int top = diff ? (63 - __clzll((long long)diff)) / 8 * 8 : 0;   // byte of the highest differing bit
-  if (t == 0) {
-    s_pre[0] = s_pre[1] = top == 56 ? 0ull : (s_mm[0] & (~0ull << (top + 8)));
-    s_msk[0] = s_msk[1] = top == 56 ? 0ull : (~0ull << (top + 8));
-  }
-  const bool two = s_k[0] != s_k[1];
-  __syncthreads();
-  // a round fixes the next byte of each rank's key; once at most 64 keys share a rank's prefix, the rank is finished
-  // among those candidates instead (most blocks need two rounds, not seven)
+  };
   for (int shift = top; shift >= 0 && !(s_done[0] && s_done[1]); shift -= 8) {
-    for (int i = t; i < 512; i += 1024) (&hist[0][0])[i] = 0;
+    for (int i = t; i < 512; i += MT) (&hist[0][0])[i] = 0;
     __syncthreads();
     const uint64_t hm = shift == 56 ? 0ull : (~0ull << (shift + 8));
     const uint64_t p0 = s_pre[0], p1 = s_pre[1];
     const bool a0 = !s_done[0], a1 = two && !s_done[1];
-    if (in_regs) {
-#pragma unroll
-      for (int i = 0; i < KR; ++i) {
-        const uint64_t key = kr[i];
-        const uint32_t bin = (uint32_t)(key >> shift) & 255u;
-        const bool live = (kv >> i) & 1u;
-        if (a0) hist_add(hist[0], bin, live && (key & hm) == p0);
-        if (a1) hist_add(hist[1], bin, live && (key & hm) == p1);
-      }
-    } else {
-      for (int r0 = 0; r0 < a.n; r0 += 1024) {
-        const int r = r0 + t;
-        const bool live = r < a.n && w.st[r];
-        const uint64_t key = live ? w.key[r] : 0;
-        const uint32_t bin = (uint32_t)(key >> shift) & 255u;
-        if (a0) hist_add(hist[0], bin, live && (key & hm) == p0);
-        if (a1) hist_add(hist[1], bin, live && (key & hm) == p1);
-      }
-    }
+    for_keys([&](uint64_t key, bool live) {
+      const uint32_t bin = (uint32_t)(key >> shift) & 255u;
+      if (a0) hist_add(hist[0], bin, live && (key & hm) == p0);
+      if (a1) hist_add(hist[1], bin, live && (key & hm) == p1);
+    });
     __syncthreads();
     if (t < (two ? 128 : 64) && !s_done[t >> 6]) {
       const int q = t >> 6;
       uint32_t k = s_k[q];
       const int bin = select_bin(hist[q], k);
       if ((t & 63) == 0) {
-        const int32_t cnt = (int32_t)hist[q][bin];
         s_pre[q] |= (uint64_t)bin << shift;
         s_msk[q] = ~0ull << shift;
         s_k[q] = k;
-        s_done[q] = cnt <= 64;
-        if (!two) {
-          s_pre[1] = s_pre[0];
-          s_msk[1] = s_msk[0];
-          s_k[1] = k;
-          s_done[1] = s_done[0];
-        }
+        s_done[q] = (int32_t)hist[q][bin] <= 64;
+        if (!two) s_done[1] = 1;
       }
     }
     __syncthreads();
   }
-  // ranks finished early: gather the keys with the rank's prefix (at most 64; any order), then the candidate whose
+  // ranks with bits still open: gather the keys with the rank's prefix (at most 64; any order), then the one whose
   // rank among them (ties broken by slot) is the remaining rank
   const bool g0 = s_msk[0] != ~0ull, g1 = two && s_msk[1] != ~0ull;
   if (g0 || g1) {
     const uint64_t m0 = s_msk[0], m1 = s_msk[1], p0 = s_pre[0], p1 = s_pre[1];
-    auto gather = [&](uint64_t key) {
-      if (g0 && (key & m0) == p0) {
+    for_keys([&](uint64_t key, bool live) {
+      if (live && g0 && (key & m0) == p0) {
         const int i = atomicAdd(&s_cn[0], 1);
         if (i < 64) s_cand[0][i] = key;
       }
-      if (g1 && (key & m1) == p1) {
+      if (live && g1 && (key & m1) == p1) {
         const int i = atomicAdd(&s_cn[1], 1);
         if (i < 64) s_cand[1][i] = key;
       }
-    };
-    if (in_regs) {
-#pragma unroll
-      for (int i = 0; i < KR; ++i)
-        if ((kv >> i) & 1u) gather(kr[i]);
-    } else {
-      for (int r = t; r < a.n; r += 1024)
-        if (w.st[r]) gather(w.key[r]);
-    }
+    });
     __syncthreads();
     if (t < 128) {
       const int q = t >> 6, l = t & 63;
       if (q == 0 ? g0 : g1) {
-        const int cn = min(s_cn[q], 64);
-        const uint64_t c = l < cn ? s_cand[q][l] : ~0ull;
+        const int cn2 = min(s_cn[q], 64);
+        const uint64_t c = l < cn2 ? s_cand[q][l] : ~0ull;
         int rank = 0;
-        for (int j = 0; j < cn; ++j) {
+        for (int j = 0; j < cn2; ++j) {
           const uint64_t o = s_cand[q][j];
           rank += (o < c) || (o == c && j < l);
         }
-        if (l < cn && rank == (int)s_k[q]) s_pre[q] = c;
+        if (l < cn2 && rank == (int)s_k[q]) s_pre[q] = c;
       }
     }
-    __syncthreads();
-    if (t == 0 && !two) s_pre[1] = s_pre[0];
     __syncthreads();
   }
   if (t == 0) {
     double v[2];
     for (int q = 0; q < 2; ++q) {
-      const uint64_t b = s_pre[q];
+      const uint64_t b = s_pre[two ? q : 0];
       v[q] = __builtin_bit_cast(double, (b >> 63) ? (b & 0x7fffffffffffffffull) : ~b);
     }
     const double med = (n & 1) ? v[1] : (v[0] + v[1]) / 2.0;
@@ -377,16 +360,30 @@ __device__ void median_select(const In &a, const Ws &w) {
 
 // strong-row count (scal[0]) and the sigma_M fit: the grid costs summed over the workgroups in order, the first
 // minimum, shrinkage towards the running value (scal[1], scal[2])
-__global__ void __launch_bounds__(64) k_ad_finish(In a, Ws w) {
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(256) k_ad_finish(In a, Ws w) {
+  __shared__ int32_t s_n;
+  __shared__ double s_part[4][64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  if (t == 0) s_n = 0;
+  __syncthreads();
   int cn = 0;
-  for (int b = lane; b < w.nprep; b += 64) cn += w.cnt[b];
-  for (int o = 32; o > 0; o >>= 1) cn += __shfl_xor(cn, o, 64);
-  const int n = cn;
-  if (lane == 0) w.scal[0] = (double)n;
-  if (n < 100) return;
-  double cst = 0.0;
-  for (int g = 0; g < NGW; ++g) cst += w.cost[g * 64 + lane];
+  for (int b = t; b < w.nprep; b += 256) cn += w.cnt[b];
+  if (cn) atomicAdd(&s_n, cn);           // integer counts: order-free
+  // wavefront q sums a quarter of the workgroup partials in order, sixteen reads in flight
+  double part = 0.0;
+  for (int g0 = wv * (NGW / 4); g0 < (wv + 1) * (NGW / 4); g0 += 16) {
+    double v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = w.cost[(g0 + u) * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) part += v[u];
+  }
+  s_part[wv][lane] = part;
+  __syncthreads();
+  const int n = s_n;
+  if (t == 0) w.scal[0] = (double)n;
+  if (n < 100 || wv > 0) return;
+  const double cst = ((s_part[0][lane] + s_part[1][lane]) + s_part[2][lane]) + s_part[3][lane];
   // first minimum over the grid points (lanes 0 .. NGRID - 1) below +inf; none: the running value
   double best = lane < NGRID ? cst : INFINITY;
   for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o, 64));
@@ -405,14 +402,10 @@ __global__ void __launch_bounds__(64) k_ad_finish(In a, Ws w) {
 // Grid costs: one wavefront per strong row (a fixed range of strong ranks per wavefront, in order), lane c for grid
 // point c.
 // The row's frame totals are one vector load (lane k: frame k < 64) broadcast by readlane; the row fields are uniform.
-__global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
-  if (blockIdx.x == 0) {                 // the median needs only k_ad_prep's outputs: it runs beside the grid
-    median_select(a, w);
-    return;
-  }
-  __shared__ double wpart[16][64];
+__global__ void __launch_bounds__(256) k_ad_grid(In a, Ws w) {
+  __shared__ double wpart[4][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int g = blockIdx.x - 1;
+  const int g = blockIdx.x;
   const int c = lane < NGRID ? lane : NGRID - 1;      // lanes past the grid repeat its last point (not summed)
   const double guess = a.sm_run * (180.0 / PI);
   const double lo_l = log(0.5), hi_l = log(2.0), step = (hi_l - lo_l) / (NGRID - 1);
@@ -426,7 +419,7 @@ __global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
     ns += cv;
   }
   int q0, q1;
-  row_range(ns, g * 16 + wv, NGW * 16, q0, q1);
+  row_range(ns, g * 4 + wv, NGW * 4, q0, q1);
   int need = q1 - q0, skip = q0, base = 0;
   for (int b0 = 0; need > 0 && b0 < w.nprep; b0 += 64) {
     const int cv = b0 + lane < w.nprep ? w.cnt[b0 + lane] : 0;
@@ -444,6 +437,7 @@ __global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
     }
     skip -= all;
   }
+  int rank = q0;                         // the strong rank of the next row taken
   uint64_t m = 0;
   base -= 64;
   double part = 0.0;
@@ -462,6 +456,8 @@ __global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
     }
     --need;
     // r is uniform, and so are the row's fields
+    if (lane == 0) w.kc[rank] = w.key[r];  // the median's keys, in row order
+    ++rank;
     const double tot = w.tot[r], phi = a.phi[r], zeta = fabs(a.zeta_k[r]);
     const int iz0 = a.iz0[r], kend = w.nzl[r];
     const float *fz = a.fz + (int64_t)r * NF;
@@ -505,7 +501,7 @@ __global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
   __syncthreads();
   if (wv == 0) {
     double t = 0.0;
-    for (int q = 0; q < 16; ++q) t += wpart[q][lane];
+    for (int q = 0; q < 4; ++q) t += wpart[q][lane];
     w.cost[g * 64 + lane] = t;
   }
 }
@@ -515,13 +511,14 @@ __global__ void __launch_bounds__(1024) k_ad_grid(In a, Ws w) {
 // One wavefront per strong row (rows of a fixed range per wavefront, in order).  Frame k contributes to the Gram
 // entries only where some basis function B_kj is non-zero; B_kj = 0 exactly when both frame boundaries saturate every
 // component's erf to the same sign, so the accumulation runs over the frames between the first and last live one.
-constexpr int GW = 4;                      // wavefronts per workgroup
+constexpr int GW = 1;                      // wavefronts per workgroup (11.6 KB of LDS each)
 __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
   __shared__ double A[GW][NF + 1][K + 1];  // boundary erfs E[b][j], then in place A[k][j] = B_kj sqrt(tot), A[k][K] = y
   double (*red)[NE] = (double (*)[NE])&A[0][0][0];   // after the rows: the wavefronts' partial sums
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   double (*Aw)[K + 1] = A[wv];
   double *Af = &A[wv][0][0];                // the same storage, flat
+  const int gb = blockIdx.x;
   const int n = (int)w.scal[0];
   int ep[4], eq[4];                        // this lane's entries e = lane + 64 i: (p, q), p <= q; q = K: right-hand side
   for (int i = 0; i < 4; ++i) {
@@ -545,7 +542,7 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
   if (n >= 100) {
     const double sm = w.scal[1];
     int r0, r1;
-    row_range(a.n, blockIdx.x * GW + wv, NG * GW, r0, r1);
+    row_range(a.n, gb * GW + wv, NG * GW, r0, r1);
     for (int r = r0; r < r1; ++r) {
       if (!w.st[r]) continue;              // uniform over the wavefront
       const double phi = a.phi[r] * (PI / 180.0);
@@ -632,7 +629,7 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
   for (int e = threadIdx.x; e < NE; e += 64 * GW) {
     double t = red[0][e];
     for (int q = 1; q < GW; ++q) t += red[q][e];
-    w.gram[(int64_t)blockIdx.x * NE + e] = t;
+    w.gram[(int64_t)gb * NE + e] = t;
   }
 }
 
@@ -640,9 +637,13 @@ __global__ void __launch_bounds__(64 * GW) k_ad_gram(In a, Ws w) {
 
 // Gram totals: entry e of each of four quarters of the NG partials summed in order (loads 16 at a time), then the
 // quarters in order
-__global__ void __launch_bounds__(256) k_ad_gsum(Ws w) {
+__global__ void __launch_bounds__(256) k_ad_gsum(In a, Ws w) {
   __shared__ double red[256];
   if ((int)w.scal[0] < 100) return;
+  if (blockIdx.x == NE) {                  // the median beside the Gram totals
+    median_select(a, w);
+    return;
+  }
   const int e = blockIdx.x, t = threadIdx.x;
   // entry e: thread t sums the partials t, t + 256, ... in order, then a fixed tree over the threads
   double s = 0.0;
@@ -1017,7 +1018,7 @@ extern "C" {
 
 int64_t xpg_adapt_workspace_bytes(int32_t n) {
   Ws w = carve(nullptr, n);
-  return (int64_t)((char *)w.scal - (char *)nullptr) + 256;
+  return (int64_t)((char *)w.kc - (char *)nullptr) + 8 * (int64_t)(n > 0 ? n : 1) + 256;
 }
 
 int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int32_t *n_over, const double *frac_rec,
@@ -1053,10 +1054,10 @@ int xpg_adapt_block(int32_t n, const double *isum, const double *vsum, const int
   Ws w = carve(workspace, n);
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_ad_prep, dim3(w.nprep), dim3(256), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_grid, dim3(1 + NGW), dim3(1024), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(64), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_grid, dim3(NGW), dim3(256), 0, st, a, w);
+  hipLaunchKernelGGL(k_ad_finish, dim3(1), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_gram, dim3(NG), dim3(64 * GW), 0, st, a, w);
-  hipLaunchKernelGGL(k_ad_gsum, dim3(NE), dim3(256), 0, st, w);
+  hipLaunchKernelGGL(k_ad_gsum, dim3(NE + 1), dim3(256), 0, st, a, w);
   hipLaunchKernelGGL(k_ad_solve, dim3(1), dim3(64), 0, st, a, w, in, out_pass2, out_next);
   return hipGetLastError() == hipSuccess ? XPG_OK : XPG_ERR_HIP;
 }
