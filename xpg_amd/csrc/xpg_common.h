@@ -100,6 +100,50 @@ __device__ __forceinline__ double wsum_d(double v) {
   v += dppd<DPP_R8>(v);
   return (rld(v, 0) + rld(v, 16)) + (rld(v, 32) + rld(v, 48));
 }
+// Sums of V float64 values over the wavefront at once (V = 1, 2, 4, 8 or 16; every lane gets every sum): butterflies
+// over lane bits 0 .. 3 in which a lane keeps half of its values at each step (so V values cost about as much as two
+// wsum_d), then the two row bits; the row sums combine as ((r0 + r1) + (r2 + r3)).
+template <int S>
+__device__ __forceinline__ double xchg_row(double x) {         // the value of lane ^ (1 << S), S < 4
+  if constexpr (S == 0) return dppd<DPP_Q1>(x);
+  else if constexpr (S == 1) return dppd<DPP_Q2>(x);
+  else if constexpr (S == 2) return __shfl_xor(x, 4, 64);
+  else return dppd<DPP_R8>(x);                                   // rotation by 8 in a row of 16: lane ^ 8
+}
+template <int S, int N>
+__device__ __forceinline__ void wsum_step(double *v, int lane) {
+  if constexpr (N == 1) {
+    v[0] = v[0] + xchg_row<S>(v[0]);
+  } else {
+    constexpr int H = N / 2;
+    const bool hi = (lane >> S) & 1;                             // keeps values H .. N - 1
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      const double keep = hi ? v[i + H] : v[i], send = hi ? v[i] : v[i + H];
+      v[i] = keep + xchg_row<S>(send);
+    }
+  }
+}
+template <int V>
+__device__ __forceinline__ void wsum_multi(double (&v)[V]) {
+  static_assert(V == 1 || V == 2 || V == 4 || V == 8 || V == 16, "V: a power of two up to 16");
+  const int lane = threadIdx.x & 63;
+  wsum_step<0, V>(v, lane);
+  wsum_step<1, (V > 1 ? V / 2 : 1)>(v, lane);
+  wsum_step<2, (V > 2 ? V / 4 : 1)>(v, lane);
+  wsum_step<3, (V > 4 ? V / 8 : 1)>(v, lane);
+  double t = v[0];                       // lane l holds the row sum of value bitreverse(l mod V)
+  t = t + __shfl_xor(t, 16, 64);
+  t = t + __shfl_xor(t, 32, 64);
+  constexpr int B = V == 16 ? 4 : (V == 8 ? 3 : (V == 4 ? 2 : (V == 2 ? 1 : 0)));
+#pragma unroll
+  for (int j = 0; j < V; ++j) {
+    int L = 0;
+#pragma unroll
+    for (int b = 0; b < B; ++b) L |= ((j >> b) & 1) << (B - 1 - b);
+    v[j] = rld(t, L);
+  }
+}
 __device__ __forceinline__ int wsum_i(int v) {
   v += dppi<DPP_Q1>(v);
   v += dppi<DPP_Q2>(v);

@@ -692,8 +692,9 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   // ---- robust background b = sum(w c) / max(sum(w T), 1e-6) with four Poisson-aware clipping rounds ----
   // A round that leaves b unchanged (bit for bit) sets the same thresholds again, so every later round would repeat
   // it: the loop stops there.
-  double Sc = wsum_d(sc0);
-  double nbg = wsum_d(sT0);
+  double s2[2] = {sc0, sT0};
+  wsum_multi(s2);
+  double Sc = s2[0], nbg = s2[1];
   double bk = Sc / fmax(nbg, 1e-6);
   for (int round = 1; round < 5 && regs; ++round) {
     double sc = 0.0, sT = 0.0;
@@ -715,8 +716,10 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
       sc += cs;
       sT += (double)cnt * (double)brq[q];
     }
-    Sc = wsum_d(sc);
-    nbg = wsum_d(sT);
+    double r2[2] = {sc, sT};
+    wsum_multi(r2);
+    Sc = r2[0];
+    nbg = r2[1];
     const double bk_prev = bk;
     bk = Sc / fmax(nbg, 1e-6);
     if (bk == bk_prev) break;
@@ -749,8 +752,10 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         sT += (double)cnt * (double)bref[p];
       }
     }
-    Sc = wsum_d(sc);
-    nbg = wsum_d(sT);
+    double r2[2] = {sc, sT};
+    wsum_multi(r2);
+    Sc = r2[0];
+    nbg = r2[1];
     const double bk_prev = bk;
     bk = Sc / fmax(nbg, 1e-6);
     if (bk == bk_prev) break;           // the next threshold equals this one: uniform over the wavefront
@@ -839,21 +844,26 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
       }
     }
   }
-  Sfgc = wsum_d(Sfgc);
-  sfb = wsum_d(sfb);
-  Swpos = wsum_d(Swpos);
-  Swx = wsum_d(Swx);
-  Swy = wsum_d(Swy);
-  Swz = wsum_d(Swz);
-  Sm1 = wsum_d(Sm1);
-  Sm2 = wsum_d(Sm2);
-  Smdd = wsum_d(Smdd);
-  Sm3 = wsum_d(Sm3);
-  Sm33 = wsum_d(Sm33);
-  n_fg = wsum_i(n_fg);
-  n_over = wsum_i(n_over);
-  n_fgm = wsum_i(n_fgm);
-  n_fgnb = wsum_i(n_fgnb);
+  {                                      // the counts are exact as float64
+    double sv[16] = {Sfgc, sfb, Swpos, Swx, Swy, Swz, Sm1, Sm2, Smdd, Sm3, Sm33,
+                     (double)n_fg, (double)n_over, (double)n_fgm, (double)n_fgnb, 0.0};
+    wsum_multi(sv);
+    Sfgc = sv[0];
+    sfb = sv[1];
+    Swpos = sv[2];
+    Swx = sv[3];
+    Swy = sv[4];
+    Swz = sv[5];
+    Sm1 = sv[6];
+    Sm2 = sv[7];
+    Smdd = sv[8];
+    Sm3 = sv[9];
+    Sm33 = sv[10];
+    n_fg = (int)sv[11];
+    n_over = (int)sv[12];
+    n_fgm = (int)sv[13];
+    n_fgnb = (int)sv[14];
+  }
 
   const double gain = call.gain;
   const double isum = Sfgc - bk * sfb;
@@ -887,10 +897,11 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         if (((inm >> k) & 1u) && cv[k] >= 0.f && !((nbmm >> k) & 1u)) fzp[j] += (double)cv[k] - bpx;
       }
     }
+    wsum_multi(fzp);
 #pragma unroll
     for (int j = 0; j < DZM; ++j) {
       if (j < Dz) {
-        const double t = wsum_d(fzp[j]);
+        const double t = fzp[j];
         bool rec = false;
 #pragma unroll
         for (int q = 0; q < NPL; ++q) rec |= ((inm >> (q * DZM + j)) & 1u) && cv[q * DZM + j] > -2.5f;
@@ -1234,7 +1245,9 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     }
   }
   wave_sync();
-  double I = (wsum_d(s_c) - wsum_d(s_b)) / fmax(wsum_d(s_pw), 1e-6);
+  double s4[4] = {s_c, s_b, s_pw, 0.0};
+  wsum_multi(s4);
+  double I = (s4[0] - s4[1]) / fmax(s4[2], 1e-6);
   double Ip = I * has_p;
 
   // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
@@ -1313,11 +1326,23 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         }
       }
     }
-    a11 = wsum_d(s11);
-    const double b1 = wsum_d(sb1);
-    a12 = partner ? wsum_d(s12) : 0.0;
-    a22 = (partner ? wsum_d(s22) : 0.0) + (1.0 - has_p) * 1.0;
-    const double b2 = partner ? wsum_d(sb2) : 0.0;
+    double b1, b2 = 0.0;
+    if (partner) {
+      double t8[8] = {s11, sb1, s12, s22, sb2, 0.0, 0.0, 0.0};
+      wsum_multi(t8);
+      a11 = t8[0];
+      b1 = t8[1];
+      a12 = t8[2];
+      a22 = t8[3] + (1.0 - has_p) * 1.0;
+      b2 = t8[4];
+    } else {
+      double t2[2] = {s11, sb1};
+      wsum_multi(t2);
+      a11 = t2[0];
+      b1 = t2[1];
+      a12 = 0.0;
+      a22 = 0.0 + (1.0 - has_p) * 1.0;
+    }
     det2 = fmax(a11 * a22 - a12 * a12, 1e-30);
     const double I_old = I;
     I = (a22 * b1 - a12 * b2) / det2;
@@ -1381,8 +1406,10 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       }
     }
   }
-  const double C = wsum_d(sC), A110 = wsum_d(s110), C0 = wsum_d(sC0), Swp = wsum_d(swp), Sres = wsum_d(sres);
-  nuse = wsum_i(nuse);
+  double t1[8] = {sC, s110, sC0, swp, sres, (double)nuse, 0.0, 0.0};   // the count is exact as float64
+  wsum_multi(t1);
+  const double C = t1[0], A110 = t1[1], C0 = t1[2], Swp = t1[3], Sres = t1[4];
+  nuse = (int)t1[5];
   const double nu = fmax((double)nuse, 1.0);
   const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
   double sdd = 0.0, sdp = 0.0, snn = 0.0, s_pv = 0.0, s_pt = 0.0, s_pfg = 0.0;
@@ -1426,8 +1453,10 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       }
     }
   }
-  const double Sdp = wsum_d(sdp), Sdd = wsum_d(sdd), Snn = wsum_d(snn);
-  const double Spv = wsum_d(s_pv), Spt = wsum_d(s_pt), Spfg = wsum_d(s_pfg);
+  double t6[8] = {sdp, sdd, snn, s_pv, s_pt, s_pfg, 0.0, 0.0};
+  wsum_multi(t6);
+  const double Sdp = t6[0], Sdd = t6[1], Snn = t6[2];
+  const double Spv = t6[3], Spt = t6[4], Spfg = t6[5];
   if (lane == 0) {
     const double var_b = gain * fmax(bgr, VAR_FLOOR) / fmax(a.nbg[r], 1.0);
     const double a11i = a22 / det2;
