@@ -281,18 +281,19 @@ __device__ __forceinline__ double sm_of(const xpg_call &c) { return c.widths ? c
 __device__ __forceinline__ bool use_rcw_of(const xpg_call &c) { return c.widths != nullptr || c.use_rcw != 0; }
 __device__ __forceinline__ float rcw_of(const xpg_call &c, int j) { return c.widths ? c.widths->rcw[j] : c.rcw[j]; }
 
-// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when no mixture weights (j == 0 only)
-__device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double hi, double phi, double zeta, int j) {
-  double s = sm_of(c) / fmax(fabs(zeta), 1e-3);
+// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when no mixture weights (j == 0 only).  s: the
+// width in phi, isw = 1 / sw with sw = (s / 4) sqrt(2) the mixture components' scale (a product instead of a quotient:
+// the erf arguments move by at most an ulp)
+__device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double hi, double phi, double s, double isw,
+                                          int j) {
   if (!use_rcw_of(c)) {
     double a = (hi - phi) / (s * sqrt(2.0));
     double b = (lo - phi) / (s * sqrt(2.0));
     return 0.5 * (erf_tab(a) - erf_tab(b));
   }
-  double sw = (s * 0.25) * sqrt(2.0);
   double cj = (double)RC_C32[j];
-  double a = ((hi - phi) - cj * s) / sw;
-  double b = ((lo - phi) - cj * s) / sw;
+  double a = ((hi - phi) - cj * s) * isw;
+  double b = ((lo - phi) - cj * s) * isw;
   return ((double)rcw_of(c, j) * 0.5) * (erf_tab(a) - erf_tab(b));
 }
 
@@ -309,6 +310,8 @@ __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb,
 __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, double zeta, int Dz, double *scratch,
                               double *R, int lane) {
   const int nt = use_rcw_of(c) ? RCK : 1;
+  const double s = sm_of(c) / fmax(fabs(zeta), 1e-3);
+  const double isw = 1.0 / ((s * 0.25) * sqrt(2.0));
   constexpr int RB = 3;                      // terms per lane at a time: their erf table reads overlap
   for (int t0 = lane; t0 < Dz * nt; t0 += RB * WAVE) {
     double v[RB];
@@ -318,7 +321,7 @@ __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, do
       const int j = t / nt, q = t - j * nt;
       double lo, hi;
       frame_bounds(c, iz0, fb, j, lo, hi);
-      v[i] = rc_term(c, lo, hi, phi, zeta, q);
+      v[i] = rc_term(c, lo, hi, phi, s, isw, q);
     }
 #pragma unroll
     for (int i = 0; i < RB; ++i)

@@ -1216,56 +1216,30 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double pmax = wmax_d(p2m) * wmax_d(Rm);
   const double ppmax = partner ? (wmax_d(pp2m) * wmax_d(Rpm)) * has_p : 0.0;
   const double pthr = 0.01 * pmax, ppthr = 0.01 * fmax(ppmax, 1e-12);
-  double s_c = 0.0, s_b = 0.0, s_pw = 0.0;
-  for (int p0 = 0; p0 < P; p0 += WAVE) {
-    const int p = p0 + lane;
-    if (p < P) {
-      const double p2p = p2[p], pp2p = pp2[p];
-      const bool fg2 = pflag[p] & PF_FG2;
-      double cs = 0.0, pw = 0.0;
-      int cnt = 0;
-      for (int j = 0; j < Dz; ++j) {
-        const int v = j * P + p;
-        const float c = vals[v];
-        const double pd = p2p * Rd[j];
-        const double ppd = partner ? (pp2p * Rpd[j]) * has_p : 0.0;
-        uint8_t fl = vflag[v];
-        if (fg2 && fflag[j]) fl |= VF_FG;
-        if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
-          fl |= VF_USE;
-          cs += (double)c;
-          pw += pd + ppd;
-          ++cnt;
-        }
-        vflag[v] = fl;
-      }
-      s_c += cs;
-      s_b += (double)cnt * (bgr * (double)bref[p]);
-      s_pw += pw;
-    }
-  }
-  wave_sync();
-  double s4[4] = {s_c, s_b, s_pw, 0.0};
-  wsum_multi(s4);
-  double I = (s4[0] - s4[1]) / fmax(s4[2], 1e-6);
-  double Ip = I * has_p;
-
-  // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
-  const float gainf = (float)gain;
-  // Rows without a partner whose voxels fit NPL pixels x DZM frames per lane keep their fit voxels' net counts and
-  // profile values in registers across the iterations (the same products, summed in the same order)
+  // Rows without a partner whose voxels fit NPL pixels x DZM frames per lane (regs) read them from LDS once and keep,
+  // from the fit mask to the tail sums, their net counts and profile values in registers and their classes as bit
+  // masks (bit k = q DZM + j for pixel lane + 64 q, frame j): the same products, summed in the same order.
   constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
   const bool regs = !partner && P <= NPL * WAVE && Dz <= DZM;
   float netr[NK], pfr[NK], b0r[NPL];
   uint32_t usem = 0, inm = 0, validm = 0, fgm = 0;   // bit k: in the box / fit voxel / valid & ~nbm / foreground
-  if (regs) {
+  double s_c = 0.0, s_b = 0.0, s_pw = 0.0;
+  if (regs) {                             // no partner: pp = +0 and the fit test is p > 0.01 max p
+    uint32_t fgz = 0;
+#pragma unroll
+    for (int j = 0; j < DZM; ++j)
+      if (j < Dz && fflag[j]) fgz |= 1u << j;
 #pragma unroll
     for (int q = 0; q < NPL; ++q) {
       const int p = lane + q * WAVE;
       const bool pin = p < P;
+      const double p2p = pin ? p2[p] : 0.0;
+      const bool fg2 = pin && (pflag[p] & PF_FG2);
       const float b0 = pin ? (float)(bgr * (double)bref[p]) : 0.f;
-      const float p2f = pin ? (float)p2[p] : 0.f;
+      const float p2f = (float)p2p;
       b0r[q] = b0;
+      double cs = 0.0, pw = 0.0;
+      int cnt = 0;
 #pragma unroll
       for (int j = 0; j < DZM; ++j) {
         const int k = q * DZM + j;
@@ -1273,18 +1247,65 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         pfr[k] = 0.f;
         if (pin && j < Dz) {
           const int v = j * P + p;
-          const uint8_t fl = vflag[v];
           const float c = vals[v];
+          const bool vn = c >= 0.f && !(vflag[v] & VF_NBM);
+          const double pd = p2p * Rd[j];
           inm |= 1u << k;
           pfr[k] = p2f * Rf[j];
           netr[k] = c - b0;
-          if (fl & VF_USE) usem |= 1u << k;
-          if (c >= 0.f && !(fl & VF_NBM)) validm |= 1u << k;
-          if (fl & VF_FG) fgm |= 1u << k;
+          if (vn) validm |= 1u << k;
+          if (fg2 && ((fgz >> j) & 1u)) fgm |= 1u << k;
+          if (vn && pd > pthr) {
+            usem |= 1u << k;
+            cs += (double)c;
+            pw += pd + 0.0;
+            ++cnt;
+          }
         }
       }
+      if (pin) {
+        s_c += cs;
+        s_b += (double)cnt * (bgr * (double)bref[p]);
+        s_pw += pw;
+      }
     }
+  } else {
+    for (int p0 = 0; p0 < P; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const double p2p = p2[p], pp2p = pp2[p];
+        const bool fg2 = pflag[p] & PF_FG2;
+        double cs = 0.0, pw = 0.0;
+        int cnt = 0;
+        for (int j = 0; j < Dz; ++j) {
+          const int v = j * P + p;
+          const float c = vals[v];
+          const double pd = p2p * Rd[j];
+          const double ppd = partner ? (pp2p * Rpd[j]) * has_p : 0.0;
+          uint8_t fl = vflag[v];
+          if (fg2 && fflag[j]) fl |= VF_FG;
+          if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
+            fl |= VF_USE;
+            cs += (double)c;
+            pw += pd + ppd;
+            ++cnt;
+          }
+          vflag[v] = fl;
+        }
+        s_c += cs;
+        s_b += (double)cnt * (bgr * (double)bref[p]);
+        s_pw += pw;
+      }
+    }
+    wave_sync();
   }
+  double s4[4] = {s_c, s_b, s_pw, 0.0};
+  wsum_multi(s4);
+  double I = (s4[0] - s4[1]) / fmax(s4[2], 1e-6);
+  double Ip = I * has_p;
+
+  // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
+  const float gainf = (float)gain;
   double a11 = 0.0, a12 = 0.0, a22 = 1.0, det2 = 1.0, Iw = I, Ipw = Ip;
   for (int it_ = 0; it_ < N_IRLS; ++it_) {
     Iw = I;
