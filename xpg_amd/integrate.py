@@ -718,6 +718,7 @@ class Sweep:
         if batched and not hasattr(self, "side"):
             self.side = torch.cuda.Stream(device)      # default priorities
             self.xs = torch.cuda.Stream(device)
+            self.aux1 = torch.cuda.Stream(device)
         side = self.side if batched else main
         rb = RunBufs(self)
         self._init_widths()
@@ -840,21 +841,35 @@ class Sweep:
             G2 = rb.acc[i]
             acc = L.ProfileAcc(G2[0].data_ptr(), G2[1].data_ptr(), o["nreg"])
         fz_base, fz_row = self.fz.data_ptr(), 4 * self.fz.shape[1]
-        for start, m, shape, mixed, pp in units:
+        # several size bins: split between main and a helper stream (largest first, each to the less loaded one), so
+        # that the few wavefronts per CU of large boxes and the many of small boxes share the CUs
+        aux = self.aux1 if (cross and len(units) > 1 and getattr(self, "aux1", None) is not None) else None
+        where = [0] * len(units)
+        if aux is not None:
+            load = [0, 0]
+            for u in sorted(range(len(units)), key=lambda u: -units[u][1] * int(np.prod(units[u][2]))):
+                where[u] = int(load[1] < load[0])
+                load[where[u]] += units[u][1] * int(np.prod(units[u][2]))
+            aux.wait_stream(main)
+        for u, (start, m, shape, mixed, pp) in enumerate(units):
             view = self._view(start, m, shape, mixed)
             out = L.Pass1Out()
             for k in K.P1_F64 + K.P1_I32:
                 setattr(out, k, rb.p1(k, start))
             out.fz = fz_base + fz_row * start
             out.fz_stride = self.fz.shape[1]
-            with (timer("pass1", b, (start, m), main) if timer else nullctx):
+            s1 = aux if where[u] else main
+            sp1 = s1.cuda_stream if where[u] else st
+            with (timer("pass1", b, (start, m), s1) if timer else nullctx):
                 a_ = C.byref(acc) if acc is not None else None
                 if pp is not None:
                     L.check(lib.xpg_pass1_prepared(C.byref(det.s), C.byref(call1), C.byref(view.s), C.byref(pp),
-                                                   C.byref(out), a_, st))
+                                                   C.byref(out), a_, sp1))
                 else:
                     L.check(lib.xpg_pass1_frames(C.byref(det.s), C.byref(call1), C.byref(view.s),
-                                                 C.byref(self.store), C.byref(out), a_, st))
+                                                 C.byref(self.store), C.byref(out), a_, sp1))
+        if aux is not None:
+            main.wait_stream(aux)
         if after_pass1 is not None:
             after_pass1()
         if o["adaptive"]:
