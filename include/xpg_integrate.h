@@ -107,6 +107,10 @@ typedef struct xpg_pass2_out {
 typedef struct xpg_profile_acc {
   int64_t *num, *den;
   int32_t n_regions;
+  /* copies > 1: num and den each hold `copies` consecutive (n_regions, 41, 41) grids; the workgroups of a launch spread
+   * their integer increments over them (fewer atomics on one address) and xpg_normalise_profiles adds them up.
+   * 0 or 1: one grid. */
+  int32_t copies;
 } xpg_profile_acc;
 
 /* Device frame store: uint32 raw counts (n_frames, ny, nx) for 0-based frames frame0 .. frame0 + n_frames - 1, and the

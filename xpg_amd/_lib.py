@@ -65,7 +65,7 @@ class Pass2Out(C.Structure):
 
 
 class ProfileAcc(C.Structure):
-    _fields_ = [("num", C.c_void_p), ("den", C.c_void_p), ("n_regions", C.c_int32)]
+    _fields_ = [("num", C.c_void_p), ("den", C.c_void_p), ("n_regions", C.c_int32), ("copies", C.c_int32)]
 
 
 class Frames(C.Structure):

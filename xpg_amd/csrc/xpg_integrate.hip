@@ -520,11 +520,15 @@ __device__ __forceinline__ void splat(int64_t *grid, int idx, double v) {
   if (q != 0) atomicAdd((unsigned long long *)(grid + idx), (unsigned long long)q);
 }
 
+// the grid copy of this workgroup (integer sums: which copy takes an increment does not change the totals)
+__device__ __forceinline__ int acc_copy(const xpg_profile_acc &acc) {
+  return acc.copies > 1 ? (int)(blockIdx.x % (unsigned)acc.copies) : 0;
+}
 __device__ __forceinline__ void splat4(const xpg_profile_acc &acc, int region, double u1, double u2, double dens) {
   Bil q = bilinear(u1, u2);
   if (!q.inside) return;
   region = min(max(region, 0), acc.n_regions - 1);   // memory safety; the host validates region
-  const int i00 = region * G * G + q.i2 * G + q.i1;
+  const int i00 = (acc_copy(acc) * acc.n_regions + region) * G * G + q.i2 * G + q.i1;
   const double w00 = (1 - q.t1) * (1 - q.t2), w10 = q.t1 * (1 - q.t2), w01 = (1 - q.t1) * q.t2, w11 = q.t1 * q.t2;
   splat(acc.num, i00, w00 * dens);
   splat(acc.den, i00, w00);
@@ -1044,13 +1048,36 @@ __global__ void k_accumulate(const float *img2, const uint8_t *ok2, const float 
 __global__ void __launch_bounds__(256) k_normalise(xpg_profile_acc acc, const float *gauss, float *prof, double *num64,
                                                    double *den64) {
   __shared__ double red[256];
+  __shared__ double s_num[G * G], s_den[G * G];
   const int reg = blockIdx.x;
   const int GG = G * G;
   const double du = (2 * UMAX / (G - 1)) * (2 * UMAX / (G - 1));
+  const int nc = acc.copies > 1 ? acc.copies : 1;
+  const int64_t cs = (int64_t)acc.n_regions * GG;      // stride of the grid copies
+  // the grids summed over their copies (integers: exact, order-free), read eight copies at a time
+  for (int i = threadIdx.x; i < GG; i += blockDim.x) {
+    long long tn = 0, td = 0;
+    for (int c0 = 0; c0 < nc; c0 += 8) {
+      long long vn[8], vd[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const bool in = c0 + u < nc;
+        vn[u] = in ? acc.num[(c0 + u) * cs + reg * GG + i] : 0;
+        vd[u] = in ? acc.den[(c0 + u) * cs + reg * GG + i] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        tn += vn[u];
+        td += vd[u];
+      }
+    }
+    s_num[i] = (double)tn / FX_SCALE;
+    s_den[i] = (double)td / FX_SCALE;
+  }
+  __syncthreads();
   double part = 0.0;
   for (int i = threadIdx.x; i < GG; i += blockDim.x) {
-    const double num = (double)acc.num[reg * GG + i] / FX_SCALE;
-    const double den = (double)acc.den[reg * GG + i] / FX_SCALE;
+    const double num = s_num[i], den = s_den[i];
     if (num64) num64[reg * GG + i] = num;
     if (den64) den64[reg * GG + i] = den;
     const double dens = den > 0 ? num / fmax(den, 1e-12) : 0.0;
@@ -1065,8 +1092,7 @@ __global__ void __launch_bounds__(256) k_normalise(xpg_profile_acc acc, const fl
   }
   const double tot = fmax(red[0] * du, 1e-12);
   for (int i = threadIdx.x; i < GG; i += blockDim.x) {
-    const double num = (double)acc.num[reg * GG + i] / FX_SCALE;
-    const double den = (double)acc.den[reg * GG + i] / FX_SCALE;
+    const double num = s_num[i], den = s_den[i];
     const double dens = den > 0 ? num / fmax(den, 1e-12) : 0.0;
     const double pr = (den * fmax(dens, 0.0) + 3.0 * (double)gauss[i]) / (den + 3.0);
     prof[reg * GG + i] = (float)(pr / tot);
@@ -1669,7 +1695,8 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
       !out->m_d || !out->m_m || !out->frac_rec || !out->nfg || !out->zeta || !out->seff || !out->n_over ||
       !out->n_fg_masked || !out->n_fg_nb)
     return fail(XPG_ERR_ARG, "missing per-reflection output");
-  if (acc && (!acc->num || !acc->den || acc->n_regions < 1)) return fail(XPG_ERR_ARG, "bad accumulation grids");
+  if (acc && (!acc->num || !acc->den || acc->n_regions < 1 || acc->copies < 0 || acc->copies > 64))
+    return fail(XPG_ERR_ARG, "bad accumulation grids");
   if (det->n_seg < 1 || !det->P0 || !det->Bx || !det->By || !det->U1 || !det->U2 || !det->UN)
     return fail(XPG_ERR_ARG, "bad detector");
   if (refl->n == 0) return XPG_OK;
@@ -1729,6 +1756,7 @@ int xpg_accumulate_profiles(const float *img2, const uint8_t *ok2, const float *
 int xpg_normalise_profiles(const xpg_profile_acc *acc, const float *gauss, float *prof, double *num64, double *den64,
                            void *stream) {
   if (!acc || !acc->num || !acc->den || acc->n_regions < 1 || !gauss || !prof) return fail(XPG_ERR_ARG, "NULL argument");
+  if (acc->copies < 0 || acc->copies > 64) return fail(XPG_ERR_ARG, "copies %d outside [0, 64]", acc->copies);
   hipLaunchKernelGGL(k_normalise, dim3(acc->n_regions), dim3(256), 0, (hipStream_t)stream, *acc, gauss, prof, num64,
                      den64);
   HIP_TRY(hipGetLastError());

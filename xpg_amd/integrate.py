@@ -234,6 +234,10 @@ P1_KEEP = ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "frac_rec", "
 # mixed shoebox shapes; its wavefronts get the bin's largest slice)
 LDS_BINS = (8192, 12288, 16384, 24576, 32768, 49152, 65536, 98304, 163840)
 
+# copies of each block's fixed-point profile grids: pass 1's workgroups spread their integer increments over them
+# (fewer atomics on one address; the normalisation adds them up, exactly)
+ACC_COPIES = 8
+
 
 class SweepPlan:
     """Everything the device needs for one integrate() call: the reflection table in processing order, uploaded once.
@@ -839,7 +843,7 @@ class Sweep:
         acc = None
         if rb.learn:
             G2 = rb.acc[i]
-            acc = L.ProfileAcc(G2[0].data_ptr(), G2[1].data_ptr(), o["nreg"])
+            acc = L.ProfileAcc(G2[0].data_ptr(), G2[1].data_ptr(), o["nreg"], ACC_COPIES)
         fz_base, fz_row = self.fz.data_ptr(), 4 * self.fz.shape[1]
         # several size bins: split between main and a helper stream (largest first, each to the less loaded one), so
         # that the few wavefronts per CU of large boxes and the many of small boxes share the CUs
@@ -1011,7 +1015,7 @@ class RunBufs:
         nb, nreg = max(len(sweep.order), 1), sweep.o["nreg"]
         self.learn = bool(sweep.o["learn_profiles"])
         if self.learn:
-            self.acc = torch.zeros((nb, 2, nreg, G, G), dtype=torch.int64, device=dev)
+            self.acc = torch.zeros((nb, 2, ACC_COPIES * nreg, G, G), dtype=torch.int64, device=dev)
             self.prof = torch.empty((nb, nreg, G, G), dtype=torch.float32, device=dev)
             self.f64 = torch.empty((2, nreg, G, G), dtype=torch.float64, device=dev)
         self.ptr1 = {k: (v.data_ptr(), v.element_size()) for k, v in self.o1.items()}

@@ -178,7 +178,7 @@ P2_F64 = ("iprf", "vprf", "vprf0", "corr", "p_valid", "p_total", "ipartner", "rh
 def new_profile_acc(n_regions, device, G=PROFILE_GRID):
     num = torch.zeros((n_regions, G, G), dtype=torch.int64, device=device)
     den = torch.zeros((n_regions, G, G), dtype=torch.int64, device=device)
-    acc = L.ProfileAcc(L.ptr(num), L.ptr(den), n_regions)
+    acc = L.ProfileAcc(L.ptr(num), L.ptr(den), n_regions, 1)
     return num, den, acc
 
 
@@ -258,7 +258,7 @@ def normalise_profiles(num_fx, den_fx, device, G=PROFILE_GRID, gauss=None):
     """Fixed-point grids -> float32 profiles (xpg/integrate.py:546-556), on the device; also the float64 grids.
     ``gauss``: the (G, G) float32 Gaussian prior already on the device (no host copy: capturable in a graph)."""
     R = int(num_fx.shape[0])
-    acc = L.ProfileAcc(L.ptr(num_fx), L.ptr(den_fx), R)
+    acc = L.ProfileAcc(L.ptr(num_fx), L.ptr(den_fx), R, 1)
     if gauss is None:
         gauss = torch.as_tensor(gaussian_profile(G), device=device)
     prof = torch.empty((R, G, G), dtype=torch.float32, device=device)
