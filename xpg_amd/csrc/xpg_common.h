@@ -318,7 +318,7 @@ __device__ void rocking_curve(const xpg_call &c, int iz0, int fb, double phi, do
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       const int t = min(t0 + i * WAVE, Dz * nt - 1);
-      const int j = t / nt, q = t - j * nt;
+      const int j = nt == RCK ? t / RCK : t, q = t - j * nt;     // a constant divisor: multiply and shift
       double lo, hi;
       frame_bounds(c, iz0, fb, j, lo, hi);
       v[i] = rc_term(c, lo, hi, phi, s, isw, q);
