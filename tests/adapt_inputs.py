@@ -3,7 +3,8 @@ import numpy as np
 from scipy.special import erf
 
 
-def make_block(n=800, seed=0, sm_true_deg=0.13, osc_deg=0.1, phi0_deg=0.0, weak_frac=0.3):
+def make_block(n=800, seed=0, sm_true_deg=0.13, osc_deg=0.1, phi0_deg=0.0, weak_frac=0.3, md_levels=0):
+    """md_levels > 0: the second moments take only that many distinct values (many equal sigma_D terms)."""
     rng = np.random.default_rng(seed)
     zeta_k = rng.choice([-1, 1], n) * rng.uniform(0.2, 1.0, n)
     z = rng.uniform(40, 200, n)
@@ -26,6 +27,8 @@ def make_block(n=800, seed=0, sm_true_deg=0.13, osc_deg=0.1, phi0_deg=0.0, weak_
     frac_rec = np.where(rng.random(n) < 0.05, 0.9, 0.999)
     fb = np.where(rng.random(n) < 0.03, 2, 1).astype(np.int32)
     m_d = np.abs(rng.normal(2.5e-7, 5e-8, n))
+    if md_levels:
+        m_d = rng.choice(np.abs(rng.normal(2.5e-7, 5e-8, md_levels)), n)
     Xlen = rng.uniform(150, 220, n)
     cosa = 150.0 / Xlen
     return dict(isum=isum, vsum=vsum, n_over=n_over, frac_rec=frac_rec, m_d=m_d, fz=fz, phi_deg=phi_deg,

@@ -41,9 +41,14 @@ def _run_device(b, sd, sm, dev, active=0):
     return L.Widths.from_buffer_copy(raw[sz:2 * sz]), L.Widths.from_buffer_copy(raw[2 * sz:3 * sz])
 
 
-@pytest.mark.parametrize("seed,n,sm_true", [(3, 900, 0.15), (4, 3000, 0.08), (5, 150, 0.2), (6, 60, 0.1)])
-def test_adapt_block_matches_oracle(seed, n, sm_true):
-    b = make_block(n=n, seed=seed, sm_true_deg=sm_true)
+@pytest.mark.parametrize("seed,n,sm_true,levels", [
+    (3, 900, 0.15, 0), (4, 3000, 0.08, 0), (5, 150, 0.2, 0), (6, 60, 0.1, 0),
+    (7, 12000, 0.12, 0),      # 6469 strong rows, more than the median keeps in registers: keys read from memory
+    (8, 2500, 0.1, 7),        # seven distinct sigma_D terms: the middle rank among many equal keys (1367 strong)
+    (10, 2500, 0.1, 7),       # the same with an even strong count (1374): two middle ranks among equal keys
+    (10, 2501, 0.1, 0)])      # an even strong count (1380) with distinct keys
+def test_adapt_block_matches_oracle(seed, n, sm_true, levels):
+    b = make_block(n=n, seed=seed, sm_true_deg=sm_true, md_levels=levels)
     sd, sm = np.radians(0.02), np.radians(0.1)
     w2, wn = _run_device(b, sd, sm, torch.device("cuda:0"))
     m, sm_new, w, sd_new, sd_fit, sm_fit = oa.adapt_block(
