@@ -81,6 +81,10 @@ __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int
   return L;
 }
 
+// frames per batch of LDS reads in the per-pixel frame loops of boxes that are not held in registers: the reads
+// of a batch are in flight together, then its voxels are used in frame order
+constexpr int LB = 8;
+
 // per-voxel flags: neighbour ellipsoid, fit mask (pass 2), background candidate, foreground (fg & valid & ~nbm),
 // foreground ellipse x foreground frame
 enum : uint8_t { VF_NBM = 1, VF_USE = 2, VF_BGM = 4, VF_FGV = 8, VF_FG = 16 };
@@ -671,20 +675,32 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         const bool fg2 = pfl & PF_FG2, bgr = pfl & PF_BGR;
         double cs = 0.0;
         int cnt = 0;
-        for (int j = 0; j < Dz; ++j) {
-          const int v = j * P + p;
-          const float c = vals[v];
-          uint8_t fl = vflag[v];
-          const bool valid = c >= 0.f, nbm = fl & VF_NBM;
-          const bool fg = fg2 && fflag[j];
-          if (fg) fl |= VF_FG;
-          if (fg && valid && !nbm) fl |= VF_FGV;
-          if (!fg && valid && !nbm && bgr) {
-            fl |= VF_BGM;
-            cs += (double)c;
-            ++cnt;
+        for (int j0 = 0; j0 < Dz; j0 += LB) {       // LB frames' reads in flight before any write
+          float cc[LB];
+          uint8_t ff[LB], fz_[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
+            fz_[u] = fflag[j];
           }
-          vflag[v] = fl;
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            if (j0 + u >= Dz) break;
+            const float c = cc[u];
+            uint8_t fl = ff[u];
+            const bool valid = c >= 0.f, nbm = fl & VF_NBM;
+            const bool fg = fg2 && fz_[u];
+            if (fg) fl |= VF_FG;
+            if (fg && valid && !nbm) fl |= VF_FGV;
+            if (!fg && valid && !nbm && bgr) {
+              fl |= VF_BGM;
+              cs += (double)c;
+              ++cnt;
+            }
+            vflag[(j0 + u) * P + p] = fl;
+          }
         }
         sc0 += cs;
         sT0 += (double)cnt * (double)bref[p];
@@ -742,12 +758,19 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         const float th = thr[p];
         double cs = 0.0;
         int cnt = 0;
-        for (int j = 0; j < Dz; ++j) {
-          const int v = j * P + p;
-          if (vflag[v] & VF_BGM) {
-            const float c = vals[v];
-            if (c <= th) {
-              cs += (double)c;
+        for (int j0 = 0; j0 < Dz; j0 += LB) {
+          float cc[LB];
+          uint8_t ff[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
+          }
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            if (j0 + u < Dz && (ff[u] & VF_BGM) && cc[u] <= th) {
+              cs += (double)cc[u];
               ++cnt;
             }
           }
@@ -823,26 +846,38 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         const double bT = bk * (double)bref[p];
         double cs = 0.0, npos = 0.0, nzz = 0.0, nall = 0.0, n3 = 0.0, n33 = 0.0;
         int cnt = 0;
-        for (int j = 0; j < Dz; ++j) {
-          const int v = j * P + p;
-          const uint8_t fl = vflag[v];
-          if (!(fl & VF_FG)) continue;
-          const float c = vals[v];
-          n_over += (c == -2.0f);
-          n_fgm += (c < 0.f && c > -2.5f);
-          n_fgnb += (fl & VF_NBM) ? 1 : 0;
-          if (!(fl & VF_FGV)) continue;
-          const double net = (double)c - bT;
-          ++cnt;
-          cs += (double)c;
-          if (net > 0.0) {
-            npos += net;
-            nzz += net * zzd[j];
+        for (int j0 = 0; j0 < Dz; j0 += LB) {
+          float cc[LB];
+          uint8_t ff[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
           }
-          const double e3 = e3d[j];
-          nall += net;
-          n3 += net * e3;
-          n33 += net * (e3 * e3);
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = j0 + u;
+            if (j >= Dz) break;
+            const uint8_t fl = ff[u];
+            if (!(fl & VF_FG)) continue;
+            const float c = cc[u];
+            n_over += (c == -2.0f);
+            n_fgm += (c < 0.f && c > -2.5f);
+            n_fgnb += (fl & VF_NBM) ? 1 : 0;
+            if (!(fl & VF_FGV)) continue;
+            const double net = (double)c - bT;
+            ++cnt;
+            cs += (double)c;
+            if (net > 0.0) {
+              npos += net;
+              nzz += net * zzd[j];
+            }
+            const double e3 = e3d[j];
+            nall += net;
+            n3 += net * e3;
+            n33 += net * (e3 * e3);
+          }
         }
         fg_pixel(p, cs, npos, nzz, nall, n3, n33, cnt);
       }
@@ -924,10 +959,24 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     double part = 0.0;
     bool rec = false;
     if (j < Dz) {
-      for (int p = q; p < P; p += g) {
-        const float c = vals[j * P + p];
-        rec |= c > -2.5f;
-        if ((pflag[p] & PF_FG2) && c >= 0.f && !(vflag[j * P + p] & VF_NBM)) part += (double)c - bk * (double)bref[p];
+      for (int p0 = q; p0 < P; p0 += 4 * g) {      // four pixels' reads in flight together, used in pixel order
+        float cc[4], br[4];
+        uint8_t pf[4], ff[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int p = min(p0 + u * g, P - 1);
+          cc[u] = vals[j * P + p];
+          ff[u] = vflag[j * P + p];
+          pf[u] = pflag[p];
+          br[u] = bref[p];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (p0 + u * g >= P) break;
+          const float c = cc[u];
+          rec |= c > -2.5f;
+          if ((pf[u] & PF_FG2) && c >= 0.f && !(ff[u] & VF_NBM)) part += (double)c - bk * (double)br[u];
+        }
       }
     }
     part += dppd<DPP_Q1>(part);
@@ -996,11 +1045,22 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     double img = 0.0;
     bool okv = true;
     const double bpx = bk * (double)bref[p];
-    for (int j = 0; j < Dz; ++j) {
-      if (!fflag[j]) continue;
-      const float c = vals[j * P + p];
-      if (c >= 0.f && !(vflag[j * P + p] & VF_NBM)) img += (double)c - bpx;
-      else okv = false;
+    for (int j0 = 0; j0 < Dz; j0 += LB) {
+      float cc[LB];
+      uint8_t ff[LB], fz_[LB];
+#pragma unroll
+      for (int u = 0; u < LB; ++u) {
+        const int j = min(j0 + u, Dz - 1);
+        cc[u] = vals[j * P + p];
+        ff[u] = vflag[j * P + p];
+        fz_[u] = fflag[j];
+      }
+#pragma unroll
+      for (int u = 0; u < LB; ++u) {
+        if (j0 + u >= Dz || !fz_[u]) continue;
+        if (cc[u] >= 0.f && !(ff[u] & VF_NBM)) img += (double)cc[u] - bpx;
+        else okv = false;
+      }
     }
     img_pixel(p, img, okv);
   }
@@ -1303,20 +1363,35 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         const bool fg2 = pflag[p] & PF_FG2;
         double cs = 0.0, pw = 0.0;
         int cnt = 0;
-        for (int j = 0; j < Dz; ++j) {
-          const int v = j * P + p;
-          const float c = vals[v];
-          const double pd = p2p * Rd[j];
-          const double ppd = partner ? (pp2p * Rpd[j]) * has_p : 0.0;
-          uint8_t fl = vflag[v];
-          if (fg2 && fflag[j]) fl |= VF_FG;
-          if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
-            fl |= VF_USE;
-            cs += (double)c;
-            pw += pd + ppd;
-            ++cnt;
+        for (int j0 = 0; j0 < Dz; j0 += LB) {       // LB frames' reads in flight before any write
+          float cc[LB];
+          uint8_t ff[LB], fz_[LB];
+          double rd[LB], rpd[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
+            fz_[u] = fflag[j];
+            rd[u] = Rd[j];
+            rpd[u] = Rpd[j];
           }
-          vflag[v] = fl;
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            if (j0 + u >= Dz) break;
+            const float c = cc[u];
+            const double pd = p2p * rd[u];
+            const double ppd = partner ? (pp2p * rpd[u]) * has_p : 0.0;
+            uint8_t fl = ff[u];
+            if (fg2 && fz_[u]) fl |= VF_FG;
+            if (c >= 0.f && !(fl & VF_NBM) && ((pd > pthr) || (ppd > ppthr))) {
+              fl |= VF_USE;
+              cs += (double)c;
+              pw += pd + ppd;
+              ++cnt;
+            }
+            vflag[(j0 + u) * P + p] = fl;
+          }
         }
         s_c += cs;
         s_b += (double)cnt * (bgr * (double)bref[p]);
@@ -1355,20 +1430,32 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       if (p < P) {
         const float b0 = (float)(bgr * (double)bref[p]);
         const float p2f = (float)p2[p], pp2f = (float)pp2[p];
-        for (int j = 0; j < Dz; ++j) {
-          const int v = j * P + p;
-          if (!(vflag[v] & VF_USE)) continue;
-          const float net = vals[v] - b0;
-          const float pf = p2f * Rf[j];
-          const float ppf = (pp2f * Rpf[j]) * hpf;
-          const float mu = (b0 + Ii * pf) + Ipi * ppf;
-          const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
-          s11 += (double)((pf * pf) * iv);
-          sb1 += (double)((pf * net) * iv);
-          if (partner) {
-            s12 += (double)((pf * ppf) * iv);
-            s22 += (double)((ppf * ppf) * iv);
-            sb2 += (double)((ppf * net) * iv);
+        for (int j0 = 0; j0 < Dz; j0 += LB) {
+          float cc[LB], rf[LB], rpf[LB];
+          uint8_t ff[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
+            rf[u] = Rf[j];
+            rpf[u] = Rpf[j];
+          }
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            if (j0 + u >= Dz || !(ff[u] & VF_USE)) continue;
+            const float net = cc[u] - b0;
+            const float pf = p2f * rf[u];
+            const float ppf = (pp2f * rpf[u]) * hpf;
+            const float mu = (b0 + Ii * pf) + Ipi * ppf;
+            const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
+            s11 += (double)((pf * pf) * iv);
+            sb1 += (double)((pf * net) * iv);
+            if (partner) {
+              s12 += (double)((pf * ppf) * iv);
+              s22 += (double)((ppf * ppf) * iv);
+              sb2 += (double)((ppf * net) * iv);
+            }
           }
         }
       }
@@ -1435,21 +1522,33 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       const float b0 = (float)(bgr * (double)bref[p]), br = bref[p];
       const float p2f = (float)p2[p], pp2f = (float)pp2[p];
       const float iv0 = __builtin_amdgcn_rcpf(gainf * fmaxf(b0, (float)VAR_FLOOR));
-      for (int j = 0; j < Dz; ++j) {
-        const int v = j * P + p;
-        if (!(vflag[v] & VF_USE)) continue;
-        const float net = vals[v] - b0;
-        const float pf = p2f * Rf[j];
-        const float ppf = (pp2f * Rpf[j]) * hpf;
-        const float mu = (b0 + Ii * pf) + Ipi * ppf;
-        const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
-        const float pb = pf * br;
-        sC += (double)(pb * iv);
-        s110 += (double)((pf * pf) * iv0);
-        sC0 += (double)(pb * iv0);
-        swp += (double)pf;
-        sres += (double)(net - Ipos * ppf);
-        nuse += 1;
+      for (int j0 = 0; j0 < Dz; j0 += LB) {
+        float cc[LB], rf[LB], rpf[LB];
+        uint8_t ff[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int j = min(j0 + u, Dz - 1);
+          cc[u] = vals[j * P + p];
+          ff[u] = vflag[j * P + p];
+          rf[u] = Rf[j];
+          rpf[u] = Rpf[j];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          if (j0 + u >= Dz || !(ff[u] & VF_USE)) continue;
+          const float net = cc[u] - b0;
+          const float pf = p2f * rf[u];
+          const float ppf = (pp2f * rpf[u]) * hpf;
+          const float mu = (b0 + Ii * pf) + Ipi * ppf;
+          const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));   // 1 ulp
+          const float pb = pf * br;
+          sC += (double)(pb * iv);
+          s110 += (double)((pf * pf) * iv0);
+          sC0 += (double)(pb * iv0);
+          swp += (double)pf;
+          sres += (double)(net - Ipos * ppf);
+          nuse += 1;
+        }
       }
     }
   }
@@ -1481,22 +1580,35 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]);
       const float p2f = (float)p2[p], pp2f = (float)pp2[p];
-      for (int j = 0; j < Dz; ++j) {
-        const int v = j * P + p;
-        const uint8_t fl = vflag[v];
-        const float pf = p2f * Rf[j];
-        const float c = vals[v];
-        s_pt += (double)pf;
-        if (c >= 0.f && !(fl & VF_NBM)) s_pv += (double)pf;
-        if (fl & VF_FG) s_pfg += (double)pf;
-        if (!(fl & VF_USE)) continue;
-        const float net = c - b0;
-        const float ppf = (pp2f * Rpf[j]) * hpf;
-        const float dp = pf - pm;
-        const float dn = (net - Ipos * ppf) - nm;
-        sdp += (double)(dp * dn);
-        sdd += (double)(dp * dp);
-        snn += (double)(dn * dn);
+      for (int j0 = 0; j0 < Dz; j0 += LB) {
+        float cc[LB], rf[LB], rpf[LB];
+        uint8_t ff[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int j = min(j0 + u, Dz - 1);
+          cc[u] = vals[j * P + p];
+          ff[u] = vflag[j * P + p];
+          rf[u] = Rf[j];
+          rpf[u] = Rpf[j];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          if (j0 + u >= Dz) break;
+          const uint8_t fl = ff[u];
+          const float pf = p2f * rf[u];
+          const float c = cc[u];
+          s_pt += (double)pf;
+          if (c >= 0.f && !(fl & VF_NBM)) s_pv += (double)pf;
+          if (fl & VF_FG) s_pfg += (double)pf;
+          if (!(fl & VF_USE)) continue;
+          const float net = c - b0;
+          const float ppf = (pp2f * rpf[u]) * hpf;
+          const float dp = pf - pm;
+          const float dn = (net - Ipos * ppf) - nm;
+          sdp += (double)(dp * dn);
+          sdd += (double)(dp * dp);
+          snn += (double)(dn * dn);
+        }
       }
     }
   }
