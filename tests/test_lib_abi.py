@@ -7,7 +7,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 STRUCTS = dict(Detector="xpg_detector", Call="xpg_call", Refl="xpg_refl", Pass1Out="xpg_pass1_out",
                Pass2Out="xpg_pass2_out", ProfileAcc="xpg_profile_acc", Frames="xpg_frames", Widths="xpg_widths",
-               Prepared="xpg_prepared")
+               Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch")
 
 
 def _declared():
@@ -31,7 +31,7 @@ def _prototypes():
                     kinds.append("ptr")
                 else:
                     kinds.append({"double": "f64", "int32_t": "i32", "int64_t": "i64", "uint32_t": "u32",
-                                  "int": "i32"}[q.split()[0]])
+                                  "int": "i32", "xpg_lz4_compress_fn": "ptr"}[q.split()[0]])
             protos[m.group(1)] = kinds
     return protos
 
@@ -66,7 +66,8 @@ def test_library_exports_declared_symbols():
 def test_ctypes_structs_match_c_layout(tmp_path):
     import ctypes
     from xpg_amd import _lib
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', "int main(void) {"]
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', '#include "xpg_decode.h"',
+             "int main(void) {"]
     for py, c in STRUCTS.items():
         lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
         for f, _ in getattr(_lib, py)._fields_:

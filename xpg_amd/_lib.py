@@ -1,4 +1,4 @@
-"""ctypes binding of libxpg_integrate.so (include/xpg_integrate.h).
+"""ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_synth.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
 is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
@@ -81,6 +81,16 @@ class Prepared(C.Structure):
                 ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p)]
 
 
+
+class Bslz4Batch(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("chunk_off", C.c_void_p), ("chunk_len", C.c_void_p), ("n_frames", C.c_int32),
+                ("esize", C.c_int32), ("nel", C.c_int64), ("block_elems", C.c_int32), ("n_blocks", C.c_int32),
+                ("table", C.c_void_p), ("status", C.c_void_p), ("out", C.c_void_p), ("out_kind", C.c_int32),
+                ("out_stride", C.c_int64), ("out_slot", C.c_void_p), ("use_min_valid", C.c_int32),
+                ("use_overload", C.c_int32), ("min_valid_ceil", C.c_int64), ("overload_ceil", C.c_int64),
+                ("static_bad", C.c_void_p)]
+
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -108,6 +118,13 @@ _SIGS = {
     "xpg_adapt_workspace_bytes": ([C.c_int32], C.c_int64),
     "xpg_synth_background": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "xpg_synth_spots_add": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "xpg_bslz4_block_elems": ([C.c_uint32, C.c_int32], C.c_int32),
+    "xpg_bslz4_n_blocks": ([C.c_int64, C.c_int32], C.c_int32),
+    "xpg_bslz4_index_host": ([C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int32),
+    "xpg_bslz4_index": ([P(Bslz4Batch), C.c_void_p], C.c_int),
+    "xpg_bslz4_decode": ([P(Bslz4Batch), C.c_void_p], C.c_int),
+    "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
+                              C.c_int64),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -30,6 +30,11 @@ static int fail(int code, const char *fmt, ...) {
   return code;
 }
 
+namespace xpg {
+// error message of the other sources of the library (xpg_decode.hip), read back by xpg_last_error()
+int report_error(int code, const char *msg) { return fail(code, "%s", msg); }
+}  // namespace xpg
+
 #define HIP_TRY(expr)                                                                              \
   do {                                                                                             \
     hipError_t e_ = (expr);                                                                        \
