@@ -287,12 +287,111 @@ def bench_serve(args, dev):
     print(json.dumps(out), flush=True)
 
 
+def bench_decode(args, dev):
+    """Bitshuffle-LZ4 decode of a sweep's frames: frames of the full-size detector (4148 x 4362 uint32, the synthetic
+    C2 sweep's first frames) written as bitshuffle-LZ4 chunks by the system's LZ4 library, the chunks and their block
+    tables resident in HBM; a step decodes every chunk into the frame store (raw counts).  Roofline: compressed bytes
+    read + 4 bytes per pixel written, per launch of k_bslz4_decode, over its HIP-event time on the launch stream."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+    from xpg_amd import _lib as L, bslz4, synth
+
+    torch.cuda.set_device(dev)
+    nfr = args.frames or 200
+    spec = synth.replace(synth.spec_full(), nframes=nfr, name=f"full[:{nfr}]")
+    t = time.time()
+    frames, _ = synth.generate(spec, dev, log=log)
+    host = frames.data.cpu().numpy().view(np.uint32)
+    log(f"generated {nfr} frames {spec.ny}x{spec.nx} in {time.time() - t:.1f}s")
+    t = time.time()
+    chunks = bslz4.encode_frames(list(host), 0, threads=min(16, os.cpu_count() or 1))
+    t_enc = time.time() - t
+    t = time.time()
+    cb = bslz4.ChunkBatch(chunks, spec.ny * spec.nx, 4, dev, index="host")
+    torch.cuda.synchronize(dev)
+    t_index = time.time() - t
+    comp = sum(len(c) for c in chunks)
+    raw_bytes = host.nbytes
+    log(f"encoded {raw_bytes / 1e9:.2f} GB -> {comp / 1e9:.2f} GB (ratio {raw_bytes / comp:.2f}) in {t_enc:.1f}s; "
+        f"host block index + upload {t_index:.2f}s")
+    out = torch.zeros_like(frames.data)
+    st = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        cb.decode_into(out, "raw")
+    cb.check()
+    assert torch.equal(out, frames.data), "decoded frames differ from the originals"
+    # device walk of the block chains (the form for chunks that arrive without a table)
+    db = bslz4.ChunkBatch(chunks, spec.ny * spec.nx, 4, dev, index="device")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(3):
+        db.status.zero_()
+        L.check(L.lib().xpg_bslz4_index(C.byref(db._batch()), L.stream_ptr(dev)))
+    e1.record(st)
+    torch.cuda.synchronize(dev)
+    index_ms = e0.elapsed_time(e1) / 3
+    assert torch.equal(db.table, cb.table)
+    del db
+    out.zero_()
+    torch.cuda.synchronize(dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.time()
+    for a, b in ev:
+        a.record(st)
+        cb.decode_into(out, "raw")
+        b.record(st)
+    torch.cuda.synchronize(dev)
+    wall = (time.time() - t0) / args.steps
+    ms = [a.elapsed_time(b) for a, b in ev]
+    cb.check()
+    assert torch.equal(out, frames.data), "decoded frames differ from the originals"
+    avg = sum(ms) / len(ms)
+    alg = comp + raw_bytes
+    achieved = alg / (avg * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_bslz4_decode_decode.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    roof = dict(bound="hbm", kernel="k_bslz4_decode", achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=round(achieved / PEAK_HBM_GBS, 4), traffic=traffic, avg_launch_ms=round(avg, 4),
+                alg_bytes_per_launch=alg, launches_per_step=1, compressed_bytes=comp, decoded_bytes=raw_bytes,
+                device_index_ms=round(index_ms, 3))
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (C, one core): LZ4 stage + bit unshuffle of whole frames, a bounded sample
+        from oracle import codec as oc
+        t = time.time()
+        n = 0
+        while n < len(chunks) and (time.time() - t < args.cpu_seconds or n < 2):
+            r, _ = oc.decode_chunk(chunks[n], spec.ny * spec.nx, 4)
+            assert r > 0
+            n += 1
+        dt = time.time() - t
+        cpu = dict(value=round(n / dt, 3), unit="frames/s", cores=1, kind="port",
+                   sample=f"{n} frames of {spec.ny}x{spec.nx} uint32 (the first of the same chunks), oracle/codec.c "
+                          f"LZ4 stage + bit unshuffle, {dt:.1f} s")
+        log(f"cpu baseline {cpu}")
+    value = nfr / (avg * 1e-3)
+    outj = dict(metric="bitshuffle-LZ4 frames decoded/s on 1 MI355X", value=round(value, 1), unit="frames/s",
+                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(avg, 3), higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="u8", data="synthetic",
+                config=dict(workload=f"bslz4_decode_{spec.nx}x{spec.ny}x{nfr}", frames=nfr,
+                            frame_shape=[spec.ny, spec.nx], element="uint32", block_bytes=8192,
+                            compression_ratio=round(raw_bytes / comp, 3), encoder="liblz4 LZ4_compress_default",
+                            decoded_GBps=round(raw_bytes / (avg * 1e-3) / 1e9, 1), wall_ms_per_step=round(wall * 1e3, 3),
+                            identical_to_source=True, parallelism="single GPU"),
+                roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(outj), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode"])
     ap.add_argument("--serve-datasets", type=int, default=32)
     ap.add_argument("--serve-frame-scale", type=float, default=0.1,
                     help="serving workload: frame counts U{50..3600} times this factor (at least 20 frames)")
@@ -351,6 +450,9 @@ def main():
         torch.cuda.synchronize(dev)
         log(f"copy calibration: {src.numel() * 4} bytes read, {dst.numel() * 4} bytes written")
         del src, dst
+    if args.workload == "decode":
+        bench_decode(args, dev)
+        return
     spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
                 stress=synth.spec_stress)[args.workload]()
     if args.frames:
