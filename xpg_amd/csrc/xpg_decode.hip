@@ -4,13 +4,15 @@
 // and the device undoes the bit shuffle and applies the value codes (unshuffle_frame_torch, xpg/io/bslz4.py:80-108).
 // Here both run in one kernel over chunks resident in HBM:
 //
-//   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block).  The wave stages
-//                    the compressed block in its LDS slice with aligned 16-byte loads, parses the sequences with
-//                    wave-uniform control flow (token, lengths and offsets in scalar registers), copies each literal
-//                    run and match with all 64 lanes, then undoes the bit shuffle from LDS -- lane l takes the groups
-//                    of 8 elements g = l, l + 64, ...: the 8 x 8 bit matrix of each output byte is gathered from 8
-//                    planes and transposed in registers -- and writes the block's pixels (raw uint32 counts for the
-//                    integrator's frame store, or value-coded float32).
+//   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block), decoding into
+//                    its LDS slice (the block only, so that 20 waves fit a CU).  The compressed bytes stay in
+//                    registers: two 256-byte windows, lane l holding a dword of each, the next one loaded ahead.  The
+//                    parse is wave-uniform (token, lengths and offset read from the window with readlane into scalar
+//                    registers); a literal run is written by the lanes that hold its bytes, with no read; a match is
+//                    copied by all 64 lanes through LDS.  Then the bit shuffle is undone from LDS -- lane l takes the
+//                    groups of 8 elements g = l, l + 64, ...: the 8 x 8 bit matrix of each output byte is gathered from
+//                    8 planes and transposed in registers -- and the block's pixels are written with 16-byte stores
+//                    (raw uint32 counts for the integrator's frame store, or value-coded float32).
 //   k_bslz4_index    one lane per frame walks the chain of block size fields into the block table (the host form,
 //                    xpg_bslz4_index_host, is the same walk over a chunk in host memory).
 //
@@ -37,6 +39,8 @@ constexpr int MINMATCH = 4;       // LZ4 block format: shortest match
 constexpr int LASTLITERALS = 5;   // the last 5 output bytes are literals
 constexpr int MFLIMIT = 12;       // a match may not start in the last 12 output bytes
 constexpr int MAX_BLOCK_BYTES = 65536;
+constexpr int DUMMY_PAD = 16;       // the dummy area starts 16 bytes past the block's bytes (dword aligned)
+constexpr int DUMMY_BYTES = 256;
 
 __host__ __device__ inline uint32_t be32(const uint8_t *p) {
   return (uint32_t)p[0] << 24 | (uint32_t)p[1] << 16 | (uint32_t)p[2] << 8 | (uint32_t)p[3];
@@ -106,8 +110,6 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
-
 struct Batch {
   const uint8_t *src;
   const int64_t *coff, *clen;
@@ -122,7 +124,7 @@ struct Batch {
   int32_t use_min, use_ovl;
   int64_t min_ceil, ovl_ceil;
   const uint8_t *sbad;
-  int32_t cap_c, wave_lds;  // LDS: staging region (16-aligned, + 16 for the source misalignment), then the block
+  int32_t wave_lds;         // LDS per wave: the decoded block, then the dummy area
   int64_t f0;               // first frame of this launch
 };
 
@@ -134,103 +136,190 @@ __global__ void __launch_bounds__(64) k_bslz4_index(Batch b) {
   b.status[f] = st;
 }
 
-// Extended length (LZ4 block format: 255-valued bytes add up until a smaller byte) read 64 bytes at a time.  The
-// sequential reading stops after the byte that brings ip to `lim` (LZ4_decompress_safe's bound checks); `stopped`
-// reports that.  Returns the added length; ip advances past the bytes read.
-__device__ __forceinline__ int ext_len(const uint8_t *cb, int cs, int &ip, int lim, bool &stopped, int lane) {
+// The compressed block as seen by the wave: two 256-byte windows in registers, lane l holding the dword at window
+// offset 4 l.  `wb` is the current window's start relative to the block (the block's first byte rounded down to a
+// dword, so wb starts in -3 .. 0); the next window is loaded ahead.  Bytes outside the chunk read as 0 and are never
+// used (every read below stays inside the block).
+struct Window {
+  const uint8_t *blk, *lo, *hi;   // block start; the chunk's bytes [lo, hi)
+  int wb;
+  uint32_t cur, nxt;
+
+  __device__ __forceinline__ uint32_t load(int base, int lane) const {
+    const uint8_t *a = blk + base + 4 * lane;
+    if (a >= lo && a + 4 <= hi) return *reinterpret_cast<const uint32_t *>(a);
+    uint32_t v = 0;
+    for (int b = 0; b < 4; ++b)
+      if (a + b >= lo && a + b < hi) v |= (uint32_t)a[b] << (8 * b);
+    return v;
+  }
+  __device__ __forceinline__ void init(const uint8_t *block, const uint8_t *chunk_lo, const uint8_t *chunk_hi,
+                                       int lane) {
+    blk = block;
+    lo = chunk_lo;
+    hi = chunk_hi;
+    wb = -(int)((uintptr_t)block & 3);
+    cur = load(wb, lane);
+    nxt = load(wb + 256, lane);
+  }
+  // make position q (uniform, q >= wb) part of the current window
+  __device__ __forceinline__ void reach(int q, int lane) {
+    while (q >= wb + 256) {
+      cur = nxt;
+      wb += 256;
+      nxt = load(wb + 256, lane);
+    }
+  }
+  // byte at position q, wb <= q < wb + 512 (uniform)
+  __device__ __forceinline__ int at(int q) const {
+    const int i = q - wb;
+    const uint32_t w = i < 256 ? (uint32_t)__builtin_amdgcn_readlane((int)cur, i >> 2)
+                               : (uint32_t)__builtin_amdgcn_readlane((int)nxt, (i - 256) >> 2);
+    return (int)((w >> (8 * (i & 3))) & 0xFF);
+  }
+};
+
+// Extended length (LZ4 block format: 255-valued bytes add up until a smaller byte).  The reading stops after the byte
+// that brings ip to `lim` (LZ4_decompress_safe's bound checks); `stopped` reports that.
+__device__ __forceinline__ int ext_len(Window &w, int &ip, int lim, bool &stopped, int lane) {
   int add = 0;
   for (;;) {
-    const int p = ip + lane;
-    const int v = p < cs ? cb[p] : 0;
-    const uint64_t nz = __ballot(v != 255);
-    const int q = nz ? __builtin_ctzll(nz) : WAVE;
-    const int istop = lim - ip - 1;   // index of the byte after which ip reaches lim
-    const int c = q < istop ? q : istop;
-    if (c >= WAVE) {                  // 64 bytes of 255 and no stop
-      add += 255 * WAVE;
-      ip += WAVE;
-      continue;
+    w.reach(ip, lane);
+    const int v = w.at(ip);
+    ++ip;
+    add += v;
+    if (ip >= lim) {
+      stopped = true;
+      return add;
     }
-    add += 255 * c + __builtin_amdgcn_readlane(v, c);
-    ip += c + 1;
-    stopped = c == istop;
-    return add;
+    if (v != 255) return add;
   }
 }
 
-// One LZ4 block of cs bytes (cb, LDS) into exactly dn bytes (db, LDS), by the whole wave; 0 or XPG_BSLZ4_LZ4.
-// Rules of the LZ4 block format as LZ4_decompress_safe applies them: literals that reach the last MFLIMIT output
-// bytes or the last 8 input bytes end the block (they must consume the input exactly); a match must lie in the
-// decoded output and end before the last LASTLITERALS bytes; offset 0 is rejected.
-__device__ int lz4_block(const uint8_t *cb, int cs, uint8_t *db, int dn, int lane) {
+// Literal run [ip, ip + lit) of the block to db[op ..]: every lane writes the bytes of its own window dword that fall
+// in the run (the others store to their own byte of the dummy area), one window at a time.  Leaves the window at
+// ip + lit.
+__device__ __forceinline__ void literals(Window &w, int ip, int lit, uint8_t *db, int op, int dummy, int lane) {
+  const int end = ip + lit;
+  while (ip < end) {
+    w.reach(ip, lane);
+    const int p0 = w.wb + 4 * lane;          // position of this lane's first byte
+    const int seg_end = min(end, w.wb + 256);
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int p = p0 + b;
+      db[(p >= ip && p < seg_end) ? op + (p - ip) : dummy + lane] = (uint8_t)(w.cur >> (8 * b));
+    }
+    op += seg_end - ip;
+    ip = seg_end;
+  }
+}
+
+// Match: db[op + j] = db[op - off + j] for j = 0 .. ml - 1 in order (an overlap repeats the period), off <= op.
+// Byte op + j depends on byte op + j - off, so steps of up to off bytes are free of dependencies inside the step;
+// LDS executes the wave's instructions in order, so each step reads what the earlier steps wrote.  Loops have uniform
+// trip counts and every lane stores each step: lanes with nothing to copy write their own slot of the dummy area
+// at `dummy` (256 bytes after the block), so no store needs an exec-mask branch.
+__device__ __forceinline__ void match_copy(uint8_t *db, int op, int off, int ml, int dummy, int lane) {
+  wave_sync();
+  if (ml <= WAVE && off >= ml) {
+    // the common case: at most 64 bytes, no overlap -- one load and one store per lane
+    const uint8_t v = db[op - off + min(lane, ml - 1)];
+    db[lane < ml ? op + lane : dummy + lane] = v;
+  } else if (off < WAVE) {
+    // period below 64: lane i < T = off * floor(64 / off) writes bytes i, i + T, ... with pattern byte i mod off
+    // (floor((x + 0.5) / off) for integers x, off < 65 is exact in float: the quotient is at least 0.5 / 64 from an
+    // integer)
+    const float inv = 1.0f / (float)off;
+    const int T = off * (int)(64.5f * inv);
+    const int r = lane - off * (int)(((float)lane + 0.5f) * inv);
+    const uint8_t v = db[op - off + r];
+    for (int base = 0; base < ml; base += T) {
+      const int j = base + lane;
+      db[(lane < T && j < ml) ? op + j : dummy + lane] = v;
+    }
+  } else if ((off & 3) == 0) {
+    // whole dwords (source and destination share the alignment): up to 64 dwords per step, at most off / 4
+    const int h = min(ml, (4 - (op & 3)) & 3);
+    {
+      const uint8_t v = db[op - off + min(lane, 3)];
+      db[lane < h ? op + lane : dummy + lane] = v;
+    }
+    const int nw = (ml - h) >> 2;
+    const int D = min(off >> 2, WAVE);
+    const int a0 = op + h;   // dword aligned, as is a0 - off
+    for (int s0 = 0; s0 < nw; s0 += D) {
+      wave_sync();
+      const int i = s0 + lane;
+      const bool ok = lane < D && i < nw;
+      const uint32_t v = *reinterpret_cast<const uint32_t *>(db + (ok ? a0 - off + 4 * i : 0));
+      *reinterpret_cast<uint32_t *>(db + (ok ? a0 + 4 * i : dummy + 4 * lane)) = v;
+    }
+    wave_sync();
+    const int t0 = h + 4 * nw;
+    const bool ok = lane < ml - t0;
+    const uint8_t v = db[ok ? op + t0 + lane - off : 0];
+    db[ok ? op + t0 + lane : dummy + lane] = v;
+  } else {
+    // bytes: steps of min(off, 256), up to 4 bytes per lane, loads before stores
+    const int S = min(off, 4 * WAVE);
+    for (int s0 = 0; s0 < ml; s0 += S) {
+      wave_sync();
+      uint8_t v[4];
+      bool ok[4];
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int j = s0 + lane + WAVE * b;
+        ok[b] = lane + WAVE * b < S && j < ml;
+        v[b] = db[ok[b] ? op + j - off : 0];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) db[ok[b] ? op + s0 + lane + WAVE * b : dummy + 4 * lane + b] = v[b];
+    }
+  }
+  wave_sync();
+}
+
+// One LZ4 block of cs bytes (window over the chunk) into exactly dn bytes (db, LDS), by the whole wave; 0 or
+// XPG_BSLZ4_LZ4.  Rules of the LZ4 block format as LZ4_decompress_safe applies them: literals that reach the last
+// MFLIMIT output bytes or the last 8 input bytes end the block (they must consume the input exactly); a match must lie
+// in the decoded output and end before the last LASTLITERALS bytes; offset 0 is rejected.
+__device__ int lz4_block(Window &w, int cs, uint8_t *db, int dn, int lane) {
   int ip = 0, op = 0;
   for (;;) {
     if (ip >= cs) return XPG_BSLZ4_LZ4;
-    const int tok = uni(cb[ip]);
+    w.reach(ip, lane);
+    const int tok = w.at(ip);
     ++ip;
     int lit = tok >> 4;
     if (lit == 15) {
       if (ip >= cs - 15) return XPG_BSLZ4_LZ4;
       bool stopped = false;
-      lit += ext_len(cb, cs, ip, cs - 15, stopped, lane);
-      ip = uni(ip);
-      lit = uni(lit);
+      lit += ext_len(w, ip, cs - 15, stopped, lane);
     }
     if (lit > cs - ip) return XPG_BSLZ4_LZ4;
     const bool last = lit > dn - MFLIMIT - op || lit > cs - 8 - ip;
     if (last && (ip + lit != cs || lit > dn - op)) return XPG_BSLZ4_LZ4;
-    for (int j = lane; j < lit; j += WAVE) db[op + j] = cb[ip + j];
+    if (lit) literals(w, ip, lit, db, op, dn + DUMMY_PAD, lane);
     ip += lit;
     op += lit;
     if (last) break;
-    const int off = uni(cb[ip] | cb[ip + 1] << 8);
+    w.reach(ip, lane);
+    const int off = w.at(ip) | w.at(ip + 1) << 8;
     ip += 2;
     int ml = tok & 15;
     if (ml == 15) {
       bool stopped = false;
-      ml += ext_len(cb, cs, ip, cs - LASTLITERALS + 1, stopped, lane);
-      ip = uni(ip);
-      ml = uni(ml);
+      ml += ext_len(w, ip, cs - LASTLITERALS + 1, stopped, lane);
       if (stopped) return XPG_BSLZ4_LZ4;
     }
     ml += MINMATCH;
     if (off == 0 || off > op || ml > dn - LASTLITERALS - op) return XPG_BSLZ4_LZ4;
-    wave_sync();
-    if (off >= WAVE || off >= ml) {
-      // every source byte was written before the pass of 64 that reads it
-      for (int j = lane; j < ml; j += WAVE) {
-        db[op + j] = db[op - off + j];
-        wave_sync();
-      }
-    } else {
-      // overlapping match of period off < 64: byte j repeats byte j mod off of the last off bytes
-      int r = lane % off;
-      const int step = WAVE % off;
-      for (int j = lane; j < ml; j += WAVE) {
-        db[op + j] = db[op - off + r];
-        r += step;
-        if (r >= off) r -= off;
-      }
-    }
+    match_copy(db, op, off, ml, dn + DUMMY_PAD, lane);
     op += ml;
     wave_sync();
   }
   return op == dn ? 0 : XPG_BSLZ4_LZ4;
-}
-
-// Global -> LDS copy of n bytes from s with aligned 16-byte loads: byte i lands at base + (s mod 16) + i, so the
-// aligned middle of the source maps to aligned LDS words.  Returns the LDS address of byte 0.
-__device__ __forceinline__ uint8_t *stage(const uint8_t *s, int n, uint8_t *base, int lane) {
-  const int mis = (int)((uintptr_t)s & 15);
-  uint8_t *d = base + mis;
-  const int head = std::min(n, (16 - mis) & 15);
-  for (int j = lane; j < head; j += WAVE) d[j] = s[j];
-  const int nw = (n - head) >> 4;
-  const uint4 *sw = reinterpret_cast<const uint4 *>(s + head);
-  uint4 *dw = reinterpret_cast<uint4 *>(d + head);
-  for (int w = lane; w < nw; w += WAVE) dw[w] = sw[w];
-  for (int j = head + (nw << 4) + lane; j < n; j += WAVE) d[j] = s[j];
-  return d;
 }
 
 template <int KIND>
@@ -238,16 +327,69 @@ __device__ __forceinline__ void put(const Batch &b, void *out, int64_t e, uint32
   if constexpr (KIND == 0) {
     static_cast<uint32_t *>(out)[e] = v;
   } else {
-    // unshuffle_frame_torch's codes (xpg/io/bslz4.py:100-108): all ones (2^n - 1) -> -1, below min_valid -> -1,
-    // static mask -> -1, at or above the overload -> -2; -1 wins
-    const uint32_t ones = b.esize == 4 ? 0xFFFFFFFFu : 0xFFFFu;
-    bool bad = v == ones;
-    if (b.use_min) bad |= (int64_t)v < b.min_ceil;
-    if (b.sbad) bad |= b.sbad[e] != 0;
-    float f = (float)v;
-    if (b.use_ovl && (int64_t)v >= b.ovl_ceil) f = -2.0f;
-    static_cast<float *>(out)[e] = bad ? -1.0f : f;
+    static_cast<float *>(out)[e] = __builtin_bit_cast(float, v);
   }
+}
+
+// unshuffle_frame_torch's codes (xpg/io/bslz4.py:100-108): all ones (2^n - 1) -> -1, below min_valid -> -1, static
+// mask -> -1, at or above the overload -> -2; -1 wins.  Returns the float32's bits.
+__device__ __forceinline__ uint32_t coded(const Batch &b, int64_t e, uint32_t v) {
+  const uint32_t ones = b.esize == 4 ? 0xFFFFFFFFu : 0xFFFFu;
+  bool bad = v == ones;
+  if (b.use_min) bad |= (int64_t)v < b.min_ceil;
+  if (b.sbad) bad |= b.sbad[e] != 0;
+  float f = (float)v;
+  if (b.use_ovl && (int64_t)v >= b.ovl_ceil) f = -2.0f;
+  return __builtin_bit_cast(uint32_t, bad ? -1.0f : f);
+}
+
+// 4 x 4 byte transpose: t[c] = byte c of w0, w1, w2, w3 (v_perm_b32: selector bytes 0-3 pick from the second
+// operand, 4-7 from the first)
+__device__ __forceinline__ void bt4(uint32_t w0, uint32_t w1, uint32_t w2, uint32_t w3, uint32_t t[4]) {
+  const uint32_t x01 = __builtin_amdgcn_perm(w1, w0, 0x05010400u), y01 = __builtin_amdgcn_perm(w1, w0, 0x07030602u);
+  const uint32_t x23 = __builtin_amdgcn_perm(w3, w2, 0x05010400u), y23 = __builtin_amdgcn_perm(w3, w2, 0x07030602u);
+  t[0] = __builtin_amdgcn_perm(x23, x01, 0x05040100u);
+  t[1] = __builtin_amdgcn_perm(x23, x01, 0x07060302u);
+  t[2] = __builtin_amdgcn_perm(y23, y01, 0x05040100u);
+  t[3] = __builtin_amdgcn_perm(y23, y01, 0x07060302u);
+}
+
+// Elements 32 q .. 32 q + 31 of a block whose planes hold m bytes, m a multiple of 4: one dword per plane gives the
+// plane's bytes of groups 4 q .. 4 q + 3.
+template <int ES>
+__device__ __forceinline__ void unshuffle32(const uint8_t *db, int m, int q, uint32_t v[32]) {
+  uint32_t X[ES][4][2];   // [output byte j][group c][half]: byte e of the 64-bit word = byte j of element 8 c + e
+#pragma unroll
+  for (int j = 0; j < ES; ++j) {
+    uint32_t w[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) w[r] = *reinterpret_cast<const uint32_t *>(db + (8 * j + r) * m + 4 * q);
+    uint32_t lo[4], hi[4];
+    bt4(w[0], w[1], w[2], w[3], lo);
+    bt4(w[4], w[5], w[6], w[7], hi);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const uint64_t x = transpose8((uint64_t)hi[c] << 32 | lo[c]);
+      X[j][c][0] = (uint32_t)x;
+      X[j][c][1] = (uint32_t)(x >> 32);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      uint32_t t[4];
+      if constexpr (ES == 4) {
+        bt4(X[0][c][h], X[1][c][h], X[2][c][h], X[3][c][h], t);
+      } else {
+        t[0] = __builtin_amdgcn_perm(X[1][c][h], X[0][c][h], 0x0C0C0400u);
+        t[1] = __builtin_amdgcn_perm(X[1][c][h], X[0][c][h], 0x0C0C0501u);
+        t[2] = __builtin_amdgcn_perm(X[1][c][h], X[0][c][h], 0x0C0C0602u);
+        t[3] = __builtin_amdgcn_perm(X[1][c][h], X[0][c][h], 0x0C0C0703u);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[8 * c + 4 * h + e] = t[e];
+    }
 }
 
 template <int ES, int KIND>
@@ -258,18 +400,16 @@ __global__ void __launch_bounds__(64) k_bslz4_decode(Batch b) {
   const int64_t f = gw / b.nbf;
   const int k = (int)(gw - f * b.nbf);
   if (f >= b.nf || b.status[f] != 0) return;
-  uint8_t *cbase = lds;
-  uint8_t *db = lds + b.cap_c;
+  uint8_t *db = lds;
   const uint32_t *t = b.table + f * (int64_t)(b.nbf + 1) * 2;
   const uint8_t *chunk = b.src + b.coff[f];
   const int64_t e0 = (int64_t)k * b.belem;
   const int n = (int)std::min<int64_t>(b.belem, b.nel - e0);
   const int n8 = n - n % 8;
   if (n8 > 0) {
-    const int cs = (int)t[2 * k + 1];
-    const uint8_t *cb = stage(chunk + t[2 * k], cs, cbase, lane);
-    wave_sync();
-    const int err = lz4_block(cb, cs, db, n8 * ES, lane);
+    Window w;
+    w.init(chunk + t[2 * k], chunk, chunk + b.clen[f], lane);
+    const int err = lz4_block(w, (int)t[2 * k + 1], db, n8 * ES, lane);
     if (err) {
       if (lane == 0) atomicOr(&b.status[f], err);
       return;
@@ -281,10 +421,29 @@ __global__ void __launch_bounds__(64) k_bslz4_decode(Batch b) {
   }
   wave_sync();
   const int64_t slot = b.oslot ? b.oslot[f] : f;
-  void *out = static_cast<uint8_t *>(b.out) + slot * b.ostride * 4;
+  uint32_t *out = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(b.out) + slot * b.ostride * 4) + e0;
+  const bool vec = ((uintptr_t)out & 15) == 0;
   // bit-shuffled part: plane p (bit p of every element) holds m = n8 / 8 bytes, element i's bit at byte i / 8, bit
   // i % 8; the 8 planes 8 j .. 8 j + 7 of byte g form the 8 x 8 matrix of output byte j of elements 8 g .. 8 g + 7
   const int m = n8 >> 3;
+  if ((m & 3) == 0) {
+    for (int q = lane; 4 * q < m; q += WAVE) {
+      uint32_t v[32];
+      unshuffle32<ES>(db, m, q, v);
+      if constexpr (KIND == 1) {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) v[i] = coded(b, e0 + 32 * q + i, v[i]);
+      }
+      if (vec) {
+        uint4 *o = reinterpret_cast<uint4 *>(out + 32 * q);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) o[i] = make_uint4(v[4 * i], v[4 * i + 1], v[4 * i + 2], v[4 * i + 3]);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 32; ++i) out[32 * q + i] = v[i];
+      }
+    }
+  } else
   for (int g = lane; g < m; g += WAVE) {
     uint32_t v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -297,15 +456,25 @@ __global__ void __launch_bounds__(64) k_bslz4_decode(Batch b) {
 #pragma unroll
       for (int c = 0; c < 8; ++c) v[c] |= (uint32_t)((x >> (8 * c)) & 0xFF) << (8 * j);
     }
+    if constexpr (KIND == 1) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) put<KIND>(b, out, e0 + 8 * g + c, v[c]);
+      for (int c = 0; c < 8; ++c) v[c] = coded(b, e0 + 8 * g + c, v[c]);
+    }
+    if (vec) {
+      uint4 *o = reinterpret_cast<uint4 *>(out + 8 * g);
+      o[0] = make_uint4(v[0], v[1], v[2], v[3]);
+      o[1] = make_uint4(v[4], v[5], v[6], v[7]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) out[8 * g + c] = v[c];
+    }
   }
   // raw tail: little-endian elements
   for (int i = n8 + lane; i < n; i += WAVE) {
     const uint8_t *p = db + i * ES;
     uint32_t v = (uint32_t)p[0] | (uint32_t)p[1] << 8;
     if (ES == 4) v |= (uint32_t)p[2] << 16 | (uint32_t)p[3] << 24;
-    put<KIND>(b, out, e0 + i, v);
+    out[i] = KIND == 1 ? coded(b, e0 + i, v) : v;
   }
 }
 
@@ -343,8 +512,7 @@ Batch make_batch(const xpg_bslz4_batch *p) {
   b.ovl_ceil = p->overload_ceil;
   b.sbad = p->static_bad;
   const int64_t bb = (int64_t)p->block_elems * p->esize;
-  b.cap_c = (int32_t)((lz4_bound(bb) + 16 + 15) / 16 * 16);
-  b.wave_lds = b.cap_c + (int32_t)((bb + 15) / 16 * 16);
+  b.wave_lds = (int32_t)((bb + DUMMY_PAD + DUMMY_BYTES + 15) / 16 * 16);
   b.f0 = 0;
   return b;
 }
