@@ -11,6 +11,8 @@
  *   bslz4_decode_chunk lz4_stage's chunk walk (xpg/io/bslz4.py:34-63) followed by the bit unshuffle of
  *                      _bitunshuffle / unshuffle_frame_torch (xpg/io/bslz4.py:28-31, :80-99), written bit by bit.
  *   bslz4_codes        the float32 value codes of unshuffle_frame_torch (xpg/io/bslz4.py:100-108).
+ *   cbf_decode         decode_byte_offset (xpg/io/cbf.py:12-44): byte-offset differences (int8, or after the escape
+ *                      0x80 an int16, int32, int64) summed in int64 and stored as int32 (read_cbf, :97).
  *
  * Parity pins: hand-built LZ4 blocks (literal-only, offset-1 runs, overlapping periodic matches, extended lengths,
  * each error rule), liblz4's compressor and decoder on random and bit-shuffled data (including LZ4_decompress_safe's
@@ -19,6 +21,7 @@
  */
 #include <stdint.h>
 #include <string.h>
+#include <limits.h>
 
 /* Decode one LZ4 block of cs bytes into exactly dn bytes.  Returns dn, -1 for a malformed block, -2 for a match of
  * offset 0 (the format calls it invalid; LZ4_decompress_safe 1.9.3 does not test for it). */
@@ -139,4 +142,43 @@ void bslz4_codes(const uint8_t *bytes, int64_t nel, int esize, int use_min, int6
     if (use_ovl && (int64_t)v >= ovl_ceil) f = -2.0f;
     out[i] = bad ? -1.0f : f;
   }
+}
+
+/* CBF byte offset: n values from a stream of len bytes.  Returns 0, or 1 when the stream ends before n elements or
+ * inside the n-th (the reference raises IndexError there, or for a cut int32 / int64 escape reads the bytes that are
+ * there). */
+int cbf_decode(const uint8_t *b, int64_t len, int64_t n, int32_t *out) {
+  int64_t pos = 0, sum = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (pos >= len) return 1;
+    int64_t d;
+    if (b[pos] != 0x80) {
+      d = (int8_t)b[pos];
+      pos += 1;
+    } else {
+      if (pos + 3 > len) return 1;
+      const int v16 = b[pos + 1] | b[pos + 2] << 8;
+      if (v16 != 0x8000) {
+        d = (int16_t)v16;
+        pos += 3;
+      } else {
+        if (pos + 7 > len) return 1;
+        const int32_t v32 = (int32_t)((uint32_t)b[pos + 3] | (uint32_t)b[pos + 4] << 8 | (uint32_t)b[pos + 5] << 16 |
+                                      (uint32_t)b[pos + 6] << 24);
+        if (v32 != INT32_MIN) {
+          d = v32;
+          pos += 7;
+        } else {
+          if (pos + 15 > len) return 1;
+          uint64_t v = 0;
+          for (int k = 0; k < 8; ++k) v |= (uint64_t)b[pos + 7 + k] << (8 * k);
+          d = (int64_t)v;
+          pos += 15;
+        }
+      }
+    }
+    sum += d;
+    out[i] = (int32_t)(uint32_t)(uint64_t)sum;
+  }
+  return 0;
 }

@@ -1,5 +1,5 @@
 """Oracle (test infrastructure only): ctypes wrapper of oracle/codec.c, the CPU restatement of the bitshuffle-LZ4
-frame decode (xpg/io/bslz4.py:28-108).  Built by __graft_entry__.build() into oracle/libxpg_oracle_codec.so; only the
+frame decode (xpg/io/bslz4.py:28-108) and of the CBF byte-offset decode (xpg/io/cbf.py:12-44).  Built by __graft_entry__.build() into oracle/libxpg_oracle_codec.so; only the
 tests, the smoke check and bench.py's CPU-baseline leg use it, as the checker."""
 from __future__ import annotations
 
@@ -28,6 +28,8 @@ def lib():
         h.bslz4_codes.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_int, C.c_int64, C.c_int, C.c_int64, C.c_void_p,
                                   C.c_void_p]
         h.bslz4_codes.restype = None
+        h.cbf_decode.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_void_p]
+        h.cbf_decode.restype = C.c_int
         _h = h
     return _h
 
@@ -70,3 +72,11 @@ def codes(raw: np.ndarray, esize: int, overload=None, min_valid=0.0, static_bad=
                       int(overload is not None), math.ceil(overload) if overload is not None else 0,
                       None if sb is None else _p(sb), _p(out))
     return out
+
+
+def cbf_decode(buf: bytes, n: int):
+    """(status, int32 values): decode_byte_offset (xpg/io/cbf.py:12-44) restated in C; status 1 = stream too short."""
+    src = np.frombuffer(bytes(buf), np.uint8).copy() if len(buf) else np.zeros(1, np.uint8)
+    out = np.zeros(max(n, 1), np.int32)
+    r = lib().cbf_decode(_p(src), len(buf), n, _p(out))
+    return int(r), out[:n]

@@ -7,7 +7,7 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 STRUCTS = dict(Detector="xpg_detector", Call="xpg_call", Refl="xpg_refl", Pass1Out="xpg_pass1_out",
                Pass2Out="xpg_pass2_out", ProfileAcc="xpg_profile_acc", Frames="xpg_frames", Widths="xpg_widths",
-               Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch")
+               Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch", CbfBatch="xpg_cbf_batch")
 
 
 def _declared():
@@ -66,7 +66,7 @@ def test_library_exports_declared_symbols():
 def test_ctypes_structs_match_c_layout(tmp_path):
     import ctypes
     from xpg_amd import _lib
-    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', '#include "xpg_decode.h"',
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', '#include "xpg_decode.h"', '#include "xpg_cbf.h"',
              "int main(void) {"]
     for py, c in STRUCTS.items():
         lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')

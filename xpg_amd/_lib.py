@@ -1,4 +1,5 @@
-"""ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_synth.h).
+"""ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_cbf.h,
+include/xpg_synth.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
 is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
@@ -91,6 +92,14 @@ class Bslz4Batch(C.Structure):
                 ("static_bad", C.c_void_p)]
 
 
+
+class CbfBatch(C.Structure):
+    _fields_ = [("src", C.c_void_p), ("stream_off", C.c_void_p), ("stream_len", C.c_void_p), ("n_elem", C.c_void_p),
+                ("chunk0", C.c_void_p), ("group0", C.c_void_p), ("n_frames", C.c_int32), ("n_chunks", C.c_int64),
+                ("n_groups", C.c_int64), ("work", C.c_void_p), ("out", C.c_void_p), ("out_off", C.c_void_p),
+                ("status", C.c_void_p)]
+
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -123,6 +132,9 @@ _SIGS = {
     "xpg_bslz4_index_host": ([C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int32),
     "xpg_bslz4_index": ([P(Bslz4Batch), C.c_void_p], C.c_int),
     "xpg_bslz4_decode": ([P(Bslz4Batch), C.c_void_p], C.c_int),
+    "xpg_cbf_workspace_bytes": ([C.c_int64, C.c_int64], C.c_int64),
+    "xpg_cbf_decode": ([P(CbfBatch), C.c_void_p], C.c_int),
+    "xpg_cbf_encode_host": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64], C.c_int64),
     "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
                               C.c_int64),
 }
