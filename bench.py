@@ -386,12 +386,96 @@ def bench_decode(args, dev):
     print(json.dumps(outj), flush=True)
 
 
+def bench_cbf(args, dev):
+    """CBF byte-offset decode of PILATUS 6M frames (2463 x 2527 int32: Poisson background, module gaps at -1, bad
+    pixels at -2, hot pixels up to the count cut-off), encoded as read_cbf reads them (encode_byte_offset); the streams
+    resident in HBM; a step decodes every frame into device int32 values (5 kernel launches).  Roofline: stream bytes
+    read + 4 bytes per pixel written, over the step's HIP-event time on the launch stream."""
+    import numpy as np
+    import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from xpg_amd import cbf
+
+    torch.cuda.set_device(dev)
+    nfr = args.frames or 200
+    ny, nx = 2527, 2463
+    g = torch.Generator(device=dev).manual_seed(1005)
+    t = time.time()
+    host = np.empty((nfr, ny, nx), np.int32)
+    for f in range(nfr):
+        lam = torch.full((ny, nx), 0.5 + 5.5 * (f % 17) / 16, device=dev)
+        a = torch.poisson(lam, generator=g).to(torch.int32)
+        a[:, 487::494] = -1
+        a[195::212, :] = -1
+        idx = torch.randint(0, ny * nx, (3000,), device=dev, generator=g)
+        a.view(-1)[idx] = torch.randint(1000, 1_048_500, (3000,), device=dev, generator=g, dtype=torch.int32)
+        host[f] = a.cpu().numpy()
+    with ThreadPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        streams = list(ex.map(cbf.encode_byte_offset, list(host)))
+    log(f"generated and encoded {nfr} frames {ny}x{nx} in {time.time() - t:.1f}s")
+    sb = cbf.StreamBatch(streams, [ny * nx] * nfr, dev)
+    ref = torch.from_numpy(host.reshape(-1)).to(dev)
+    out = torch.zeros(nfr * ny * nx, dtype=torch.int32, device=dev)
+    st = torch.cuda.current_stream(dev)
+    for _ in range(args.warmup):
+        sb.decode_into(out)
+    sb.check()
+    assert torch.equal(out, ref), "decoded frames differ from the originals"
+    out.zero_()
+    torch.cuda.synchronize(dev)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.time()
+    for a, b in ev:
+        a.record(st)
+        sb.decode_into(out)
+        b.record(st)
+    torch.cuda.synchronize(dev)
+    wall = (time.time() - t0) / args.steps
+    ms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    sb.check()
+    assert torch.equal(out, ref), "decoded frames differ from the originals"
+    raw = nfr * ny * nx * 4
+    alg = sb.stream_bytes + raw
+    achieved = alg / (ms * 1e-3) / 1e9
+    traffic = None
+    tf = ROOT / "profiles" / "traffic_cbf_decode_cbf.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    roof = dict(bound="hbm", kernel="xpg_cbf_decode (k_cbf_maps, k_cbf_groups, k_cbf_frames, k_cbf_chunks, k_cbf_write)",
+                achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s", frac=round(achieved / PEAK_HBM_GBS, 4),
+                traffic=traffic, avg_launch_ms=round(ms, 4), alg_bytes_per_launch=alg, launches_per_step=5,
+                stream_bytes=sb.stream_bytes, decoded_bytes=raw)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import codec as oc
+        t = time.time()
+        n = 0
+        while n < nfr and (time.time() - t < args.cpu_seconds or n < 2):
+            r, _ = oc.cbf_decode(streams[n], ny * nx)
+            assert r == 0
+            n += 1
+        dt = time.time() - t
+        cpu = dict(value=round(n / dt, 3), unit="frames/s", cores=1, kind="port",
+                   sample=f"{n} frames of {ny}x{nx} (the first of the same streams), oracle/codec.c cbf_decode, "
+                          f"{dt:.1f} s")
+        log(f"cpu baseline {cpu}")
+    outj = dict(metric="CBF byte-offset frames decoded/s on 1 MI355X", value=round(nfr / (ms * 1e-3), 1),
+                unit="frames/s", n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 3),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="int64", data="synthetic",
+                config=dict(workload=f"cbf_decode_{nx}x{ny}x{nfr}", frames=nfr, frame_shape=[ny, nx],
+                            bytes_per_pixel=round(sb.stream_bytes / (nfr * ny * nx), 3),
+                            decoded_GBps=round(raw / (ms * 1e-3) / 1e9, 1), wall_ms_per_step=round(wall * 1e3, 3),
+                            identical_to_source=True, parallelism="single GPU"),
+                roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(outj), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode", "cbf"])
     ap.add_argument("--serve-datasets", type=int, default=32)
     ap.add_argument("--serve-frame-scale", type=float, default=0.1,
                     help="serving workload: frame counts U{50..3600} times this factor (at least 20 frames)")
@@ -452,6 +536,9 @@ def main():
         del src, dst
     if args.workload == "decode":
         bench_decode(args, dev)
+        return
+    if args.workload == "cbf":
+        bench_cbf(args, dev)
         return
     spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
                 stress=synth.spec_stress)[args.workload]()

@@ -4,16 +4,19 @@
 // all elements before it (1, 3, 7 or 15 bytes) and its value on the sum of all differences before it.  Both are
 // recovered in parallel from 256-byte chunks:
 //
-//   k_cbf_maps    one lane per (chunk, entry offset e = 0 .. 14): parse the elements that start in the chunk when the
-//                 first one starts e bytes into it; record the exit offset into the next chunk, the element count and
-//                 the sum of the differences (the chunk's map).
+//   k_cbf_maps    one wavefront per chunk, staged in LDS: for each entry offset e = 0 .. 14 (the first element
+//                 starting e bytes into the chunk) the exit offset into the next chunk, the element count and the sum
+//                 of the differences (the chunk's map).  Only the 0x80 escape bytes carry a dependency: a scalar walk
+//                 over them finds the escaped elements; counts and sums follow from the bytes' int8 prefix sums.
 //   k_cbf_groups  one lane per (group of 64 chunks, entry offset): compose the group's chunk maps.
 //   k_cbf_frames  one lane per frame: walk the frame's groups from entry 0 -- each group's true entry, element base
 //                 and value base; the frame's element total.
 //   k_cbf_chunks  one lane per group: walk its chunks from the group's entry -- each chunk's entry and bases.
-//   k_cbf_write   one lane per chunk: parse from the true entry and write the running sums (int64, stored as int32).
+//   k_cbf_write   one wavefront per chunk: from the true entry, the element starts (bytes not inside an escaped
+//                 element) and their running sums (int64, stored as int32) by wave scans in position order; stores
+//                 contiguous across lanes.
 //
-// HBM traffic: the stream read about twice (the maps' 15 parses hit the cache), the values written once.
+// HBM traffic: the stream read twice (maps and write), the values written once.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string.h>
@@ -34,6 +37,7 @@ constexpr int G = XPG_CBF_GROUP;
 constexpr int NE = 15;          // entry offsets 0 .. 14 (an element is at most 15 bytes)
 constexpr int NES = 16;         // map stride
 constexpr uint8_t STOP = 255;   // exit of a map whose last element runs past the stream's end
+constexpr int WAVE64 = 64;
 
 struct Map {                    // per (chunk or group, entry): structure of arrays in the workspace
   uint8_t *exit;
@@ -109,6 +113,22 @@ __device__ __forceinline__ int element(const uint8_t *s, int64_t L, int64_t p, i
   return 15;
 }
 
+// One dword of the bytes [c0, c0 + STAGE) of a frame's stream (zero past its end L) into LDS: an aligned dword load
+// (streams placed on 4-byte boundaries, c0 a multiple of 256), else byte loads.
+constexpr int STAGE = C + 16;   // a chunk plus the longest element that starts in it and ends past it
+
+__device__ __forceinline__ void stage_word(const uint8_t *s, int64_t L, int64_t c0, int w, uint8_t *dst) {
+  const int64_t q = c0 + 4 * w;
+  uint32_t v = 0;
+  if (q + 4 <= L && (reinterpret_cast<uintptr_t>(s + q) & 3) == 0) {
+    v = *reinterpret_cast<const uint32_t *>(s + q);
+  } else {
+    for (int k = 0; k < 4; ++k)
+      if (q + k < L) v |= (uint32_t)s[q + k] << (8 * k);
+  }
+  *reinterpret_cast<uint32_t *>(dst + 4 * w) = v;
+}
+
 // frame of a global chunk (or group) index: the last f with first[f] <= i (binary search over n_frames + 1 bounds)
 __device__ __forceinline__ int frame_of(const int64_t *first, int nf, int64_t i) {
   int lo = 0, hi = nf - 1;
@@ -119,34 +139,175 @@ __device__ __forceinline__ int frame_of(const int64_t *first, int nf, int64_t i)
   return lo;
 }
 
-__global__ void __launch_bounds__(256) k_cbf_maps(Batch b) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t c = t / NES;
-  const int e = (int)(t % NES);
-  if (c >= b.nc || e >= NE) return;
-  const int f = frame_of(b.chunk0, b.nf, c);
-  const uint8_t *s = b.src + b.soff[f];
-  const int64_t L = b.slen[f];
-  const int64_t c0 = (c - b.chunk0[f]) * C;
-  int64_t p = c0 + e;
-  int32_t n = 0;
-  int64_t sum = 0;
-  uint8_t ex = 0;
-  while (p < c0 + C && p < L) {
-    int64_t d;
-    const int len = element(s, L, p, d);
-    if (!len) {
-      ex = STOP;
-      break;
-    }
-    sum += d;
-    ++n;
-    p += len;
+// ---- byte-parallel chunk parse: one wavefront per chunk ------------------------------------------------------------
+// Inside a chunk every byte that is not 0x80 and not inside an escaped element is a one-byte element, so only the
+// escape bytes carry the sequential dependency: a scalar walk over the chunk's 0x80 bytes in order (skipping those
+// inside the previous escaped element) finds the escaped elements that start there; everything else follows from
+// per-byte prefix sums.
+constexpr int WPB = 4;   // chunks (wavefronts) per workgroup
+
+struct ChunkView {
+  const uint8_t *bytes;   // LDS: STAGE bytes from the chunk's start, zero past the stream's end
+  int E;                  // bytes of the chunk inside the stream: min(C, L - c0)
+  int Lr;                 // stream end relative to the chunk (may exceed C)
+  uint64_t esc[4];        // 0x80 bytes among positions 64 k + lane < E
+  int32_t *P;             // LDS (C + 1): P[q] = sum of int8(b[i]) for i < q, i < E
+};
+
+__device__ __forceinline__ int64_t shfl_up64(int64_t v, int d, int lane) {
+  const int lo = __shfl_up((int)(uint32_t)v, d), hi = __shfl_up((int)((uint64_t)v >> 32), d);
+  const int64_t u = (int64_t)((uint64_t)(uint32_t)hi << 32 | (uint32_t)lo);
+  return lane >= d ? u : 0;
+}
+
+__device__ __forceinline__ int64_t wscan64(int64_t v, int lane) {   // inclusive scan over the wave
+#pragma unroll
+  for (int d = 1; d < WAVE64; d <<= 1) v += shfl_up64(v, d, lane);
+  return v;
+}
+
+__device__ __forceinline__ int wscan32(int v, int lane) {
+#pragma unroll
+  for (int d = 1; d < WAVE64; d <<= 1) {
+    const int u = __shfl_up(v, d);
+    if (lane >= d) v += u;
   }
-  if (ex != STOP) ex = (uint8_t)(p >= c0 + C ? p - (c0 + C) : 0);
-  b.w.cm.exit[c * NES + e] = ex;
-  b.w.cm.cnt[c * NES + e] = n;
-  b.w.cm.sum[c * NES + e] = sum;
+  return v;
+}
+
+// the escaped element at a (bytes[a] == 0x80, a < E): its length (3, 7, 15; 0 when it runs past the stream's end)
+// and difference (decode_byte_offset, xpg/io/cbf.py:30-40), read by lanes 0 .. 14 and gathered uniformly
+__device__ __forceinline__ int escaped(const ChunkView &v, int a, int64_t &d, int lane) {
+  const int x = lane < 15 ? v.bytes[a + lane] : 0;
+  const int b1 = __builtin_amdgcn_readlane(x, 1), b2 = __builtin_amdgcn_readlane(x, 2);
+  const int v16 = b1 | b2 << 8;
+  if (a + 3 > v.Lr) return 0;
+  if (v16 != 0x8000) {
+    d = (int16_t)v16;
+    return 3;
+  }
+  if (a + 7 > v.Lr) return 0;
+  const uint32_t w32 = (uint32_t)__builtin_amdgcn_readlane(x, 3) | (uint32_t)__builtin_amdgcn_readlane(x, 4) << 8 |
+                       (uint32_t)__builtin_amdgcn_readlane(x, 5) << 16 | (uint32_t)__builtin_amdgcn_readlane(x, 6) << 24;
+  if ((int32_t)w32 != INT32_MIN) {
+    d = (int32_t)w32;
+    return 7;
+  }
+  if (a + 15 > v.Lr) return 0;
+  uint64_t w = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) w |= (uint64_t)(uint32_t)__builtin_amdgcn_readlane(x, 7 + k) << (8 * k);
+  d = (int64_t)w;
+  return 15;
+}
+
+// Stage the chunk (by the wave), find its 0x80 bytes and the int8 prefix sums
+__device__ __forceinline__ void view_chunk(ChunkView &v, uint8_t *buf, int32_t *P, const uint8_t *s, int64_t L,
+                                           int64_t c0, int lane) {
+  for (int w = lane; w < STAGE / 4; w += WAVE64) stage_word(s, L, c0, w, buf);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  v.bytes = buf;
+  v.P = P;
+  v.Lr = (int)std::min<int64_t>(L - c0, 1 << 20);
+  v.E = std::min(C, v.Lr);
+  int carry = 0;
+  if (lane == 0) P[0] = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = 64 * k + lane;
+    const int byte = q < v.E ? buf[q] : 0;
+    v.esc[k] = __ballot(q < v.E && byte == 0x80);
+    const int inc = wscan32((int)(int8_t)byte, lane) + carry;
+    P[q + 1] = inc;
+    carry = __builtin_amdgcn_readlane(inc, 63);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Walk from entry e: the escaped elements that start in [e, E), in order; `fn(a, len, d)` for each; stops at one that
+// runs past the stream's end (returns false then).  `pos` ends at the position after the last element.
+template <typename F>
+__device__ __forceinline__ bool walk(const ChunkView &v, int e, int &pos, int lane, F fn) {
+  pos = e;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint64_t m = v.esc[k];
+    while (m) {
+      const int a = 64 * k + __builtin_ctzll(m);
+      m &= m - 1;
+      if (a < pos) continue;               // inside the previous escaped element (or before the entry)
+      int64_t d = 0;
+      const int len = escaped(v, a, d, lane);
+      if (!len) {
+        fn(a, 0, 0);
+        return false;
+      }
+      fn(a, len, d);
+      pos = a + len;
+    }
+  }
+  if (pos < v.E) pos = v.E;                // one-byte elements up to the chunk's (or stream's) end
+  return true;
+}
+
+__global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
+  __shared__ __align__(16) uint8_t buf[WPB][STAGE];
+  __shared__ int32_t Ps[WPB][C + 1];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * WPB + wv;
+  if (c >= b.nc) return;
+  const int f = frame_of(b.chunk0, b.nf, c);
+  const int64_t c0 = (c - b.chunk0[f]) * C;
+  ChunkView v;
+  view_chunk(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
+  // lane e < 15 follows entry e through one pass over the 0x80 bytes: each escaped element's length and difference
+  // are the same for every entry; an entry takes the ones at or past its position.  Starts in [e, end) = bytes there
+  // minus the escaped elements' other bytes; sums from the prefix sums, each escaped element's bytes replaced by its
+  // difference.
+  const int e = lane;
+  int pos = e, end = v.E, payload = 0;
+  int64_t corr = 0;
+  bool cut = false;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    uint64_t m = v.esc[k];
+    while (m) {
+      const int a = 64 * k + __builtin_ctzll(m);
+      m &= m - 1;
+      int64_t d = 0;
+      const int len = escaped(v, a, d, lane);
+      const int hi = std::min(a + (len ? len : 1), v.E);
+      const int64_t pa = (int64_t)(v.P[hi] - v.P[a]);
+      if (e < NE && !cut && a >= pos) {
+        if (!len) {
+          end = a;
+          cut = true;
+        } else {
+          payload += hi - a - 1;
+          corr += d - pa;
+          pos = a + len;
+        }
+      }
+    }
+  }
+  if (e < NE) {
+    int n = 0;
+    int64_t sum = 0;
+    uint8_t ex = 0;
+    if (e < v.E) {
+      if (pos < v.E) pos = v.E;
+      n = end - e - payload;
+      sum = (int64_t)(v.P[end] - v.P[e]) + corr;
+      ex = cut ? STOP : (uint8_t)(pos > C ? pos - C : 0);
+    }
+    b.w.cm.exit[c * NES + e] = ex;
+    b.w.cm.cnt[c * NES + e] = n;
+    b.w.cm.sum[c * NES + e] = sum;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_cbf_groups(Batch b) {
@@ -207,27 +368,56 @@ __global__ void __launch_bounds__(64) k_cbf_chunks(Batch b) {
   }
 }
 
-__global__ void __launch_bounds__(256) k_cbf_write(Batch b) {
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
+  // one wavefront per chunk: the starts of elements are the bytes from the entry on that are not inside an escaped
+  // element; their indices and running sums are wave scans in position order, so the stores are contiguous
+  __shared__ __align__(16) uint8_t buf[WPB][STAGE];
+  __shared__ int32_t Ps[WPB][C + 1];
+  __shared__ int64_t dq[WPB][C];
+  __shared__ uint8_t pay[WPB][C];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = (int64_t)blockIdx.x * WPB + wv;
   if (c >= b.nc) return;
   const int f = frame_of(b.chunk0, b.nf, c);
-  if (b.status[f]) return;
   const int e = b.w.c_entry[c];
   const int64_t nmax = b.nel[f];
-  int64_t i = b.w.c_base[c];
-  if (e == STOP || i >= nmax) return;
-  const uint8_t *s = b.src + b.soff[f];
-  const int64_t L = b.slen[f];
+  const int64_t i0 = b.w.c_base[c];
+  if (b.status[f] || e == STOP || i0 >= nmax) return;
   const int64_t c0 = (c - b.chunk0[f]) * C;
+  ChunkView v;
+  view_chunk(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) pay[wv][64 * k + lane] = 0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int end = v.E, pos = 0;
+  walk(v, e, pos, lane, [&](int a, int len, int64_t d) {
+    if (!len) {
+      end = a;
+      return;
+    }
+    if (lane < len - 1 && a + 1 + lane < C) pay[wv][a + 1 + lane] = 1;
+    if (lane == 0) dq[wv][a] = d;
+  });
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int32_t *o = b.out + b.ooff[f];
-  int64_t sum = b.w.c_vbase[c];
-  for (int64_t p = c0 + e; p < c0 + C && p < L && i < nmax;) {
-    int64_t d;
-    const int len = element(s, L, p, d);
-    if (!len) break;              // cannot happen for i < n_elem once the frame's status is 0
-    sum += d;
-    o[i++] = (int32_t)(uint32_t)(uint64_t)sum;
-    p += len;
+  int64_t cn = 0, csum = b.w.c_vbase[c];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int q = 64 * k + lane;
+    const bool start = q >= e && q < end && !pay[wv][q];
+    const int byte = v.bytes[q];
+    const int64_t d = start ? (byte == 0x80 ? dq[wv][q] : (int64_t)(int8_t)byte) : 0;
+    const int rn = wscan32(start ? 1 : 0, lane);
+    const int64_t rs = wscan64(d, lane);
+    const int64_t idx = i0 + cn + rn - 1;
+    if (start && idx < nmax) o[idx] = (int32_t)(uint32_t)(uint64_t)(csum + rs);
+    cn += __builtin_amdgcn_readlane(rn, 63);
+    csum += (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rs >> 32), 63) << 32 |
+                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rs, 63));
   }
 }
 
@@ -262,11 +452,11 @@ int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
   b.status = p->status;
   hipStream_t s = (hipStream_t)stream;
   const int64_t tm = b.nc * NES, tg = b.ng * NES;
-  if (tm > 0) hipLaunchKernelGGL(k_cbf_maps, dim3((uint32_t)((tm + 255) / 256)), dim3(256), 0, s, b);
+  if (tm > 0) hipLaunchKernelGGL(k_cbf_maps, dim3((uint32_t)((b.nc + WPB - 1) / WPB)), dim3(64 * WPB), 0, s, b);
   if (tg > 0) hipLaunchKernelGGL(k_cbf_groups, dim3((uint32_t)((tg + 255) / 256)), dim3(256), 0, s, b);
   hipLaunchKernelGGL(k_cbf_frames, dim3((uint32_t)((b.nf + 63) / 64)), dim3(64), 0, s, b);
   if (b.ng > 0) hipLaunchKernelGGL(k_cbf_chunks, dim3((uint32_t)((b.ng + 63) / 64)), dim3(64), 0, s, b);
-  if (b.nc > 0) hipLaunchKernelGGL(k_cbf_write, dim3((uint32_t)((b.nc + 255) / 256)), dim3(256), 0, s, b);
+  if (b.nc > 0) hipLaunchKernelGGL(k_cbf_write, dim3((uint32_t)((b.nc + WPB - 1) / WPB)), dim3(64 * WPB), 0, s, b);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return xpg::report_error(XPG_ERR_HIP, hipGetErrorString(e));
   return 0;
