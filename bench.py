@@ -437,10 +437,10 @@ def bench_cbf(args, dev):
     raw = nfr * ny * nx * 4
     alg = sb.stream_bytes + raw
     achieved = alg / (ms * 1e-3) / 1e9
-    traffic = None
-    tf = ROOT / "profiles" / "traffic_cbf_decode_cbf.json"
-    if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+    # measured HBM bytes of the decode's two large kernels (maps and write; the three walks move a few MB)
+    tfs = [ROOT / "profiles" / f"traffic_cbf_{k}_cbf.json" for k in ("maps", "write")]
+    traffic = (sum(json.loads(t.read_text())["bytes_per_launch"] for t in tfs) if all(t.exists() for t in tfs)
+               else None)
     roof = dict(bound="hbm", kernel="xpg_cbf_decode (k_cbf_maps, k_cbf_groups, k_cbf_frames, k_cbf_chunks, k_cbf_write)",
                 achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit="GB/s", frac=round(achieved / PEAK_HBM_GBS, 4),
                 traffic=traffic, avg_launch_ms=round(ms, 4), alg_bytes_per_launch=alg, launches_per_step=5,

@@ -1,7 +1,7 @@
-/* Oracle (test infrastructure only): CPU restatement of the bitshuffle-LZ4 frame decode.
+/* Oracle (test infrastructure only): CPU restatement of the bitshuffle-LZ4 and CBF byte-offset frame decodes.
  *
  * Only the tests, the smoke check and bench.py's CPU-baseline leg load this library, as the checker; the product
- * decode is xpg_amd/csrc/xpg_decode.hip.
+ * decodes are xpg_amd/csrc/xpg_decode.hip and xpg_amd/csrc/xpg_cbf.hip.
  *
  *   lz4_block_decode   the LZ4 block format (lz4 1.9.3, doc/lz4_Block_format.md), with the end-of-block rules of
  *                      LZ4_decompress_safe, the routine python-lz4's lz4.block.decompress calls for the reference's
@@ -17,7 +17,9 @@
  * Parity pins: hand-built LZ4 blocks (literal-only, offset-1 runs, overlapping periodic matches, extended lengths,
  * each error rule), liblz4's compressor and decoder on random and bit-shuffled data (including LZ4_decompress_safe's
  * accept / reject decision on corrupted blocks), numpy's unpackbits / packbits for the bit transpose, and closed forms
- * for the codes.
+ * for the codes; cbf_decode by hand-built streams of each escape class with the values the format defines, int32 wrap
+ * of the int64 sum, cut and short streams, and the reference test's vector (tests/test_core.py:15-22 of the
+ * reference) through the product's encoder.
  */
 #include <stdint.h>
 #include <string.h>

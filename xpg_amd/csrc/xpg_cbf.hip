@@ -201,7 +201,8 @@ __device__ __forceinline__ int escaped(const ChunkView &v, int a, int64_t &d, in
   return 15;
 }
 
-// Stage the chunk (by the wave), find its 0x80 bytes and the int8 prefix sums
+// Stage the chunk (by the wave), find its 0x80 bytes and (PREFIX) the int8 prefix sums
+template <bool PREFIX>
 __device__ __forceinline__ void view_chunk(ChunkView &v, uint8_t *buf, int32_t *P, const uint8_t *s, int64_t L,
                                            int64_t c0, int lane) {
   for (int w = lane; w < STAGE / 4; w += WAVE64) stage_word(s, L, c0, w, buf);
@@ -213,15 +214,17 @@ __device__ __forceinline__ void view_chunk(ChunkView &v, uint8_t *buf, int32_t *
   v.Lr = (int)std::min<int64_t>(L - c0, 1 << 20);
   v.E = std::min(C, v.Lr);
   int carry = 0;
-  if (lane == 0) P[0] = 0;
+  if (PREFIX && lane == 0) P[0] = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int q = 64 * k + lane;
     const int byte = q < v.E ? buf[q] : 0;
     v.esc[k] = __ballot(q < v.E && byte == 0x80);
-    const int inc = wscan32((int)(int8_t)byte, lane) + carry;
-    P[q + 1] = inc;
-    carry = __builtin_amdgcn_readlane(inc, 63);
+    if constexpr (PREFIX) {
+      const int inc = wscan32((int)(int8_t)byte, lane) + carry;
+      P[q + 1] = inc;
+      carry = __builtin_amdgcn_readlane(inc, 63);
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
   const int f = frame_of(b.chunk0, b.nf, c);
   const int64_t c0 = (c - b.chunk0[f]) * C;
   ChunkView v;
-  view_chunk(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
+  view_chunk<true>(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
   // lane e < 15 follows entry e through one pass over the 0x80 bytes: each escaped element's length and difference
   // are the same for every entry; an entry takes the ones at or past its position.  Starts in [e, end) = bytes there
   // minus the escaped elements' other bytes; sums from the prefix sums, each escaped element's bytes replaced by its
@@ -372,7 +375,6 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
   // one wavefront per chunk: the starts of elements are the bytes from the entry on that are not inside an escaped
   // element; their indices and running sums are wave scans in position order, so the stores are contiguous
   __shared__ __align__(16) uint8_t buf[WPB][STAGE];
-  __shared__ int32_t Ps[WPB][C + 1];
   __shared__ int64_t dq[WPB][C];
   __shared__ uint8_t pay[WPB][C];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
   if (b.status[f] || e == STOP || i0 >= nmax) return;
   const int64_t c0 = (c - b.chunk0[f]) * C;
   ChunkView v;
-  view_chunk(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
+  view_chunk<false>(v, buf[wv], nullptr, b.src + b.soff[f], b.slen[f], c0, lane);
 #pragma unroll
   for (int k = 0; k < 4; ++k) pay[wv][64 * k + lane] = 0;
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
