@@ -35,3 +35,16 @@ def test_integrate_needs_a_gpu(monkeypatch):
     model = I.IntegrationModel(0.02, 0.1, box_xy=9, box_z=5)
     with pytest.raises(XpgError):
         I.integrate(img, [exp, exp], synth.block_edges(spec), model, 3.8, verbose=False)
+
+
+def test_frame_decoders_have_no_cpu_path():
+    """The bitshuffle-LZ4 and CBF decoders run on the device only: without a GPU (or with host tensors) they raise."""
+    from xpg_amd import bslz4, cbf
+    from xpg_amd._lib import XpgError
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    chunk = bslz4.encode_frame(np.arange(64, dtype=np.uint32))
+    with pytest.raises((XpgError, RuntimeError)):
+        bslz4.decode_frame(chunk, 64, 4, device="cpu")
+    with pytest.raises((XpgError, RuntimeError)):
+        cbf.decode_byte_offset(cbf.encode_byte_offset(np.arange(10, dtype=np.int32)), 10, device="cpu")
