@@ -42,15 +42,17 @@ constexpr int WAVE64 = 64;
 struct Map {                    // per (chunk or group, entry): structure of arrays in the workspace
   uint8_t *exit;
   int32_t *cnt;
-  int64_t *sum;
+  uint32_t *sum;               // mod 2^32: the values are the int32 cast of the running int64 sum
 };
 
 struct Work {
   Map cm, gm;                   // chunk maps, group maps
   uint8_t *c_entry;             // per chunk: true entry offset
-  int64_t *c_base, *c_vbase;    // per chunk: elements and value sum before it
+  int64_t *c_base;             // per chunk: elements before it
+  uint32_t *c_vbase;           // per chunk: value sum before it, mod 2^32
   uint8_t *g_entry;
-  int64_t *g_base, *g_vbase;
+  int64_t *g_base;
+  uint32_t *g_vbase;
 };
 
 __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
@@ -58,12 +60,12 @@ __host__ __device__ inline int64_t align256(int64_t x) { return (x + 255) / 256 
 __host__ __device__ inline Work carve(void *w, int64_t nc, int64_t ng) {
   uint8_t *p = static_cast<uint8_t *>(w);
   Work k;
-  k.cm.sum = reinterpret_cast<int64_t *>(p); p += align256(8 * NES * nc);
-  k.gm.sum = reinterpret_cast<int64_t *>(p); p += align256(8 * NES * ng);
+  k.cm.sum = reinterpret_cast<uint32_t *>(p); p += align256(4 * NES * nc);
+  k.gm.sum = reinterpret_cast<uint32_t *>(p); p += align256(4 * NES * ng);
   k.c_base = reinterpret_cast<int64_t *>(p); p += align256(8 * nc);
-  k.c_vbase = reinterpret_cast<int64_t *>(p); p += align256(8 * nc);
+  k.c_vbase = reinterpret_cast<uint32_t *>(p); p += align256(4 * nc);
   k.g_base = reinterpret_cast<int64_t *>(p); p += align256(8 * ng);
-  k.g_vbase = reinterpret_cast<int64_t *>(p); p += align256(8 * ng);
+  k.g_vbase = reinterpret_cast<uint32_t *>(p); p += align256(4 * ng);
   k.cm.cnt = reinterpret_cast<int32_t *>(p); p += align256(4 * NES * nc);
   k.gm.cnt = reinterpret_cast<int32_t *>(p); p += align256(4 * NES * ng);
   k.cm.exit = p; p += align256(NES * nc);
@@ -153,18 +155,6 @@ struct ChunkView {
   uint64_t esc[4];        // 0x80 bytes among positions 64 k + lane < E
   int32_t *P;             // LDS (C + 1): P[q] = sum of int8(b[i]) for i < q, i < E
 };
-
-__device__ __forceinline__ int64_t shfl_up64(int64_t v, int d, int lane) {
-  const int lo = __shfl_up((int)(uint32_t)v, d), hi = __shfl_up((int)((uint64_t)v >> 32), d);
-  const int64_t u = (int64_t)((uint64_t)(uint32_t)hi << 32 | (uint32_t)lo);
-  return lane >= d ? u : 0;
-}
-
-__device__ __forceinline__ int64_t wscan64(int64_t v, int lane) {   // inclusive scan over the wave
-#pragma unroll
-  for (int d = 1; d < WAVE64; d <<= 1) v += shfl_up64(v, d, lane);
-  return v;
-}
 
 __device__ __forceinline__ int wscan32(int v, int lane) {
 #pragma unroll
@@ -273,7 +263,7 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
   // difference.
   const int e = lane;
   int pos = e, end = v.E, payload = 0;
-  int64_t corr = 0;
+  uint32_t corr = 0;
   bool cut = false;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
@@ -284,14 +274,14 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
       int64_t d = 0;
       const int len = escaped(v, a, d, lane);
       const int hi = std::min(a + (len ? len : 1), v.E);
-      const int64_t pa = (int64_t)(v.P[hi] - v.P[a]);
+      const uint32_t pa = (uint32_t)(v.P[hi] - v.P[a]);
       if (e < NE && !cut && a >= pos) {
         if (!len) {
           end = a;
           cut = true;
         } else {
           payload += hi - a - 1;
-          corr += d - pa;
+          corr += (uint32_t)d - pa;
           pos = a + len;
         }
       }
@@ -299,12 +289,12 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
   }
   if (e < NE) {
     int n = 0;
-    int64_t sum = 0;
+    uint32_t sum = 0;
     uint8_t ex = 0;
     if (e < v.E) {
       if (pos < v.E) pos = v.E;
       n = end - e - payload;
-      sum = (int64_t)(v.P[end] - v.P[e]) + corr;
+      sum = (uint32_t)(v.P[end] - v.P[e]) + corr;
       ex = cut ? STOP : (uint8_t)(pos > C ? pos - C : 0);
     }
     b.w.cm.exit[c * NES + e] = ex;
@@ -323,7 +313,7 @@ __global__ void __launch_bounds__(256) k_cbf_groups(Batch b) {
   const int64_t cend = std::min<int64_t>(cfirst + G, b.chunk0[f + 1]);
   int e = e0;
   int32_t n = 0;
-  int64_t sum = 0;
+  uint32_t sum = 0;
   for (int64_t c = cfirst; c < cend && e != STOP; ++c) {
     n += b.w.cm.cnt[c * NES + e];
     sum += b.w.cm.sum[c * NES + e];
@@ -338,7 +328,8 @@ __global__ void __launch_bounds__(64) k_cbf_frames(Batch b) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= b.nf) return;
   int e = 0;
-  int64_t n = 0, sum = 0;
+  int64_t n = 0;
+  uint32_t sum = 0;
   for (int64_t g = b.group0[f]; g < b.group0[f + 1]; ++g) {
     b.w.g_entry[g] = (uint8_t)e;
     b.w.g_base[g] = n;
@@ -359,7 +350,8 @@ __global__ void __launch_bounds__(64) k_cbf_chunks(Batch b) {
   const int64_t cfirst = b.chunk0[f] + (g - b.group0[f]) * G;
   const int64_t cend = std::min<int64_t>(cfirst + G, b.chunk0[f + 1]);
   int e = b.w.g_entry[g];
-  int64_t n = b.w.g_base[g], sum = b.w.g_vbase[g];
+  int64_t n = b.w.g_base[g];
+  uint32_t sum = b.w.g_vbase[g];
   for (int64_t c = cfirst; c < cend; ++c) {
     b.w.c_entry[c] = (uint8_t)e;
     b.w.c_base[c] = n;
@@ -375,7 +367,7 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
   // one wavefront per chunk: the starts of elements are the bytes from the entry on that are not inside an escaped
   // element; their indices and running sums are wave scans in position order, so the stores are contiguous
   __shared__ __align__(16) uint8_t buf[WPB][STAGE];
-  __shared__ int64_t dq[WPB][C];
+  __shared__ uint32_t dq[WPB][C];
   __shared__ uint8_t pay[WPB][C];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t c = (int64_t)blockIdx.x * WPB + wv;
@@ -400,26 +392,26 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
       return;
     }
     if (lane < len - 1 && a + 1 + lane < C) pay[wv][a + 1 + lane] = 1;
-    if (lane == 0) dq[wv][a] = d;
+    if (lane == 0) dq[wv][a] = (uint32_t)d;
   });
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int32_t *o = b.out + b.ooff[f];
-  int64_t cn = 0, csum = b.w.c_vbase[c];
+  int64_t cn = 0;
+  uint32_t csum = b.w.c_vbase[c];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     const int q = 64 * k + lane;
     const bool start = q >= e && q < end && !pay[wv][q];
     const int byte = v.bytes[q];
-    const int64_t d = start ? (byte == 0x80 ? dq[wv][q] : (int64_t)(int8_t)byte) : 0;
+    const uint32_t d = start ? (byte == 0x80 ? dq[wv][q] : (uint32_t)(int32_t)(int8_t)byte) : 0u;
     const int rn = wscan32(start ? 1 : 0, lane);
-    const int64_t rs = wscan64(d, lane);
+    const uint32_t rs = (uint32_t)wscan32((int)d, lane);   // wraps mod 2^32 like the int32 cast of the int64 sum
     const int64_t idx = i0 + cn + rn - 1;
-    if (start && idx < nmax) o[idx] = (int32_t)(uint32_t)(uint64_t)(csum + rs);
+    if (start && idx < nmax) o[idx] = (int32_t)(csum + rs);
     cn += __builtin_amdgcn_readlane(rn, 63);
-    csum += (int64_t)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)((uint64_t)rs >> 32), 63) << 32 |
-                      (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)rs, 63));
+    csum += (uint32_t)__builtin_amdgcn_readlane((int)rs, 63);
   }
 }
 
