@@ -5,8 +5,9 @@
 // Here both run in one kernel over chunks resident in HBM:
 //
 //   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block), decoding into
-//                    its LDS slice (the block only, so that 20 waves fit a CU).  The compressed bytes stay in
-//                    registers: two 256-byte windows, lane l holding a dword of each, the next one loaded ahead.  The
+//                    its LDS slice (the block and a 256-byte dummy area, so that 19 waves fit a CU).  The compressed
+//                    bytes stay in registers: two 256-byte windows, lane l holding a dword of each, the next one loaded
+//                    ahead.  The
 //                    parse is wave-uniform (token, lengths and offset read from the window with readlane into scalar
 //                    registers); a literal run is written by the lanes that hold its bytes, with no read; a match is
 //                    copied by all 64 lanes through LDS.  Then the bit shuffle is undone from LDS -- lane l takes the
