@@ -7,13 +7,13 @@
 //   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block), decoding into
 //                    its LDS slice (the block and a 256-byte dummy area, so that 19 waves fit a CU).  The compressed
 //                    bytes stay in registers: two 256-byte windows, lane l holding a dword of each, the next one loaded
-//                    ahead.  The
-//                    parse is wave-uniform (token, lengths and offset read from the window with readlane into scalar
-//                    registers); a literal run is written by the lanes that hold its bytes, with no read; a match is
-//                    copied by all 64 lanes through LDS.  Then the bit shuffle is undone from LDS -- lane l takes the
-//                    groups of 8 elements g = l, l + 64, ...: the 8 x 8 bit matrix of each output byte is gathered from
-//                    8 planes and transposed in registers -- and the block's pixels are written with 16-byte stores
-//                    (raw uint32 counts for the integrator's frame store, or value-coded float32).
+//                    ahead.  The parse is wave-uniform (token, lengths and offset read from the window with readlane
+//                    into scalar registers); a literal run is written by the lanes that hold its bytes, with no read; a
+//                    match is copied through LDS in steps free of dependencies (short matches in one load and store).
+//                    Then the bit shuffle is undone from LDS: for a full block lane l reads one dword per bit plane for
+//                    its 4 groups of 8 elements, 4 x 4 byte transposes (v_perm) and one 8 x 8 bit transpose per output
+//                    byte give the 32 elements, written with 16-byte stores (raw uint32 counts for the integrator's
+//                    frame store, or value-coded float32); a last, shorter block goes one group per lane.
 //   k_bslz4_index    one lane per frame walks the chain of block size fields into the block table (the host form,
 //                    xpg_bslz4_index_host, is the same walk over a chunk in host memory).
 //
