@@ -146,7 +146,8 @@ __device__ __forceinline__ int frame_of(const int64_t *first, int nf, int64_t i)
 // escape bytes carry the sequential dependency: a scalar walk over the chunk's 0x80 bytes in order (skipping those
 // inside the previous escaped element) finds the escaped elements that start there; everything else follows from
 // per-byte prefix sums.
-constexpr int WPB = 4;   // chunks (wavefronts) per workgroup
+constexpr int WPB = 4;   // wavefronts per workgroup
+constexpr int CPW = 4;   // consecutive chunks per wavefront in k_cbf_maps and k_cbf_write
 
 struct ChunkView {
   const uint8_t *bytes;   // LDS: STAGE bytes from the chunk's start, zero past the stream's end
@@ -251,55 +252,67 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_maps(Batch b) {
   __shared__ __align__(16) uint8_t buf[WPB][STAGE];
   __shared__ int32_t Ps[WPB][C + 1];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t c = (int64_t)blockIdx.x * WPB + wv;
-  if (c >= b.nc) return;
-  const int f = frame_of(b.chunk0, b.nf, c);
-  const int64_t c0 = (c - b.chunk0[f]) * C;
-  ChunkView v;
-  view_chunk<true>(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
-  // lane e < 15 follows entry e through one pass over the 0x80 bytes: each escaped element's length and difference
-  // are the same for every entry; an entry takes the ones at or past its position.  Starts in [e, end) = bytes there
-  // minus the escaped elements' other bytes; sums from the prefix sums, each escaped element's bytes replaced by its
-  // difference.
-  const int e = lane;
-  int pos = e, end = v.E, payload = 0;
-  uint32_t corr = 0;
-  bool cut = false;
+  const int64_t cfirst = ((int64_t)blockIdx.x * WPB + wv) * CPW;
+  int f = 0;
+  int64_t flo = 0, fhi = 0;                 // the chunks of frame f: [flo, fhi)
+  for (int t = 0; t < CPW; ++t) {
+    const int64_t c = cfirst + t;
+    if (c >= b.nc) break;
+    if (c < flo || c >= fhi) {
+      f = frame_of(b.chunk0, b.nf, c);
+      flo = b.chunk0[f];
+      fhi = b.chunk0[f + 1];
+    }
+    const int64_t c0 = (c - flo) * C;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the previous chunk's LDS reads precede the new stage
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    ChunkView v;
+    view_chunk<true>(v, buf[wv], Ps[wv], b.src + b.soff[f], b.slen[f], c0, lane);
+    // lane e < 15 follows entry e through one pass over the 0x80 bytes: each escaped element's length and difference
+    // are the same for every entry; an entry takes the ones at or past its position.  Starts in [e, end) = bytes there
+    // minus the escaped elements' other bytes; sums from the prefix sums, each escaped element's bytes replaced by its
+    // difference.
+    const int e = lane;
+    int pos = e, end = v.E, payload = 0;
+    uint32_t corr = 0;
+    bool cut = false;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    uint64_t m = v.esc[k];
-    while (m) {
-      const int a = 64 * k + __builtin_ctzll(m);
-      m &= m - 1;
-      int64_t d = 0;
-      const int len = escaped(v, a, d, lane);
-      const int hi = std::min(a + (len ? len : 1), v.E);
-      const uint32_t pa = (uint32_t)(v.P[hi] - v.P[a]);
-      if (e < NE && !cut && a >= pos) {
-        if (!len) {
-          end = a;
-          cut = true;
-        } else {
-          payload += hi - a - 1;
-          corr += (uint32_t)d - pa;
-          pos = a + len;
+    for (int k = 0; k < 4; ++k) {
+      uint64_t m = v.esc[k];
+      while (m) {
+        const int a = 64 * k + __builtin_ctzll(m);
+        m &= m - 1;
+        int64_t d = 0;
+        const int len = escaped(v, a, d, lane);
+        const int hi = std::min(a + (len ? len : 1), v.E);
+        const uint32_t pa = (uint32_t)(v.P[hi] - v.P[a]);
+        if (e < NE && !cut && a >= pos) {
+          if (!len) {
+            end = a;
+            cut = true;
+          } else {
+            payload += hi - a - 1;
+            corr += (uint32_t)d - pa;
+            pos = a + len;
+          }
         }
       }
     }
-  }
-  if (e < NE) {
-    int n = 0;
-    uint32_t sum = 0;
-    uint8_t ex = 0;
-    if (e < v.E) {
-      if (pos < v.E) pos = v.E;
-      n = end - e - payload;
-      sum = (uint32_t)(v.P[end] - v.P[e]) + corr;
-      ex = cut ? STOP : (uint8_t)(pos > C ? pos - C : 0);
+    if (e < NE) {
+      int n = 0;
+      uint32_t sum = 0;
+      uint8_t ex = 0;
+      if (e < v.E) {
+        if (pos < v.E) pos = v.E;
+        n = end - e - payload;
+        sum = (uint32_t)(v.P[end] - v.P[e]) + corr;
+        ex = cut ? STOP : (uint8_t)(pos > C ? pos - C : 0);
+      }
+      b.w.cm.exit[c * NES + e] = ex;
+      b.w.cm.cnt[c * NES + e] = n;
+      b.w.cm.sum[c * NES + e] = sum;
     }
-    b.w.cm.exit[c * NES + e] = ex;
-    b.w.cm.cnt[c * NES + e] = n;
-    b.w.cm.sum[c * NES + e] = sum;
   }
 }
 
@@ -364,54 +377,67 @@ __global__ void __launch_bounds__(64) k_cbf_chunks(Batch b) {
 }
 
 __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
-  // one wavefront per chunk: the starts of elements are the bytes from the entry on that are not inside an escaped
-  // element; their indices and running sums are wave scans in position order, so the stores are contiguous
+  // one wavefront per chunk (CPW consecutive chunks in turn): the starts of elements are the bytes from the entry on
+  // that are not inside an escaped element; their indices and running sums are wave scans in position order, so the
+  // stores are contiguous
   __shared__ __align__(16) uint8_t buf[WPB][STAGE];
   __shared__ uint32_t dq[WPB][C];
   __shared__ uint8_t pay[WPB][C];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t c = (int64_t)blockIdx.x * WPB + wv;
-  if (c >= b.nc) return;
-  const int f = frame_of(b.chunk0, b.nf, c);
-  const int e = b.w.c_entry[c];
-  const int64_t nmax = b.nel[f];
-  const int64_t i0 = b.w.c_base[c];
-  if (b.status[f] || e == STOP || i0 >= nmax) return;
-  const int64_t c0 = (c - b.chunk0[f]) * C;
-  ChunkView v;
-  view_chunk<false>(v, buf[wv], nullptr, b.src + b.soff[f], b.slen[f], c0, lane);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) pay[wv][64 * k + lane] = 0;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int end = v.E, pos = 0;
-  walk(v, e, pos, lane, [&](int a, int len, int64_t d) {
-    if (!len) {
-      end = a;
-      return;
+  const int64_t cfirst = ((int64_t)blockIdx.x * WPB + wv) * CPW;
+  int f = 0;
+  int64_t flo = 0, fhi = 0;                 // the chunks of frame f: [flo, fhi)
+  for (int t = 0; t < CPW; ++t) {
+    const int64_t c = cfirst + t;
+    if (c >= b.nc) break;
+    if (c < flo || c >= fhi) {
+      f = frame_of(b.chunk0, b.nf, c);
+      flo = b.chunk0[f];
+      fhi = b.chunk0[f + 1];
     }
-    if (lane < len - 1 && a + 1 + lane < C) pay[wv][a + 1 + lane] = 1;
-    if (lane == 0) dq[wv][a] = (uint32_t)d;
-  });
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int32_t *o = b.out + b.ooff[f];
-  int64_t cn = 0;
-  uint32_t csum = b.w.c_vbase[c];
+    const int e = b.w.c_entry[c];
+    const int64_t nmax = b.nel[f];
+    const int64_t i0 = b.w.c_base[c];
+    if (b.status[f] || e == STOP || i0 >= nmax) continue;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");   // the previous chunk's LDS reads precede the new stage
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int64_t c0 = (c - flo) * C;
+    ChunkView v;
+    view_chunk<false>(v, buf[wv], nullptr, b.src + b.soff[f], b.slen[f], c0, lane);
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    const int q = 64 * k + lane;
-    const bool start = q >= e && q < end && !pay[wv][q];
-    const int byte = v.bytes[q];
-    const uint32_t d = start ? (byte == 0x80 ? dq[wv][q] : (uint32_t)(int32_t)(int8_t)byte) : 0u;
-    const int rn = wscan32(start ? 1 : 0, lane);
-    const uint32_t rs = (uint32_t)wscan32((int)d, lane);   // wraps mod 2^32 like the int32 cast of the int64 sum
-    const int64_t idx = i0 + cn + rn - 1;
-    if (start && idx < nmax) o[idx] = (int32_t)(csum + rs);
-    cn += __builtin_amdgcn_readlane(rn, 63);
-    csum += (uint32_t)__builtin_amdgcn_readlane((int)rs, 63);
+    for (int k = 0; k < 4; ++k) pay[wv][64 * k + lane] = 0;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int end = v.E, pos = 0;
+    walk(v, e, pos, lane, [&](int a, int len, int64_t d) {
+      if (!len) {
+        end = a;
+        return;
+      }
+      if (lane < len - 1 && a + 1 + lane < C) pay[wv][a + 1 + lane] = 1;
+      if (lane == 0) dq[wv][a] = (uint32_t)d;
+    });
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int32_t *o = b.out + b.ooff[f];
+    int64_t cn = 0;
+    uint32_t csum = b.w.c_vbase[c];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int q = 64 * k + lane;
+      const bool start = q >= e && q < end && !pay[wv][q];
+      const int byte = v.bytes[q];
+      const uint32_t d = start ? (byte == 0x80 ? dq[wv][q] : (uint32_t)(int32_t)(int8_t)byte) : 0u;
+      const int rn = wscan32(start ? 1 : 0, lane);
+      const uint32_t rs = (uint32_t)wscan32((int)d, lane);   // wraps mod 2^32 like the int32 cast of the int64 sum
+      const int64_t idx = i0 + cn + rn - 1;
+      if (start && idx < nmax) o[idx] = (int32_t)(csum + rs);
+      cn += __builtin_amdgcn_readlane(rn, 63);
+      csum += (uint32_t)__builtin_amdgcn_readlane((int)rs, 63);
+    }
   }
 }
 
@@ -446,11 +472,13 @@ int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
   b.status = p->status;
   hipStream_t s = (hipStream_t)stream;
   const int64_t tm = b.nc * NES, tg = b.ng * NES;
-  if (tm > 0) hipLaunchKernelGGL(k_cbf_maps, dim3((uint32_t)((b.nc + WPB - 1) / WPB)), dim3(64 * WPB), 0, s, b);
+  if (tm > 0)
+    hipLaunchKernelGGL(k_cbf_maps, dim3((uint32_t)((b.nc + WPB * CPW - 1) / (WPB * CPW))), dim3(64 * WPB), 0, s, b);
   if (tg > 0) hipLaunchKernelGGL(k_cbf_groups, dim3((uint32_t)((tg + 255) / 256)), dim3(256), 0, s, b);
   hipLaunchKernelGGL(k_cbf_frames, dim3((uint32_t)((b.nf + 63) / 64)), dim3(64), 0, s, b);
   if (b.ng > 0) hipLaunchKernelGGL(k_cbf_chunks, dim3((uint32_t)((b.ng + 63) / 64)), dim3(64), 0, s, b);
-  if (b.nc > 0) hipLaunchKernelGGL(k_cbf_write, dim3((uint32_t)((b.nc + WPB - 1) / WPB)), dim3(64 * WPB), 0, s, b);
+  if (b.nc > 0)
+    hipLaunchKernelGGL(k_cbf_write, dim3((uint32_t)((b.nc + WPB * CPW - 1) / (WPB * CPW))), dim3(64 * WPB), 0, s, b);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return xpg::report_error(XPG_ERR_HIP, hipGetErrorString(e));
   return 0;
