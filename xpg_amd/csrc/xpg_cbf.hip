@@ -147,7 +147,7 @@ __device__ __forceinline__ int frame_of(const int64_t *first, int nf, int64_t i)
 // inside the previous escaped element) finds the escaped elements that start there; everything else follows from
 // per-byte prefix sums.
 constexpr int WPB = 4;   // wavefronts per workgroup
-constexpr int CPW = 4;   // consecutive chunks per wavefront in k_cbf_maps and k_cbf_write
+constexpr int CPW = 8;   // consecutive chunks per wavefront in k_cbf_maps and k_cbf_write
 
 struct ChunkView {
   const uint8_t *bytes;   // LDS: STAGE bytes from the chunk's start, zero past the stream's end
