@@ -171,6 +171,17 @@ struct Window {
       nxt = load(wb + 256, lane);
     }
   }
+  // bytes q .. q + 4 as the low 40 bits, after reach(q) (so wb <= q < wb + 256): dword d of the current window and
+  // the next dword (lane 0 of the next window for d = 63), both read and one selected
+  __device__ __forceinline__ uint64_t at5(int q) const {
+    const int i = q - wb;
+    const int d = i >> 2;
+    const uint32_t w0 = (uint32_t)__builtin_amdgcn_readlane((int)cur, d);
+    const uint32_t a1 = (uint32_t)__builtin_amdgcn_readlane((int)cur, (d + 1) & 63);
+    const uint32_t b1 = (uint32_t)__builtin_amdgcn_readlane((int)nxt, 0);
+    const uint32_t w1 = d == 63 ? b1 : a1;
+    return ((uint64_t)w1 << 32 | w0) >> (8 * (i & 3));
+  }
   // byte at position q, wb <= q < wb + 512 (uniform)
   __device__ __forceinline__ int at(int q) const {
     const int i = q - wb;
@@ -287,10 +298,20 @@ __device__ __forceinline__ void match_copy(uint8_t *db, int op, int off, int ml,
 // in the decoded output and end before the last LASTLITERALS bytes; offset 0 is rejected.
 __device__ int lz4_block(Window &w, int cs, uint8_t *db, int dn, int lane) {
   int ip = 0, op = 0;
+  // the parse reads through a 5-byte cache: the read at a match's offset also holds the next token
+  int px = -8;
+  uint64_t x = 0;
+  auto get = [&](int q) -> int {
+    if (q - px > 4) {
+      w.reach(q, lane);
+      x = w.at5(q);
+      px = q;
+    }
+    return (int)((x >> (8 * (q - px))) & 0xFF);
+  };
   for (;;) {
     if (ip >= cs) return XPG_BSLZ4_LZ4;
-    w.reach(ip, lane);
-    const int tok = w.at(ip);
+    const int tok = get(ip);
     ++ip;
     int lit = tok >> 4;
     if (lit == 15) {
@@ -305,8 +326,7 @@ __device__ int lz4_block(Window &w, int cs, uint8_t *db, int dn, int lane) {
     ip += lit;
     op += lit;
     if (last) break;
-    w.reach(ip, lane);
-    const int off = w.at(ip) | w.at(ip + 1) << 8;
+    const int off = get(ip) | get(ip + 1) << 8;
     ip += 2;
     int ml = tok & 15;
     if (ml == 15) {
@@ -318,8 +338,8 @@ __device__ int lz4_block(Window &w, int cs, uint8_t *db, int dn, int lane) {
     if (off == 0 || off > op || ml > dn - LASTLITERALS - op) return XPG_BSLZ4_LZ4;
     match_copy(db, op, off, ml, dn + DUMMY_PAD, lane);
     op += ml;
-    wave_sync();
   }
+  wave_sync();
   return op == dn ? 0 : XPG_BSLZ4_LZ4;
 }
 
