@@ -157,12 +157,22 @@ struct ChunkView {
   int32_t *P;             // LDS (C + 1): P[q] = sum of int8(b[i]) for i < q, i < E
 };
 
+// inclusive scan over the wave with DPP: within rows of 16 lanes by row_shr 1, 2, 4, 8 (lanes without a source add
+// 0), then row 0's last lane into row 1 and row 2's into row 3 (row_bcast:15), then lane 31 into rows 2 and 3
+// (row_bcast:31)
+template <int CTRL, int ROWS>
+__device__ __forceinline__ int dpp_add(int v) {
+  return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false);
+}
+
 __device__ __forceinline__ int wscan32(int v, int lane) {
-#pragma unroll
-  for (int d = 1; d < WAVE64; d <<= 1) {
-    const int u = __shfl_up(v, d);
-    if (lane >= d) v += u;
-  }
+  (void)lane;
+  v = dpp_add<0x111, 0xF>(v);   // row_shr:1
+  v = dpp_add<0x112, 0xF>(v);   // row_shr:2
+  v = dpp_add<0x114, 0xF>(v);   // row_shr:4
+  v = dpp_add<0x118, 0xF>(v);   // row_shr:8
+  v = dpp_add<0x142, 0xA>(v);   // row_bcast:15 into rows 1 and 3
+  v = dpp_add<0x143, 0xC>(v);   // row_bcast:31 into rows 2 and 3
   return v;
 }
 
