@@ -4,7 +4,8 @@
 // and the device undoes the bit shuffle and applies the value codes (unshuffle_frame_torch, xpg/io/bslz4.py:80-108).
 // Here both run in one kernel over chunks resident in HBM:
 //
-//   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block), decoding into
+//   k_bslz4_decode   one wavefront per LZ4 block (block_elems elements, 8 KB for the default block; BPW consecutive
+//                    blocks per wavefront in turn), decoding into
 //                    its LDS slice (the block and a 256-byte dummy area, so that 19 waves fit a CU).  The compressed
 //                    bytes stay in registers: two 256-byte windows, lane l holding a dword of each, the next one loaded
 //                    ahead.  The parse is wave-uniform (token, lengths and offset read from the window with readlane
@@ -126,7 +127,7 @@ struct Batch {
   int64_t min_ceil, ovl_ceil;
   const uint8_t *sbad;
   int32_t wave_lds;         // LDS per wave: the decoded block, then the dummy area
-  int64_t f0;               // first frame of this launch
+  int64_t f0, f1;           // frames of this launch: [f0, f1)
 };
 
 __global__ void __launch_bounds__(64) k_bslz4_index(Batch b) {
@@ -413,14 +414,35 @@ __device__ __forceinline__ void unshuffle32(const uint8_t *db, int m, int q, uin
     }
 }
 
+constexpr int BPW = 4;   // blocks per wavefront
+
+template <int ES, int KIND>
+__device__ __forceinline__ void decode_block(const Batch &b, uint8_t *lds, int64_t f, int k, int lane);
+
 template <int ES, int KIND>
 __global__ void __launch_bounds__(64) k_bslz4_decode(Batch b) {
   extern __shared__ __align__(16) uint8_t lds[];
   const int lane = threadIdx.x;
-  const int64_t gw = b.f0 * b.nbf + blockIdx.x;
-  const int64_t f = gw / b.nbf;
-  const int k = (int)(gw - f * b.nbf);
+  // BPW consecutive blocks per wavefront, in turn
+  const int64_t gw0 = b.f0 * b.nbf + (int64_t)blockIdx.x * BPW;
+  const int64_t f = gw0 / b.nbf;
+  const int k0 = (int)(gw0 - f * b.nbf);
+  for (int t = 0; t < BPW; ++t) {
+    int64_t fb = f;
+    int k = k0 + t;
+    if (k >= b.nbf) {
+      fb += k / b.nbf;
+      k %= b.nbf;
+    }
+    if (fb >= b.f1) break;   // past this launch's frames
+    decode_block<ES, KIND>(b, lds, fb, k, lane);
+  }
+}
+
+template <int ES, int KIND>
+__device__ __forceinline__ void decode_block(const Batch &b, uint8_t *lds, int64_t f, int k, int lane) {
   if (f >= b.nf || b.status[f] != 0) return;
+  wave_sync();   // the previous block's LDS reads precede this block's writes
   uint8_t *db = lds;
   const uint32_t *t = b.table + f * (int64_t)(b.nbf + 1) * 2;
   const uint8_t *chunk = b.src + b.coff[f];
@@ -535,6 +557,7 @@ Batch make_batch(const xpg_bslz4_batch *p) {
   const int64_t bb = (int64_t)p->block_elems * p->esize;
   b.wave_lds = (int32_t)((bb + DUMMY_PAD + DUMMY_BYTES + 15) / 16 * 16);
   b.f0 = 0;
+  b.f1 = p->n_frames;
   return b;
 }
 
@@ -587,7 +610,9 @@ int xpg_bslz4_decode(const xpg_bslz4_batch *p, void *stream) {
   for (int64_t f0 = 0; f0 < p->n_frames; f0 += per) {
     const int64_t nfr = std::min<int64_t>(per, p->n_frames - f0);
     b.f0 = f0;
-    hipLaunchKernelGGL(kern, dim3((uint32_t)(nfr * b.nbf)), dim3(64), b.wave_lds, (hipStream_t)stream, b);
+    b.f1 = f0 + nfr;
+    hipLaunchKernelGGL(kern, dim3((uint32_t)((nfr * b.nbf + BPW - 1) / BPW)), dim3(64), b.wave_lds,
+                       (hipStream_t)stream, b);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return xpg::report_error(XPG_ERR_HIP, hipGetErrorString(e));
   }
