@@ -95,11 +95,21 @@ typedef struct xpg_pass1_out {
   int32_t fz_stride;       /* 0: Dz */
   float *img2, *J, *u1, *u2;
   uint8_t *all_valid2;
+  /* Per-voxel classes (n, Dz, Dy, Dx) at each row's voxel offset, or NULL (not written; for mask comparisons):
+   * XPG_VM_FG foreground (ellipse x foreground frame), XPG_VM_BGM background candidate, XPG_VM_NBM inside a neighbour
+   * ellipsoid, XPG_VM_BGW in the final background set (after the last clipping round), XPG_VM_FGV foreground, valid and
+   * not claimed by a neighbour. */
+  uint8_t *vmask;
 } xpg_pass1_out;
+
+enum { XPG_VM_FG = 1, XPG_VM_BGM = 2, XPG_VM_NBM = 4, XPG_VM_BGW = 8, XPG_VM_FGV = 16, XPG_VM_USE = 32 };
 
 /* pass 2 outputs (xpg/integrate_kernels.py:429, :482), float64 (n). */
 typedef struct xpg_pass2_out {
   double *iprf, *vprf, *vprf0, *corr, *p_valid, *p_total, *ipartner, *rho, *p_fg;
+  /* Per-voxel classes (n, Dz, Dy, Dx) at each row's voxel offset, or NULL: XPG_VM_USE in the fit mask, XPG_VM_NBM
+   * inside a neighbour ellipsoid other than the modelled partner's. */
+  uint8_t *vmask;
 } xpg_pass2_out;
 
 /* Fixed-point profile accumulation grids: int64 (n_regions, 41, 41) each, value * 2^32, zeroed by the caller.
@@ -210,6 +220,9 @@ typedef struct xpg_prepared {
   uint8_t *nbm;    /* (n, Dz, Dy, Dx) bit 0: inside a neighbour ellipsoid; bit 1: the same with column 0 skipped */
   double *geo_p;   /* (n, 3, Dy, Dx): the overlapping partner's eps1, eps2, J on this box (rows with has_p only) */
   double *zeta_p;  /* (n,) the partner's zeta (rows with has_p only) */
+  /* (n,) or NULL: the effective spot width seff of each row, given instead of computed from sigma_D and the centre
+   * pixel's area (the reference's pass 2 with pass 1's ``pix``, xpg/integrate_kernels.py:259-260: eps = u seff) */
+  const double *seff;
 } xpg_prepared;
 
 /* Shoebox gather (host gather xpg/integrate.py:704-803 with _decode_bins xpg/shoebox.py:264-268), the widths-independent
@@ -217,6 +230,11 @@ typedef struct xpg_prepared {
  * (_neighbour_mask, :186-213) of a chunk, into prep. */
 int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
                 const xpg_prepared *prep, void *stream);
+
+/* xpg_prepare from a chunk of value-coded shoeboxes sb (n, Dz, Dy, Dx) instead of the frame store (the kernel boundary
+ * of xpg/integrate_kernels.py:216-485, whose closures take the chunk); prep->sb may be sb itself. */
+int xpg_prepare_sb(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+                   const xpg_prepared *prep, void *stream);
 
 /* xpg_pass1 / xpg_pass2 reading the prepared data of xpg_prepare (results identical to xpg_pass1_frames /
  * xpg_pass2_frames on the same frames). */

@@ -17,6 +17,14 @@ from . import kernels as ok
 STRONG_KEYS = ("isum", "vsum", "n_over", "frac_rec", "n_fg_nb", "n_fg_masked")
 
 
+def profile_strong(o1):
+    """The rows whose projected images build the reference profiles (xpg/integrate.py:530-533): I/sigma > 10 by
+    summation, no overload, > 95% of the rocking curve recorded, no foreground pixel claimed by a neighbour or masked.
+    Pinned by rows just over and under each threshold (tests/test_oracle_pins.py)."""
+    return ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10) & (o1["n_over"] == 0)
+            & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0) & (o1["n_fg_masked"] == 0))
+
+
 def integrate_sweep(coded_frames, read, hp, exp_blocks, model, det, adaptive=True, learn_profiles=True, mask=None,
                     n_regions=9):
     """coded_frames: {0-based frame: float32 value-coded frame}; read: 0-based frames read; hp: the host plan of
@@ -58,9 +66,7 @@ def integrate_sweep(coded_frames, read, hp, exp_blocks, model, det, adaptive=Tru
         den = np.zeros((n_regions, 41, 41))
         if learn_profiles:
             for sel, sb, rc, nb, o1, B in groups:
-                strong = ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10)
-                          & (o1["n_over"] == 0) & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0)
-                          & (o1["n_fg_masked"] == 0))
+                strong = profile_strong(o1)
                 if strong.any():
                     nn, dd = ok.accumulate_profiles(o1["img2"], o1["all_valid2"], o1["J"], o1["u1"], o1["u2"],
                                                     rc["region"], o1["isum"], strong, o1["seff"], n_regions)

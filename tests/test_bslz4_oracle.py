@@ -267,3 +267,20 @@ def test_codes_match_imageset_rules(esize):
             img[raw >= overload] = -2.0
         img[bad] = -1.0
         np.testing.assert_array_equal(got, img)
+
+
+def test_decode_output_capacity_checked_before_launch():
+    # ChunkBatch.decode_into validates the output with output_frames() before any launch: the kernel writes nel
+    # elements per frame at slot * stride and has no capacity argument of its own
+    from xpg_amd.bslz4 import output_frames
+    assert output_frames((4, 37, 61), 37 * 61, 4, False) == (37 * 61, 4)
+    assert output_frames((37 * 61,), 37 * 61, 1, False) == (37 * 61, 1)
+    assert output_frames((2, 3000), 2257, 9, True) == (3000, 2)            # slots pick the frames
+    with pytest.raises(ValueError):
+        output_frames((37 * 61 - 1,), 37 * 61, 1, False)                   # 1-D output one element short
+    with pytest.raises(ValueError):
+        output_frames((0,), 16, 1, False)
+    with pytest.raises(ValueError):
+        output_frames((4, 100), 101, 4, False)                             # frames smaller than the chunks'
+    with pytest.raises(ValueError):
+        output_frames((2, 37 * 61), 37 * 61, 3, False)                     # more chunks than frames

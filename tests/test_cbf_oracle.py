@@ -55,3 +55,15 @@ def test_encoder_bytes_and_roundtrip():
     big = np.array([2 ** 31 - 1, -2 ** 31 + 1], np.int32)          # a difference beyond int32 is refused
     with pytest.raises(ValueError):
         cbf.encode_byte_offset(big)
+
+
+def test_stream_cut_inside_the_last_wide_escape_is_short():
+    # The n-th element's int32 (7-byte) or int64 (15-byte) escape cut short at the stream's end: the reference reads
+    # the bytes that are there as a shorter integer (xpg/io/cbf.py:35-41 slice without a length check); the oracle and
+    # the device decode flag the frame as short instead (a cut stream's value is not the pixel's).
+    int32_tail = bytes([3]) + b"\x80\x00\x80" + le(100000, 4)
+    int64_tail = bytes([3]) + b"\x80\x00\x80\x00\x00\x00\x80" + le(2 ** 40, 8)
+    for full in (int32_tail, int64_tail):
+        assert oc.cbf_decode(full, 2)[0] == 0
+        for cut in range(1, 4):
+            assert oc.cbf_decode(full[:-cut], 2)[0] == 1

@@ -158,3 +158,141 @@ def test_fused_accumulation_matches_strong_selection():
     K.run_pass1(det, call, refl, sb=sb, acc=acc2)
     torch.cuda.synchronize()
     assert torch.equal(num2, num_fx) and torch.equal(den2, den_fx)
+
+
+def _masks(c, dev):
+    """Per-voxel classes of pass 1 and pass 2 from the device (XPG_VM_* bits), through the C ABI."""
+    from xpg_amd import integrate_kernels as K
+    det = K.device_detector(c["det"], dev)
+    call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"], c["rcw"])
+    refl = K.DeviceRefl(c["rc"], c["nb"], (c["Dz"], c["Dy"], c["Dx"]), dev)
+    sb = torch.as_tensor(c["sb"], dtype=torch.float32, device=dev)
+    r1 = K.run_pass1(det, call, refl, sb=sb, masks=True)
+    r2 = K.run_pass2(det, call, refl, c["prof"], r1["bg"], r1["nbg"], sb=sb, masks=True)
+    torch.cuda.synchronize()
+    return r1["vmask"].cpu().numpy(), r2["vmask"].cpu().numpy()
+
+
+MASK_CASES = [("golden", n) for n in cases()] + [
+    ("synthetic", (201, (7, 11, 11), {})), ("synthetic", (202, (9, 9, 9), dict(fb=3))),
+    ("synthetic", (203, (5, 13, 13), dict(pxs=3))), ("synthetic", (204, (17, 21, 21), dict(intensity=(10.0, 1e6)))),
+    ("synthetic", (205, (33, 31, 31), dict(n=12, sd_deg=0.06, sm_deg=0.5))),
+    ("synthetic", (206, (5, 9, 9), dict(intensity=(1.0, 2 ** 20 * 20), background=0.5)))]
+
+
+@pytest.mark.parametrize("kind,case", MASK_CASES, ids=[str(c[1] if c[0] == "golden" else c[1][0]) for c in MASK_CASES])
+def test_voxel_masks_bit_exact(kind, case):
+    """Every voxel's class, bit for bit against the oracle's masks: foreground (ellipse x foreground frame),
+    background candidate, neighbour ellipsoid, final background set after the clipping rounds, valid foreground;
+    pass 2's fit mask and neighbour mask (partner column skipped).  Boxes too large for LDS (33 x 31 x 31) included."""
+    from xpg_amd.integrate_kernels import P2_F64  # noqa: F401  (the library loads)
+    if kind == "golden":
+        c, _ = load(case)
+    else:
+        from chunks import make_chunk
+        seed, shape, kw = case
+        kw = dict(kw)
+        c = make_chunk(n=kw.pop("n", 96), Dz=shape[0], Dy=shape[1], Dx=shape[2], seed=seed, **kw)
+    print("voxel masks", kind, case if kind == "golden" else case[0], flush=True)
+    m1, m2 = _masks(c, torch.device("cuda:0"))
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    o2 = ok.pass2(c["sb"], *_args(c), c["prof"], o1["bg"], o1["nbg"], c["nb"], c["rcw"], irls_rule="reflection")
+    valid = np.asarray(c["sb"]) >= 0
+    want1 = {1: o1["_fg"], 2: o1["_bgm"], 4: o1["_nbm"], 8: o1["_w"], 16: o1["_fg"] & valid & ~o1["_nbm"]}
+    for bit, w in want1.items():
+        got = (m1 & bit) != 0
+        assert np.array_equal(got, np.broadcast_to(w, got.shape)), (case, bit, int((got != w).sum()))
+    assert np.array_equal((m2 & 32) != 0, o2["_use"]), (case, int((((m2 & 32) != 0) != o2["_use"]).sum()))
+    assert np.array_equal((m2 & 4) != 0, np.broadcast_to(o2["_nbm"], m2.shape)), case
+    assert o1["_fg"].any() and o2["_use"].any()
+    assert o1["_w"].any() or not o1["_bgm"].any()      # (the 3 x 2 x 3 boxes have no background ring at all)
+
+
+def test_make_kernels_with_the_reference_argument_types():
+    """The closures of make_kernels called as xpg/integrate.py:466-563 calls them: a chunk on the device, per-reflection
+    arrays as float32 / int64 device tensors, the detector dict of _detector_consts (float32 device tensors, (1, 1)
+    placeholders, Python scalars), axis / s0 / rcw as float32 device tensors, numpy float32 scalars, and pass 2 given
+    pass 1's per-pixel geometry ``pix``.  Results against the oracle run on the same float32 inputs."""
+    from chunks import make_chunk
+    from xpg_amd.integrate_kernels import make_kernels
+    dev = torch.device("cuda:0")
+    c = make_chunk(n=96, seed=211)
+    f32d = lambda a: torch.as_tensor(np.asarray(a, np.float32), device=dev)
+    rc = dict(S=f32d(c["rc"]["S"]), phi=f32d(c["rc"]["phi"]), ix0=torch.as_tensor(c["rc"]["ix0"], device=dev),
+              iy0=torch.as_tensor(c["rc"]["iy0"], device=dev), iz0=torch.as_tensor(c["rc"]["iz0"], device=dev),
+              fb=f32d(c["rc"]["fb"]), region=torch.as_tensor(c["rc"]["region"], device=dev),
+              has_p=f32d(c["rc"]["has_p"]), pS=f32d(c["rc"]["pS"]), pphi=f32d(c["rc"]["pphi"]))
+    d = c["det"]
+    det = dict(orgx=d["orgx"], orgy=d["orgy"], qx=d["qx"], qy=d["qy"], wl=d["wl"], tsens=d["tsens"], mu=d["mu"],
+               plx=d["plx"], segmap=torch.zeros((1, 1), dtype=torch.int16, device=dev), gx=f32d(np.zeros((1, 1))),
+               gy=f32d(np.zeros((1, 1))), bkg=f32d(np.ones((1, 1))),
+               **{k: f32d(d[k]) for k in ("P0", "Bx", "By", "U1", "U2", "UN")})
+    nb = {k: f32d(v) for k, v in c["nb"].items()}
+    scal = [np.float32(v) for v in (c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"])]
+    p1, acc, p2 = make_kernels(7, 11, 11, 9)
+    sb = torch.as_tensor(c["sb"], dtype=torch.float32, device=dev)
+    prof0 = f32d(c["prof"])
+    r1 = p1(sb, rc, det, f32d(c["axis"]), f32d(c["s0"]), *scal, prof0, nb, f32d(c["rcw"]))
+    pix = {k: r1[k] for k in ("zeta", "J", "u1", "u2", "seff")}
+    r2 = p2(sb, rc, det, f32d(c["axis"]), f32d(c["s0"]), *scal, prof0, r1["bg"], r1["nbg"], nb, f32d(c["rcw"]), pix)
+    torch.cuda.synchronize()
+    # the oracle on the same float32-rounded inputs
+    r32 = lambda a: np.asarray(a, np.float32).astype(np.float64)
+    orc = dict(c["rc"])
+    for k in ("S", "phi", "pS", "pphi"):
+        orc[k] = r32(orc[k])
+    odet = dict(d)
+    for k in ("P0", "Bx", "By", "U1", "U2", "UN"):
+        odet[k] = r32(d[k])
+    oargs = (orc, odet, r32(c["axis"]), r32(c["s0"]), *[float(v) for v in scal])
+    o1 = ok.pass1(c["sb"], *oargs, c["nb"], c["rcw"])
+    for k in ("n_over", "n_fg_masked", "n_fg_nb"):
+        assert np.array_equal(r1[k].cpu().numpy(), np.asarray(o1[k])), k
+    opix = dict(zeta=r1["zeta"].cpu().numpy(), J=r1["J"].cpu().numpy().astype(np.float64),
+                u1=r1["u1"].cpu().numpy().astype(np.float64), u2=r1["u2"].cpu().numpy().astype(np.float64),
+                seff=r1["seff"].cpu().numpy().astype(np.float32).astype(np.float64))
+    o2 = ok.pass2(c["sb"], *oargs, c["prof"], r1["bg"].cpu().numpy(), r1["nbg"].cpu().numpy(), c["nb"], c["rcw"],
+                  pix=opix, irls_rule="reflection")
+    _check_pass2({k: v.cpu().numpy() for k, v in r2.items() if torch.is_tensor(v)}, o2, "reference argument types")
+    # pix is honoured: a pass 2 given another width than its own sigma_D follows the given one
+    pix_w = dict(pix, seff=pix["seff"] * 1.25, u1=pix["u1"] / 1.25, u2=pix["u2"] / 1.25)
+    r2w = p2(sb, rc, det, f32d(c["axis"]), f32d(c["s0"]), *scal, prof0, r1["bg"], r1["nbg"], nb, f32d(c["rcw"]),
+             pix_w)
+    torch.cuda.synchronize()
+    ow = ok.pass2(c["sb"], *oargs, c["prof"], r1["bg"].cpu().numpy(), r1["nbg"].cpu().numpy(), c["nb"], c["rcw"],
+                  pix=dict(opix, seff=(pix["seff"] * 1.25).cpu().numpy().astype(np.float32).astype(np.float64),
+                           u1=(r1["u1"] / 1.25).cpu().numpy().astype(np.float64),
+                           u2=(r1["u2"] / 1.25).cpu().numpy().astype(np.float64)), irls_rule="reflection")
+    _check_pass2({k: v.cpu().numpy() for k, v in r2w.items() if torch.is_tensor(v)}, ow, "pix with another width")
+    assert not np.allclose(r2w["iprf"].cpu().numpy(), r2["iprf"].cpu().numpy(), rtol=1e-3)
+
+
+def test_kernels_on_a_segmented_detector_with_parallax_and_distortion():
+    """Two tilted, shifted segments with a gap, 0.45 mm sensor parallax and geometric-correction tables, routed from
+    the host Geometry through detector_consts to the kernels (segment map, per-segment maps, gx / gy): pass 1 and
+    pass 2 against the oracle, masks bit-exact, boxes that straddle the gap included."""
+    from chunks import make_chunk
+    from xpg_amd.geometry import Geometry, Segment, detector_consts
+    th = np.radians(0.4)
+    segs = [Segment(1, 2068, 1, 1000),
+            Segment(1, 2068, 1040, 2162, orgx=0.4, orgy=-0.3, distance=0.2,
+                    ex=np.array([np.cos(th), np.sin(th), 0.0]), ey=np.array([-np.sin(th), np.cos(th), 0.0]))]
+    yy, xx = np.mgrid[0:2162, 0:2068]
+    geo = ((0.5 * np.sin(xx / 150.0)).astype(np.float32), (0.4 * np.cos(yy / 200.0)).astype(np.float32))
+    g = Geometry(wavelength=1.0, beam=np.array([0, 0, 1.0]), axis=np.array([1.0, 0, 0]), orgx=1030, orgy=1080,
+                 distance=150, ed1=np.array([1.0, 0, 0]), ed2=np.array([0, 1.0, 0]), qx=0.075, qy=0.075, nx=2068,
+                 ny=2162, phi0=0, osc=0.1, sensor_thickness=0.45, mu=3.7, segments=segs, geo_corr=geo)
+    det = dict(detector_consts(g), nx=g.nx, ny=g.ny)
+    c = make_chunk(n=128, seed=221, det=det)
+    print("segmented detector chunk", flush=True)
+    h1, h2 = _run(c, torch.device("cuda:0"))
+    o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
+    _check_pass1(h1, o1, "segmented")
+    o2 = ok.pass2(c["sb"], *_args(c), c["prof"], o1["bg"], o1["nbg"], c["nb"], c["rcw"], irls_rule="reflection")
+    _check_pass2(h2, o2, "segmented")
+    m1, m2 = _masks(c, torch.device("cuda:0"))
+    assert np.array_equal((m1 & 1) != 0, o1["_fg"]) and np.array_equal((m1 & 8) != 0, o1["_w"])
+    assert np.array_equal((m2 & 32) != 0, o2["_use"])
+    # the detector really is segmented where the boxes are: some boxes reach into the second segment or the gap
+    y_box = c["rc"]["iy0"][:, None] + np.arange(c["Dy"])[None] + 1.0
+    assert (y_box > 1000).any()

@@ -57,12 +57,12 @@ class Pass1Out(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("bg", "nbg", "isum", "vsum", "vsum0", "xo", "yo", "zo", "m_d", "m_m",
                                           "frac_rec", "nfg", "zeta", "seff", "n_over", "n_fg_masked", "n_fg_nb",
                                           "fz")] + [("fz_stride", C.c_int32)] + \
-               [(k, C.c_void_p) for k in ("img2", "J", "u1", "u2", "all_valid2")]
+               [(k, C.c_void_p) for k in ("img2", "J", "u1", "u2", "all_valid2", "vmask")]
 
 
 class Pass2Out(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("iprf", "vprf", "vprf0", "corr", "p_valid", "p_total", "ipartner", "rho",
-                                          "p_fg")]
+                                          "p_fg", "vmask")]
 
 
 class ProfileAcc(C.Structure):
@@ -79,7 +79,7 @@ class Frames(C.Structure):
 
 class Prepared(C.Structure):
     _fields_ = [("sb", C.c_void_p), ("geo", C.c_void_p), ("zeta", C.c_void_p), ("nbm", C.c_void_p),
-                ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p)]
+                ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p), ("seff", C.c_void_p)]
 
 
 
@@ -116,6 +116,7 @@ _SIGS = {
     "xpg_pass2_frames": ([P(Detector), P(Call), P(Refl), P(Frames), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                           C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_prepare": ([P(Detector), P(Call), P(Refl), P(Frames), P(Prepared), C.c_void_p], C.c_int),
+    "xpg_prepare_sb": ([P(Detector), P(Call), P(Refl), C.c_void_p, P(Prepared), C.c_void_p], C.c_int),
     "xpg_pass1_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), P(Pass1Out), P(ProfileAcc), C.c_void_p],
                            C.c_int),
     "xpg_pass2_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,

@@ -60,6 +60,25 @@ def _status_error(st: np.ndarray, frames) -> ValueError:
     return ValueError("not a decodable bitshuffle-LZ4 chunk: " + "; ".join(what) + more)
 
 
+def output_frames(shape, nel: int, n_chunks: int, slotted: bool):
+    """(elements between output frames, output frames) of a decode output of ``shape``: a 2-D or higher output holds
+    shape[0] frames of prod(shape[1:]) >= nel elements; a 1-D output one frame, of at least nel elements.  Raises
+    ValueError for an output that cannot hold the frames (checked before any launch: the kernel writes nel elements per
+    frame at slot * stride)."""
+    numel = int(np.prod(shape)) if len(shape) else 1
+    if len(shape) > 1:
+        stride, nslot = int(np.prod(shape[1:])), int(shape[0])
+    else:
+        stride, nslot = nel, 1
+    if stride < nel:
+        raise ValueError("output frames smaller than the chunks' frames")
+    if numel < (nslot - 1) * stride + nel:
+        raise ValueError(f"output of {numel} elements cannot hold a frame of {nel}")
+    if not slotted and n_chunks > nslot:
+        raise ValueError("more chunks than output frames")
+    return stride, nslot
+
+
 class ChunkBatch:
     """Compressed chunks of frames of ``nel`` elements of ``esize`` bytes, resident on the device with their block
     table.  ``index="host"`` walks each chunk's block chain on the host while it is in host memory (one pass over 4
@@ -105,6 +124,9 @@ class ChunkBatch:
         else:
             raise ValueError("index must be 'host' or 'device'")
         self._host = host          # the pinned staging buffer lives until the upload has completed
+        # decodes may run on another stream than the uploads above: they wait for this event first
+        self._uploaded = torch.cuda.Event()
+        self._uploaded.record(torch.cuda.current_stream(self.device))
 
     @property
     def compressed_bytes(self) -> int:
@@ -132,12 +154,7 @@ class ChunkBatch:
         k = {"raw": 0, "coded": 1}[kind]
         if out.dtype != (torch.float32 if k else torch.int32) or not out.is_contiguous():
             raise ValueError("output: contiguous int32 (raw uint32 counts) or float32 (coded) frames")
-        stride = out[0].numel() if out.dim() > 1 else self.nel
-        if stride < self.nel:
-            raise ValueError("output frames smaller than the chunks' frames")
-        nslot = out.shape[0] if out.dim() > 1 else 1
-        if slots is None and self.n > nslot:
-            raise ValueError("more chunks than output frames")
+        stride, nslot = output_frames(tuple(out.shape), self.nel, self.n, slots is not None)
         codes = None
         if k == 1:
             sb = None
@@ -158,6 +175,7 @@ class ChunkBatch:
                 raise ValueError("slot outside the output")
         self._slots = slots
         b = self._batch(out, k, stride, slots, codes)
+        torch.cuda.current_stream(self.device).wait_event(self._uploaded)
         L.check(L.lib().xpg_bslz4_decode(C.byref(b), L.stream_ptr(self.device)))
         return out
 

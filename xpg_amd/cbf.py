@@ -5,8 +5,11 @@ The reference decodes every CBF frame on the host (``decode_byte_offset``, xpg/i
 one batch of five kernels (include/xpg_cbf.h) rebuilds the int32 pixel values of every frame: ``read_cbf``'s ``data``
 (``decode_byte_offset(buf, n).astype(np.int32)``), written straight into device memory.
 
-Errors: a stream with fewer than n elements raises ``ValueError`` naming the frames (the reference raises
-``IndexError`` from the same condition).
+Errors: a stream with fewer than n elements raises ``ValueError`` naming the frames.  The reference raises
+``IndexError`` when a stream runs out before its n-th element or inside an int16 escape; when the stream ends inside
+the int32 or int64 payload of its n-th element, the reference reads the bytes that are there as a shorter signed
+integer and returns a value.  Here that case is an error too (deliberately stricter: such a stream is cut, and the
+value it would give is not the pixel's).
 """
 from __future__ import annotations
 
@@ -50,7 +53,7 @@ class StreamBatch:
         self.n_chunks, self.n_groups = int(chunk0[-1]), int(group0[-1])
         dev = self.device
         self.src = host.to(dev, non_blocking=True)
-        self._host = host
+        self._host = host          # the pinned staging buffer lives until the upload has completed
         self.stream_off = torch.as_tensor(offs, device=dev)
         self.stream_len = torch.as_tensor(lens, device=dev)
         self.n_elem_d = torch.as_tensor(self.n_elem, device=dev)
@@ -60,6 +63,9 @@ class StreamBatch:
         self.work = torch.empty(max(wb, 256), dtype=torch.uint8, device=dev)
         self.status = torch.zeros(self.n, dtype=torch.int32, device=dev)
         self.stream_bytes = total
+        # decodes may run on another stream than the uploads above: they wait for this event first
+        self._uploaded = torch.cuda.Event()
+        self._uploaded.record(torch.cuda.current_stream(dev))
 
     def decode_into(self, out: torch.Tensor, out_off=None):
         """Decode every frame into ``out`` (contiguous int32): frame i at element ``out_off[i]`` (default: frames back
@@ -78,6 +84,7 @@ class StreamBatch:
         b.n_elem, b.chunk0, b.group0 = L.ptr(self.n_elem_d), L.ptr(self.chunk0), L.ptr(self.group0)
         b.n_frames, b.n_chunks, b.n_groups = self.n, self.n_chunks, self.n_groups
         b.work, b.out, b.out_off, b.status = L.ptr(self.work), L.ptr(out), L.ptr(self._out_off), L.ptr(self.status)
+        torch.cuda.current_stream(self.device).wait_event(self._uploaded)
         L.check(L.lib().xpg_cbf_decode(C.byref(b), L.stream_ptr(self.device)))
         return out
 

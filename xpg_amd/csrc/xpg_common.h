@@ -213,7 +213,7 @@ __device__ void pixel_S(const xpg_detector &d, double x, double y, double *S) {
     long ix = (long)floor(x - 0.5), iy = (long)floor(y - 0.5);
     ix = ix < 0 ? 0 : (ix > d.seg_nx - 1 ? d.seg_nx - 1 : ix);
     iy = iy < 0 ? 0 : (iy > d.seg_ny - 1 ? d.seg_ny - 1 : iy);
-    k = max((int)d.segmap[iy * d.seg_nx + ix], 0);
+    k = min(max((int)d.segmap[iy * d.seg_nx + ix], 0), d.n_seg - 1);   // gaps (-1) use segment 0
   }
   const double *P0 = d.P0 + 3 * k, *Bx = d.Bx + 3 * k, *By = d.By + 3 * k;
   double xc = x, yc = y;
@@ -412,6 +412,16 @@ __device__ double box_u(int Dy, int Dx, double sd, double *U1, double *U2, const
   double psz = sqrt(Jd[(Dy / 2) * Dx + Dx / 2]);
   double t = psz / sd;
   double seff = sd * sqrt(1.0 + (t * t) / 12.0);
+  for (int p = lane; p < P; p += WAVE) {
+    U1[p] = U1[p] / seff;
+    U2[p] = U2[p] / seff;
+  }
+  wave_sync();
+  return seff;
+}
+
+// u = eps / seff in place for a given seff (pass 1's effective width of the row); returns seff
+__device__ __forceinline__ double box_u_given(int P, double seff, double *U1, double *U2, int lane) {
   for (int p = lane; p < P; p += WAVE) {
     U1[p] = U1[p] / seff;
     U2[p] = U2[p] / seff;

@@ -451,8 +451,9 @@ __host__ __device__ static inline PrepLayout prep_layout(int Dz, int Dy, int Dx,
   return L;
 }
 
+// sb: the shoeboxes from a chunk buffer instead of the frame store (NULL: from F)
 __global__ void __launch_bounds__(256) k_prepare(xpg_detector det, xpg_call call, xpg_refl R, xpg_frames F,
-                                                 xpg_prepared pp, int wave_bytes, int big) {
+                                                 const float *sb, xpg_prepared pp, int wave_bytes, int big) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x & (WAVE - 1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);   // wave-uniform: row data in SGPRs
@@ -468,7 +469,7 @@ __global__ void __launch_bounds__(256) k_prepare(xpg_detector det, xpg_call call
   double *U1 = (double *)(base + L.U1), *U2 = (double *)(base + L.U2), *Jd = (double *)(base + L.Jd);
   double *co = (double *)(base + L.co), *tab = (double *)(base + L.tab);
   const int P = b.P, V = b.V, Dz = b.Dz, Dy = b.Dy, Dx = b.Dx;
-  stage_values(b, nullptr, F, vals, vflag, prow, base + L.tmp, base + L.ftmp, lane);
+  stage_values(b, sb, F, vals, vflag, prow, base + L.tmp, base + L.ftmp, lane);
   Frame f;
   kabsch(R.S + 3 * (int64_t)r, call.s0, call.axis, f);
   box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
@@ -590,7 +591,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
   }
   rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
-  const double seff = box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+  const double seff = (PREP && a.prep.seff) ? box_u_given(P, uni_d(a.prep.seff[r]), U1, U2, lane)
+                                            : box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   wave_sync();
   for (int p = lane; p < P; p += WAVE) {
@@ -721,7 +723,9 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   wsum_multi(s2);
   double Sc = s2[0], nbg = s2[1];
   double bk = Sc / fmax(nbg, 1e-6);
+  double bk_thr = bk;                   // the level whose thresholds made the final background set
   for (int round = 1; round < 5 && regs; ++round) {
+    bk_thr = bk;
     double sc = 0.0, sT = 0.0;
 #pragma unroll
     for (int q = 0; q < NPL; ++q) {
@@ -750,6 +754,7 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     if (bk == bk_prev) break;
   }
   for (int round = 1; round < 5 && !regs; ++round) {
+    bk_thr = bk;
     for (int p = lane; p < P; p += WAVE) {
       const double bp = bk * (double)bref[p];
       const double t = (bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0;
@@ -791,6 +796,40 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     const double bk_prev = bk;
     bk = Sc / fmax(nbg, 1e-6);
     if (bk == bk_prev) break;           // the next threshold equals this one: uniform over the wavefront
+  }
+
+  // ---- per-voxel classes for mask comparisons (only when asked for) ----
+  if (a.out.vmask) {
+    uint8_t *vm = a.out.vmask + b.voff;
+    if (regs) {
+#pragma unroll
+      for (int q = 0; q < NPL; ++q) {
+        const int p = lane + q * WAVE;
+        if (p >= P) continue;
+        const double bp = bk_thr * (double)brq[q];
+        const float th = __double2float_rd((bp + 3.0 * sqrt(call.gain * fmax(bp, 0.5))) + 1.0);
+#pragma unroll
+        for (int j = 0; j < DZM; ++j) {
+          const int k = q * DZM + j;
+          if (j >= Dz) continue;
+          const bool bgm = (bgmm >> k) & 1u;
+          vm[j * P + p] = (((fgm >> k) & 1u) ? XPG_VM_FG : 0) | (bgm ? XPG_VM_BGM : 0) |
+                          (((nbmm >> k) & 1u) ? XPG_VM_NBM : 0) | ((bgm && cv[k] <= th) ? XPG_VM_BGW : 0) |
+                          (((fgvm >> k) & 1u) ? XPG_VM_FGV : 0);
+        }
+      }
+    } else {
+      // thr[] holds the thresholds of the last clipping round (those of bk_thr)
+      for (int p = lane; p < P; p += WAVE) {
+        const float th = thr[p];
+        for (int j = 0; j < Dz; ++j) {
+          const uint8_t fl = vflag[j * P + p];
+          const bool bgm = fl & VF_BGM;
+          vm[j * P + p] = ((fl & VF_FG) ? XPG_VM_FG : 0) | (bgm ? XPG_VM_BGM : 0) | ((fl & VF_NBM) ? XPG_VM_NBM : 0) |
+                          ((bgm && vals[j * P + p] <= th) ? XPG_VM_BGW : 0) | ((fl & VF_FGV) ? XPG_VM_FGV : 0);
+        }
+      }
+    }
   }
 
   // ---- sums over the foreground ----
@@ -1245,7 +1284,8 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
   }
   rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
-  const double seff = box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
+  const double seff = (PREP && a.prep.seff) ? box_u_given(P, uni_d(a.prep.seff[r]), U1, U2, lane)
+                                            : box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
   profile_pixels(a.prof, b.region, U1, U2, Jd, seff, P, p2, lane);
   wave_sync();
@@ -1404,6 +1444,17 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       }
     }
     wave_sync();
+  }
+  if (a.out.vmask) {                      // per-voxel classes for mask comparisons (only when asked for)
+    uint8_t *vm = a.out.vmask + b.voff;
+    for (int p = lane; p < P; p += WAVE) {
+      const int q = (p - lane) / WAVE;
+      for (int j = 0; j < Dz; ++j) {
+        const uint8_t fl = vflag[j * P + p];
+        const bool use = regs ? ((usem >> (q * DZM + j)) & 1u) != 0 : (fl & VF_USE) != 0;
+        vm[j * P + p] = (use ? XPG_VM_USE : 0) | ((fl & VF_NBM) ? XPG_VM_NBM : 0);
+      }
+    }
   }
   double s4[4] = {s_c, s_b, s_pw, 0.0};
   wsum_multi(s4);
@@ -1772,13 +1823,15 @@ static int check_prep(const xpg_prepared *prep) {
   return XPG_OK;
 }
 
-int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
-                const xpg_prepared *prep, void *stream) {
+static int prepare_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                        const float *sb, const xpg_prepared *prep, void *stream) {
   int rc = check_refl(refl, false);
   if (rc) return rc;
   if (!det || !call) return fail(XPG_ERR_ARG, "det / call is NULL");
-  rc = check_frames(frames);
-  if (rc) return rc;
+  if (!sb) {
+    rc = check_frames(frames);
+    if (rc) return rc;
+  }
   rc = check_prep(prep);
   if (rc) return rc;
   if (refl->n == 0) return XPG_OK;
@@ -1795,9 +1848,20 @@ int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *r
   rc = set_lds(k_prepare, (size_t)w * wb);
   if (rc) return rc;
   hipLaunchKernelGGL(k_prepare, dim3((refl->n + w - 1) / w), dim3(64 * w), (size_t)w * wb, (hipStream_t)stream, *det,
-                     *call, *refl, *frames, *prep, (int)wb, big);
+                     *call, *refl, sb ? kNoFrames : *frames, sb, *prep, (int)wb, big);
   HIP_TRY(hipGetLastError());
   return XPG_OK;
+}
+
+int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_frames *frames,
+                const xpg_prepared *prep, void *stream) {
+  return prepare_impl(det, call, refl, frames, nullptr, prep, stream);
+}
+
+int xpg_prepare_sb(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
+                   const xpg_prepared *prep, void *stream) {
+  if (!sb) return fail(XPG_ERR_ARG, "sb is NULL");
+  return prepare_impl(det, call, refl, nullptr, sb, prep, stream);
 }
 
 static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,

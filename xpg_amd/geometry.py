@@ -3,9 +3,10 @@
 Follows the conventions of xpg/geometry.py:1-14 (Kabsch 1988/2010): right-handed lab frame, S0 = beam / lambda,
 pixel (x, y) continuous with array element ix centred at x = ix + 1 (PIX0), lab position
 X = QX (x - ORGX) ED1 + QY (y - ORGY) ED2 + F ED3, frame i covering z in [i - 1, i), phi(z) = phi0 + osc (z - (first - 1)),
-A* columns = a*, b*, c* at phi = 0.  Only what the integrator reads is provided (the Geometry fields of xpg/geometry.py:129-407 it uses): a flat single panel with
-optional sensor parallax; the integration kernels themselves also accept segmented detectors through
-``detector_consts``.
+A* columns = a*, b*, c* at phi = 0.  Only what the integrator reads is provided (the Geometry fields of
+xpg/geometry.py:129-407 it uses): one flat panel or several segments (the SEGMENT= model of xpg/detector.py:1-19),
+optional geometric-correction tables (X-/Y-GEO_CORR, pixel offsets added to raw coordinates) and sensor parallax.
+``detector_consts`` hands all of it to the kernels (per-segment affine maps, segment map, correction tables).
 """
 from __future__ import annotations
 
@@ -48,6 +49,48 @@ def random_rotation(rng):
     return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+@dataclass
+class Segment:
+    """One detector segment (module) of the SEGMENT= model (xpg/detector.py:36-80): raw pixels x1..x2, y1..y2
+    (1-based, inclusive) map to detector-system points qx (x - orgx) ex + qy (y - orgy) ey + distance (ex x ey), with
+    ex, ey the segment's axes in the detector system (ey made orthogonal to ex)."""
+    x1: int
+    x2: int
+    y1: int
+    y2: int
+    orgx: float = 0.0
+    orgy: float = 0.0
+    distance: float = 0.0
+    ex: np.ndarray = field(default_factory=lambda: np.array([1.0, 0.0, 0.0]))
+    ey: np.ndarray = field(default_factory=lambda: np.array([0.0, 1.0, 0.0]))
+
+    def axes(self, qx, qy):
+        """(offset c, unit ex, unit ey) in the detector system: a raw pixel (x, y) is at qx x ex + qy y ey + c."""
+        ex = unit(self.ex)
+        ey = np.asarray(self.ey, float)
+        ey = unit(ey - (ey @ ex) * ex)
+        ez = np.cross(ex, ey)
+        return -qx * self.orgx * ex - qy * self.orgy * ey + self.distance * ez, ex, ey
+
+    @property
+    def rect(self):
+        return (self.x1, self.x2, self.y1, self.y2)
+
+
+def bilinear_table(tab, x, y):
+    """Bilinear interpolation of a per-pixel table whose element ix sits at x = ix + 1 (coordinates clamped to the
+    table, xpg/detector.py:141-153)."""
+    ny, nx = tab.shape
+    fx = np.clip(np.asarray(x, float) - 1.0, 0, nx - 1.000001)
+    fy = np.clip(np.asarray(y, float) - 1.0, 0, ny - 1.000001)
+    ix, iy = np.floor(fx).astype(np.int64), np.floor(fy).astype(np.int64)
+    ix1, iy1 = np.minimum(ix + 1, nx - 1), np.minimum(iy + 1, ny - 1)
+    tx, ty = fx - ix, fy - iy
+    t = np.asarray(tab, float)
+    return ((t[iy, ix] * (1 - tx)) * (1 - ty) + (t[iy, ix1] * tx) * (1 - ty)) + (t[iy1, ix] * (1 - tx)) * ty + \
+        (t[iy1, ix1] * tx) * ty
 
 
 @dataclass
@@ -96,20 +139,61 @@ class Geometry:
     def phi_to_z(self, phi):
         return (np.asarray(phi) - self.phi0) / self.osc + (self.starting_frame - 1)
 
-    def panels(self):
-        """Affine lab maps (P0, Bx, By), each (1, 3), of the single panel: X = P0 + (x - orgx) Bx + (y - orgy) By."""
-        e1, e2 = unit(self.ed1), unit(self.ed2)
-        return (self.distance * np.cross(e1, e2))[None], (self.qx * e1)[None], (self.qy * e2)[None]
+    def segment_list(self):
+        return list(self.segments) if self.segments else [Segment(1, max(self.nx, 1), 1, max(self.ny, 1))]
 
-    def _parallax(self, e):
-        """Shift (panel axes, mm) of the absorption-weighted mean interaction depth along unit rays e (sensor of
+    def panels(self):
+        """Affine lab maps (P0, Bx, By), each (K, 3), one per segment: X = P0 + (x - orgx) Bx + (y - orgy) By.  The
+        detector system's axes are ED1, ED2 and ED3 = ED1 x ED2, its origin -qx orgx ED1 - qy orgy ED2 + F ED3 from
+        the crystal (xpg/detector.py:84-95)."""
+        E = np.stack([unit(self.ed1), unit(self.ed2)])
+        E = np.vstack([E, np.cross(E[0], E[1])])
+        origin = np.array([-self.qx * self.orgx, -self.qy * self.orgy, self.distance])
+        P0, Bx, By = [], [], []
+        for seg in self.segment_list():
+            c, ex, ey = seg.axes(self.qx, self.qy)
+            bx, by = self.qx * (ex @ E), self.qy * (ey @ E)
+            P0.append((c + origin) @ E + self.orgx * bx + self.orgy * by)
+            Bx.append(bx)
+            By.append(by)
+        return np.array(P0), np.array(Bx), np.array(By)
+
+    def segmap(self):
+        """(ny, nx) int16 segment index of every raw pixel, -1 in the gaps between segments (None: one panel)."""
+        if not self.segmented:
+            return None
+        m = np.full((self.ny, self.nx), -1, np.int16)
+        for k, seg in enumerate(self.segments):
+            m[max(seg.y1 - 1, 0):min(seg.y2, self.ny), max(seg.x1 - 1, 0):min(seg.x2, self.nx)] = k
+        return m
+
+    def segment_of(self, x, y):
+        """Segment of raw continuous coordinates (the pixel containing them), -1 in gaps; 0 for one panel."""
+        x = np.asarray(x, float)
+        if not self.segmented:
+            return np.zeros(x.shape, np.int64)
+        m = self.segmap()
+        ix = np.clip(np.floor(x - 0.5).astype(np.int64), 0, m.shape[1] - 1)
+        iy = np.clip(np.floor(np.asarray(y, float) - 0.5).astype(np.int64), 0, m.shape[0] - 1)
+        return m[iy, ix].astype(np.int64)
+
+    def corrected_xy(self, x, y):
+        """Raw coordinates with the geometric-correction offsets added (unchanged without tables)."""
+        x, y = np.asarray(x, float), np.asarray(y, float)
+        if self.geo_corr is None:
+            return x, y
+        return x + bilinear_table(self.geo_corr[0], x, y), y + bilinear_table(self.geo_corr[1], x, y)
+
+    def _parallax(self, e, u1=None, u2=None, un=None):
+        """Shift (panel axes u1, u2, mm) of the absorption-weighted mean interaction depth along unit rays e (sensor of
         thickness t, attenuation mu): lbar = 1/mu - L e^{-mu L} / (1 - e^{-mu L}), L = t / cos(incidence)."""
-        un = self.ed3
-        cosa = np.clip(e @ un, 1e-3, 1.0)
+        if u1 is None:
+            u1, u2, un = unit(self.ed1), unit(self.ed2), self.ed3
+        cosa = np.clip(np.sum(e * un, axis=-1), 1e-3, 1.0)
         L = self.sensor_thickness / cosa
         muL = self.mu * L
         lbar = 1.0 / self.mu - L * np.exp(-muL) / np.maximum(-np.expm1(-muL), 1e-12)
-        return lbar * (e @ unit(self.ed1)), lbar * (e @ unit(self.ed2))
+        return lbar * np.sum(e * u1, axis=-1), lbar * np.sum(e * u2, axis=-1)
 
     def _plane(self, x, y):
         x, y = np.asarray(x, float), np.asarray(y, float)
@@ -117,14 +201,27 @@ class Geometry:
                 * unit(self.ed2) + self.distance * self.ed3)
 
     def pixel_to_lab(self, x, y):
-        if self.segmented or self.geo_corr is not None:
-            raise NotImplementedError("host geometry supports a single flat panel")
-        X = self._plane(x, y)
+        """Lab position (mm) where the ray recorded at raw pixel (x, y) enters the sensor: segment map, geometric
+        correction, then four parallax iterations (xpg/geometry.py:246-262)."""
+        x, y = np.asarray(x, float), np.asarray(y, float)
+        if not self.segmented and self.geo_corr is None:
+            X = self._plane(x, y)
+            if self.has_parallax:
+                for _ in range(4):
+                    d1, d2 = self._parallax(X / np.linalg.norm(X, axis=-1, keepdims=True))
+                    X = self._plane(x - d1 / self.qx, y - d2 / self.qy)
+            return X
+        k = np.maximum(self.segment_of(x, y), 0)
+        xc, yc = self.corrected_xy(x, y)
+        P0, Bx, By = (a[k] for a in self.panels())
+        lab = lambda a, b: P0 + (a - self.orgx)[..., None] * Bx + (b - self.orgy)[..., None] * By
+        X = lab(xc, yc)
         if self.has_parallax:
-            x, y = np.asarray(x, float), np.asarray(y, float)
+            u1, u2 = Bx / self.qx, By / self.qy
+            un = np.cross(u1, u2)
             for _ in range(4):
-                d1, d2 = self._parallax(X / np.linalg.norm(X, axis=-1, keepdims=True))
-                X = self._plane(x - d1 / self.qx, y - d2 / self.qy)
+                d1, d2 = self._parallax(X / np.linalg.norm(X, axis=-1, keepdims=True), u1, u2, un)
+                X = lab(xc - d1 / self.qx, yc - d2 / self.qy)
         return X
 
     def pixel_to_s(self, x, y):
@@ -132,20 +229,49 @@ class Geometry:
         return X / (np.linalg.norm(X, axis=-1, keepdims=True) * self.wavelength)
 
     def s_to_pixel(self, S):
-        """Pixel coordinates where rays S hit the panel (nan when they miss the plane)."""
+        """Raw pixel coordinates where rays S are recorded (nan when they miss the plane, or every segment): the
+        intersection with each segment's plane, the parallax shift, the inverse geometric correction (fixed-point
+        iteration), the first segment whose pixels contain the point (xpg/geometry.py:268-304)."""
         S = np.asarray(S, float)
-        n = self.ed3
-        sn = S @ n
-        with np.errstate(divide="ignore", invalid="ignore"):
-            t = self.distance / sn
-        P = t[..., None] * S - self.distance * n
-        x = self.orgx + (P @ unit(self.ed1)) / self.qx
-        y = self.orgy + (P @ unit(self.ed2)) / self.qy
-        if self.has_parallax:
-            d1, d2 = self._parallax(S / np.linalg.norm(S, axis=-1, keepdims=True))
-            x, y = x + d1 / self.qx, y + d2 / self.qy
-        ok = np.isfinite(t) & (t > 0)
-        return np.where(ok, x, np.nan), np.where(ok, y, np.nan)
+        if not self.segmented and self.geo_corr is None:
+            n = self.ed3
+            sn = S @ n
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = self.distance / sn
+            P = t[..., None] * S - self.distance * n
+            x = self.orgx + (P @ unit(self.ed1)) / self.qx
+            y = self.orgy + (P @ unit(self.ed2)) / self.qy
+            if self.has_parallax:
+                d1, d2 = self._parallax(S / np.linalg.norm(S, axis=-1, keepdims=True))
+                x, y = x + d1 / self.qx, y + d2 / self.qy
+            ok = np.isfinite(t) & (t > 0)
+            return np.where(ok, x, np.nan), np.where(ok, y, np.nan)
+        P0, Bx, By = self.panels()
+        x = np.full(S.shape[:-1], np.nan)
+        y = np.full(S.shape[:-1], np.nan)
+        for k, seg in enumerate(self.segment_list()):
+            nrm = np.cross(Bx[k], By[k])
+            with np.errstate(divide="ignore", invalid="ignore"):
+                t = (P0[k] @ nrm) / (S @ nrm)
+            P = t[..., None] * S - P0[k]
+            xk = self.orgx + (P @ Bx[k]) / (Bx[k] @ Bx[k])
+            yk = self.orgy + (P @ By[k]) / (By[k] @ By[k])
+            if self.has_parallax:
+                u1, u2 = Bx[k] / self.qx, By[k] / self.qy
+                with np.errstate(invalid="ignore"):
+                    d1, d2 = self._parallax(S / np.linalg.norm(S, axis=-1, keepdims=True), u1, u2, np.cross(u1, u2))
+                xk, yk = xk + d1 / self.qx, yk + d2 / self.qy
+            if self.geo_corr is not None:              # raw (x, y) whose corrected position is (xk, yk)
+                xr, yr = xk, yk
+                for _ in range(4):
+                    xr = xk - bilinear_table(self.geo_corr[0], xr, yr)
+                    yr = yk - bilinear_table(self.geo_corr[1], xr, yr)
+                xk, yk = xr, yr
+            ok = np.isfinite(t) & (t > 0) & np.isnan(x)
+            if self.segmented:
+                ok &= (xk >= seg.x1 - 0.5) & (xk < seg.x2 + 0.5) & (yk >= seg.y1 - 0.5) & (yk < seg.y2 + 0.5)
+            x, y = np.where(ok, xk, x), np.where(ok, yk, y)
+        return x, y
 
     def lp_factor(self, S):
         """(Lorentz zeta, polarisation P) of diffracted beams S."""
@@ -175,12 +301,16 @@ class Experiment:
 
 
 def detector_consts(g: Geometry, bkg_template=None):
-    """Detector constants for the kernels (float64 host arrays), the fields of xpg/integrate_kernels.py:48-65."""
+    """Detector constants for the kernels (float64 host arrays), the fields of xpg/integrate_kernels.py:48-65: one
+    affine map per segment, the segment map of a segmented detector and the geometric-correction tables."""
     P0, Bx, By = g.panels()
     U1, U2 = Bx / g.qx, By / g.qy
+    gx = gy = None
+    if g.geo_corr is not None:
+        gx, gy = (np.asarray(t, np.float32) for t in g.geo_corr)
     return dict(orgx=g.orgx, orgy=g.orgy, P0=P0, Bx=Bx, By=By, U1=U1, U2=U2, UN=np.cross(U1, U2), qx=g.qx, qy=g.qy,
                 wl=g.wavelength, tsens=float(max(g.sensor_thickness, 1e-6)), mu=float(max(g.mu, 1e-6)),
-                plx=float(g.has_parallax), segmap=None, gx=None, gy=None,
+                plx=float(g.has_parallax), segmap=g.segmap(), gx=gx, gy=gy,
                 bkg=None if bkg_template is None else np.asarray(bkg_template, np.float32))
 
 

@@ -1048,144 +1048,21 @@ class _Null:
         return False
 
 
-# ---------------------------------------------------------------------------------------------------------------------
-# host fits of estimate_sigmas (the per-block adaptive fits run on the device: xpg_adapt_block)
-# ---------------------------------------------------------------------------------------------------------------------
-
-def _nnls_gram(A, y):
-    """argmin ||A w - y||, w >= 0, through the Cholesky factor of A^T A (xpg/integrate.py:1095-1104)."""
-    from scipy.linalg import LinAlgError, cholesky, solve_triangular
-    from scipy.optimize import nnls
-    try:
-        R = cholesky(A.T @ A)
-        return nnls(R, solve_triangular(R.T, A.T @ y, lower=True))[0]
-    except (LinAlgError, ValueError):
-        return nnls(A, y)[0]
-
-
-def _rocking_resolved(r, max_edge=0.5):
-    """Reflections whose net counts show a rocking curve within the box (xpg/integrate.py:1107-1127)."""
-    fz = r.get("fz")
-    if fz is None:
-        return np.ones(len(r["x"]), bool)
-    n = len(fz)
-    dz = np.minimum(np.asarray(r["dz"], int), fz.shape[1])
-    rows = np.arange(n)
-    c = np.clip(np.floor((r["z"] - r["iz0"]) / np.maximum(r["fb"], 1)).astype(int), 0, np.maximum(dz - 1, 0))
-    centre = fz[rows, c]
-    k = np.arange(fz.shape[1])[None]
-    rec = (fz != 0) & (k < dz[:, None])
-    first = np.where(rec.any(1), np.argmax(rec, 1), 0)
-    last = np.where(rec.any(1), fz.shape[1] - 1 - np.argmax(rec[:, ::-1], 1), 0)
-    edge = 0.5 * (fz[rows, first] + fz[rows, last])
-    return (dz < 3) | ((centre > 0) & (edge < max_edge * centre) & (last - first >= 2))
-
-
-def _fit_rocking_width(sel_fit, g: Geometry, sm_guess, lo_fac=0.2, hi_fac=3.0):
-    """Grid search (61 points in log sigma_M) of the frame-integrated Gaussian rocking curve against the per-frame
-    fractions of strong reflections (xpg/integrate.py:1130-1154)."""
-    from scipy.special import erf
-    fz = np.concatenate([s[0] for s in sel_fit])
-    iz0 = np.concatenate([s[1] for s in sel_fit])
-    phi = np.concatenate([s[2] for s in sel_fit])
-    zeta = np.abs(np.concatenate([s[3] for s in sel_fit]))
-    tot = fz.sum(1, keepdims=True)
-    ok = tot[:, 0] > 0
-    fz, iz0, phi, zeta, tot = fz[ok], iz0[ok], phi[ok], zeta[ok], tot[ok]
-    frac = fz / tot
-    k = iz0[:, None] + np.arange(fz.shape[1])[None]
-    lo = g.z_to_phi(k.astype(float))
-    hi = g.z_to_phi(k + 1.0)
-    lo, hi = np.minimum(lo, hi), np.maximum(lo, hi)
-    best = (np.inf, sm_guess)
-    for smv in sm_guess * np.exp(np.linspace(np.log(lo_fac), np.log(hi_fac), 61)):
-        s = smv / zeta[:, None] * np.sqrt(2)
-        R = 0.5 * (erf((hi - phi[:, None]) / s) - erf((lo - phi[:, None]) / s))
-        R = R / np.maximum(R.sum(1, keepdims=True), 1e-9)
-        cost = np.sum(tot * (frac - R) ** 2)
-        if cost < best[0]:
-            best = (cost, smv)
-    return float(best[1])
-
-
 def _estimate_sigmas(images, exp, model0: IntegrationModel, dmin, frames, n_sample_frames=30, verbose=True, log=print,
                      extra_exps=(), bkg_template=None, foreign_spots=None, max_iter=None, guard=None):
-    """sigma_D / sigma_M from truncation-corrected second moments of strong spots in five phi ranges, iterated
-    (xpg/integrate.py:1002-1092)."""
+    """sigma_D / sigma_M from strong spots of five sampled phi ranges, iterated (xpg/integrate.py:1002-1092): each
+    range runs as a moments-only device sweep; the selection and fits are in xpg_amd/widths.py."""
+    from . import widths as W
     g = exp.geom
     f0, f1 = frames
-    nf = f1 - f0 + 1
-    nr = 5
-    span = max(int(round(n_sample_frames / 3)), int(np.ceil(5.0 / max(abs(g.osc), 1e-6))), 3)
-    span = min(span, max(nf // nr, 3))
-    ranges = []
-    for i in range(nr):
-        a = f0 + int(i * (nf - span) / max(nr - 1, 1))
-        ranges.append((a, min(a + span - 1, f1)))
-    model = prev = model0
-    n_iter = max_iter or 3
-    history = []
-    for it in range(n_iter):
-        sel_md, sel_mm, sel_fit, per_range = [], [], [], []
-        for a, b in ranges:
-            edges = np.array([a - 1.0, float(b)])
-            res = integrate(images, [exp], edges, model, dmin, frames=(max(f0, a - 5), min(f1, b + 5)), verbose=False,
-                            learn_profiles=False, adaptive=False, bkg_template=bkg_template,
-                            extra_lattices=[[e] for e in extra_exps] or None, foreign_spots=foreign_spots,
-                            moments_only=True)
-            r = res.refl
-            z_ok = (r["z"] >= a - 1) & (r["z"] < b) & (r.get("lattice", 0) == 0) & ~r["overlap"]
-            isg = r["isum"] / np.sqrt(np.maximum(r["vsum"], 1e-6))
-            base = z_ok & np.isfinite(r["isum"]) & (r["isum"] > 0) & (r["n_over"] == 0) & (r["frac_rec"] > 0.98)
-            base &= _rocking_resolved(r)
-            per_range.append((r, base, isg))
-        sels = []
-        for r, base, isg in per_range:
-            strong = base & (isg > 20)
-            if strong.sum() < 150:
-                strong = base & (isg > 10)
-            sels.append(strong if strong.sum() >= 20 else None)
-        if all(x is None for x in sels) and sum(int((bs & (ig > 5.0)).sum()) for _, bs, ig in per_range) >= 30:
-            sels = [bs & (ig > 5.0) for _, bs, ig in per_range]
-        for (r, base, isg), strong in zip(per_range, sels):
-            if strong is None or not strong.any():
-                continue
-            Xv = g.pixel_to_lab(r["x"][strong], r["y"][strong])
-            Xlen = np.linalg.norm(Xv, axis=1)
-            cosa = np.abs(Xv @ g.ed3) / Xlen
-            pix = (g.qx / Xlen) ** 2 * (1 + cosa ** 2) / 24.0
-            if "pxs" in r:
-                pix = pix * r["pxs"][strong] ** 2
-            sel_md.append(np.c_[r["m_d"][strong], pix])
-            fbs = r["fb"][strong] if "fb" in r else 1
-            sel_mm.append(r["m_m"][strong] - (np.radians(g.osc) * fbs * r["zeta_k"][strong]) ** 2 / 12.0)
-            one = strong & (r["fb"] == 1) if "fb" in r else strong
-            sel_fit.append((r["fz"][one], r["iz0"][one], r["phi"][one], r["zeta_k"][one]))
-        if not sel_md:
-            break
-        md2 = np.concatenate(sel_md)
-        mm = np.concatenate(sel_mm)
-        t2 = 1.0 - (FG_R ** 2 / 2) * np.exp(-FG_R ** 2 / 2) / (1 - np.exp(-FG_R ** 2 / 2))
-        md = md2[:, 0] / t2 - md2[:, 1]
-        sd_new = np.degrees(np.sqrt(max(np.median(md), 1e-12)))
-        sm_new = _fit_rocking_width(sel_fit, g, max(np.degrees(np.sqrt(max(np.median(mm), 1e-12))),
-                                                    0.5 * model.sigma_m))
-        new = IntegrationModel(sigma_d=float(np.clip(sd_new, 0.005, 1.0)), sigma_m=float(np.clip(sm_new, 0.005, 5.0)),
-                               gain=model.gain)
-        history.append((float(new.sigma_d), float(new.sigma_m), int(md.size)))
-        if verbose:
-            log(f"  INTEGRATE: estimated sigma_D {new.sigma_d:.4f} deg, sigma_M {new.sigma_m:.4f} deg "
-                f"from {md.size} strong spots (iteration {it + 1})")
-        if (guard is None or guard) and it >= 1 and (new.sigma_d > 1.15 * model.sigma_d or
-                                                     new.sigma_m > 1.15 * model.sigma_m) and \
-                (model.sigma_d > 1.15 * prev.sigma_d or model.sigma_m > 1.15 * prev.sigma_m):
-            if verbose:
-                log("  INTEGRATE: spot-width estimate keeps growing with the box (contamination): keeping iteration "
-                    f"{it} values")
-            break
-        prev, model = model, new
-        if max_iter and it >= 2 and abs(model.sigma_d / prev.sigma_d - 1) < 0.03 and \
-                abs(model.sigma_m / prev.sigma_m - 1) < 0.03:
-            break
-    model.history = history
-    return model
+
+    def run_range(model, a, b):
+        res = integrate(images, [exp], np.array([a - 1.0, float(b)]), model, dmin,
+                        frames=(max(f0, a - 5), min(f1, b + 5)), verbose=False, learn_profiles=False, adaptive=False,
+                        bkg_template=bkg_template, extra_lattices=[[e] for e in extra_exps] or None,
+                        foreign_spots=foreign_spots, moments_only=True)
+        return res.refl
+
+    return W.estimate(run_range, g, model0, lambda sd, sm: IntegrationModel(sigma_d=sd, sigma_m=sm, gain=model0.gain),
+                      W.sample_ranges(f0, f1, g.osc, n_sample_frames), max_iter=max_iter, guard=guard,
+                      log=log if verbose else None)

@@ -4,12 +4,15 @@
 signatures (xpg/integrate_kernels.py:216-485); each runs one HIP kernel on the current stream and returns device
 tensors.  Constants and host helpers (rocking-curve basis, Gaussian prior) keep the reference's names and values.
 
+Arguments are taken as the reference's ``process_block`` passes them (xpg/integrate.py:466-519): device tensors
+(float32 ``S``/``phi``/``fb``/``pxs``, integer ``ix0``/``iy0``/``iz0``/``region``, the ``_detector_consts`` dict of
+device tensors and Python scalars, ``axis``/``s0``/``rcw`` device tensors, numpy float32 scalars) or numpy arrays.
+
 Differences from the reference that a caller can observe, all deliberate:
   * geometry is evaluated in float64 and voxel sums are compensated, so results match the float64 restatement of
     the algorithm to ~1e-7 instead of carrying float32 cancellation errors (|dI|/sigma up to ~3e-3 in the reference);
   * pass2's IRLS stops per reflection (results do not depend on which reflections share a call); the reference's
     chunk-global stop (xpg/integrate_kernels.py:464-467) differs from it by < 1e-3 sigma;
-  * pass2 recomputes the per-pixel geometry instead of reading pass1's ``pix`` (same values: same sigma_D);
   * per-reflection outputs are float64.
 """
 from __future__ import annotations
@@ -83,13 +86,29 @@ def _dev(x, dtype, device):
     return torch.as_tensor(np.ascontiguousarray(np.asarray(x)), dtype=dtype, device=device)
 
 
+def _host64(x):
+    """float64 host copy of an array, a device or host tensor, or a scalar."""
+    if torch.is_tensor(x):
+        return x.detach().to("cpu", torch.float64).numpy()
+    return np.asarray(x, np.float64)
+
+
+def _shape(x):
+    return tuple(x.shape) if hasattr(x, "shape") else tuple(np.shape(x))
+
+
+def _table(x):
+    """A per-pixel detector table, or None for an absent one (None or the reference's (1, 1) placeholder)."""
+    return None if x is None or _shape(x) == (1, 1) else x
+
+
 class DeviceDetector:
     """Detector constants on the device (xpg/integrate_kernels.py:48-65).  ``d`` is a dict with P0, Bx, By, U1, U2, UN
     (K,3), orgx, orgy, qx, qy, wl, tsens, mu, plx and optional segmap, gx, gy, bkg tables (None when absent)."""
 
     def __init__(self, d, device):
         self.device = device
-        f64 = lambda k: _dev(np.atleast_2d(np.asarray(d[k], np.float64)), torch.float64, device)
+        f64 = lambda k: _dev(_host64(d[k]).reshape(-1, 3), torch.float64, device)
         self.t = {k: f64(k) for k in ("P0", "Bx", "By", "U1", "U2", "UN")}
         s = L.Detector()
         s.n_seg = int(self.t["P0"].shape[0])
@@ -97,18 +116,22 @@ class DeviceDetector:
             setattr(s, k, L.ptr(self.t[k]))
         for k in ("orgx", "orgy", "qx", "qy", "wl", "tsens", "mu", "plx"):
             setattr(s, k, float(d[k]))
-        seg = d.get("segmap")
-        if seg is not None and np.asarray(seg).shape != (1, 1):
-            self.t["segmap"] = _dev(np.asarray(seg, np.int16), torch.int16, device)
-            s.segmap, s.seg_ny, s.seg_nx = L.ptr(self.t["segmap"]), *self.t["segmap"].shape
-        gx = d.get("gx")
-        if gx is not None and np.asarray(gx).shape != (1, 1):
-            self.t["gx"] = _dev(np.asarray(gx, np.float32), torch.float32, device)
-            self.t["gy"] = _dev(np.asarray(d["gy"], np.float32), torch.float32, device)
+        seg = _table(d.get("segmap"))
+        if seg is not None:
+            self.t["segmap"] = _dev(seg, torch.int16, device)
+            if int(self.t["segmap"].max()) >= s.n_seg:
+                raise L.XpgError(f"segment map names segment {int(self.t['segmap'].max())} of {s.n_seg} affine maps")
+            s.segmap, (s.seg_ny, s.seg_nx) = L.ptr(self.t["segmap"]), self.t["segmap"].shape
+        gx = _table(d.get("gx"))
+        if gx is not None:
+            self.t["gx"] = _dev(gx, torch.float32, device)
+            self.t["gy"] = _dev(d["gy"], torch.float32, device)
+            if self.t["gy"].shape != self.t["gx"].shape:
+                raise L.XpgError("geometric-correction tables gx and gy differ in shape")
             s.gx, s.gy = L.ptr(self.t["gx"]), L.ptr(self.t["gy"])
             s.geo_ny, s.geo_nx = self.t["gx"].shape
-        bkg = d.get("bkg")
-        if bkg is not None and tuple(np.shape(bkg)) != (1, 1):
+        bkg = _table(d.get("bkg"))
+        if bkg is not None:
             self.t["bkg"] = _dev(bkg, torch.float32, device)
             s.bkg, (s.bkg_ny, s.bkg_nx) = L.ptr(self.t["bkg"]), self.t["bkg"].shape
         self.s = s
@@ -121,15 +144,15 @@ def device_detector(det, device):
 
 
 def make_call(axis, s0, frames_phi0, osc, sd, sm, gain, rcw):
+    """Scalars of one kernel call (xpg_call); vectors may be device tensors, scalars numpy / 0-d tensors."""
     c = L.Call()
-    c.axis[:] = [float(v) for v in np.asarray(axis, np.float64)]
-    c.s0[:] = [float(v) for v in np.asarray(s0, np.float64)]
+    c.axis[:] = [float(v) for v in _host64(axis).reshape(3)]
+    c.s0[:] = [float(v) for v in _host64(s0).reshape(3)]
     c.frames_phi0, c.osc, c.sd, c.sm, c.gain = (float(v) for v in (frames_phi0, osc, sd, sm, gain))
     if rcw is None:
         c.use_rcw = 0
     else:
-        w = rcw.detach().cpu().numpy() if torch.is_tensor(rcw) else np.asarray(rcw)
-        c.rcw[:] = [float(v) for v in np.asarray(w, np.float32)]
+        c.rcw[:] = [float(v) for v in _host64(rcw).astype(np.float32).reshape(-1)]
         c.use_rcw = 1
     return c
 
@@ -199,9 +222,38 @@ def run_prepare(det, call, refl, frames, device):
     return ps, bufs
 
 
-def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=True, prep=None):
+def prepared_from_pix(det, call, refl, sb, pix):
+    """Prepared data of a chunk (sb, neighbour masks, the partners' geometry: xpg_prepare_sb) whose own per-pixel
+    geometry is pass 1's ``pix`` (xpg/integrate_kernels.py:236-260): eps = u seff in float64 from u and seff rounded
+    to float32 (the reference's pass-1 dtype), so that the kernel's eps / seff gives back u exactly."""
+    n, Dz, Dy, Dx = refl.n, refl.s.Dz, refl.s.Dy, refl.s.Dx
+    device = det.device
+    g = lambda k, dt: _dev(pix[k], dt, device)
+    seff = g("seff", torch.float32).reshape(n).to(torch.float64)
+    geo = torch.empty((n, 3, Dy, Dx), dtype=torch.float64, device=device)
+    bufs = dict(sb=sb, geo=geo, zeta=torch.empty(n, dtype=torch.float64, device=device),
+                nbm=torch.empty((n, Dz, Dy, Dx), dtype=torch.uint8, device=device),
+                geo_p=torch.empty((n, 3, Dy, Dx), dtype=torch.float64, device=device),
+                zeta_p=torch.empty(n, dtype=torch.float64, device=device), seff=seff)
+    ps = L.Prepared()
+    for k, t in bufs.items():
+        if k != "seff":
+            setattr(ps, k, L.ptr(t))
+    L.check(L.lib().xpg_prepare_sb(C.byref(det.s), C.byref(call), C.byref(refl.s), L.ptr(sb), C.byref(ps),
+                                   L.stream_ptr(device)))
+    u1 = g("u1", torch.float32).reshape(n, Dy, Dx).to(torch.float64)
+    u2 = g("u2", torch.float32).reshape(n, Dy, Dx).to(torch.float64)
+    geo[:, 0] = u1 * seff[:, None, None]
+    geo[:, 1] = u2 * seff[:, None, None]
+    geo[:, 2] = g("J", torch.float64).reshape(n, Dy, Dx)
+    bufs["zeta"].copy_(g("zeta", torch.float64).reshape(n))
+    ps.seff = L.ptr(seff)
+    return ps, bufs
+
+
+def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=True, prep=None, masks=False):
     """One pass-1 launch; ``frames`` (a FrameStore view) instead of ``sb`` extracts from the frame store in-kernel,
-    ``prep`` (run_prepare) reads prepared data."""
+    ``prep`` (run_prepare) reads prepared data.  ``masks``: also the per-voxel classes (``vmask``, XPG_VM_* bits)."""
     n, Dz, Dy, Dx = refl.n, refl.s.Dz, refl.s.Dy, refl.s.Dx
     device = det.device
     out = {k: torch.empty(n, dtype=torch.float64, device=device) for k in P1_F64}
@@ -211,6 +263,8 @@ def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=Tru
         for k in ("img2", "J", "u1", "u2"):
             out[k] = torch.empty((n, Dy, Dx), dtype=torch.float32, device=device)
         out["all_valid2"] = torch.empty((n, Dy, Dx), dtype=torch.uint8, device=device)
+    if masks:
+        out["vmask"] = torch.zeros((n, Dz, Dy, Dx), dtype=torch.uint8, device=device)
     o = L.Pass1Out()
     for k, _ in L.Pass1Out._fields_:
         if k != "fz_stride":
@@ -230,13 +284,16 @@ def run_pass1(det, call, refl, sb=None, frames=None, acc=None, pixel_outputs=Tru
     return out
 
 
-def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0, prep=None):
+def run_pass2(det, call, refl, prof, bg, nbg, sb=None, frames=None, irls_rule=0, prep=None, masks=False):
+    """One pass-2 launch (sources as run_pass1); ``masks``: also the per-voxel fit mask and neighbour bits."""
     n = refl.n
     device = det.device
     out = {k: torch.empty(n, dtype=torch.float64, device=device) for k in P2_F64}
+    if masks:
+        out["vmask"] = torch.zeros((n, refl.s.Dz, refl.s.Dy, refl.s.Dx), dtype=torch.uint8, device=device)
     o = L.Pass2Out()
     for k, _ in L.Pass2Out._fields_:
-        setattr(o, k, L.ptr(out[k]))
+        setattr(o, k, L.ptr(out.get(k)))
     prof = _dev(prof, torch.float32, device)
     bg = _dev(bg, torch.float64, device)
     nbg = _dev(nbg, torch.float64, device)
@@ -302,9 +359,15 @@ def make_kernels(Dz, Dy, Dx, n_regions, G=PROFILE_GRID):
 
     def pass2(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, prof, bg, nbm_x, nb, rcw, pix=None):
         """Profile fitting with fixed background and 2x2 overlap deconvolution (xpg/integrate_kernels.py:382-483).
-        ``pix`` is accepted for signature compatibility; the kernel recomputes the same geometry in float64."""
+        ``pix``: pass 1's per-pixel geometry of these reflections (zeta, J, u1, u2, seff), used instead of
+        recomputing it, as the reference does; the overlapping partners' geometry is computed afresh."""
         d, call, refl, sbf = _setup(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, nb, rcw)
-        return run_pass2(d, call, refl, prof, bg, nbm_x, sb=sbf)
+        if pix is None:
+            return run_pass2(d, call, refl, prof, bg, nbm_x, sb=sbf)
+        ps, keep = prepared_from_pix(d, call, refl, sbf, pix)
+        out = run_pass2(d, call, refl, prof, bg, nbm_x, prep=ps)
+        out["_prepared"] = keep                  # the buffers stay alive until the launch has completed
+        return out
 
     return pass1, accumulate_profiles, pass2
 
