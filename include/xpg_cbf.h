@@ -46,6 +46,17 @@ int64_t xpg_cbf_workspace_bytes(int64_t n_chunks, int64_t n_groups);
 
 int xpg_cbf_decode(const xpg_cbf_batch *b, void *stream);
 
+/* xpg_cbf_decode without a caller workspace and without any allocation: no counterpart in the reference (it decodes
+ * on the host); for callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The workspace is a file-scope device array of XPG_CBF_STATIC_SCRATCH_BYTES
+ * (64 MiB) on each GPU the library runs on, held until the process ends; b->work is ignored (may be NULL).  A batch
+ * whose xpg_cbf_workspace_bytes exceeds it is rejected with XPG_ERR_TOO_LARGE (about 420,000 chunks of 256 bytes:
+ * some 107 MB of byte-offset streams per call).  Every call on a GPU shares that GPU's array, so calls that overlap
+ * on one GPU (on different streams, or from several threads) race: serialise them.  Nothing in the array needs to
+ * start at zero: every call writes each workspace entry before reading it. */
+#define XPG_CBF_STATIC_SCRATCH_BYTES (64LL << 20)
+int xpg_cbf_decode_static(const xpg_cbf_batch *b, void *stream);
+
 /* Host: encode_byte_offset (xpg/io/cbf.py:47-70) of n int32 values into out (at most 7 n bytes); returns the length.
  * The writer side of the format (write_cbf, :150-194); used to make workloads. */
 int64_t xpg_cbf_encode_host(const int32_t *values, int64_t n, uint8_t *out, int64_t cap);

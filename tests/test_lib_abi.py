@@ -86,3 +86,29 @@ def test_ctypes_structs_match_c_layout(tmp_path):
         assert got == int(v), (py, f, got, v)
         seen += 1
     assert seen > 60
+
+
+def test_static_scratch_decode_refuses_oversized_batches_before_any_launch():
+    """xpg_cbf_decode_static checks the batch's workspace against its fixed scratch before it touches a device (so this
+    runs without a GPU: the pointers are never dereferenced)."""
+    import ctypes as C
+    from xpg_amd import _lib
+    h = _lib.lib()
+    b = _lib.CbfBatch()
+    for k in ("src", "stream_off", "stream_len", "n_elem", "chunk0", "group0", "out", "out_off", "status"):
+        setattr(b, k, C.c_void_p(0x1000))
+    b.n_frames = 1
+    limit = 64 << 20
+    nc = 1
+    while h.xpg_cbf_workspace_bytes(nc * 2, (nc * 2 + 63) // 64) <= limit:
+        nc *= 2
+    lo, hi = nc, nc * 2                       # the largest chunk count that fits lies in [lo, hi)
+    while hi - lo > 1:
+        mid = (lo + hi) // 2
+        lo, hi = (mid, hi) if h.xpg_cbf_workspace_bytes(mid, (mid + 63) // 64) <= limit else (lo, mid)
+    assert 400_000 < lo < 450_000
+    b.n_chunks, b.n_groups = hi, (hi + 63) // 64
+    assert h.xpg_cbf_decode_static(C.byref(b), None) == -3              # XPG_ERR_TOO_LARGE
+    assert b"static scratch" in h.xpg_last_error()
+    b.n_chunks = -1
+    assert h.xpg_cbf_decode_static(C.byref(b), None) == -1              # XPG_ERR_ARG

@@ -135,6 +135,7 @@ _SIGS = {
     "xpg_bslz4_decode": ([P(Bslz4Batch), C.c_void_p], C.c_int),
     "xpg_cbf_workspace_bytes": ([C.c_int64, C.c_int64], C.c_int64),
     "xpg_cbf_decode": ([P(CbfBatch), C.c_void_p], C.c_int),
+    "xpg_cbf_decode_static": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_encode_host": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64], C.c_int64),
     "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
                               C.c_int64),

@@ -59,17 +59,19 @@ class StreamBatch:
         self.n_elem_d = torch.as_tensor(self.n_elem, device=dev)
         self.chunk0 = torch.as_tensor(chunk0, device=dev)
         self.group0 = torch.as_tensor(group0, device=dev)
-        wb = int(L.lib().xpg_cbf_workspace_bytes(self.n_chunks, self.n_groups))
-        self.work = torch.empty(max(wb, 256), dtype=torch.uint8, device=dev)
+        self.work_bytes = int(L.lib().xpg_cbf_workspace_bytes(self.n_chunks, self.n_groups))
+        self.work = None               # allocated by the first decode that uses a caller workspace
         self.status = torch.zeros(self.n, dtype=torch.int32, device=dev)
         self.stream_bytes = total
         # decodes may run on another stream than the uploads above: they wait for this event first
         self._uploaded = torch.cuda.Event()
         self._uploaded.record(torch.cuda.current_stream(dev))
 
-    def decode_into(self, out: torch.Tensor, out_off=None):
+    def decode_into(self, out: torch.Tensor, out_off=None, static_scratch: bool = False):
         """Decode every frame into ``out`` (contiguous int32): frame i at element ``out_off[i]`` (default: frames back
-        to back).  Asynchronous on the current stream; :meth:`check` raises for short streams."""
+        to back).  Asynchronous on the current stream; :meth:`check` raises for short streams.  ``static_scratch``:
+        through xpg_cbf_decode_static (the library's per-GPU scratch array instead of a workspace allocated here;
+        batches up to XPG_CBF_STATIC_SCRATCH_BYTES of workspace, calls on one GPU must not overlap)."""
         L.require_device(out)
         if out.dtype != torch.int32 or not out.is_contiguous():
             raise ValueError("output: contiguous int32")
@@ -83,9 +85,13 @@ class StreamBatch:
         b.src, b.stream_off, b.stream_len = L.ptr(self.src), L.ptr(self.stream_off), L.ptr(self.stream_len)
         b.n_elem, b.chunk0, b.group0 = L.ptr(self.n_elem_d), L.ptr(self.chunk0), L.ptr(self.group0)
         b.n_frames, b.n_chunks, b.n_groups = self.n, self.n_chunks, self.n_groups
-        b.work, b.out, b.out_off, b.status = L.ptr(self.work), L.ptr(out), L.ptr(self._out_off), L.ptr(self.status)
+        if not static_scratch and self.work is None:
+            self.work = torch.empty(max(self.work_bytes, 256), dtype=torch.uint8, device=self.device)
+        b.work = None if static_scratch else L.ptr(self.work)
+        b.out, b.out_off, b.status = L.ptr(out), L.ptr(self._out_off), L.ptr(self.status)
         torch.cuda.current_stream(self.device).wait_event(self._uploaded)
-        L.check(L.lib().xpg_cbf_decode(C.byref(b), L.stream_ptr(self.device)))
+        decode = L.lib().xpg_cbf_decode_static if static_scratch else L.lib().xpg_cbf_decode
+        L.check(decode(C.byref(b), L.stream_ptr(self.device)))
         return out
 
     def check(self):

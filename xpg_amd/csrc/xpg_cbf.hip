@@ -21,7 +21,10 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <stdio.h>
+
 #include <algorithm>
+#include <mutex>
 
 #include "xpg_cbf.h"
 #include "xpg_integrate.h"
@@ -451,21 +454,42 @@ __global__ void __launch_bounds__(64 * WPB) k_cbf_write(Batch b) {
   }
 }
 
-}  // namespace
+// The workspace of xpg_cbf_decode_static: one array per GPU, held until the process ends.  Host code never passes
+// the array's name to a kernel (on the host it is a stand-in, not a device address): it takes the device address
+// from hipGetSymbolAddress once per device and uses that.
+static __device__ __align__(256) uint8_t scratch[XPG_CBF_STATIC_SCRATCH_BYTES];
 
-extern "C" {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[MAX_DEVICES] = {};
 
-int64_t xpg_cbf_workspace_bytes(int64_t n_chunks, int64_t n_groups) {
-  if (n_chunks < 0 || n_groups < 0) return -1;
-  const Work k = carve(nullptr, n_chunks, n_groups);
-  return (int64_t)(reinterpret_cast<uintptr_t>(k.g_entry) + align256(n_groups));
+// the current device's scratch address, looked up on the device's first call (device current)
+int scratch_address(void **ptr) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return xpg::report_error(XPG_ERR_HIP, hipGetErrorString(e));
+  if (dev < 0 || dev >= MAX_DEVICES) return xpg::report_error(XPG_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return xpg::report_error(XPG_ERR_HIP, e != hipSuccess ? hipGetErrorString(e)
+                                                                                   : "scratch has no device address");
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return 0;
 }
 
-int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
-  if (!p || !p->src || !p->stream_off || !p->stream_len || !p->n_elem || !p->chunk0 || !p->group0 || !p->work ||
-      !p->out || !p->out_off || !p->status || p->n_frames < 0 || p->n_chunks < 0 || p->n_groups < 0)
+int check_batch(const xpg_cbf_batch *p, bool need_work) {
+  if (!p || !p->src || !p->stream_off || !p->stream_len || !p->n_elem || !p->chunk0 || !p->group0 ||
+      (need_work && !p->work) || !p->out || !p->out_off || !p->status || p->n_frames < 0 || p->n_chunks < 0 ||
+      p->n_groups < 0)
     return xpg::report_error(XPG_ERR_ARG, "cbf batch: missing pointer or negative size");
-  if (p->n_frames == 0) return 0;
+  return 0;
+}
+
+int launch_decode(const xpg_cbf_batch *p, void *work, void *stream) {
   Batch b;
   b.src = p->src;
   b.soff = p->stream_off;
@@ -476,7 +500,7 @@ int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
   b.nf = p->n_frames;
   b.nc = p->n_chunks;
   b.ng = p->n_groups;
-  b.w = carve(p->work, p->n_chunks, p->n_groups);
+  b.w = carve(work, p->n_chunks, p->n_groups);
   b.out = p->out;
   b.ooff = p->out_off;
   b.status = p->status;
@@ -492,6 +516,40 @@ int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return xpg::report_error(XPG_ERR_HIP, hipGetErrorString(e));
   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t xpg_cbf_workspace_bytes(int64_t n_chunks, int64_t n_groups) {
+  if (n_chunks < 0 || n_groups < 0) return -1;
+  const Work k = carve(nullptr, n_chunks, n_groups);
+  return (int64_t)(reinterpret_cast<uintptr_t>(k.g_entry) + align256(n_groups));
+}
+
+int xpg_cbf_decode(const xpg_cbf_batch *p, void *stream) {
+  const int rc = check_batch(p, true);
+  if (rc) return rc;
+  if (p->n_frames == 0) return 0;
+  return launch_decode(p, p->work, stream);
+}
+
+int xpg_cbf_decode_static(const xpg_cbf_batch *p, void *stream) {
+  int rc = check_batch(p, false);
+  if (rc) return rc;
+  const int64_t need = xpg_cbf_workspace_bytes(p->n_chunks, p->n_groups);
+  if (need < 0 || need > XPG_CBF_STATIC_SCRATCH_BYTES) {
+    char msg[160];
+    snprintf(msg, sizeof(msg), "cbf batch of %lld chunks needs %lld bytes of workspace > the %lld-byte static scratch",
+             (long long)p->n_chunks, (long long)need, (long long)XPG_CBF_STATIC_SCRATCH_BYTES);
+    return xpg::report_error(XPG_ERR_TOO_LARGE, msg);
+  }
+  if (p->n_frames == 0) return 0;
+  void *work = nullptr;
+  rc = scratch_address(&work);
+  if (rc) return rc;
+  return launch_decode(p, work, stream);
 }
 
 int64_t xpg_cbf_encode_host(const int32_t *values, int64_t n, uint8_t *out, int64_t cap) {
