@@ -362,14 +362,15 @@ __global__ void __launch_bounds__(256) k_extract(xpg_frames F, xpg_refl R, float
 
 // prepared data of reflection r: shoebox values, neighbour-mask bit `bit` as VF_NBM, eps1 / eps2 / J into U1, U2, Jd;
 // returns zeta
+// vox = false: the voxel values and mask bits stay in global memory (rows whose voxels are read into registers)
 __device__ __forceinline__ double stage_prepared(const Box &b, const xpg_prepared &pp, int bit, float *vals,
                                                  uint8_t *vflag, uint8_t *prow, double *U1, double *U2, double *Jd,
-                                                 int lane) {
+                                                 int lane, bool vox = true) {
   for (int p = lane; p < b.P; p += WAVE) prow[p] = (uint8_t)(p / b.Dx);
   const float *src = pp.sb + b.voff;
   const uint8_t *nsrc = pp.nbm + b.voff;
   constexpr int NB = 16;
-  for (int v0 = lane; v0 < b.V; v0 += NB * WAVE) {
+  for (int v0 = lane; v0 < b.V && vox; v0 += NB * WAVE) {
     float t[NB];
     uint8_t m[NB];
 #pragma unroll
@@ -580,15 +581,34 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const xpg_detector &det = a.det;
   const xpg_call &call = a.call;
 
+  // Boxes of at most NPL pixels x DZM frames per lane (regs) read their voxels once, into registers (value,
+  // neighbour bit): from the prepared buffers directly, or from LDS after staging.
+  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
+  const bool regs = P <= NPL * WAVE && Dz <= DZM;
+  const bool direct = PREP && !VG && regs;
   double zeta;
   if constexpr (PREP) {
-    zeta = stage_prepared(b, a.prep, 0, vals, vflag, prow, U1, U2, Jd, lane);
+    zeta = stage_prepared(b, a.prep, 0, vals, vflag, prow, U1, U2, Jd, lane, !direct);
   } else {
     stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
     Frame f;
     kabsch(a.R.S + 3 * (int64_t)r, call.s0, call.axis, f);
     zeta = f.zeta;
     box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
+  }
+  float dv[NK];                          // direct rows: values and neighbour bytes in flight during the setup below
+  uint8_t dn[NK];
+  if (direct) {
+    const float *src = a.prep.sb + b.voff;
+    const uint8_t *nsrc = a.prep.nbm + b.voff;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q)
+#pragma unroll
+      for (int j = 0; j < DZM; ++j) {
+        const int v = min(j, Dz - 1) * P + min(lane + q * WAVE, P - 1);   // in the row, whatever the lane
+        dv[q * DZM + j] = src[v];
+        dn[q * DZM + j] = nsrc[v];
+      }
   }
   rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
   const double seff = (PREP && a.prep.seff) ? box_u_given(P, uni_d(a.prep.seff[r]), U1, U2, lane)
@@ -614,11 +634,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   // Lanes run over pixels (p = lane, lane + 64, ...) and each lane over the frames of its pixel: per-pixel terms stay
   // in registers, per-frame terms are broadcasts.  Sums are float64; sums of counts and of template values are exact.
   //
-  // Boxes of at most NPL pixels x DZM frames per lane (regs) read their voxels from LDS once, into registers (value,
-  // neighbour bit), and keep the per-voxel classes as bit masks (bit k = q DZM + j for pixel lane + 64 q, frame j);
-  // the sums run over the same voxels in the same order as the LDS loops of larger boxes.
-  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
-  const bool regs = P <= NPL * WAVE && Dz <= DZM;
+  // Register rows keep the per-voxel classes as bit masks (bit k = q DZM + j for pixel lane + 64 q, frame j); the
+  // sums run over the same voxels in the same order as the LDS loops of larger boxes.
   float cv[NK], brq[NPL];
   uint32_t inm = 0, nbmm = 0, fgm = 0, fgvm = 0, bgmm = 0, fg2q = 0;   // in box / nbm / fg / fgv / bg / bit q: fg2
   uint32_t fgz = 0;                                                   // bit j: foreground frame (uniform)
@@ -639,8 +656,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         cv[k] = 0.f;
         if (pin && j < Dz) {
           const int v = j * P + p;
-          const float c = vals[v];
-          const bool nbm = vflag[v] & VF_NBM;
+          const float c = direct ? dv[k] : vals[v];
+          const bool nbm = direct ? (dn[k] & 1) != 0 : (vflag[v] & VF_NBM) != 0;
           cv[k] = c;
           inm |= 1u << k;
           if (nbm) nbmm |= 1u << k;
@@ -1273,9 +1290,28 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double bgr = a.bg[r];
   const double gain = call.gain;
 
+  // Rows without a partner whose voxels fit NPL pixels x DZM frames per lane (regs) keep, from the fit mask to the
+  // tail sums, their net counts and profile values in registers; prepared rows read their voxels straight into them.
+  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
+  const bool regs = !partner && P <= NPL * WAVE && Dz <= DZM;
+  const bool direct = PREP && !VG && regs;
   double zeta;
+  float dv[NK];
+  uint8_t dn[NK];
   if constexpr (PREP) {
-    zeta = stage_prepared(b, a.prep, partner ? 1 : 0, vals, vflag, prow, U1, U2, Jd, lane);
+    zeta = stage_prepared(b, a.prep, partner ? 1 : 0, vals, vflag, prow, U1, U2, Jd, lane, !direct);
+    if (direct) {
+      const float *src = a.prep.sb + b.voff;
+      const uint8_t *nsrc = a.prep.nbm + b.voff;
+#pragma unroll
+      for (int q = 0; q < NPL; ++q)
+#pragma unroll
+        for (int j = 0; j < DZM; ++j) {
+          const int v = min(j, Dz - 1) * P + min(lane + q * WAVE, P - 1);   // in the row, whatever the lane
+          dv[q * DZM + j] = src[v];
+          dn[q * DZM + j] = nsrc[v];
+        }
+    }
   } else {
     stage_values(b, a.sb, a.F, vals, vflag, prow, base + L.pa, base + L.R, lane);
     Frame f;
@@ -1347,13 +1383,11 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const double pmax = wmax_d(p2m) * wmax_d(Rm);
   const double ppmax = partner ? (wmax_d(pp2m) * wmax_d(Rpm)) * has_p : 0.0;
   const double pthr = 0.01 * pmax, ppthr = 0.01 * fmax(ppmax, 1e-12);
-  // Rows without a partner whose voxels fit NPL pixels x DZM frames per lane (regs) read them from LDS once and keep,
-  // from the fit mask to the tail sums, their net counts and profile values in registers and their classes as bit
-  // masks (bit k = q DZM + j for pixel lane + 64 q, frame j): the same products, summed in the same order.
-  constexpr int NPL = 2, DZM = 8, NK = NPL * DZM;
-  const bool regs = !partner && P <= NPL * WAVE && Dz <= DZM;
+  // Register rows keep their classes as bit masks (bit k = q DZM + j for pixel lane + 64 q, frame j): the same
+  // products, summed in the same order as the LDS loops.
   float netr[NK], pfr[NK], b0r[NPL];
   uint32_t usem = 0, inm = 0, validm = 0, fgm = 0;   // bit k: in the box / fit voxel / valid & ~nbm / foreground
+  uint32_t nbmr = 0;                                  // bit k: inside a neighbour ellipsoid (mask export only)
   double s_c = 0.0, s_b = 0.0, s_pw = 0.0;
   if (regs) {                             // no partner: pp = +0 and the fit test is p > 0.01 max p
     uint32_t fgz = 0;
@@ -1378,10 +1412,12 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         pfr[k] = 0.f;
         if (pin && j < Dz) {
           const int v = j * P + p;
-          const float c = vals[v];
-          const bool vn = c >= 0.f && !(vflag[v] & VF_NBM);
+          const float c = direct ? dv[k] : vals[v];
+          const bool nbm = direct ? (dn[k] & 1) != 0 : (vflag[v] & VF_NBM) != 0;
+          const bool vn = c >= 0.f && !nbm;
           const double pd = p2p * Rd[j];
           inm |= 1u << k;
+          if (nbm) nbmr |= 1u << k;
           pfr[k] = p2f * Rf[j];
           netr[k] = c - b0;
           if (vn) validm |= 1u << k;
@@ -1450,9 +1486,16 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     for (int p = lane; p < P; p += WAVE) {
       const int q = (p - lane) / WAVE;
       for (int j = 0; j < Dz; ++j) {
-        const uint8_t fl = vflag[j * P + p];
-        const bool use = regs ? ((usem >> (q * DZM + j)) & 1u) != 0 : (fl & VF_USE) != 0;
-        vm[j * P + p] = (use ? XPG_VM_USE : 0) | ((fl & VF_NBM) ? XPG_VM_NBM : 0);
+        bool use, nbm;
+        if (regs) {                       // register rows: their bits (their LDS flags may not be staged)
+          use = ((usem >> (q * DZM + j)) & 1u) != 0;
+          nbm = ((nbmr >> (q * DZM + j)) & 1u) != 0;
+        } else {
+          const uint8_t fl = vflag[j * P + p];
+          use = (fl & VF_USE) != 0;
+          nbm = (fl & VF_NBM) != 0;
+        }
+        vm[j * P + p] = (use ? XPG_VM_USE : 0) | (nbm ? XPG_VM_NBM : 0);
       }
     }
   }
