@@ -223,6 +223,10 @@ typedef struct xpg_prepared {
   /* (n,) or NULL: the effective spot width seff of each row, given instead of computed from sigma_D and the centre
    * pixel's area (the reference's pass 2 with pass 1's ``pix``, xpg/integrate_kernels.py:259-260: eps = u seff) */
   const double *seff;
+  /* (n, r_stride) or NULL: the rows' frame-integrated rocking curves R for the widths of the launch that reads them,
+   * written by xpg_rocking (NULL: the pass kernels evaluate them) */
+  double *R;
+  int32_t r_stride;        /* doubles per row of R (>= every row's Dz) */
 } xpg_prepared;
 
 /* Shoebox gather (host gather xpg/integrate.py:704-803 with _decode_bins xpg/shoebox.py:264-268), the widths-independent
@@ -235,6 +239,11 @@ int xpg_prepare(const xpg_detector *det, const xpg_call *call, const xpg_refl *r
  * of xpg/integrate_kernels.py:216-485, whose closures take the chunk); prep->sb may be sb itself. */
 int xpg_prepare_sb(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const float *sb,
                    const xpg_prepared *prep, void *stream);
+
+/* The rocking-curve half of geometry_terms (xpg/integrate_kernels.py:246-252 with _phi_rc :130-141): each row's
+ * frame-integrated rocking curve R (Dz values) for the widths of call, into prep->R (one lane per row and frame; the
+ * same arithmetic as the pass kernels' own evaluation, so their results do not change).  Reads prep->zeta. */
+int xpg_rocking(const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep, void *stream);
 
 /* xpg_pass1 / xpg_pass2 reading the prepared data of xpg_prepare (results identical to xpg_pass1_frames /
  * xpg_pass2_frames on the same frames). */

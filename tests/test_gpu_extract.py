@@ -101,3 +101,44 @@ def test_prepared_passes_give_identical_results():
     b2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], frames=store)
     for k in K.P2_F64:
         assert torch.equal(a2[k], b2[k]), k
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mixture", [True, False])
+def test_rocking_table_gives_identical_results(mixture):
+    """xpg_rocking's per-row rocking-curve table read by the prepared passes == the passes' own evaluation, bit for bit
+    (mixture weights and the plain Gaussian; a table wider than the boxes)."""
+    import ctypes as C
+    from chunks import make_chunk
+    from xpg_amd import _lib as L
+    from xpg_amd import integrate_kernels as K
+    from xpg_amd.frames import DeviceFrames
+    dev = torch.device("cuda:0")
+    c = make_chunk(n=80, seed=31)
+    nfr, ny, nx = 420, 1024, 1024
+    store_np = np.random.default_rng(5).poisson(3.0, (nfr, ny, nx)).astype(np.uint32)
+    frames = DeviceFrames(torch.as_tensor(store_np.view(np.int32), device=dev), first=1)
+    store = frames.store()
+    det = K.device_detector(c["det"], dev)
+    call = K.make_call(c["axis"], c["s0"], c["frames_phi0"], c["osc"], c["sd"], c["sm"], c["gain"],
+                       c["rcw"] if mixture else None)
+    refl = K.DeviceRefl(c["rc"], c["nb"], (7, 11, 11), dev)
+    print(f"rocking table n=80 7x11x11 mixture={mixture}", flush=True)
+    ps, bufs = K.run_prepare(det, call, refl, store, dev)
+    R = torch.full((80, 9), float("nan"), dtype=torch.float64, device=dev)
+    pt = L.Prepared.from_buffer_copy(ps)
+    pt.R, pt.r_stride = L.ptr(R), 9
+    L.check(L.lib().xpg_rocking(C.byref(call), C.byref(refl.s), C.byref(pt), L.stream_ptr(dev)))
+    assert torch.isfinite(R[:, :7]).all() and torch.isnan(R[:, 7:]).all()
+    a = K.run_pass1(det, call, refl, prep=pt)
+    b = K.run_pass1(det, call, refl, prep=ps)
+    for k in K.P1_F64 + K.P1_I32 + ("fz", "img2"):
+        assert torch.equal(a[k], b[k]), k
+    prof = torch.as_tensor(c["prof"], device=dev)
+    a2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], prep=pt)
+    b2 = K.run_pass2(det, call, refl, prof, a["bg"], a["nbg"], prep=ps)
+    for k in K.P2_F64:
+        assert torch.equal(torch.nan_to_num(a2[k], nan=-7.0), torch.nan_to_num(b2[k], nan=-7.0)), k
+    pt.r_stride = 6
+    with pytest.raises(L.XpgError, match="r_stride"):
+        K.run_pass1(det, call, refl, prep=pt)

@@ -79,7 +79,8 @@ class Frames(C.Structure):
 
 class Prepared(C.Structure):
     _fields_ = [("sb", C.c_void_p), ("geo", C.c_void_p), ("zeta", C.c_void_p), ("nbm", C.c_void_p),
-                ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p), ("seff", C.c_void_p)]
+                ("geo_p", C.c_void_p), ("zeta_p", C.c_void_p), ("seff", C.c_void_p), ("R", C.c_void_p),
+                ("r_stride", C.c_int32)]
 
 
 
@@ -117,6 +118,7 @@ _SIGS = {
                           C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_prepare": ([P(Detector), P(Call), P(Refl), P(Frames), P(Prepared), C.c_void_p], C.c_int),
     "xpg_prepare_sb": ([P(Detector), P(Call), P(Refl), C.c_void_p, P(Prepared), C.c_void_p], C.c_int),
+    "xpg_rocking": ([P(Call), P(Refl), P(Prepared), C.c_void_p], C.c_int),
     "xpg_pass1_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), P(Pass1Out), P(ProfileAcc), C.c_void_p],
                            C.c_int),
     "xpg_pass2_prepared": ([P(Detector), P(Call), P(Refl), P(Prepared), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,

@@ -104,11 +104,31 @@ __device__ __forceinline__ double wsum_d(double v) {
 // over lane bits 0 .. 3 in which a lane keeps half of its values at each step (so V values cost about as much as two
 // wsum_d), then the two row bits; the row sums combine as ((r0 + r1) + (r2 + r3)).
 template <int S>
-__device__ __forceinline__ double xchg_row(double x) {         // the value of lane ^ (1 << S), S < 4
+__device__ __forceinline__ double xchg_row(double x) {
+  // the value of the lane paired with this one at step S: lane ^ 1, lane ^ 2, then a rotation by 4 within the row of
+  // 16 (a partner whose bits 0-1 match and bit 2 differs: the halving steps below stay exact), then lane ^ 8
   if constexpr (S == 0) return dppd<DPP_Q1>(x);
   else if constexpr (S == 1) return dppd<DPP_Q2>(x);
-  else if constexpr (S == 2) return __shfl_xor(x, 4, 64);
+  else if constexpr (S == 2) return dppd<DPP_R4>(x);
   else return dppd<DPP_R8>(x);                                   // rotation by 8 in a row of 16: lane ^ 8
+}
+// x[lane] + x[lane ^ 16] (and ^ 32) on every lane, by the gfx950 row / half-wave swaps: the pair a, b of a swap holds
+// this lane's value and its partner's, and IEEE addition is commutative, so both lanes of a pair get the same bits
+__device__ __forceinline__ double add_xor16(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const auto lo = __builtin_amdgcn_permlane16_swap((uint32_t)b, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+  const double a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double c = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return a + c;
+}
+__device__ __forceinline__ double add_xor32(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)b, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+  const double a = __builtin_bit_cast(double, ((uint64_t)hi[0] << 32) | lo[0]);
+  const double c = __builtin_bit_cast(double, ((uint64_t)hi[1] << 32) | lo[1]);
+  return a + c;
 }
 template <int S, int N>
 __device__ __forceinline__ void wsum_step(double *v, int lane) {
@@ -133,8 +153,8 @@ __device__ __forceinline__ void wsum_multi(double (&v)[V]) {
   wsum_step<2, (V > 2 ? V / 4 : 1)>(v, lane);
   wsum_step<3, (V > 4 ? V / 8 : 1)>(v, lane);
   double t = v[0];                       // lane l holds the row sum of value bitreverse(l mod V)
-  t = t + __shfl_xor(t, 16, 64);
-  t = t + __shfl_xor(t, 32, 64);
+  t = add_xor16(t);
+  t = add_xor32(t);
   constexpr int B = V == 16 ? 4 : (V == 8 ? 3 : (V == 4 ? 2 : (V == 2 ? 1 : 0)));
 #pragma unroll
   for (int j = 0; j < V; ++j) {
@@ -303,6 +323,19 @@ __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb,
   double phi_ = plo + c.osc * (double)fb;
   lo = fmin(plo, phi_);
   hi = fmax(plo, phi_);
+}
+
+// R[j] of one frame: the mixture terms summed in component order (the arithmetic of rocking_curve for one frame)
+__device__ __forceinline__ double rocking_frame(const xpg_call &c, int iz0, int fb, double phi, double zeta, int j) {
+  const int nt = use_rcw_of(c) ? RCK : 1;
+  const double s = sm_of(c) / fmax(fabs(zeta), 1e-3);
+  const double isw = 1.0 / ((s * 0.25) * sqrt(2.0));
+  double lo, hi;
+  frame_bounds(c, iz0, fb, j, lo, hi);
+  if (nt == 1) return rc_term(c, lo, hi, phi, s, isw, 0);
+  double out = 0.0;
+  for (int q = 0; q < nt; ++q) out = out + rc_term(c, lo, hi, phi, s, isw, q);
+  return out;
 }
 
 // R[j] for all frames of a reflection into R (LDS), using scratch (>= Dz * 21 doubles); the mixture terms are summed

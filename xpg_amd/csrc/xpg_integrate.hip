@@ -397,6 +397,24 @@ __device__ __forceinline__ double stage_prepared(const Box &b, const xpg_prepare
   return zeta;
 }
 
+// a row's rocking curve computed ahead by xpg_rocking, into Rd (LDS)
+__device__ __forceinline__ void load_rocking(const xpg_prepared &pp, int r, int Dz, double *Rd, int lane) {
+  for (int j = lane; j < Dz; j += WAVE) Rd[j] = pp.R[(int64_t)r * pp.r_stride + j];
+  wave_sync();
+}
+
+// rocking curves of a launch's rows ahead of a pass: one lane per (row, frame)
+__global__ void __launch_bounds__(256) k_rocking(xpg_call call, xpg_refl R, xpg_prepared pp) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = pp.r_stride;
+  const int64_t r = t / stride;
+  const int j = (int)(t - r * stride);
+  if (r >= R.n) return;
+  const int Dz = R.rdz ? R.rdz[r] : R.Dz;
+  if (j >= Dz) return;
+  pp.R[t] = rocking_frame(call, R.iz0[r], R.fb[r], R.phi[r], pp.zeta[r], j);
+}
+
 // box coordinates: co[0..Dx) pixel-centre x, co[Dx..Dx+Dy) y, then the Dz frame centres iz0 + fb (j + 1/2)
 __device__ __forceinline__ void box_coords(const Box &b, double *co, int lane) {
   for (int i = lane; i < b.Dx + b.Dy + b.Dz; i += WAVE) {
@@ -610,7 +628,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
         dn[q * DZM + j] = nsrc[v];
       }
   }
-  rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
+  if (PREP && a.prep.R) load_rocking(a.prep, r, Dz, Rd, lane);
+  else rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
   const double seff = (PREP && a.prep.seff) ? box_u_given(P, uni_d(a.prep.seff[r]), U1, U2, lane)
                                             : box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
@@ -1239,6 +1258,7 @@ struct P2Args {
   xpg_pass2_out out;
   int waves;
   int wave_bytes;
+  int only_partner;         // rows without an overlapping partner are left to k_pass2_small
 };
 
 // p2 = P2(u) J / seff^2 per pixel, float64 (xpg/integrate_kernels.py:278-285)
@@ -1286,6 +1306,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   const xpg_call &call = a.call;
   const double has_p = a.R.has_p ? (double)uni_f(a.R.has_p[r]) : 0.0;
   const bool partner = has_p > 0.5;
+  if (a.only_partner && !partner) return;
   const float hpf = (float)has_p;
   const double bgr = a.bg[r];
   const double gain = call.gain;
@@ -1319,7 +1340,8 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
     zeta = f.zeta;
     box_eps_J(det, f, b.ix0, b.iy0, b.s, Dy, Dx, U1, U2, Jd, lane);
   }
-  rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
+  if (PREP && a.prep.R) load_rocking(a.prep, r, Dz, Rd, lane);
+  else rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
   const double seff = (PREP && a.prep.seff) ? box_u_given(P, uni_d(a.prep.seff[r]), U1, U2, lane)
                                             : box_u(Dy, Dx, sd_of(call), U1, U2, Jd, lane);
   box_coords(b, co, lane);
@@ -1736,6 +1758,336 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
+// register kernel of pass 2: prepared rows of at most SPL x 64 pixels and SDZ frames without an overlapping partner.
+// Each lane holds its pixels' voxels, neighbour bits and geometry in registers, read straight from the prepared
+// buffers; LDS holds only the rocking curve and the per-frame values.  The arithmetic is that of the LDS kernel's
+// register-row path (same masks, exact integer sums, same per-pixel accumulation order).  (A pass-1 form of it was
+// measured slower than the LDS kernel's register rows on full-size blocks and is not kept.)
+// ---------------------------------------------------------------------------------------------------------------------
+
+constexpr int SPL = 2, SDZ = 8, SNK = SPL * SDZ;
+
+// LDS of one wavefront: rocking-curve terms (21 Dz doubles), then R, e3 and frame centres (Dz doubles each)
+__host__ __device__ static inline size_t small_wave_bytes(int Dz) { return al8(8 * (21 + 3) * (size_t)Dz); }
+
+__host__ __device__ static inline bool small_shape(int Dz, int Dy, int Dx) { return Dz <= SDZ && Dy * Dx <= SPL * WAVE; }
+
+// one row's prepared data in registers: lane l holds pixels l and l + 64 (clamped reads past the box; voxels outside
+// it are -3, unrecorded, and belong to no class)
+struct SmallRow {
+  float c[SNK];
+  uint32_t nbm;            // bit k = q SDZ + j
+  double e1[SPL], e2[SPL], J[SPL], Jc;
+};
+
+__device__ __forceinline__ void small_load(const Box &b, const xpg_prepared &pp, int bit, SmallRow &w, int lane) {
+  const float *src = pp.sb + b.voff;
+  const uint8_t *nsrc = pp.nbm + b.voff;
+  const double *g = pp.geo + 3 * b.poff;
+  uint8_t nb[SNK];
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const int pc = min(lane + q * WAVE, b.P - 1);
+    w.e1[q] = g[pc];
+    w.e2[q] = g[b.P + pc];
+    w.J[q] = g[2 * b.P + pc];
+#pragma unroll
+    for (int j = 0; j < SDZ; ++j) {
+      const int v = min(j, b.Dz - 1) * b.P + pc;
+      w.c[q * SDZ + j] = src[v];
+      nb[q * SDZ + j] = nsrc[v];
+    }
+  }
+  w.Jc = g[2 * b.P + (b.Dy / 2) * b.Dx + b.Dx / 2];
+  w.nbm = 0;
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const bool pin = lane + q * WAVE < b.P;
+#pragma unroll
+    for (int j = 0; j < SDZ; ++j) {
+      const int k = q * SDZ + j;
+      if (!pin || j >= b.Dz) w.c[k] = -3.0f;
+      else if ((nb[k] >> bit) & 1) w.nbm |= 1u << k;
+    }
+  }
+}
+
+// seff and u = eps / seff as box_u / box_u_given compute them
+__device__ __forceinline__ double small_u(const xpg_prepared &pp, int r, double sd, const SmallRow &w, double *u1,
+                                          double *u2) {
+  double seff;
+  if (pp.seff) {
+    seff = uni_d(pp.seff[r]);
+  } else {
+    const double psz = sqrt(w.Jc);
+    const double t = psz / sd;
+    seff = sd * sqrt(1.0 + (t * t) / 12.0);
+  }
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    u1[q] = w.e1[q] / seff;
+    u2[q] = w.e2[q] / seff;
+  }
+  return seff;
+}
+
+// box row of pixel p by a multiply and shift (p < 128, Dx <= 128: exact)
+__device__ __forceinline__ int small_row(int p, uint32_t mul) { return (int)(((uint32_t)p * mul) >> 16); }
+
+// template value of pixel (x, y) (xpg/integrate_kernels.py:68-75)
+__device__ __forceinline__ float small_template(const xpg_detector &det, double x, double y) {
+  if (!det.bkg) return 1.0f;
+  const int kx = min(max((int)rint(x - 1.0), 0), det.bkg_nx - 1);
+  const int ky = min(max((int)rint(y - 1.0), 0), det.bkg_ny - 1);
+  return det.bkg[(int64_t)ky * det.bkg_nx + kx];
+}
+
+// per-frame values into LDS (lane j): e3 = zeta (phi_c - phi) and the frame centre iz0 + fb (j + 1/2); the
+// foreground frames (R >= 1e-3 max R + 1e-6) as bits
+__device__ __forceinline__ uint32_t small_frames(const xpg_call &call, const Box &b, double zeta, const double *Rd,
+                                                 double *e3d, double *zzd, int lane) {
+  double Rm = -INFINITY;
+  for (int j = 0; j < b.Dz; ++j) Rm = fmax(Rm, Rd[j]);
+  const double thr = 1e-3 * Rm + 1e-6;
+  uint32_t fgz = 0;
+  for (int j = 0; j < b.Dz; ++j)
+    if (Rd[j] >= thr) fgz |= 1u << j;
+  if (lane < b.Dz) {
+    double lo, hi;
+    frame_bounds(call, b.iz0, b.fb, lane, lo, hi);
+    e3d[lane] = zeta * (0.5 * (lo + hi) - b.phi);
+    zzd[lane] = (double)b.iz0 + (double)b.fb * ((double)lane + 0.5);
+  }
+  wave_sync();
+  return fgz;
+}
+
+__global__ void __launch_bounds__(256) k_pass2_small(P2Args a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int r = blockIdx.x * a.waves + wid;
+  if (r >= a.R.n) return;
+  Box b = box_of(a.R, r);
+  if (!small_shape(b.Dz, b.Dy, b.Dx) || small_wave_bytes(b.Dz) > (size_t)a.wave_bytes) return;
+  if (a.R.has_p && uni_f(a.R.has_p[r]) > 0.5f) return;     // rows with a partner: the LDS kernel
+  unsigned char *base = smem + (size_t)wid * a.wave_bytes;
+  double *scr = (double *)base, *Rd = scr + 21 * b.Dz, *e3d = Rd + b.Dz, *zzd = e3d + b.Dz;
+  b.region = min(max(b.region, 0), a.n_regions - 1);       // memory safety; the host validates region
+  const int P = b.P, Dz = b.Dz, Dx = b.Dx;
+  const xpg_detector &det = a.det;
+  const xpg_call &call = a.call;
+  const double bgr = a.bg[r];
+  const double gain = call.gain;
+
+  SmallRow w;
+  small_load(b, a.prep, 0, w, lane);
+  const double zeta = uni_d(a.prep.zeta[r]);
+  if (a.prep.R) load_rocking(a.prep, r, Dz, Rd, lane);
+  else rocking_curve(call, b.iz0, b.fb, b.phi, zeta, Dz, scr, Rd, lane);
+  double u1[SPL], u2[SPL];
+  const double seff = small_u(a.prep, r, sd_of(call), w, u1, u2);
+  const uint32_t fgz = small_frames(call, b, zeta, Rd, e3d, zzd, lane);
+
+  // per pixel: profile value P2(u) J / seff^2 (xpg/integrate_kernels.py:278-285), template, foreground ellipse
+  const float *pr = a.prof + (int64_t)b.region * G * G;
+  const double s2 = seff * seff;
+  const uint32_t rmul = (65536u + (uint32_t)Dx - 1u) / (uint32_t)Dx;
+  double p2[SPL];
+  float brq[SPL];
+  uint32_t fg2q = 0;
+  double p2m = 0.0, Rm = 0.0;
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const int p = lane + q * WAVE;
+    const int pc = min(p, P - 1);
+    const int iy = small_row(pc, rmul), ix = pc - iy * Dx;
+    brq[q] = p < P ? small_template(det, pix_x(b.ix0, ix, b.s), pix_x(b.iy0, iy, b.s)) : 0.f;
+    const Bil bq = bilinear(u1[q], u2[q]);
+    const double va = (double)pr[bq.i2 * G + bq.i1], vb = (double)pr[bq.i2 * G + bq.i1 + 1];
+    const double vc = (double)pr[(bq.i2 + 1) * G + bq.i1], vd = (double)pr[(bq.i2 + 1) * G + bq.i1 + 1];
+    const double v = ((((1 - bq.t1) * (1 - bq.t2)) * va + (bq.t1 * (1 - bq.t2)) * vb) + ((1 - bq.t1) * bq.t2) * vc) +
+                     (bq.t1 * bq.t2) * vd;
+    p2[q] = (bq.inside ? v : 0.0) * (w.J[q] / s2);
+    if (p < P) {
+      p2m = fmax(p2m, p2[q]);
+      if (u1[q] * u1[q] + u2[q] * u2[q] <= 9.0) fg2q |= 1u << q;
+    } else {
+      p2[q] = 0.0;
+    }
+  }
+  for (int j = lane; j < Dz; j += WAVE) Rm = fmax(Rm, Rd[j]);
+  const double pmax = wmax_d(p2m) * wmax_d(Rm);
+  const double pthr = 0.01 * pmax;
+
+  // fit mask and the starting intensity: p > 0.01 max p on valid voxels outside other spots (no partner: pp = +0)
+  float netr[SNK], pfr[SNK], b0r[SPL];
+  uint32_t usem = 0, validm = 0, fgm = 0;
+  double s_c = 0.0, s_b = 0.0, s_pw = 0.0;
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const int p = lane + q * WAVE;
+    const bool pin = p < P;
+    const bool fg2 = (fg2q >> q) & 1u;
+    const float b0 = pin ? (float)(bgr * (double)brq[q]) : 0.f;
+    const float p2f = (float)p2[q];
+    b0r[q] = b0;
+    double cs = 0.0, pw = 0.0;
+    int cnt = 0;
+#pragma unroll
+    for (int j = 0; j < SDZ; ++j) {
+      const int k = q * SDZ + j;
+      netr[k] = 0.f;
+      pfr[k] = 0.f;
+      if (pin && j < Dz) {
+        const float c = w.c[k];
+        const bool vn = c >= 0.f && !((w.nbm >> k) & 1u);
+        const double Rj = Rd[j];
+        const double pd = p2[q] * Rj;
+        pfr[k] = p2f * (float)Rj;
+        netr[k] = c - b0;
+        if (vn) validm |= 1u << k;
+        if (fg2 && ((fgz >> j) & 1u)) fgm |= 1u << k;
+        if (vn && pd > pthr) {
+          usem |= 1u << k;
+          cs += (double)c;
+          pw += pd + 0.0;
+          ++cnt;
+        }
+      }
+    }
+    if (pin) {
+      s_c += cs;
+      s_b += (double)cnt * (bgr * (double)brq[q]);
+      s_pw += pw;
+    }
+  }
+  if (a.out.vmask) {                      // per-voxel classes for mask comparisons (only when asked for)
+    uint8_t *vm = a.out.vmask + b.voff;
+#pragma unroll
+    for (int q = 0; q < SPL; ++q) {
+      const int p = lane + q * WAVE;
+      if (p >= P) continue;
+      for (int j = 0; j < Dz; ++j) {
+        const int k = q * SDZ + j;
+        vm[j * P + p] = (((usem >> k) & 1u) ? XPG_VM_USE : 0) | (((w.nbm >> k) & 1u) ? XPG_VM_NBM : 0);
+      }
+    }
+  }
+  double s4[4] = {s_c, s_b, s_pw, 0.0};
+  wsum_multi(s4);
+  double I = (s4[0] - s4[1]) / fmax(s4[2], 1e-6);
+  double Ip = I * 0.0;
+
+  // ---- IRLS (xpg/integrate_kernels.py:445-467): each reflection stops on its own test ----
+  const float gainf = (float)gain;
+  double a11 = 0.0, a12 = 0.0, a22 = 1.0, det2 = 1.0, Iw = I;
+  for (int it_ = 0; it_ < N_IRLS; ++it_) {
+    Iw = I;
+    const float Ii = (float)fmax(I, 0.0);
+    double s11 = 0.0, sb1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < SNK; ++k) {
+      if ((usem >> k) & 1u) {
+        const float pf = pfr[k], net = netr[k];
+        const float mu = b0r[k / SDZ] + Ii * pf;         // the partner term is +0 here
+        const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));
+        s11 += (double)((pf * pf) * iv);
+        sb1 += (double)((pf * net) * iv);
+      }
+    }
+    double t2[2] = {s11, sb1};
+    wsum_multi(t2);
+    a11 = t2[0];
+    const double b1 = t2[1];
+    a12 = 0.0;
+    a22 = 0.0 + (1.0 - 0.0) * 1.0;
+    det2 = fmax(a11 * a22 - a12 * a12, 1e-30);
+    const double I_old = I;
+    I = (a22 * b1 - a12 * 0.0) / det2;
+    Ip = (a11 * 0.0 - a12 * b1) / det2 * 0.0;
+    if (IRLS_TOL > 0 && it_ >= 1 && it_ < N_IRLS - 1) {
+      const double dz = fabs(I - I_old) * sqrt(fmax(det2 / fmax(a22, 1e-30), 0.0));
+      if (dz < IRLS_TOL) break;     // uniform: every lane holds the same reduced values
+    }
+  }
+
+  // ---- tail (xpg/integrate_kernels.py:405-429): weights of the last pass (intensity Iw) ----
+  const float Ii = (float)fmax(Iw, 0.0);
+  double sC = 0.0, s110 = 0.0, sC0 = 0.0, swp = 0.0, sres = 0.0;
+  int nuse = 0;
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    const float br = lane + q * WAVE < P ? brq[q] : 0.f;
+    const float iv0 = __builtin_amdgcn_rcpf(gainf * fmaxf(b0r[q], (float)VAR_FLOOR));
+#pragma unroll
+    for (int j = 0; j < SDZ; ++j) {
+      const int k = q * SDZ + j;
+      if ((usem >> k) & 1u) {
+        const float pf = pfr[k], net = netr[k];
+        const float mu = b0r[q] + Ii * pf;
+        const float iv = __builtin_amdgcn_rcpf(gainf * fmaxf(mu, (float)VAR_FLOOR));
+        const float pb = pf * br;
+        sC += (double)(pb * iv);
+        s110 += (double)((pf * pf) * iv0);
+        sC0 += (double)(pb * iv0);
+        swp += (double)pf;
+        sres += (double)net;
+        nuse += 1;
+      }
+    }
+  }
+  double t1[8] = {sC, s110, sC0, swp, sres, (double)nuse, 0.0, 0.0};   // the count is exact as float64
+  wsum_multi(t1);
+  const double C = t1[0], A110 = t1[1], C0 = t1[2], Swp = t1[3], Sres = t1[4];
+  nuse = (int)t1[5];
+  const double nu = fmax((double)nuse, 1.0);
+  const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
+  double sdd = 0.0, sdp = 0.0, snn = 0.0, s_pv = 0.0, s_pt = 0.0, s_pfg = 0.0;
+#pragma unroll
+  for (int q = 0; q < SPL; ++q) {
+    if (lane + q * WAVE >= P) continue;
+#pragma unroll
+    for (int j = 0; j < SDZ; ++j) {
+      if (j >= Dz) continue;
+      const int k = q * SDZ + j;
+      const float pf = pfr[k];
+      s_pt += (double)pf;
+      if ((validm >> k) & 1u) s_pv += (double)pf;
+      if ((fgm >> k) & 1u) s_pfg += (double)pf;
+      if (!((usem >> k) & 1u)) continue;
+      const float dp = pf - pm;
+      const float dn = netr[k] - nm;       // (net - Ipos pp) - nm with pp = +0
+      sdp += (double)(dp * dn);
+      sdd += (double)(dp * dp);
+      snn += (double)(dn * dn);
+    }
+  }
+  double t6[8] = {sdp, sdd, snn, s_pv, s_pt, s_pfg, 0.0, 0.0};
+  wsum_multi(t6);
+  const double Sdp = t6[0], Sdd = t6[1], Snn = t6[2];
+  const double Spv = t6[3], Spt = t6[4], Spfg = t6[5];
+  if (lane == 0) {
+    const double var_b = gain * fmax(bgr, VAR_FLOOR) / fmax(a.nbg[r], 1.0);
+    const double a11i = a22 / det2;
+    const double varI = a11i + (C * a11i) * (C * a11i) * var_b;
+    const double a110 = fmax(A110, 1e-12);
+    const double infl = a11i * a11;
+    const double var0 = (1.0 / a110 + (C0 / a110) * (C0 / a110) * var_b) * fmax(infl, 1.0);
+    const xpg_pass2_out &o = a.out;
+    o.iprf[r] = I;
+    o.vprf[r] = varI;
+    o.vprf0[r] = var0;
+    o.corr[r] = Sdp / sqrt(fmax(Sdd * Snn, 1e-20));
+    o.p_valid[r] = Spv;
+    o.p_total[r] = Spt;
+    o.ipartner[r] = Ip;
+    o.rho[r] = 0.0;
+    o.p_fg[r] = Spfg / fmax(Spt, 1e-9);
+  }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------------------------------------------------
 
@@ -1860,9 +2212,11 @@ static int check_frames(const xpg_frames *frames) {
   return XPG_OK;
 }
 
-static int check_prep(const xpg_prepared *prep) {
+static int check_prep(const xpg_prepared *prep, const xpg_refl *refl = nullptr) {
   if (!prep || !prep->sb || !prep->geo || !prep->zeta || !prep->nbm || !prep->geo_p || !prep->zeta_p)
     return fail(XPG_ERR_ARG, "incomplete xpg_prepared");
+  if (refl && prep->R && prep->r_stride < refl->Dz)
+    return fail(XPG_ERR_ARG, "rocking-curve table: r_stride %d < Dz %d", prep->r_stride, refl->Dz);
   return XPG_OK;
 }
 
@@ -2017,6 +2371,21 @@ static int pass2_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   a.bg = bg;
   a.nbg = nbg;
   a.out = *out;
+  a.only_partner = 0;
+  if (prep && small_shape(refl->Dz, refl->Dy, refl->Dx)) {
+    // prepared rows of at most 128 pixels x 8 frames: rows without a partner in the register kernel, rows with one
+    // (if the table has partners) in the LDS kernel, launched over the same rows after it
+    P2Args s = a;
+    s.waves = refl->wg_waves > 0 && refl->wg_waves < 4 ? refl->wg_waves : 4;
+    s.wave_bytes = (int)small_wave_bytes(refl->Dz);
+    s.vals_global = 0;
+    const size_t slds = (size_t)s.waves * s.wave_bytes;
+    hipLaunchKernelGGL(k_pass2_small, dim3((refl->n + s.waves - 1) / s.waves), dim3(64 * s.waves), slds,
+                       (hipStream_t)stream, s);
+    HIP_TRY(hipGetLastError());
+    if (!refl->has_p) return XPG_OK;
+    a.only_partner = 1;
+  }
   size_t lds;
   rc = launch_shape(refl, 2, sb != nullptr, &a.waves, &a.wave_bytes, &lds, &a.vals_global);
   if (rc) return rc;
@@ -2046,7 +2415,7 @@ int xpg_pass2_frames(const xpg_detector *det, const xpg_call *call, const xpg_re
 
 int xpg_pass1_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
                        const xpg_pass1_out *out, const xpg_profile_acc *acc, void *stream) {
-  const int rc = check_prep(prep);
+  const int rc = check_prep(prep, refl);
   if (rc) return rc;
   return pass1_impl(det, call, refl, prep->sb, nullptr, out, acc, stream, prep);
 }
@@ -2054,9 +2423,23 @@ int xpg_pass1_prepared(const xpg_detector *det, const xpg_call *call, const xpg_
 int xpg_pass2_prepared(const xpg_detector *det, const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep,
                        const float *prof, int32_t n_regions, const double *bg, const double *nbg, int32_t irls_rule,
                        const xpg_pass2_out *out, void *stream) {
-  const int rc = check_prep(prep);
+  const int rc = check_prep(prep, refl);
   if (rc) return rc;
   return pass2_impl(det, call, refl, prep->sb, nullptr, prof, n_regions, bg, nbg, irls_rule, out, stream, prep);
+}
+
+int xpg_rocking(const xpg_call *call, const xpg_refl *refl, const xpg_prepared *prep, void *stream) {
+  int rc = check_refl(refl, false);
+  if (rc) return rc;
+  if (!call || !prep || !prep->R || !prep->zeta) return fail(XPG_ERR_ARG, "call / prep / prep->R / prep->zeta is NULL");
+  if (prep->r_stride < refl->Dz) return fail(XPG_ERR_ARG, "r_stride %d < Dz %d", prep->r_stride, refl->Dz);
+  if (refl->n == 0) return XPG_OK;
+  const int64_t total = (int64_t)refl->n * prep->r_stride;
+  if (total > ((int64_t)1 << 31) - 256) return fail(XPG_ERR_ARG, "too many rocking-curve values in one launch");
+  hipLaunchKernelGGL(k_rocking, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, (hipStream_t)stream, *call, *refl,
+                     *prep);
+  HIP_TRY(hipGetLastError());
+  return XPG_OK;
 }
 
 int xpg_lds_limit(void) { return lds_limit(); }

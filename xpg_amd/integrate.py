@@ -622,10 +622,12 @@ class Sweep:
                         zeta=torch.empty(m, dtype=torch.float64, device=dev),
                         nbm=torch.empty(nV, dtype=torch.uint8, device=dev),
                         geo_p=torch.empty(3 * nP if hasp else 1, dtype=torch.float64, device=dev),
-                        zeta_p=torch.empty(m if hasp else 1, dtype=torch.float64, device=dev))
+                        zeta_p=torch.empty(m if hasp else 1, dtype=torch.float64, device=dev),
+                        R=torch.empty(m * mx[0], dtype=torch.float64, device=dev))
             ps = L.Prepared()
             for k, t in bufs.items():
                 setattr(ps, k, L.ptr(t))
+            ps.r_stride = int(mx[0])
             self.prep[bi] = (ps, bufs)
         return self.prep[bi][0]
 
@@ -640,6 +642,7 @@ class Sweep:
             setattr(g, k, getattr(ps, k))
         g.zeta = ps.zeta + 8 * d
         g.zeta_p = ps.zeta_p + 8 * d if self.prep[bi][1]["zeta_p"].numel() > 1 else ps.zeta_p
+        g.R, g.r_stride = ps.R + 8 * d * ps.r_stride, ps.r_stride
         return g
 
     def _widths_ptr(self, slot):
@@ -864,6 +867,8 @@ class Sweep:
             out.fz_stride = self.fz.shape[1]
             s1 = aux if where[u] else main
             sp1 = s1.cuda_stream if where[u] else st
+            if pp is not None:          # the rows' rocking curves for pass 1's widths, ahead of it
+                L.check(lib.xpg_rocking(C.byref(call1), C.byref(view.s), C.byref(pp), sp1))
             with (timer("pass1", b, (start, m), s1) if timer else nullctx):
                 a_ = C.byref(acc) if acc is not None else None
                 if pp is not None:
@@ -901,6 +906,8 @@ class Sweep:
                 for k in K.P2_F64:
                     setattr(out, k, rb.p2(k, start))
                 bg, nbg = rb.p1("bg", start), rb.p1("nbg", start)
+                if pp is not None:      # the rows' rocking curves for pass 2's widths (pass 1 has read its own)
+                    L.check(lib.xpg_rocking(C.byref(call2), C.byref(view.s), C.byref(pp), sst))
                 with (timer("pass2", b, (start, m), side) if timer else nullctx):
                     if pp is not None:
                         L.check(lib.xpg_pass2_prepared(C.byref(det.s), C.byref(call2), C.byref(view.s), C.byref(pp),
@@ -927,12 +934,17 @@ class Sweep:
                     zeta=torch.empty(max(nR, 1), dtype=torch.float64, device=dev),
                     nbm=torch.empty(max(nV, 1), dtype=torch.uint8, device=dev),
                     geo_p=torch.empty(max(3 * nP, 1), dtype=torch.float64, device=dev),
-                    zeta_p=torch.empty(max(nR, 1), dtype=torch.float64, device=dev))
+                    zeta_p=torch.empty(max(nR, 1), dtype=torch.float64, device=dev),
+                    R=torch.empty(max(nR, 1) * self.rocking_stride(), dtype=torch.float64, device=dev))
         return bufs
+
+    def rocking_stride(self):
+        """Doubles per row of the arena's rocking-curve table: the deepest box of the sweep."""
+        return int(max((mx[0] for b, lb, start, m, mx, wl in self.sp.bins), default=1))
 
     def arena_bytes(self):
         nV, nP, nR = self.sp.block_arena
-        return 5 * nV + 48 * nP + 16 * nR
+        return 5 * nV + 48 * nP + 16 * nR + 8 * nR * self.rocking_stride()
 
     def enqueue_block(self, i, store, arena, rb):
         """The i-th block of the order on the current stream only, outputs into ``rb`` (RunBufs), reading frames
@@ -957,6 +969,8 @@ class Sweep:
             pp.geo_p = arena["geo_p"].data_ptr() + 24 * p0
             pp.zeta = arena["zeta"].data_ptr() + 8 * r0
             pp.zeta_p = arena["zeta_p"].data_ptr() + 8 * r0
+            pp.r_stride = self.rocking_stride()
+            pp.R = arena["R"].data_ptr() + 8 * r0 * pp.r_stride
             L.check(L.lib().xpg_prepare(C.byref(det.s), C.byref(callp), C.byref(sp.refl_view(start, m, shape, True, wl).s),
                                         C.byref(store), C.byref(pp), st))
             units.append((start, m, shape, wl, pp))
