@@ -2,12 +2,14 @@
 """Benchmark: profile-fitted reflections/s of the MI355X integration path on one GPU.
 
 A step is one pass of the integration hot path over the whole sweep with the frames resident in HBM: for every phi
-block, pass 1 with in-kernel shoebox extraction and fused profile accumulation, the adaptive-width fits, the float64
-profile normalisation and pass 2 (Poisson profile fit).  Host planning (prediction, box plan, neighbour lists) and
-the upload of the reflection table happen once before the timed region, like the frames.
+block, the shoebox gather, pass 1 with fused profile accumulation, the adaptive-width fits, the float64 profile
+normalisation and pass 2 (Poisson profile fit), then every per-reflection result copied to pinned host memory.  Host
+planning (prediction, box plan, neighbour lists) and the upload of the reflection table happen once before the timed
+region, like the frames; their time is reported beside the value (``plan``).
 
-Prints one JSON line (metric, value = reflections/s, roofline of the dominant kernel from HIP events on the launch
-stream, CPU baseline = the float64 oracle on a bounded sample timed on this host's cores, parity of that sample).
+Prints one JSON line: metric, value = reflections/s, the roofline of the kernel with the most device time (HIP events
+on its launch stream, algorithmic bytes per launch), the CPU baseline (the reference CPU path restated: host shoebox
+gather plus the oracle's passes in float32 on every host core, over a bounded sample) and the parity of that sample.
 """
 from __future__ import annotations
 
@@ -34,10 +36,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def per_refl_bytes(V, Dz, K, kind):
-    """Algorithmic HBM bytes of one reflection in one pass launch: half of the per-reflection figure 8 V + 160 B of
-    the step's byte count below (each pass reads the shoebox's V voxels once as 4-byte values, plus 80 B of
-    parameters in and results out)."""
+def per_refl_bytes(V, Dz, kind):
+    """Algorithmic HBM bytes of one reflection in one launch.  Shoebox gather (prepare), pass 1 and pass 2: 4 V + 80,
+    the shoebox's V voxels read once as 4-byte values plus 80 B of parameters in and results out (each pass is half of
+    the step's per-reflection figure 8 V + 160 below; the gather reads what the fused form's pass 1 reads).  Rocking
+    curves: 8 Dz + 48, the curve written and the row's parameters read."""
+    if kind == "rocking":
+        return 8 * Dz + 48
     return 4 * V + 80
 
 
@@ -79,71 +84,168 @@ def step_bytes(hp, nframes, ny, nx, device):
     return dict(B_alg=b_alg, B_min=b_min, union_voxels=U, sum_V=int(V.sum()), reflections=n)
 
 
-def cpu_baseline(sweep, hp, spec, n_sample, seconds):
-    """The float64 oracle (oracle/kernels.py, numpy, one core) on a bounded sample of the same workload: the
-    shoeboxes of n_sample reflections of one block, extracted on the device and copied to the host, run through
-    pass 1, profile accumulation, normalisation and pass 2 (IRLS per-reflection rule); also the HIP results of the
-    same rows for parity."""
+def host_cores():
+    """Host cores this process may use (affinity), at most the 16 of one GPU's share of the machine."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+CHUNK_ELEMENTS = 15_000_000       # voxels per kernel call of the reference (xpg/integrate.py:56)
+
+
+def cpu_baseline(sweep, hp, spec, frames, n_sample, seconds, threads):
+    """The reference's CPU path restated and timed on this host: the host shoebox gather (frame codes, padded frames,
+    shoebox windows; xpg/integrate.py:648-803, on min(8, cores) gather threads as :671) and the oracle's pass 1,
+    profile accumulation, normalisation and pass 2 in float32, the precision of the reference's tensor program, over
+    chunks of at most 1.5e7 voxels (xpg/integrate.py:56), each chunk's rows split over ``threads`` threads (the
+    reference's torch intra-op threads, xpg/backend.py:60-69).  The sample is one block's largest shoebox group (all
+    of its frames, so that the per-frame host work is amortised as in a whole run).
+
+    Also returns the parity record of the same rows: the float64 oracle against the HIP passes (same shoeboxes, default
+    widths, the oracle's profiles), the float32 restatement against the float64 oracle (the reference precision's own
+    drift), and whether the host gather equals the device extraction."""
     import numpy as np
     import torch
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import extract as ox
     from oracle import kernels as ok
     from xpg_amd import integrate_kernels as K
     from xpg_amd.geometry import detector_consts
 
     sp = sweep.sp
-    # the largest group of the middle block
     b_mid = hp["block_order"][len(hp["block_order"]) // 2]
     gi, (B, bx, s), start, m = max(sweep.by_block[b_mid], key=lambda t: t[3])
-    m = min(m, n_sample)
+    if n_sample:
+        m = min(m, n_sample)
     view = sp.refl_view(start, m, (B, bx, bx))
-    sb = K.run_extract(sweep.store, view, sweep.device).cpu().numpy().astype(np.float64)
     rows = sp.perm[start:start + m]
     refl = hp["refl"]
     exp = sweep.exp_blocks[b_mid]
     g = exp.geom
     det = detector_consts(g)
     det.update(nx=g.nx, ny=g.ny)
-    rc = dict(S=refl["S"][rows], phi=np.radians(refl["phi"][rows]), ix0=hp["ix0"][rows], iy0=hp["iy0"][rows],
-              iz0=hp["iz0"][rows], fb=hp["fb"][rows].astype(float), region=hp["region"][rows],
-              has_p=hp["has_partner"][rows].astype(float), pS=refl["S"][hp["pj"][rows]],
-              pphi=np.radians(refl["phi"][hp["pj"][rows]]))
+    ix0, iy0, iz0, fb = hp["ix0"][rows], hp["iy0"][rows], hp["iz0"][rows], hp["fb"][rows]
+    z0 = int(max(0, iz0.min()))                  # boxes may start before the first frame or end after the last
+    z1 = int(min(frames.n, (iz0 + B * fb).max()))
+    # the block's frames in host memory as the reference reads them (raw uint32 counts): outside the timed region
+    raw = {z: frames.data[z].cpu().numpy().view(np.uint32) for z in range(z0, z1)}
+    rc = dict(S=refl["S"][rows], phi=np.radians(refl["phi"][rows]), ix0=ix0, iy0=iy0, iz0=iz0,
+              fb=fb.astype(float), region=hp["region"][rows], has_p=hp["has_partner"][rows].astype(float),
+              pS=refl["S"][hp["pj"][rows]], pphi=np.radians(refl["phi"][hp["pj"][rows]]))
+    if s > 1:
+        rc["pxs"] = np.full(m, float(s))
     nb = {k: v[rows] for k, v in hp["nb"].items()}
     sd, sm = np.radians(sweep.model.sigma_d), np.radians(sweep.model.sigma_m)
     frames_phi0, osc = np.radians(g.z_to_phi(0.0)), np.radians(g.osc)
     rcw = ok.rc_default_weights()
-    a = (rc, det, g.axis, g.s0, frames_phi0, osc, sd, sm, 1.0)
+    a = (det, g.axis, g.s0, frames_phi0, osc, sd, sm, 1.0)
+    pool = ThreadPoolExecutor(threads)
+    gpool = ThreadPoolExecutor(min(8, threads))
+    # row ranges: chunks of the reference's size, each split over the threads
+    per_chunk = max(1, CHUNK_ELEMENTS // (B * bx * bx))
+    parts = []
+    for c0 in range(0, m, per_chunk):
+        c1 = min(m, c0 + per_chunk)
+        step = -(-(c1 - c0) // threads)
+        parts.append([(r0, min(c1, r0 + step)) for r0 in range(c0, c1, step)])
+    sub = lambda d, r0, r1: {k: (v[r0:r1] if np.ndim(v) else v) for k, v in d.items()}
+    pad = bx // 2 + 1
+
+    def host_gather():
+        def one(z):
+            P = ox.prepared_frame(ox.frame_codes(raw[z], flag_value=frames.flag_value, overload=frames.overload), pad)
+            if s > 1:
+                P = ox.superpixel_frame(P[pad:pad + g.ny, pad:pad + g.nx], s, pad)
+            return z, P
+        padded = dict(gpool.map(one, range(z0, z1)))
+        sb = np.zeros((m, B, bx, bx), np.float32)
+        ny_p, nx_p = next(iter(padded.values())).shape
+
+        def rows_of(r0, r1):
+            for r in range(r0, r1):
+                iy = int(np.clip(iy0[r] + pad, 0, ny_p - bx))
+                ix = int(np.clip(ix0[r] + pad, 0, nx_p - bx))
+                for k in range(B):
+                    z = int(iz0[r]) + k
+                    if fb[r] == 1 and z in padded:
+                        sb[r, k] = padded[z][iy:iy + bx, ix:ix + bx] + ox.SB_OFF
+            return None
+        list(gpool.map(lambda t: rows_of(*t), [t for ch in parts for t in ch]))
+        if (fb > 1).any():                        # binned rows: the oracle's own gather
+            sel = np.flatnonzero(fb > 1)
+            sb[sel] = ox.gather({z: ox.frame_codes(raw[z], flag_value=frames.flag_value, overload=frames.overload)
+                                 for z in range(z0, z1)}, range(z0, z1), ix0[sel], iy0[sel], iz0[sel], fb[sel], s, B,
+                                bx) + ox.SB_OFF
+        return sb - ox.SB_OFF
+
+    def one_run(ft):
+        ok.FT = ft
+        try:
+            sb = host_gather()
+            o1s = [None] * m
+            res1 = {}
+            for ch in parts:
+                for (r0, r1), o1 in zip(ch, pool.map(lambda t: ok.pass1(sb[t[0]:t[1]], sub(rc, *t), *a,
+                                                                         sub(nb, *t), rcw), ch)):
+                    res1[(r0, r1)] = o1
+            cat = lambda k: np.concatenate([res1[t][k] for ch in parts for t in ch])
+            o1 = {k: cat(k) for k in ("isum", "vsum", "n_over", "frac_rec", "n_fg_nb", "n_fg_masked", "bg", "nbg")}
+            strong = ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10)
+                      & (o1["n_over"] == 0) & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0)
+                      & (o1["n_fg_masked"] == 0))
+
+            def acc(t):
+                r = res1[t]
+                return ok.accumulate_profiles(r["img2"], r["all_valid2"], r["J"], r["u1"], r["u2"],
+                                              rc["region"][t[0]:t[1]], r["isum"], strong[t[0]:t[1]], r["seff"], 9)
+            grids = list(pool.map(acc, [t for ch in parts for t in ch]))
+            prof = ok.normalise_profiles(sum(n_ for n_, _ in grids), sum(d_ for _, d_ in grids))
+            res2 = {}
+            for ch in parts:
+                for t, o2 in zip(ch, pool.map(lambda t: ok.pass2(sb[t[0]:t[1]], sub(rc, *t), *a, prof,
+                                                                  o1["bg"][t[0]:t[1]], o1["nbg"][t[0]:t[1]],
+                                                                  sub(nb, *t), rcw, irls_rule="reflection"), ch)):
+                    res2[t] = o2
+            o2 = {k: np.concatenate([res2[t][k] for ch in parts for t in ch]) for k in ("iprf", "vprf")}
+            return sb, o1, prof, o2
+        finally:
+            ok.FT = np.float64
+
+    # the float64 oracle once (parity), then the float32 restatement timed
+    sb64, o1_64, prof64, o2_64 = one_run(np.float64)
     t0 = time.perf_counter()
     reps = 0
     while True:
-        o1 = ok.pass1(sb, *a, nb, rcw)
-        strong = ((o1["isum"] > 0) & (o1["isum"] / np.sqrt(np.maximum(o1["vsum"], 1e-6)) > 10) & (o1["n_over"] == 0)
-                  & (o1["frac_rec"] > 0.95) & (o1["n_fg_nb"] == 0) & (o1["n_fg_masked"] == 0))
-        num, den = ok.accumulate_profiles(o1["img2"], o1["all_valid2"], o1["J"], o1["u1"], o1["u2"], rc["region"],
-                                          o1["isum"], strong, o1["seff"], 9)
-        prof = ok.normalise_profiles(num, den)
-        o2 = ok.pass2(sb, *a, prof, o1["bg"], o1["nbg"], nb, rcw, irls_rule="reflection")
+        sb32, o1_32, prof32, o2_32 = one_run(np.float32)
         reps += 1
         el = time.perf_counter() - t0
-        if el > seconds or reps >= 50:
+        if el > seconds or reps >= 20:
             break
+    pool.shutdown()
+    gpool.shutdown()
     rate = reps * m / el
-    # the same rows on the device with the same inputs (default widths, these profiles)
+    # the same rows on the device with the same inputs (default widths, the oracle's profiles)
     dev = sweep.device
+    sb_dev = K.run_extract(sweep.store, view, dev).cpu().numpy()
     detd = K.device_detector(det, dev)
     call = K.make_call(g.axis, g.s0, frames_phi0, osc, sd, sm, 1.0, rcw)
     r1 = K.run_pass1(detd, call, view, frames=sweep.store, pixel_outputs=False)
-    r2 = K.run_pass2(detd, call, view, torch.as_tensor(prof, device=dev), r1["bg"], r1["nbg"], frames=sweep.store)
+    r2 = K.run_pass2(detd, call, view, torch.as_tensor(prof64, device=dev), r1["bg"], r1["nbg"], frames=sweep.store)
     torch.cuda.synchronize(dev)
     hI = r2["iprf"].cpu().numpy()
     hV = r2["vprf"].cpu().numpy()
-    sig = np.sqrt(np.maximum(o2["vprf"], 1e-30))
-    dI = np.abs(hI - o2["iprf"]) / sig
-    ok_ = np.isfinite(dI) & (o2["vprf"] > 0)
-    # per decile of the oracle's intensity: |dI|/sigma and the error of I/sigma
+    sig = np.sqrt(np.maximum(o2_64["vprf"], 1e-30))
+    ok_ = np.isfinite(hI) & np.isfinite(o2_64["iprf"]) & (o2_64["vprf"] > 0)
+    dI = np.abs(hI - o2_64["iprf"]) / sig
+    d32 = np.abs(o2_32["iprf"] - o2_64["iprf"]) / sig
+    ok32 = ok_ & np.isfinite(o2_32["iprf"])
     deciles = []
     if ok_.sum() >= 10:
-        I0 = o2["iprf"][ok_]
-        zs = o2["iprf"][ok_] / sig[ok_] - hI[ok_] / np.sqrt(np.maximum(hV[ok_], 1e-30))
+        I0 = o2_64["iprf"][ok_]
+        zs = o2_64["iprf"][ok_] / sig[ok_] - hI[ok_] / np.sqrt(np.maximum(hV[ok_], 1e-30))
         edges = np.quantile(I0, np.linspace(0, 1, 11))
         idx = np.clip(np.searchsorted(edges, I0, side="right") - 1, 0, 9)
         for d in range(10):
@@ -155,14 +257,17 @@ def cpu_baseline(sweep, hp, spec, n_sample, seconds):
                                     max_d_I_over_sigma=float(np.abs(zs[sel]).max())))
     parity = dict(sample=int(m), max_dI_over_sigma=float(np.max(dI[ok_])) if ok_.any() else None,
                   median_dI_over_sigma=float(np.median(dI[ok_])) if ok_.any() else None,
-                  n_over_equal=bool(np.array_equal(r1["n_over"].cpu().numpy(), o1["n_over"])),
-                  n_fg_nb_equal=bool(np.array_equal(r1["n_fg_nb"].cpu().numpy(), o1["n_fg_nb"])),
-                  n_fg_masked_equal=bool(np.array_equal(r1["n_fg_masked"].cpu().numpy(), o1["n_fg_masked"])),
+                  host_gather_equal_device=bool(np.array_equal(sb64, sb_dev)),
+                  n_over_equal=bool(np.array_equal(r1["n_over"].cpu().numpy(), o1_64["n_over"])),
+                  n_fg_nb_equal=bool(np.array_equal(r1["n_fg_nb"].cpu().numpy(), o1_64["n_fg_nb"])),
+                  n_fg_masked_equal=bool(np.array_equal(r1["n_fg_masked"].cpu().numpy(), o1_64["n_fg_masked"])),
+                  cpu_float32_vs_oracle_max_dI_over_sigma=float(np.max(d32[ok32])) if ok32.any() else None,
                   by_intensity_decile=deciles)
-    base = dict(value=round(rate, 2), unit="reflections/s", cores=1, kind="port",
-                sample=f"{m} reflections ({B}x{bx}x{bx} shoeboxes, block {b_mid}) x {reps} repetitions, float64 "
-                       f"oracle pass1 + profile accumulation + normalisation + pass2 on shoeboxes extracted on the "
-                       f"device; {el:.1f} s")
+    base = dict(value=round(rate, 2), unit="reflections/s", cores=threads, kind="port",
+                sample=f"block {b_mid}: {m} reflections ({B}x{bx}x{bx} shoeboxes) x {reps} repetitions; host gather "
+                       f"from {z1 - z0} raw frames in host memory ({min(8, threads)} threads) + float32 oracle pass 1, "
+                       f"profile accumulation, normalisation, pass 2 in {len(parts)} chunk(s) of <= 1.5e7 voxels over "
+                       f"{threads} threads; {el:.1f} s")
     return base, parity
 
 
@@ -219,6 +324,47 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
         f"- dispatches: {len(fk)}\n- calibration: one copy-like kernel over {copy_mib} MiB read and "
         f"written (`--calibrate-copy {copy_mib}`), counters read {f_fac:.4f} x and {w_fac:.4f} x the known bytes\n")
     return rec
+
+
+def summarize_sq(csv_path, workload, command, kernels):
+    """Per-wavefront instruction counts of the sweep's kernels from one rocprofv3 --pmc run of SQ counters
+    (SQ_WAVES, SQ_INSTS_VALU, _SALU, _LDS, _VMEM_RD, _VMEM_WR, SQ_WAVE_CYCLES, SQ_BUSY_CYCLES): writes
+    profiles/counters_<workload>.md and .json."""
+    import csv
+    import re
+    tot, disp, meta = {}, {}, {}
+    for r in csv.DictReader(open(csv_path)):
+        name = r["Kernel_Name"]
+        k = next((k for k in kernels if re.search(r"\b" + re.escape(k) + r"\b", name)), None)
+        if k is None:
+            continue
+        tot.setdefault(k, {}).setdefault(r["Counter_Name"], 0.0)
+        tot[k][r["Counter_Name"]] += float(r["Counter_Value"])
+        disp.setdefault(k, set()).add(r["Dispatch_Id"])
+        meta[k] = dict(vgpr=int(r["VGPR_Count"]), sgpr=int(r["SGPR_Count"]), workgroup=int(r["Workgroup_Size"]))
+    rows = []
+    for k in kernels:
+        if k not in tot:
+            continue
+        t, n = tot[k], len(disp[k])
+        w = max(t.get("SQ_WAVES", 0.0), 1.0)
+        rows.append(dict(kernel=k, dispatches=n, waves_per_dispatch=round(w / n),
+                         valu_per_wave=round(t.get("SQ_INSTS_VALU", 0) / w), salu_per_wave=round(t.get("SQ_INSTS_SALU", 0) / w),
+                         lds_per_wave=round(t.get("SQ_INSTS_LDS", 0) / w), vmem_rd_per_wave=round(t.get("SQ_INSTS_VMEM_RD", 0) / w),
+                         vmem_wr_per_wave=round(t.get("SQ_INSTS_VMEM_WR", 0) / w),
+                         wave_cycles_per_wave=round(t.get("SQ_WAVE_CYCLES", 0) / w), **meta[k]))
+    base = ROOT / "profiles" / f"counters_{workload}"
+    base.with_suffix(".json").write_text(json.dumps(dict(workload=workload, command=command, kernels=rows), indent=1) + "\n")
+    hdr = "| kernel | dispatches | waves / dispatch | VALU / wave | SALU / wave | LDS / wave | VMEM rd / wave | VMEM wr / wave | SQ_WAVE_CYCLES / wave | VGPR | SGPR | workgroup |"
+    lines = [f"# Instruction counters per wavefront ({workload})", "", f"Command: `{command}`", "",
+             "One wavefront per reflection in every kernel but k_rocking (one lane per reflection and frame).  "
+             "SQ_WAVE_CYCLES per wavefront is the counter as reported, summed over the dispatch's wavefronts and divided by their number.", "", hdr, "|" + "---|" * 12]
+    for r in rows:
+        lines.append(f"| {r['kernel']} | {r['dispatches']} | {r['waves_per_dispatch']} | {r['valu_per_wave']} | "
+                     f"{r['salu_per_wave']} | {r['lds_per_wave']} | {r['vmem_rd_per_wave']} | {r['vmem_wr_per_wave']} | "
+                     f"{r['wave_cycles_per_wave']} | {r['vgpr']} | {r['sgpr']} | {r['workgroup']} |")
+    base.with_suffix(".md").write_text("\n".join(lines) + "\n")
+    return rows
 
 
 def bench_serve(args, dev):
@@ -482,8 +628,10 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=0.8, help="offered load as a fraction of measured capacity")
     ap.add_argument("--frames", type=int, default=0, help="override the number of frames (smaller runs)")
-    ap.add_argument("--cpu-sample", type=int, default=2000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="CPU baseline: at most this many reflections of the sample group (0: the whole group)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (0: every host core, <= 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "batched", "plain"],
                     help="batched (the reported value) and plain (reported beside it), or one of them")
@@ -501,12 +649,18 @@ def main():
                          "the reference of known traffic for hardware byte counters")
     ap.add_argument("--summarize-stats", nargs=3, metavar=("STATS_CSV", "COMMAND", "OUT_MD"),
                     help="write a markdown summary of a rocprofv3 kernel-stats csv and exit")
+    ap.add_argument("--summarize-sq", nargs=3, metavar=("PMC_CSV", "WORKLOAD", "COMMAND"),
+                    help="write profiles/counters_<workload>.md/.json from a rocprofv3 --pmc run of SQ counters and exit")
     ap.add_argument("--summarize-pmc", nargs=6, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_MIB", "KERNEL", "WORKLOAD",
                                                           "COMMAND"),
                     help="write profiles/traffic_<kernel>_<workload>.json/.md from two rocprofv3 --pmc runs and exit")
     args = ap.parse_args()
     if args.summarize_stats:
         summarize_stats(*args.summarize_stats)
+        return
+    if args.summarize_sq:
+        print(json.dumps(summarize_sq(*args.summarize_sq, kernels=("k_prepare", "k_rocking", "k_pass1", "k_pass2_small",
+                                                                    "k_pass2", "k_normalise", "k_extract"))))
         return
     if args.summarize_pmc:
         f, w, mib, kern, wl, cmd = args.summarize_pmc
@@ -554,6 +708,7 @@ def main():
     t = time.time()
     hp = _integrate(frames, [exp] * (len(edges) - 1), edges, model, spots["dmin"], plan_only=True, verbose=False,
                     _box_draw=synth.box_draw_for(spec))
+    t_host_plan = time.time() - t
     from xpg_amd.integrate import DZ_BUCKETS  # noqa: F401
     opts = dict(frames=(1, spec.nframes), mask=None, exclude_frames=(), learn_profiles=True, adaptive=True,
                 bkg_template=None, moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
@@ -568,11 +723,15 @@ def main():
                      all_identical=all(r["identical"] for r in table))
         if args.autotune_save:
             autotune.save(best, table, spec.name, args.autotune_save)
+    t = time.time()
     sweep = Sweep(frames, [exp] * (len(edges) - 1), model, hp, opts)
+    torch.cuda.synchronize(dev)
+    t_setup = time.time() - t
     N = sweep.sp.n
+    # pinned host memory for the results: every timed step ends with them copied there
+    hostbuf = sweep.host_results(dict(zip(("o1", "o2"), sweep.alloc_outputs())))
     log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups ({len(sweep.sp.bins)} size-bin launches) "
-        f"over {len(edges) - 1} blocks "
-        f"({time.time() - t:.1f}s)")
+        f"over {len(edges) - 1} blocks (host plan {t_host_plan:.1f}s, device setup {t_setup:.1f}s)")
 
     events = []
 
@@ -600,8 +759,9 @@ def main():
         t0 = time.perf_counter()
         last = None
         for i in range(steps):
-            last = sweep.run(timer=timer)
-        torch.cuda.synchronize(dev)
+            last = sweep.enqueue(timer)
+            sweep.copy_results(last, hostbuf)
+            torch.cuda.synchronize(dev)
         el = time.perf_counter() - t0
         integrated = int(torch.isfinite(last["o2"]["iprf"]).sum().item())
         log(f"{form}: timed {steps} steps: {el / steps * 1e3:.1f} ms/step, {integrated} profile-fitted "
@@ -630,9 +790,11 @@ def main():
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         for i in range(args.steps):
-            last_graph = sweep.replay()
-        torch.cuda.synchronize(dev)
+            sweep.graph.replay()
+            sweep.copy_results(sweep.graph_out, hostbuf)
+            torch.cuda.synchronize(dev)
         t_graph = (time.perf_counter() - t0) / args.steps
+        last_graph = dict(sweep.graph_out)
         n_graph = int(torch.isfinite(last_graph["o2"]["iprf"]).sum().item())
         log(f"batched (graph): timed {args.steps} replays: {t_graph * 1e3:.1f} ms/step, {n_graph} profile-fitted "
             f"reflections/step")
@@ -656,18 +818,21 @@ def main():
     ms = t_step * 1e3
     value = integrated / t_step
 
-    # dominant kernel and its roofline (HIP events on the launch stream, algorithmic bytes per launch)
+    # the last step's results as copied to pinned host memory
+    fin = lambda t: torch.nan_to_num(t.cpu(), nan=-1.0)
+    host_copy_identical = bool(torch.equal(fin(hostbuf["iprf"]), fin(last["o2"]["iprf"])) and
+                               torch.equal(fin(hostbuf["isum"]), fin(last["o1"]["isum"])))
+
+    # dominant kernel and its roofline, over every kernel of the sweep (HIP events on the launch stream, algorithmic
+    # bytes per launch)
     per = {}
-    K_nb = sweep.sp.K
     n_timed = 1
     for kind, (start, m), e0, e1 in events:
-        if kind == "prepare":
-            continue
         Bs = sweep.sp.rows_dz[start:start + m]
         bxs = sweep.sp.rows_bx[start:start + m]
         d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
         d["ms"] += e0.elapsed_time(e1)
-        d["bytes"] += float(np.sum(per_refl_bytes(Bs * bxs * bxs, Bs, K_nb, kind)))
+        d["bytes"] += float(np.sum(per_refl_bytes(Bs * bxs * bxs, Bs, kind)))
         d["n"] += 1
     dom = max(per, key=lambda k: per[k]["ms"])
     dd = per[dom]
@@ -697,12 +862,18 @@ def main():
     cpu, parity = (None, None)
     if not args.no_cpu:
         t = time.time()
-        cpu, parity = cpu_baseline(sweep, hp, spec, args.cpu_sample, args.cpu_seconds)
+        cpu, parity = cpu_baseline(sweep, hp, spec, frames, args.cpu_sample, args.cpu_seconds,
+                                   args.cpu_threads or host_cores())
         log(f"cpu baseline {cpu['value']} refl/s ({time.time() - t:.1f}s); parity {parity}")
 
     out = dict(metric="profile-fitted reflections/s on 1 MI355X", value=round(value, 1), unit="reflections/s",
                n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(ms, 3), higher_is_better=True,
-               scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+               scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+               timed_region="every launch of the sweep from shoebox gather to pass 2, then the per-reflection results "
+                            "copied to pinned host memory (host_copy_identical); frames and the plan resident before",
+               plan=dict(host_plan_s=round(t_host_plan, 3), device_setup_s=round(t_setup, 3),
+                         value_incl_plan=round(integrated / (t_step + t_host_plan + t_setup), 1)),
+               host_copy_identical=host_copy_identical,
                config=dict(workload=spec.name, form=FORMS[form],
                            frames=spec.nframes, frame_shape=[spec.ny, spec.nx], reflections=N,
                            integrated=integrated, blocks=len(edges) - 1, box=[spec.box_z, spec.box_xy, spec.box_xy]

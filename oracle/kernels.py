@@ -45,6 +45,10 @@ RC_W = 0.25
 UMAX = N_BOX * 0.9
 # the reference builds the mixture centres of its device rocking curve as a float32 tensor
 RC_C32 = RC_C.astype(np.float32).astype(np.float64)
+# working precision of the per-voxel arithmetic (geometry, masks, sums, fits): float64 for the oracle.  The CPU
+# throughput baseline sets float32, the precision of the reference's tensor program; profile accumulation and
+# normalisation stay float64 either way.
+FT = np.float64
 
 
 def _dot3(a, b):
@@ -103,7 +107,7 @@ def _bilinear_table(tab, x, y):
 def pixel_S(det, x, y):
     """S of detector points (x, y): xpg/integrate_kernels.py:78-105.  ``det`` holds float64 arrays P0, Bx, By, U1, U2,
     UN (K,3), scalars orgx, orgy, qx, qy, wl, tsens, mu, plx, and optional segmap / gx, gy / bkg tables (None)."""
-    x, y = np.broadcast_arrays(np.asarray(x, np.float64), np.asarray(y, np.float64))
+    x, y = np.broadcast_arrays(np.asarray(x, FT), np.asarray(y, FT))
     if det.get("segmap") is None:
         k = np.zeros(x.shape, np.int64)
     else:
@@ -146,8 +150,8 @@ def phi_rc(phi_lo, phi_hi, phi_c, zeta, sm, rcw=None):
         b = (phi_lo - phi_c) / (s * np.sqrt(2.0))
         return 0.5 * (erf(a) - erf(b))
     sw = s * RC_W * np.sqrt(2.0)
-    out = np.zeros(np.shape(phi_lo))
-    rcw = np.asarray(rcw, np.float64)
+    out = np.zeros(np.shape(phi_lo), FT)
+    rcw = np.asarray(rcw, FT)
     for j in range(RC_C32.size):
         a = ((phi_hi - phi_c) - RC_C32[j] * s) / sw
         b = ((phi_lo - phi_c) - RC_C32[j] * s) / sw
@@ -181,11 +185,11 @@ def neighbour_mask(xx, yy, zz, nb, shape, skip_first=None):
     skip of columns that reach no box is exact (monotone rounding), so every column is evaluated here."""
     nbm = np.zeros(shape, bool)
     for kk in range(nb["x"].shape[1]):
-        sx = nb["sx"][:, kk].astype(np.float64)[:, None, None, None]
-        sz = nb["sz"][:, kk].astype(np.float64)[:, None, None, None]
-        dx = (xx - nb["x"][:, kk].astype(np.float64)[:, None, None, None]) / sx
-        dy = (yy - nb["y"][:, kk].astype(np.float64)[:, None, None, None]) / sx
-        dz = (zz - nb["z"][:, kk].astype(np.float64)[:, None, None, None]) / sz
+        sx = nb["sx"][:, kk].astype(FT)[:, None, None, None]
+        sz = nb["sz"][:, kk].astype(FT)[:, None, None, None]
+        dx = (xx - nb["x"][:, kk].astype(FT)[:, None, None, None]) / sx
+        dy = (yy - nb["y"][:, kk].astype(FT)[:, None, None, None]) / sx
+        dz = (zz - nb["z"][:, kk].astype(FT)[:, None, None, None]) / sz
         m = ((dx * dx + dy * dy) + dz * dz) < 1.0
         if kk == 0 and skip_first is not None:
             m = m & ~np.asarray(skip_first, bool)[:, None, None, None]
@@ -201,7 +205,7 @@ def bkg_ref(det, x, y, Dy, Dx):
     ny, nx = det["bkg"].shape
     ix = np.clip(np.round(x - 1.0).astype(np.int64), 0, nx - 1)
     iy = np.clip(np.round(y - 1.0).astype(np.int64), 0, ny - 1)
-    return det["bkg"].astype(np.float64)[iy[:, :, None], ix[:, None, :]][:, None]
+    return det["bkg"].astype(FT)[iy[:, :, None], ix[:, None, :]][:, None]
 
 
 # ---------------------------------------------------------------------------------------------------------------------
@@ -221,18 +225,18 @@ def gradient2(f, axis):
 
 def pixel_coords(rc, Dy, Dx):
     """Pixel (or super-pixel) centres of the box in detector pixels: xpg/integrate_kernels.py:224-230."""
-    s_ = np.asarray(rc.get("pxs", np.ones(len(rc["ix0"]))), np.float64)[:, None]
-    x = (np.asarray(rc["ix0"], np.float64)[:, None] + np.arange(Dx, dtype=np.float64)[None]) * s_ + (s_ - 1) * 0.5 + PIX0
-    y = (np.asarray(rc["iy0"], np.float64)[:, None] + np.arange(Dy, dtype=np.float64)[None]) * s_ + (s_ - 1) * 0.5 + PIX0
+    s_ = np.asarray(rc.get("pxs", np.ones(len(rc["ix0"]))), FT)[:, None]
+    x = (np.asarray(rc["ix0"], FT)[:, None] + np.arange(Dx, dtype=FT)[None]) * s_ + (s_ - 1) * 0.5 + PIX0
+    y = (np.asarray(rc["iy0"], FT)[:, None] + np.arange(Dy, dtype=FT)[None]) * s_ + (s_ - 1) * 0.5 + PIX0
     return x, y
 
 
 def geometry_terms(det, axis, s0, rc, frames_phi0, osc, sd, sm, rcw, Dz, Dy, Dx, pix=None):
     """xpg/integrate_kernels.py:219-276 in float64."""
-    S = np.asarray(rc["S"], np.float64)
+    S = np.asarray(rc["S"], FT)
     x, y = pixel_coords(rc, Dy, Dx)
-    axis = np.asarray(axis, np.float64)
-    s0 = np.asarray(s0, np.float64)
+    axis = np.asarray(axis, FT)
+    s0 = np.asarray(s0, FT)
     if pix is not None:
         zeta = pix["zeta"]
     else:
@@ -254,12 +258,12 @@ def geometry_terms(det, axis, s0, rc, frames_phi0, osc, sd, sm, rcw, Dz, Dy, Dx,
         c1_, c2_ = eps12(det, S, e1, e2, x, y + 0.5)
         d1, d2 = eps12(det, S, e1, e2, x, y - 0.5)
         J = np.abs((a1 - b1) * (c2_ - d2) - (a2 - b2) * (c1_ - d1))
-    fb = np.asarray(rc["fb"], np.float64)[:, None]
-    k = np.asarray(rc["iz0"], np.float64)[:, None] + fb * np.arange(Dz, dtype=np.float64)[None]
+    fb = np.asarray(rc["fb"], FT)[:, None]
+    k = np.asarray(rc["iz0"], FT)[:, None] + fb * np.arange(Dz, dtype=FT)[None]
     phi_lo = frames_phi0 + osc * k
     phi_hi = phi_lo + osc * fb
     lo, hi = np.minimum(phi_lo, phi_hi), np.maximum(phi_lo, phi_hi)
-    phi = np.asarray(rc["phi"], np.float64)
+    phi = np.asarray(rc["phi"], FT)
     R = phi_rc(lo, hi, np.broadcast_to(phi[:, None], lo.shape), np.broadcast_to(zeta[:, None], lo.shape), sm, rcw)
     if pix is not None:
         seff, u1c, u2c = pix["seff"], pix["u1"], pix["u2"]
@@ -275,7 +279,7 @@ def geometry_terms(det, axis, s0, rc, frames_phi0, osc, sd, sm, rcw, Dz, Dy, Dx,
 
 def profile_values(prof, region, gt):
     """p2 = P2(u) J / seff^2: xpg/integrate_kernels.py:278-285.  prof (R,G,G) float32 values."""
-    prof = np.asarray(prof, np.float64)
+    prof = np.asarray(prof, FT)
     inside, i1, i2, t1, t2 = bilinear(gt["u1"], gt["u2"], prof.shape[-1])
     r = np.asarray(region)[:, None, None, None]
     v = ((((1 - t1) * (1 - t2)) * prof[r, i2, i1] + (t1 * (1 - t2)) * prof[r, i2, i1 + 1])
@@ -284,8 +288,8 @@ def profile_values(prof, region, gt):
 
 
 def _zz(rc, Dz):
-    fb = np.asarray(rc["fb"], np.float64)[:, None]
-    return (np.asarray(rc["iz0"], np.float64)[:, None] + fb * (np.arange(Dz, dtype=np.float64)[None] + 0.5))[:, :, None, None]
+    fb = np.asarray(rc["fb"], FT)[:, None]
+    return (np.asarray(rc["iz0"], FT)[:, None] + fb * (np.arange(Dz, dtype=FT)[None] + 0.5))[:, :, None, None]
 
 
 # ---------------------------------------------------------------------------------------------------------------------
@@ -295,7 +299,7 @@ def _zz(rc, Dz):
 def pass1(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, nb, rcw):
     """Background, summation, centroids, moments and projected images: xpg/integrate_kernels.py:287-365.
     sb: (n,Dz,Dy,Dx) value codes (-3 unrecorded, -2 overloaded, -1 masked, >= 0 counts)."""
-    sb = np.asarray(sb, np.float64)
+    sb = np.asarray(sb, FT)
     n, Dz, Dy, Dx = sb.shape
     gt = geometry_terms(det, axis, s0, rc, frames_phi0, osc, sd, sm, rcw, Dz, Dy, Dx)
     valid = sb >= 0
@@ -314,16 +318,16 @@ def pass1(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, nb, rcw):
     c = np.where(valid, sb, 0.0)
     bref = bkg_ref(det, gt["x"], gt["y"], Dy, Dx)
     ax = (1, 2, 3)
-    w = bgm.astype(np.float64)
+    w = bgm.astype(FT)
     b = np.sum(w * c, axis=ax) / np.maximum(np.sum(w * bref, axis=ax), 1e-6)
     for _ in range(4):
         bp = b[:, None, None, None] * bref
         thr = (bp + 3.0 * np.sqrt(gain * np.maximum(bp, 0.5))) + 1.0
-        w = (bgm & (c <= thr)).astype(np.float64)
+        w = (bgm & (c <= thr)).astype(FT)
         b = np.sum(w * c, axis=ax) / np.maximum(np.sum(w * bref, axis=ax), 1e-6)
     nbg = np.sum(w * bref, axis=ax)
     fgv = fg & valid & ~nbm
-    fgw = fgv.astype(np.float64)
+    fgw = fgv.astype(FT)
     bpix = b[:, None, None, None] * bref
     net = c - bpix
     nfg = np.sum(fgw, axis=ax)
@@ -344,7 +348,7 @@ def pass1(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, nb, rcw):
     mu1 = np.sum(wsg * e1v, axis=ax) / sws
     mu2 = np.sum(wsg * e2v, axis=ax) / sws
     m_d = (np.sum(wsg * (e1v ** 2 + e2v ** 2), axis=ax) / sws - mu1 ** 2 - mu2 ** 2) / 2.0
-    e3 = gt["zeta"][:, None] * (gt["phi_c"] - np.asarray(rc["phi"], np.float64)[:, None])
+    e3 = gt["zeta"][:, None] * (gt["phi_c"] - np.asarray(rc["phi"], FT)[:, None])
     mu3 = np.sum(wsg * e3[:, :, None, None], axis=ax) / sws
     m_m = np.sum(wsg * (e3 ** 2)[:, :, None, None], axis=ax) / sws - mu3 ** 2
     clean = valid & ~nbm
@@ -408,15 +412,15 @@ def pass2(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, prof, bg, nbm_x
     """Poisson-weighted profile fit with the 2x2 partner deconvolution: xpg/integrate_kernels.py:382-483.
     irls_rule "chunk": stop when no reflection of the call moved by more than IRLS_TOL sigma (the reference,
     xpg/integrate_kernels.py:464-467); "reflection": each reflection stops on its own test."""
-    sb = np.asarray(sb, np.float64)
+    sb = np.asarray(sb, FT)
     n, Dz, Dy, Dx = sb.shape
     gain = float(gain)
-    bg, nbm_x = np.asarray(bg, np.float64), np.asarray(nbm_x, np.float64)
+    bg, nbm_x = np.asarray(bg, FT), np.asarray(nbm_x, FT)
     gt = geometry_terms(det, axis, s0, rc, frames_phi0, osc, sd, sm, rcw, Dz, Dy, Dx, pix)
     valid = sb >= 0
     region = np.asarray(rc["region"])
     p = profile_values(prof, region, gt) * gt["R"][:, :, None, None]
-    has_p = np.asarray(rc["has_p"], np.float64)
+    has_p = np.asarray(rc["has_p"], FT)
     anyp = bool(np.any(has_p > 0.5))
     if anyp:
         rc_p = dict(S=rc["pS"], phi=rc["pphi"], ix0=rc["ix0"], iy0=rc["iy0"], iz0=rc["iz0"], region=rc["region"],
@@ -441,7 +445,7 @@ def pass2(sb, rc, det, axis, s0, frames_phi0, osc, sd, sm, gain, prof, bg, nbm_x
     pmax = np.max(p, axis=ax, keepdims=True)
     ppmax = np.max(pp, axis=ax, keepdims=True)
     use = valid & ~nbm & ((p > 0.01 * pmax) | (pp > 0.01 * np.maximum(ppmax, 1e-12)))
-    wuse = use.astype(np.float64)
+    wuse = use.astype(FT)
     wp = wuse * p
     I = np.sum(wuse * net, axis=ax) / np.maximum(np.sum(wp + wuse * pp, axis=ax), 1e-6)
     Ip = I * has_p

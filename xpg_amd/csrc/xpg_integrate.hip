@@ -61,14 +61,19 @@ struct Layout {
   size_t vals, vflag, scr, U1, U2, Jd, pa, pb, pc, prow, pflag, R, Rp, fr, co, tab, bytes;
 };
 
-// vals_lds = false: the shoebox values are read in place from a device buffer (boxes too large for LDS)
-__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int pass, bool vals_lds = true) {
+// vals_lds = false: the shoebox values are read in place from a device buffer (boxes too large for LDS).
+// compact: pass-1 register rows over prepared data with their rocking curve given (compact_rows): voxels and their
+// bits live in registers, the curve comes ready and the neighbour masks were made ahead, so no LDS for voxels, curve
+// terms or neighbour tables (scr keeps Dz doubles, the per-frame scratch) -- a fraction of the full layout, and with
+// it more wavefronts per CU.
+__host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int pass, bool vals_lds = true,
+                                                     bool compact = false) {
   const size_t P = (size_t)Dy * Dx, V = (size_t)Dz * P, C = (size_t)Dx + Dy + Dz;
   Layout L;
   size_t o = 0;
-  L.vals = o; if (vals_lds) o = al8(o + 4 * V);
-  L.vflag = o; o = al8(o + V);
-  L.scr = o; o += 8 * 21 * (size_t)Dz;
+  L.vals = o; if (vals_lds && !compact) o = al8(o + 4 * V);
+  L.vflag = o; if (!compact) o = al8(o + V);
+  L.scr = o; o += 8 * (compact ? 1 : 21) * (size_t)Dz;         // compact: the per-frame scratch of pass 1 only
   L.U1 = o; o += 8 * P;
   L.U2 = o; o += 8 * P;
   L.Jd = o; o += 8 * P;
@@ -81,9 +86,14 @@ __host__ __device__ static inline Layout make_layout(int Dz, int Dy, int Dx, int
   L.Rp = o; if (pass == 2) o += 8 * Dz;
   L.fr = o; o = al8(o + (pass == 2 ? 9 : 17) * (size_t)Dz);
   L.co = o; o += 8 * C;
-  L.tab = o; o += 8 * C;
+  L.tab = o; if (!compact) o += 8 * C;
   L.bytes = al8(o);
   return L;
+}
+
+// pass-1 rows of this shape over prepared data with a rocking-curve table use the compact layout (make_layout)
+__host__ __device__ static inline bool compact_rows(int Dz, int Dy, int Dx, bool prep, bool rtab) {
+  return prep && rtab && Dy * Dx <= 2 * 64 && Dz <= 8;
 }
 
 // frames per batch of LDS reads in the per-pixel frame loops of boxes that are not held in registers: the reads
@@ -580,7 +590,8 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
   const int r = blockIdx.x * a.waves + wid;
   if (r >= a.R.n) return;
   const Box b = box_of(a.R, r);
-  const Layout L = make_layout(b.Dz, b.Dy, b.Dx, 1, !VG);
+  const Layout L = make_layout(b.Dz, b.Dy, b.Dx, 1, !VG,
+                               !VG && compact_rows(b.Dz, b.Dy, b.Dx, PREP, PREP && a.prep.R != nullptr));
   if (L.bytes > (size_t)a.wave_bytes) return;
   unsigned char *base = smem + (size_t)wid * a.wave_bytes;
   const int P = b.P;
@@ -2136,11 +2147,14 @@ static int pack_waves(size_t bytes, int lim, int cap) {
 
 // in_place_ok: the shoebox values are in a device buffer the kernel may read in place when they do not fit in LDS
 static int launch_shape(const xpg_refl *R, int pass, bool in_place_ok, int *waves, int *wave_bytes, size_t *lds,
-                        int *vals_global) {
-  Layout L = make_layout(R->Dz, R->Dy, R->Dx, pass);
+                        int *vals_global, bool compact = false) {
+  // compact: every row of the launch (shapes up to the maxima R->Dz, Dy, Dx) takes the compact layout, which grows
+  // with the shape, so that of the maxima bounds them all
+  Layout L = make_layout(R->Dz, R->Dy, R->Dx, pass, true, compact);
   const int lim = lds_limit();
   *vals_global = 0;
-  if (R->wave_lds > 0 && R->wave_lds <= lim && (size_t)R->wave_lds < L.bytes) L.bytes = al8((size_t)R->wave_lds);
+  if (!compact && R->wave_lds > 0 && R->wave_lds <= lim && (size_t)R->wave_lds < L.bytes)
+    L.bytes = al8((size_t)R->wave_lds);
   if ((long long)L.bytes > lim && in_place_ok) {
     L = make_layout(R->Dz, R->Dy, R->Dx, pass, false);
     *vals_global = 1;
@@ -2293,7 +2307,8 @@ static int pass1_impl(const xpg_detector *det, const xpg_call *call, const xpg_r
   if (acc) a.acc = *acc;
   else memset(&a.acc, 0, sizeof(a.acc));
   size_t lds;
-  rc = launch_shape(refl, 1, sb != nullptr, &a.waves, &a.wave_bytes, &lds, &a.vals_global);
+  rc = launch_shape(refl, 1, sb != nullptr, &a.waves, &a.wave_bytes, &lds, &a.vals_global,
+                    compact_rows(refl->Dz, refl->Dy, refl->Dx, prep != nullptr, prep && prep->R));
   if (rc) return rc;
   void (*kern)(P1Args) = prep ? (a.vals_global ? k_pass1<true, true> : k_pass1<true, false>)
                                : (a.vals_global ? k_pass1<false, true> : k_pass1<false, false>);

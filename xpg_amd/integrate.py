@@ -681,6 +681,21 @@ class Sweep:
             self.graph_out = self.enqueue(None)
         return self.graph_out
 
+    RESULT_KEYS = OUT_F64 + ("n_over", "n_fg_masked")
+
+    def host_results(self, run):
+        """Pinned host tensors for the per-reflection results of ``run`` (copy_results' destination)."""
+        src = {**run["o1"], **run["o2"]}
+        return {k: torch.empty(src[k].shape, dtype=src[k].dtype, pin_memory=True) for k in self.RESULT_KEYS}
+
+    def copy_results(self, run, host):
+        """The per-reflection results of ``run`` (table order) into the pinned tensors ``host``: asynchronous copies
+        on the current stream, complete at its next synchronisation."""
+        src = {**run["o1"], **run["o2"]}
+        for k in self.RESULT_KEYS:
+            host[k].copy_(src[k], non_blocking=True)
+        return host
+
     def replay(self):
         t_dev0 = time.time()
         self.graph.replay()
@@ -868,7 +883,8 @@ class Sweep:
             s1 = aux if where[u] else main
             sp1 = s1.cuda_stream if where[u] else st
             if pp is not None:          # the rows' rocking curves for pass 1's widths, ahead of it
-                L.check(lib.xpg_rocking(C.byref(call1), C.byref(view.s), C.byref(pp), sp1))
+                with (timer("rocking", b, (start, m), s1) if timer else nullctx):
+                    L.check(lib.xpg_rocking(C.byref(call1), C.byref(view.s), C.byref(pp), sp1))
             with (timer("pass1", b, (start, m), s1) if timer else nullctx):
                 a_ = C.byref(acc) if acc is not None else None
                 if pp is not None:
@@ -907,7 +923,8 @@ class Sweep:
                     setattr(out, k, rb.p2(k, start))
                 bg, nbg = rb.p1("bg", start), rb.p1("nbg", start)
                 if pp is not None:      # the rows' rocking curves for pass 2's widths (pass 1 has read its own)
-                    L.check(lib.xpg_rocking(C.byref(call2), C.byref(view.s), C.byref(pp), sst))
+                    with (timer("rocking", b, (start, m), side) if timer else nullctx):
+                        L.check(lib.xpg_rocking(C.byref(call2), C.byref(view.s), C.byref(pp), sst))
                 with (timer("pass2", b, (start, m), side) if timer else nullctx):
                     if pp is not None:
                         L.check(lib.xpg_pass2_prepared(C.byref(det.s), C.byref(call2), C.byref(view.s), C.byref(pp),
