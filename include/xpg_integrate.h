@@ -112,7 +112,8 @@ typedef struct xpg_pass2_out {
   uint8_t *vmask;
 } xpg_pass2_out;
 
-/* Fixed-point profile accumulation grids: int64 (n_regions, 41, 41) each, value * 2^32, zeroed by the caller.
+/* Fixed-point profile accumulation grids: int64 (n_regions, 41, 41) each, value * 2^44, zeroed by the caller (a cell
+ * holds up to 2^19 in value units).
  * Integer addition is associative, so the grids are bit-identical for any launch order. */
 typedef struct xpg_profile_acc {
   int64_t *num, *den;

@@ -29,7 +29,7 @@ noise-free b*T + I*p model by the Poisson fit, Cramer's rule for the 2x2 partner
 from __future__ import annotations
 
 import numpy as np
-from scipy.special import erf
+from scipy.special import erf, erfc
 
 PIX0 = 1.0
 N_FG = 3.0
@@ -142,20 +142,31 @@ def eps12(det, S, e1, e2, x, y):
     return _dot3(d, e1[:, None, None, :]) * invS, _dot3(d, e2[:, None, None, :]) * invS
 
 
+def erf_diff(a, b):
+    """erf(a) - erf(b) for a >= b.  Where both arguments have one sign it is computed as erfc(b) - erfc(a) (or
+    erfc(-a) - erfc(-b)), the same number without the cancellation of two erf values rounded towards +-1: the tails
+    of a rocking curve keep their relative precision (the reference's erf(a) - erf(b) keeps an absolute 1e-16)."""
+    a, b = np.broadcast_arrays(np.asarray(a), np.asarray(b))
+    pos, neg = b >= 0, a <= 0
+    return np.where(pos, erfc(np.where(pos, b, 0.0)) - erfc(np.where(pos, a, 0.0)),
+                    np.where(neg, erfc(-np.where(neg, a, 0.0)) - erfc(-np.where(neg, b, 0.0)), erf(a) - erf(b)))
+
+
 def phi_rc(phi_lo, phi_hi, phi_c, zeta, sm, rcw=None):
-    """Recorded fraction of the rocking curve between phi_lo and phi_hi: xpg/integrate_kernels.py:130-141."""
+    """Recorded fraction of the rocking curve between phi_lo and phi_hi: xpg/integrate_kernels.py:130-141 (the erf
+    differences by erf_diff)."""
     s = sm / np.maximum(np.abs(zeta), 1e-3)
     if rcw is None:
         a = (phi_hi - phi_c) / (s * np.sqrt(2.0))
         b = (phi_lo - phi_c) / (s * np.sqrt(2.0))
-        return 0.5 * (erf(a) - erf(b))
+        return 0.5 * erf_diff(a, b)
     sw = s * RC_W * np.sqrt(2.0)
     out = np.zeros(np.shape(phi_lo), FT)
     rcw = np.asarray(rcw, FT)
     for j in range(RC_C32.size):
         a = ((phi_hi - phi_c) - RC_C32[j] * s) / sw
         b = ((phi_lo - phi_c) - RC_C32[j] * s) / sw
-        out = out + (rcw[j] * 0.5) * (erf(a) - erf(b))
+        out = out + (rcw[j] * 0.5) * erf_diff(a, b)
     return out
 
 

@@ -104,7 +104,7 @@ def test_kernels_match_oracle_synthetic(seed, shape, kw):
 
 
 def test_accumulate_and_normalise_match_oracle():
-    from xpg_amd.integrate_kernels import make_kernels, new_profile_acc, normalise_profiles
+    from xpg_amd.integrate_kernels import FX_SCALE, make_kernels, new_profile_acc, normalise_profiles
     c, _ = load("a_7x11x11")
     dev = torch.device("cuda:0")
     o1 = ok.pass1(c["sb"], *_args(c), c["nb"], c["rcw"])
@@ -123,8 +123,8 @@ def test_accumulate_and_normalise_match_oracle():
     assert np.max(np.abs(num - onum)) < 1e-8 * max(np.max(np.abs(onum)), 1)
     assert np.max(np.abs(den - oden)) < 1e-8 * max(np.max(np.abs(oden)), 1)
     # normalisation on the device against the float64 recipe
-    num_fx = torch.as_tensor(np.rint(onum * 2 ** 32).astype(np.int64), device=dev)
-    den_fx = torch.as_tensor(np.rint(oden * 2 ** 32).astype(np.int64), device=dev)
+    num_fx = torch.as_tensor(np.rint(onum * FX_SCALE).astype(np.int64), device=dev)
+    den_fx = torch.as_tensor(np.rint(oden * FX_SCALE).astype(np.int64), device=dev)
     prof, _, _ = normalise_profiles(num_fx, den_fx, dev)
     oprof = ok.normalise_profiles(onum, oden)
     assert np.max(np.abs(prof.cpu().numpy() - oprof)) < 1e-6 * np.max(oprof)
@@ -149,10 +149,10 @@ def test_fused_accumulation_matches_strong_selection():
     assert strong.sum() > 20
     onum, oden = ok.accumulate_profiles(o1["img2"], o1["all_valid2"], o1["J"], o1["u1"], o1["u2"], c["rc"]["region"],
                                         o1["isum"], strong, o1["seff"], 9)
-    num = num_fx.cpu().numpy() / 2 ** 32
-    den = den_fx.cpu().numpy() / 2 ** 32
-    assert np.max(np.abs(den - oden)) < 1e-6 * np.max(oden)
-    assert np.max(np.abs(num - onum)) < 1e-5 * np.max(np.abs(onum))
+    num = num_fx.cpu().numpy() / K.FX_SCALE
+    den = den_fx.cpu().numpy() / K.FX_SCALE
+    assert np.max(np.abs(den - oden)) < 1e-9 * np.max(oden)
+    assert np.max(np.abs(num - onum)) < 1e-9 * np.max(np.abs(onum))
     # bit-identical on a second launch (integer accumulation is order-independent)
     num2, den2, acc2 = K.new_profile_acc(9, dev)
     K.run_pass1(det, call, refl, sb=sb, acc=acc2)

@@ -3,7 +3,9 @@
 arithmetic and written to xpg_erf_table.h:
 
   nodes x_i = i / 256, i = 0 .. 1536 (x <= 6; float64 erf(x) rounds to 1 beyond 5.93),
-  per node f0 = erf(x_i) and f1 = erf'(x_i) = (2 / sqrt(pi)) exp(-x_i^2), each correctly rounded to float64.
+  per node f0 = erf(x_i) and f1 = erf'(x_i) = (2 / sqrt(pi)) exp(-x_i^2), each correctly rounded to float64,
+  and (ERFC_TAB) c0 = erfc(x_i) = 1 - erf(x_i), correctly rounded: erfc_tab expands about it, which keeps its
+  relative precision in the tail (erfc(6) = 2.2e-17), where 1 - erf_tab would have none.
 
 erf near a node is its Taylor expansion to degree 5, erf(x_i + d) = f0 + f1 d (1 + c2 d + c3 d^2 + c4 d^3 + c5 d^4),
 with the higher derivatives erf^(k) = f1 (-1)^(k-1) H_(k-1)(x) (Hermite polynomials), so that
@@ -63,6 +65,11 @@ def table():
     return out
 
 
+def erfc_table():
+    """erfc at the nodes, correctly rounded (1 - the 80-digit erf keeps > 40 digits up to x = 6)."""
+    return [float(1 - erf_dec(D(i) / INV_H)) for i in range(N)]
+
+
 def rnd(q):
     """Round an exact rational to the nearest float64 (ties to even)."""
     return float(q)
@@ -103,8 +110,30 @@ def erf_model(tab, x):
     return math.copysign(r, x)
 
 
+def _q(x0, d):
+    x2 = mul(x0, x0)
+    c2 = -x0
+    c3 = mul(add(mul(2.0, x2), -1.0), 1.0 / 3.0)
+    c4 = mul(mul(x0, add(3.0, -mul(2.0, x2))), 1.0 / 6.0)
+    c5 = mul(add(mul(x2, add(mul(4.0, x2), -12.0)), 3.0), 1.0 / 30.0)
+    q = fma(c5, d, c4)
+    q = fma(q, d, c3)
+    q = fma(q, d, c2)
+    return fma(q, d, 1.0)
+
+
+def erfc_model(tab, ctab, x):
+    """erfc_tab of xpg_erf.h for 0 <= x <= 6, operation for operation: erfc(x0) - erf'(x0) d q(d)."""
+    v = x * INV_H
+    i = int(round(v)) if v < N - 1 else N - 1
+    x0 = i / INV_H
+    d = x - x0
+    return fma(-mul(tab[i][1], d), _q(x0, d), ctab[i])
+
+
 def main():
     tab = table()
+    ctab = erfc_table()
     random.seed(11)
     pts = [random.uniform(0.0, 6.5) for _ in range(3000)] + [random.uniform(0.0, 0.01) for _ in range(300)]
     pts += [i / INV_H + s / 512 for i in range(0, N - 1, 37) for s in (-1, 1) if i / INV_H + s / 512 >= 0]
@@ -117,15 +146,26 @@ def main():
         worst_ulp = max(worst_ulp, float(err) / math.ulp(float(ref)) if float(ref) else 0.0)
     print(f"erf_tab model: worst |error| {worst_abs:.3e}, {worst_ulp:.3f} ulp at {len(pts)} points in [0, 6.5]",
           file=sys.stderr)
+    worst_rel = 0.0
+    for x in pts:
+        if x <= 6.0:
+            ref = 1 - erf_dec(x)
+            worst_rel = max(worst_rel, float(abs(D(erfc_model(tab, ctab, x)) - ref) / ref))
+    print(f"erfc_tab model: worst relative error {worst_rel:.3e} in [0, 6]", file=sys.stderr)
     w = sys.stdout.write
-    w("// generated by gen_erf_table.py: erf and erf' at x = i / 256, i = 0 .. 1536, correctly rounded to float64;\n")
-    w(f"// worst |error| of erf_tab (xpg_erf.h) {worst_abs:.2e} ({worst_ulp:.2f} ulp) at {len(pts)} points in [0, 6.5]\n")
+    w("// generated by gen_erf_table.py: erf, erf' and erfc at x = i / 256, i = 0 .. 1536, correctly rounded to float64;\n")
+    w(f"// worst |error| of erf_tab (xpg_erf.h) {worst_abs:.2e} ({worst_ulp:.2f} ulp) at {len(pts)} points in [0, 6.5];\n")
+    w(f"// worst relative error of erfc_tab {worst_rel:.2e} in [0, 6]\n")
     w("#pragma once\n\nnamespace xpg {\n\n")
     w("struct alignas(16) ErfNode {\n  double f0, f1;\n};\n")
     w(f"constexpr int ERF_TAB_INV_H = {INV_H}, ERF_TAB_N = {N};\n")
     w(f"constexpr ErfNode ERF_TAB[{N}] = {{\n")
     for i in range(0, N, 2):
         w("    " + " ".join(f"{{{f0!r}, {f1!r}}}," for f0, f1 in tab[i:i + 2]) + "\n")
+    w("};\n")
+    w(f"constexpr double ERFC_TAB[{N}] = {{\n")
+    for i in range(0, N, 4):
+        w("    " + " ".join(f"{c!r}," for c in ctab[i:i + 4]) + "\n")
     w("};\n\n}  // namespace xpg\n")
 
 

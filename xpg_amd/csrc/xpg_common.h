@@ -17,7 +17,11 @@ constexpr double PIX0 = 1.0;        // coordinate of the centre of array element
 constexpr double UMAX = 4.5;        // N_BOX * 0.9: half-width of the profile grid (sigma units)
 constexpr int G = XPG_PROFILE_GRID;
 constexpr int RCK = XPG_RC_K;
-constexpr double FX_SCALE = 4294967296.0;   // 2^32: fixed-point profile grids
+// fixed-point profile grids: value * 2^44 in int64.  A contribution is rounded to 2^-44 (6e-14), so that the learned
+// float32 profiles stay within ~1e-12 of exact float64 sums (deblended intensities next to partners 1e5 x brighter
+// depend on them to that level); a grid cell holds up to 2^19 (524288) in value units: the weights and densities of
+// that many strong-row pixels
+constexpr double FX_SCALE = 17592186044416.0;   // 2^44
 constexpr double VAR_FLOOR = 1e-3;
 constexpr int N_IRLS = 6;
 constexpr double IRLS_TOL = 1e-3;
@@ -301,20 +305,20 @@ __device__ __forceinline__ double sm_of(const xpg_call &c) { return c.widths ? c
 __device__ __forceinline__ bool use_rcw_of(const xpg_call &c) { return c.widths != nullptr || c.use_rcw != 0; }
 __device__ __forceinline__ float rcw_of(const xpg_call &c, int j) { return c.widths ? c.widths->rcw[j] : c.rcw[j]; }
 
-// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)); plain Gaussian when no mixture weights (j == 0 only).  s: the
-// width in phi, isw = 1 / sw with sw = (s / 4) sqrt(2) the mixture components' scale (a product instead of a quotient:
-// the erf arguments move by at most an ulp)
+// term j of frame: (w_j * 0.5) * (erf(a) - erf(b)) (erf_diff: tail values keep their relative precision); plain
+// Gaussian when no mixture weights (j == 0 only).  s: the width in phi, isw = 1 / sw with sw = (s / 4) sqrt(2) the
+// mixture components' scale (a product instead of a quotient: the erf arguments move by at most an ulp)
 __device__ __forceinline__ double rc_term(const xpg_call &c, double lo, double hi, double phi, double s, double isw,
                                           int j) {
   if (!use_rcw_of(c)) {
     double a = (hi - phi) / (s * sqrt(2.0));
     double b = (lo - phi) / (s * sqrt(2.0));
-    return 0.5 * (erf_tab(a) - erf_tab(b));
+    return 0.5 * erf_diff(a, b);
   }
   double cj = (double)RC_C32[j];
   double a = ((hi - phi) - cj * s) * isw;
   double b = ((lo - phi) - cj * s) * isw;
-  return ((double)rcw_of(c, j) * 0.5) * (erf_tab(a) - erf_tab(b));
+  return ((double)rcw_of(c, j) * 0.5) * erf_diff(a, b);
 }
 
 __device__ __forceinline__ void frame_bounds(const xpg_call &c, int iz0, int fb, int j, double &lo, double &hi) {

@@ -1110,7 +1110,9 @@ __global__ void __launch_bounds__(256) k_pass1(P1Args a) {
     if (a.out.u1) a.out.u1[o] = (float)U1[p];
     if (a.out.u2) a.out.u2[o] = (float)U2[p];
     if (strong && okv) {
-      const double dens = ((double)(float)img / fmax(isum, 1e-6)) / fmax(Jd[p] / seff2, 1e-12);
+      // the projected image in float64 (the standalone accumulation takes pass 1's float32 img2, as the reference's
+      // accumulate_profiles does; fused, nothing forces the rounding and the grids stay at the float64 sums)
+      const double dens = (img / fmax(isum, 1e-6)) / fmax(Jd[p] / seff2, 1e-12);
       splat4(a.acc, b.region, U1[p], U2[p], dens);
     }
   };
@@ -1557,7 +1559,44 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
         }
       }
     }
-    for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
+    // rows with an overlapping partner: the 2 x 2 fit in float64 throughout.  With a partner orders of magnitude
+    // brighter and overlapping profiles (rho -> 1) the system is nearly singular, and float32 products and weights
+    // would move I by more than 1e-3 sigma from the float64 result.
+    for (int p0 = 0; p0 < P && !regs && partner; p0 += WAVE) {
+      const int p = p0 + lane;
+      if (p < P) {
+        const double b0 = bgr * (double)bref[p];
+        const double p2d = p2[p], pp2d = pp2[p], Iid = fmax(I, 0.0), Ipid = fmax(Ip, 0.0);
+        for (int j0 = 0; j0 < Dz; j0 += LB) {
+          float cc[LB];
+          uint8_t ff[LB];
+          double rd[LB], rpd[LB];
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            const int j = min(j0 + u, Dz - 1);
+            cc[u] = vals[j * P + p];
+            ff[u] = vflag[j * P + p];
+            rd[u] = Rd[j];
+            rpd[u] = Rpd[j];
+          }
+#pragma unroll
+          for (int u = 0; u < LB; ++u) {
+            if (j0 + u >= Dz || !(ff[u] & VF_USE)) continue;
+            const double net = (double)cc[u] - b0;
+            const double pf = p2d * rd[u];
+            const double ppf = (pp2d * rpd[u]) * has_p;
+            const double mu = (b0 + Iid * pf) + Ipid * ppf;
+            const double iv = 1.0 / (gain * fmax(mu, VAR_FLOOR));
+            s11 += (pf * pf) * iv;
+            sb1 += (pf * net) * iv;
+            s12 += (pf * ppf) * iv;
+            s22 += (ppf * ppf) * iv;
+            sb2 += (ppf * net) * iv;
+          }
+        }
+      }
+    }
+    for (int p0 = 0; p0 < P && !regs && !partner; p0 += WAVE) {
       const int p = p0 + lane;
       if (p < P) {
         const float b0 = (float)(bgr * (double)bref[p]);
@@ -1648,7 +1687,45 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       }
     }
   }
-  for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
+  for (int p0 = 0; p0 < P && !regs && partner; p0 += WAVE) {       // partner rows: float64 (see the IRLS)
+    const int p = p0 + lane;
+    if (p < P) {
+      const double b0 = bgr * (double)bref[p], br = (double)bref[p];
+      const double p2d = p2[p], pp2d = pp2[p];
+      const double Iid = fmax(Iw, 0.0), Ipid = fmax(Ipw, 0.0), Iposd = fmax(Ip, 0.0);
+      const double iv0 = 1.0 / (gain * fmax(b0, VAR_FLOOR));
+      for (int j0 = 0; j0 < Dz; j0 += LB) {
+        float cc[LB];
+        uint8_t ff[LB];
+        double rd[LB], rpd[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int j = min(j0 + u, Dz - 1);
+          cc[u] = vals[j * P + p];
+          ff[u] = vflag[j * P + p];
+          rd[u] = Rd[j];
+          rpd[u] = Rpd[j];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          if (j0 + u >= Dz || !(ff[u] & VF_USE)) continue;
+          const double net = (double)cc[u] - b0;
+          const double pf = p2d * rd[u];
+          const double ppf = (pp2d * rpd[u]) * has_p;
+          const double mu = (b0 + Iid * pf) + Ipid * ppf;
+          const double iv = 1.0 / (gain * fmax(mu, VAR_FLOOR));
+          const double pb = pf * br;
+          sC += pb * iv;
+          s110 += (pf * pf) * iv0;
+          sC0 += pb * iv0;
+          swp += pf;
+          sres += net - Iposd * ppf;
+          nuse += 1;
+        }
+      }
+    }
+  }
+  for (int p0 = 0; p0 < P && !regs && !partner; p0 += WAVE) {
     const int p = p0 + lane;
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]), br = bref[p];
@@ -1690,6 +1767,7 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
   nuse = (int)t1[5];
   const double nu = fmax((double)nuse, 1.0);
   const float pm = (float)(Swp / nu), nm = (float)(Sres / nu);
+  const double pmd = Swp / nu, nmd = Sres / nu;
   double sdd = 0.0, sdp = 0.0, snn = 0.0, s_pv = 0.0, s_pt = 0.0, s_pfg = 0.0;
   if (regs) {
 #pragma unroll
@@ -1707,7 +1785,45 @@ __global__ void __launch_bounds__(256) k_pass2(P2Args a) {
       snn += (double)(dn * dn);
     }
   }
-  for (int p0 = 0; p0 < P && !regs; p0 += WAVE) {
+  for (int p0 = 0; p0 < P && !regs && partner; p0 += WAVE) {       // partner rows: float64 (see the IRLS)
+    const int p = p0 + lane;
+    if (p < P) {
+      const double b0 = bgr * (double)bref[p];
+      const double p2d = p2[p], pp2d = pp2[p], Iposd = fmax(Ip, 0.0);
+      for (int j0 = 0; j0 < Dz; j0 += LB) {
+        float cc[LB];
+        uint8_t ff[LB];
+        double rd[LB], rpd[LB];
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          const int j = min(j0 + u, Dz - 1);
+          cc[u] = vals[j * P + p];
+          ff[u] = vflag[j * P + p];
+          rd[u] = Rd[j];
+          rpd[u] = Rpd[j];
+        }
+#pragma unroll
+        for (int u = 0; u < LB; ++u) {
+          if (j0 + u >= Dz) break;
+          const uint8_t fl = ff[u];
+          const double pf = p2d * rd[u];
+          const float c = cc[u];
+          s_pt += pf;
+          if (c >= 0.f && !(fl & VF_NBM)) s_pv += pf;
+          if (fl & VF_FG) s_pfg += pf;
+          if (!(fl & VF_USE)) continue;
+          const double net = (double)c - b0;
+          const double ppf = (pp2d * rpd[u]) * has_p;
+          const double dp = pf - pmd;
+          const double dn = (net - Iposd * ppf) - nmd;
+          sdp += dp * dn;
+          sdd += dp * dp;
+          snn += dn * dn;
+        }
+      }
+    }
+  }
+  for (int p0 = 0; p0 < P && !regs && !partner; p0 += WAVE) {
     const int p = p0 + lane;
     if (p < P) {
       const float b0 = (float)(bgr * (double)bref[p]);

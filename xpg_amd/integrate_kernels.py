@@ -34,7 +34,7 @@ IRLS_TOL = 1e-3
 VAR_FLOOR = 1e-3
 RC_C = np.linspace(-3.5, 3.5, 21)
 RC_W = 0.25
-FX_SCALE = float(2 ** 32)
+FX_SCALE = float(2 ** 44)          # fixed-point profile grids (xpg_common.h)
 
 
 def rc_default_weights():
