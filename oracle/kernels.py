@@ -142,14 +142,19 @@ def eps12(det, S, e1, e2, x, y):
     return _dot3(d, e1[:, None, None, :]) * invS, _dot3(d, e2[:, None, None, :]) * invS
 
 
+def _erfc6(x):
+    """erfc, 0 beyond 6 (erfc < 2.2e-17 there; float64 erf rounds to 1, so erf differences are 0 as well)."""
+    return np.where(x > 6.0, 0.0, erfc(np.minimum(x, 6.0)))
+
+
 def erf_diff(a, b):
-    """erf(a) - erf(b) for a >= b.  Where both arguments have one sign it is computed as erfc(b) - erfc(a) (or
-    erfc(-a) - erfc(-b)), the same number without the cancellation of two erf values rounded towards +-1: the tails
-    of a rocking curve keep their relative precision (the reference's erf(a) - erf(b) keeps an absolute 1e-16)."""
+    """erf(a) - erf(b) for a >= b, from ea = erfc|a| and eb = erfc|b|: eb - ea where both arguments are >= 0 and
+    ea - eb where both are <= 0 -- the same number as the reference's erf(a) - erf(b) without the cancellation of two
+    erf values rounded towards +-1, so that the tails of a rocking curve keep their relative precision (the
+    reference's keeps an absolute 1e-16) -- and (2 - ea) - eb across zero."""
     a, b = np.broadcast_arrays(np.asarray(a), np.asarray(b))
-    pos, neg = b >= 0, a <= 0
-    return np.where(pos, erfc(np.where(pos, b, 0.0)) - erfc(np.where(pos, a, 0.0)),
-                    np.where(neg, erfc(-np.where(neg, a, 0.0)) - erfc(-np.where(neg, b, 0.0)), erf(a) - erf(b)))
+    ea, eb = _erfc6(np.abs(a)), _erfc6(np.abs(b))
+    return np.where(b >= 0, eb - ea, np.where(a <= 0, ea - eb, (2.0 - ea) - eb))
 
 
 def phi_rc(phi_lo, phi_hi, phi_c, zeta, sm, rcw=None):

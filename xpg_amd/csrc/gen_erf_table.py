@@ -4,8 +4,9 @@ arithmetic and written to xpg_erf_table.h:
 
   nodes x_i = i / 256, i = 0 .. 1536 (x <= 6; float64 erf(x) rounds to 1 beyond 5.93),
   per node f0 = erf(x_i) and f1 = erf'(x_i) = (2 / sqrt(pi)) exp(-x_i^2), each correctly rounded to float64,
-  and (ERFC_TAB) c0 = erfc(x_i) = 1 - erf(x_i), correctly rounded: erfc_tab expands about it, which keeps its
-  relative precision in the tail (erfc(6) = 2.2e-17), where 1 - erf_tab would have none.
+  and (ERFC_TAB, with f1 again so that erfc_tab reads one 16-byte node) c0 = erfc(x_i) = 1 - erf(x_i), correctly
+  rounded: erfc_tab expands about it, which keeps its relative precision in the tail (erfc(6) = 2.2e-17), where
+  1 - erf_tab would have none.
 
 erf near a node is its Taylor expansion to degree 5, erf(x_i + d) = f0 + f1 d (1 + c2 d + c3 d^2 + c4 d^3 + c5 d^4),
 with the higher derivatives erf^(k) = f1 (-1)^(k-1) H_(k-1)(x) (Hermite polynomials), so that
@@ -163,9 +164,10 @@ def main():
     for i in range(0, N, 2):
         w("    " + " ".join(f"{{{f0!r}, {f1!r}}}," for f0, f1 in tab[i:i + 2]) + "\n")
     w("};\n")
-    w(f"constexpr double ERFC_TAB[{N}] = {{\n")
-    for i in range(0, N, 4):
-        w("    " + " ".join(f"{c!r}," for c in ctab[i:i + 4]) + "\n")
+    w("struct alignas(16) ErfcNode {\n  double c0, f1;\n};\n")
+    w(f"constexpr ErfcNode ERFC_TAB[{N}] = {{\n")
+    for i in range(0, N, 2):
+        w("    " + " ".join(f"{{{c!r}, {f1!r}}}," for c, (_, f1) in zip(ctab[i:i + 2], tab[i:i + 2])) + "\n")
     w("};\n\n}  // namespace xpg\n")
 
 

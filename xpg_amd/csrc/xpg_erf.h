@@ -32,12 +32,13 @@ __host__ __device__ __forceinline__ double erf_tab(double x) {
 // float64 erfc for x >= 0 with relative precision in the tail: the same expansion taken from erfc at the node
 // (ERFC_TAB), erfc(x0 + d) = erfc(x0) - erf'(x0) d q(d), whose correction stays below 3% of erfc(x0) up to x = 6, so
 // that nothing cancels (worst relative error 1.9e-13, near x = 6; 1 - erf_tab would carry an absolute 1e-16).  Beyond
-// the last node (erfc < 2.2e-17) and for NaN, the library erfc.
+// the last node 0 (erfc < 2.2e-17 there, and float64 erf rounds to 1, so differences of erf values are 0 as well);
+// NaN propagates.
 __host__ __device__ __forceinline__ double erfc_tab(double x) {
-  if (!(x <= 6.0)) return ::erfc(x);
-  const int i = (int)rint(x * (double)ERF_TAB_INV_H);
+  const int i = (int)fmin(rint(x * (double)ERF_TAB_INV_H), (double)(ERF_TAB_N - 1));   // NaN -> the last node
   const double x0 = (double)i / (double)ERF_TAB_INV_H;
   const double d = x - x0;
+  const ErfcNode e = ERFC_TAB[i];
   const double x2 = x0 * x0;
   const double c2 = -x0;
   const double c3 = (2.0 * x2 - 1.0) * (1.0 / 3.0);
@@ -47,16 +48,17 @@ __host__ __device__ __forceinline__ double erfc_tab(double x) {
   q = fma(q, d, c3);
   q = fma(q, d, c2);
   q = fma(q, d, 1.0);
-  return fma(-(ERF_TAB[i].f1 * d), q, ERFC_TAB[i]);
+  const double r = fma(-(e.f1 * d), q, e.c0);
+  return x <= 6.0 ? r : (x > 6.0 ? 0.0 : x);          // selects, no branch
 }
 
-// erf(a) - erf(b) for a >= b.  With both arguments of one sign it is the difference of erfc values (erfc(b) - erfc(a),
-// or erfc(-a) - erfc(-b)): there erf rounds towards +-1 and the difference of erf values would keep only an absolute
-// 1e-16, nothing of a tail value of 1e-12; across zero the erf values do not cancel.
+// erf(a) - erf(b) for a >= b, from ea = erfc|a| and eb = erfc|b|: eb - ea with both arguments >= 0, ea - eb with
+// both <= 0 (there erf rounds towards +-1 and a difference of erf values would keep only an absolute 1e-16, nothing
+// of a tail value of 1e-12), (2 - ea) - eb across zero (no cancellation).  The same two table evaluations on every
+// lane.
 __host__ __device__ __forceinline__ double erf_diff(double a, double b) {
-  if (b >= 0.0) return erfc_tab(b) - erfc_tab(a);
-  if (a <= 0.0) return erfc_tab(-a) - erfc_tab(-b);
-  return erf_tab(a) - erf_tab(b);
+  const double ea = erfc_tab(fabs(a)), eb = erfc_tab(fabs(b));
+  return b >= 0.0 ? eb - ea : (a <= 0.0 ? ea - eb : (2.0 - ea) - eb);
 }
 
 }  // namespace xpg
