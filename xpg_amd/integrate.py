@@ -671,14 +671,17 @@ class Sweep:
         out["t_gpu"] = time.time() - t_dev0
         return out
 
-    def capture(self):
+    def capture(self, host=None):
         """Capture the batched form of the whole sweep in a HIP graph (after one eager run that allocates and
-        uploads everything the launches need); ``replay()`` then reruns it.  Returns the graph's output tensors."""
+        uploads everything the launches need); ``replay()`` then reruns it.  Returns the graph's output tensors.
+        ``host``: the per-block copies of the results to these pinned tensors are captured too (enqueue)."""
         self.o["form"] = "batched"
         self.run()
+        if host is not None:
+            self.copy_stream()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
-            self.graph_out = self.enqueue(None)
+            self.graph_out = self.enqueue(None, host)
         return self.graph_out
 
     RESULT_KEYS = OUT_F64 + ("n_over", "n_fg_masked")
@@ -729,9 +732,10 @@ class Sweep:
         o2 = {k: f64() for k in K.P2_F64}
         return o1, o2
 
-    def enqueue(self, timer=None):
+    def enqueue(self, timer=None, host=None):
         """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
-        and no host-to-device copy."""
+        and no host-to-device copy.  ``host`` (host_results()): each block's results are copied there as soon as its
+        pass 2 has run, on a copy stream, while later blocks compute; the current stream waits for the last copy."""
         from . import _lib as L
         o, sp, device, model = self.o, self.sp, self.device, self.model
         main = torch.cuda.current_stream(device)
@@ -804,14 +808,21 @@ class Sweep:
                     held.pop()()
 
             f = self._block_chain(i, b, units, rb, main, side, timer, cross=batched, after_pass1=after_pass1,
-                                  defer=defer)
+                                  defer=defer, host=host)
             if f is not None:
                 held.append(f)
         while held:
             held.pop()()
         if batched:
             main.wait_stream(side)
+        if host is not None and not o["moments_only"]:
+            main.wait_stream(self.copy_stream())
         return rb.result()
+
+    def copy_stream(self):
+        if not hasattr(self, "cs"):
+            self.cs = torch.cuda.Stream(self.device)      # default priority
+        return self.cs
 
     def _view(self, start, m, shape, mixed):
         """Cached ctypes table view: mixed None for one shape with its own row layout, else the LDS bytes per
@@ -847,10 +858,12 @@ class Sweep:
             c = self._call_cache[i] = (call1, call2, fit)
         return c
 
-    def _block_chain(self, i, b, units, rb, main, side, timer=None, cross=False, after_pass1=None, defer=False):
+    def _block_chain(self, i, b, units, rb, main, side, timer=None, cross=False, after_pass1=None, defer=False,
+                     host=None):
         """The i-th block (table block b): pass 1 of every launch unit, the adaptive fits and the profile
         normalisation on ``main``, pass 2 on ``side`` (after ``main`` when ``cross``); outputs into ``rb``
-        (RunBufs).  ``defer``: pass 2 is not launched but returned as a callable, for the caller to launch later."""
+        (RunBufs), and with ``host`` the block's rows copied there after its pass 2.  ``defer``: pass 2 is not
+        launched but returned as a callable, for the caller to launch later."""
         from . import _lib as L
         o, device = self.o, self.device
         nullctx = _Null()
@@ -934,6 +947,14 @@ class Sweep:
                         L.check(lib.xpg_pass2_frames(C.byref(det.s), C.byref(call2), C.byref(view.s),
                                                      C.byref(self.store), L.ptr(prof), int(prof.shape[0]), bg, nbg,
                                                      0, C.byref(out), sst))
+            if host is not None:        # the block's rows (contiguous in the table) to the host, behind its pass 2
+                cs = self.copy_stream()
+                cs.wait_stream(side)
+                s0, n_b = fit[0], fit[1]
+                src = {**rb.o1, **rb.o2}
+                with torch.cuda.stream(cs):
+                    for k in self.RESULT_KEYS:
+                        host[k][s0:s0 + n_b].copy_(src[k][s0:s0 + n_b], non_blocking=True)
 
         if defer:
             return pass2
