@@ -26,7 +26,7 @@ class DeviceFrames:
     trusted, as ImageSet.static_mask (untrusted -> -1 even when overloaded).
     """
 
-    def __init__(self, data, first=1, overload=None, min_valid=0, flag_value=None, static_mask=None):
+    def __init__(self, data, first=1, overload=None, min_valid=0, flag_value=None, static_mask=None, n_frames=None):
         L.require_device(data)
         if data.dim() != 3:
             raise ValueError("frames must be (n, ny, nx)")
@@ -36,7 +36,10 @@ class DeviceFrames:
             raise ValueError("raw frames must be 32-bit")
         self.device = self.data.device
         self.first = int(first)
-        self.n = int(self.data.shape[0])
+        # n_frames > len(data): a ring of len(data) slots holding frames first .. first + n_frames - 1 as they are
+        # streamed in (frame first + t in slot t % len(data)); the caller keeps the frames a launch reads resident
+        self.ring = int(self.data.shape[0]) if n_frames is not None and n_frames > self.data.shape[0] else 0
+        self.n = int(n_frames) if self.ring else int(self.data.shape[0])
         self.last = self.first + self.n - 1
         self.ny, self.nx = int(self.data.shape[1]), int(self.data.shape[2])
         self.overload = overload
@@ -75,6 +78,7 @@ class DeviceFrames:
             t = torch.as_tensor(np.asarray(mask, np.uint8), device=self.device)
             keep.append(t)
             s.ceil_mask = L.ptr(t)
+        s.ring = self.ring
         s.min_valid = self.min_valid
         s.use_flag_value = int(self.flag_value is not None)
         s.flag_value = int(self.flag_value or 0)
@@ -87,11 +91,41 @@ class DeviceFrames:
         return self.data.numel() * self.data.element_size()
 
 
-def from_imageset(images, frames, device):
-    """Upload the value-coded float32 frames of an ImageSet-like source (``iter_frames``) into a device store."""
+def from_imageset(images, frames, device, batch=10):
+    """Upload the value-coded float32 frames of an ImageSet-like source (``iter_frames``) into a device store, through
+    two pinned host buffers of ``batch`` frames: the host fills one while the other's copy runs."""
     f0, f1 = frames
     n = f1 - f0 + 1
     data = torch.empty((n, images.ny, images.nx), dtype=torch.float32, device=device)
-    for f, img in images.iter_frames(range(f0, f1 + 1), copy=False):
-        data[f - f0].copy_(torch.from_numpy(np.ascontiguousarray(img, np.float32)), non_blocking=False)
+    up = FrameUploader(images, device, batch)
+    for k0 in range(0, n, batch):
+        up.upload(f0 + k0, min(batch, n - k0), data[k0:k0 + batch])
+    torch.cuda.current_stream(device).synchronize()
     return DeviceFrames(data, first=f0)
+
+
+class FrameUploader:
+    """Host frames to the device in batches, through two pinned buffers: filling one waits only for the copy that
+    last read it (an event), so reading the next batch on the host overlaps the previous batch's copy."""
+
+    def __init__(self, images, device, batch):
+        self.images, self.device, self.batch = images, device, int(batch)
+        self.buf = [torch.empty((self.batch, images.ny, images.nx), dtype=torch.float32, pin_memory=True)
+                    for _ in range(2)]
+        self.done = [None, None]
+        self.k = 0
+
+    def upload(self, f0, nf, dst):
+        """Frames f0 .. f0 + nf - 1 into ``dst`` (nf, ny, nx) float32 on the device, asynchronously on the current
+        stream."""
+        i = self.k % 2
+        self.k += 1
+        if self.done[i] is not None:
+            self.done[i].synchronize()
+        hb = self.buf[i]
+        for j, (f, img) in enumerate(self.images.iter_frames(range(f0, f0 + nf), copy=False)):
+            hb[j].copy_(torch.from_numpy(np.ascontiguousarray(img, np.float32)))
+        dst.copy_(hb[:nf], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        self.done[i] = ev

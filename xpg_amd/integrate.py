@@ -393,10 +393,12 @@ class SweepPlan:
 def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, dmax=1e9, frames=None, mask=None,
                minimum_zeta=0.05, verbose=True, log=print, n_regions_xy=3, learn_profiles=True, exclude_frames=(),
                adaptive=True, deconvolve_overlaps=True, extra_lattices=None, bkg_template=None, foreign_spots=None,
-               n_foreign=4, moments_only=False, plan_only=False, timing=None, _box_draw=None):
+               n_foreign=4, moments_only=False, plan_only=False, timing=None, hbm_budget=None, _box_draw=None):
     """Integrate a sweep (xpg/integrate.py:209-999).  ``images``: a DeviceFrames store, or an ImageSet-like object
-    (first, last, nx, ny, iter_frames) whose frames are uploaded once.  ``plan_only`` returns the host plan without
-    device work (planning tests); ``timing`` (dict) receives host / device times."""
+    (first, last, nx, ny, iter_frames) whose frames are uploaded through pinned buffers -- all of them when they and
+    the batched form's prepared data fit ``hbm_budget`` (bytes; default 85% of the device's free memory), else
+    streamed through a ring of frames block by block (xpg_amd/passes.py; bit-identical results).  ``plan_only``
+    returns the host plan without device work (planning tests); ``timing`` (dict) receives host / device times."""
     lattice_of = [0] * len(exp_blocks)
     if extra_lattices:
         exp_blocks = list(exp_blocks)
@@ -491,11 +493,25 @@ def _integrate(images, exp_blocks, block_edges, model: IntegrationModel, dmin, d
                      need_lo=need_lo, need_hi=need_hi, pj=pj)
     if plan_only:
         return host_plan
-    sw = Sweep(images, exp_blocks, model, host_plan, dict(
-        frames=frames, mask=mask, exclude_frames=exclude_frames, learn_profiles=learn_profiles, adaptive=adaptive,
-        bkg_template=bkg_template, moments_only=moments_only, nreg=nreg, sd=sd, sm=sm))
-    sw.t_plan = time.time() - t0
-    run = sw.run()
+    opts = dict(frames=frames, mask=mask, exclude_frames=exclude_frames, learn_profiles=learn_profiles,
+                adaptive=adaptive, bkg_template=bkg_template, moments_only=moments_only, nreg=nreg, sd=sd, sm=sm)
+    mp = None
+    if not isinstance(images, DeviceFrames):
+        from . import passes
+        device = _kernel_device(images)
+        host_plan["sweep_plan"] = sweep_plan(host_plan, exp_blocks)
+        budget = hbm_budget if hbm_budget is not None else int(0.85 * torch.cuda.mem_get_info(device)[0])
+        mp = passes.plan_memory(host_plan, host_plan["sweep_plan"], frames, images.ny, images.nx, budget,
+                                frame_offset=f_first - 1)
+        host_plan["memory_plan"] = mp
+    if mp is not None and mp.streamed:
+        sw = Sweep(passes.stream_store(images, mp, frames, device), exp_blocks, model, host_plan, opts)
+        sw.t_plan = time.time() - t0
+        run = passes.run_streamed(sw, images, mp, frames)
+    else:
+        sw = Sweep(images, exp_blocks, model, host_plan, opts)
+        sw.t_plan = time.time() - t0
+        run = sw.run()
     res = sw.finish(run, t0)
     if timing is not None:
         timing.update(res.timing)

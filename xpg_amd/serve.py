@@ -35,6 +35,7 @@ from . import _lib as L
 from . import synth
 from .frames import DeviceFrames
 from .integrate import IntegrationModel, RunBufs, Sweep, _integrate, sweep_plan
+from .passes import ring_frames  # noqa: F401  (the ring of frames a dataset needs; re-exported)
 
 
 @dataclass
@@ -61,21 +62,6 @@ class Plan:
     done_batch: np.ndarray       # per reflection (table order of hp): the batch that completes it
     spot_idx: np.ndarray = None  # the generator's spots reaching each batch (batch_spot_index)
     spot_off: np.ndarray = None
-
-
-def ring_frames(order, need_lo, need_hi, batch, nframes):
-    """Frames the ring must hold, and each block's enqueue batch.  Blocks run in ``order`` (the adaptive chain's
-    order), each after the batch holding its last needed frame and after the blocks before it; when block i is
-    enqueued after batch j, the frames written so far are [0, (j + 1) batch) and it still needs need_lo[i] ..."""
-    ready, last = [], -1
-    ring = batch
-    for b in order:
-        j = max(last, min(need_hi[b], nframes - 1) // batch)
-        last = j
-        ready.append(j)
-        ring = max(ring, (j + 1) * batch - max(need_lo[b], 0))
-    ring = -(-ring // batch) * batch
-    return min(ring, -(-nframes // batch) * batch), ready
 
 
 def plan_request(req: Request, batch_frames=10):
