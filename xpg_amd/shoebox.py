@@ -3,7 +3,7 @@
 SB_OFF and the stored-value convention (value + 3, 0 = unrecorded) are xpg/shoebox.py:11; ``_decode_bins`` runs the
 bin decoding of xpg/shoebox.py:264-268 on the device (HIP kernel ``xpg_decode_bins``).  The reference's host shoebox
 buffers and their memory planner (xpg/shoebox.py:19-261) have no counterpart here: frames stay in HBM, and sweeps
-larger than the HBM budget are planned as passes over the frames by xpg_amd/passes.py.
+larger than the HBM budget stream their frames through a ring on the device (xpg_amd/passes.py).
 """
 from __future__ import annotations
 
