@@ -729,7 +729,7 @@ def main():
     t_setup = time.time() - t
     N = sweep.sp.n
     # pinned host memory for the results: every timed step ends with them copied there
-    hostbuf = sweep.host_results(dict(zip(("o1", "o2"), sweep.alloc_outputs())))
+    hostbuf = sweep.host_results()
     log(f"planned {N} reflections in {len(sweep.sp.groups)} shoebox groups ({len(sweep.sp.bins)} size-bin launches) "
         f"over {len(edges) - 1} blocks (host plan {t_host_plan:.1f}s, device setup {t_setup:.1f}s)")
 
@@ -769,11 +769,10 @@ def main():
 
     plain = None
     graph_info = None
-    if args.form in ("both", "plain"):
-        t_plain, n_plain, last_plain = time_form("plain", args.steps, args.warmup, None)
-        plain = dict(value=round(n_plain / t_plain, 1), ms_per_step=round(t_plain * 1e3, 3))
     form = "plain" if args.form == "plain" else "batched"
     if form == "plain":
+        t_plain, n_plain, last_plain = time_form("plain", args.steps, args.warmup, None)
+        plain = dict(value=round(n_plain / t_plain, 1), ms_per_step=round(t_plain * 1e3, 3))
         t_step, integrated, last = t_plain, n_plain, last_plain
         events.clear()
         time_form("plain", 1, 0, Timer)
@@ -803,13 +802,8 @@ def main():
         else:
             t_step, integrated, last = t_eager, n_eager, last_eager
             graph_info["reported"] = "eager launches"
-        ref = last_plain if plain is not None else eager
-        a, b_ = ref["o2"]["iprf"], last["o2"]["iprf"]
-        same = bool(torch.equal(torch.isfinite(a), torch.isfinite(b_)) and
-                    torch.equal(a[torch.isfinite(a)], b_[torch.isfinite(b_)]))
-        if plain is not None:
-            plain["identical_results"] = same
-        for other in (last_eager, last_graph):
+        b_ = last["o2"]["iprf"]
+        for other in (eager, last_eager, last_graph):
             if not torch.equal(torch.nan_to_num(other["o2"]["iprf"], nan=-1.0), torch.nan_to_num(b_, nan=-1.0)):
                 graph_info["identical_results"] = False
         graph_info.setdefault("identical_results", True)
@@ -820,6 +814,20 @@ def main():
     fin = lambda t: torch.nan_to_num(t.cpu(), nan=-1.0)
     src = {**last["o1"], **last["o2"]}
     host_copy_identical = all(torch.equal(fin(hostbuf[k]), fin(src[k])) for k in sweep.RESULT_KEYS)
+
+    # the plain form beside it, timed after the batched one with the batched form's prepared data and graph released
+    # (each form is timed with only its own buffers allocated: with both held, the largest workloads brought the
+    # allocator close to the device's memory, and the form timed second ran up to 15% slower)
+    if form == "batched" and args.form == "both":
+        sweep.prep, sweep.prep_g = {}, {}
+        del sweep.graph, sweep.graph_out, eager, last_eager, last_graph
+        torch.cuda.synchronize(dev)
+        torch.cuda.empty_cache()
+        t_plain, n_plain, last_plain = time_form("plain", args.steps, args.warmup, None)
+        a = last_plain["o2"]["iprf"]
+        plain = dict(value=round(n_plain / t_plain, 1), ms_per_step=round(t_plain * 1e3, 3),
+                     identical_results=bool(torch.equal(torch.nan_to_num(a, nan=-1.0),
+                                                        torch.nan_to_num(b_, nan=-1.0))))
 
     # dominant kernel and its roofline, over every kernel of the sweep (HIP events on the launch stream, algorithmic
     # bytes per launch)

@@ -261,6 +261,13 @@ int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx);
  * gathered values (xpg_prepare / xpg_extract), read in place; the *_frames entry points reject them. */
 int xpg_lds_limit(void);
 
+/* The same byte range of `height` rows of a row-major block, between device and (pinned) host memory or within one
+ * of them, as one asynchronous 2-D copy on `stream`: dst + i * dst_pitch <- src + i * src_pitch, `width` bytes each.
+ * No counterpart in the reference (its results are host arrays already): a sweep moves each phi block's rows of every
+ * per-reflection result to the host with one such copy per dtype, behind the block's pass 2. */
+int xpg_copy_rows(void *dst, int64_t dst_pitch, const void *src, int64_t src_pitch, int64_t width, int64_t height,
+                  void *stream);
+
 #ifdef __cplusplus
 }
 #endif

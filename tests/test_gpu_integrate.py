@@ -165,7 +165,7 @@ def test_results_copied_to_pinned_host_memory_per_block():
                                                  moments_only=False, nreg=9, sd=np.radians(model.sigma_d),
                                                  sm=np.radians(model.sigma_m)))
     assert len(sw.order) >= 2
-    host = sw.host_results(dict(zip(("o1", "o2"), sw.alloc_outputs())))
+    host = sw.host_results()
     for t in host.values():
         t.fill_(-7)
     run = sw.enqueue(None, host)

@@ -125,6 +125,7 @@ _SIGS = {
                             C.c_int32, P(Pass2Out), C.c_void_p], C.c_int),
     "xpg_lds_bytes": ([C.c_int32] * 3, C.c_int),
     "xpg_lds_limit": ([], C.c_int),
+    "xpg_copy_rows": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64, C.c_int64, C.c_int64, C.c_void_p], C.c_int),
     "xpg_adapt_block": ([C.c_int32] + [C.c_void_p] * 12 + [C.c_double] * 5 + [C.c_void_p] * 4 + [C.c_int64,
                                                                                          C.c_void_p], C.c_int),
     "xpg_adapt_workspace_bytes": ([C.c_int32], C.c_int64),

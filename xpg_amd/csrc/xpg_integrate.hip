@@ -2573,6 +2573,16 @@ int xpg_rocking(const xpg_call *call, const xpg_refl *refl, const xpg_prepared *
   return XPG_OK;
 }
 
+int xpg_copy_rows(void *dst, int64_t dst_pitch, const void *src, int64_t src_pitch, int64_t width, int64_t height,
+                  void *stream) {
+  if (!dst || !src || width < 0 || height < 0 || dst_pitch < width || src_pitch < width)
+    return fail(XPG_ERR_ARG, "copy_rows: bad pointers or sizes");
+  if (width == 0 || height == 0) return XPG_OK;
+  HIP_TRY(hipMemcpy2DAsync(dst, (size_t)dst_pitch, src, (size_t)src_pitch, (size_t)width, (size_t)height,
+                           hipMemcpyDefault, (hipStream_t)stream));
+  return XPG_OK;
+}
+
 int xpg_lds_limit(void) { return lds_limit(); }
 
 int xpg_lds_bytes(int32_t Dz, int32_t Dy, int32_t Dx) {

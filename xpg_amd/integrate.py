@@ -700,19 +700,33 @@ class Sweep:
             self.graph_out = self.enqueue(None, host)
         return self.graph_out
 
-    RESULT_KEYS = OUT_F64 + ("n_over", "n_fg_masked")
+    RESULT_F64 = OUT_F64
+    RESULT_I32 = ("n_over", "n_fg_masked")
+    RESULT_KEYS = RESULT_F64 + RESULT_I32
 
-    def host_results(self, run):
-        """Pinned host tensors for the per-reflection results of ``run`` (copy_results' destination)."""
-        src = {**run["o1"], **run["o2"]}
-        return {k: torch.empty(src[k].shape, dtype=src[k].dtype, pin_memory=True) for k in self.RESULT_KEYS}
+    def host_results(self):
+        """Pinned host rows for the per-reflection results (copy_results' destination): a dict of per-key views of
+        one (len(RESULT_F64), N) float64 and one (len(RESULT_I32), N) int32 block, the blocks as ``.blocks``."""
+        N = self.sp.n
+        F = torch.empty((len(self.RESULT_F64), N), dtype=torch.float64, pin_memory=True)
+        I = torch.empty((len(self.RESULT_I32), N), dtype=torch.int32, pin_memory=True)
+        h = _Rows({k: F[i] for i, k in enumerate(self.RESULT_F64)})
+        h.update({k: I[i] for i, k in enumerate(self.RESULT_I32)})
+        h.blocks = (F, I)
+        return h
 
-    def copy_results(self, run, host):
-        """The per-reflection results of ``run`` (table order) into the pinned tensors ``host``: asynchronous copies
-        on the current stream, complete at its next synchronisation."""
-        src = {**run["o1"], **run["o2"]}
-        for k in self.RESULT_KEYS:
-            host[k].copy_(src[k], non_blocking=True)
+    def copy_results(self, run, host, rows=None, stream=None):
+        """Rows ``rows`` = (first, n) (default: all) of every per-reflection result of ``run`` (table order) into
+        ``host`` (host_results()): one asynchronous 2-D copy per dtype on ``stream`` (default: the current one),
+        complete at its next synchronisation."""
+        from . import _lib as L
+        s0, n = rows if rows is not None else (0, self.sp.n)
+        st = stream.cuda_stream if stream is not None else torch.cuda.current_stream(self.device).cuda_stream
+        for dst, src in zip(host.blocks, run["blocks"]):
+            es, N = dst.element_size(), self.sp.n
+            L.check(L.lib().xpg_copy_rows(C.c_void_p(dst.data_ptr() + es * s0), es * N,
+                                          C.c_void_p(src.data_ptr() + es * s0), es * N, es * n, dst.shape[0],
+                                          C.c_void_p(st)))
         return host
 
     def replay(self):
@@ -740,13 +754,23 @@ class Sweep:
             if gi not in self.prep_g:
                 self.prep_g[gi] = self._prep_group(gi)
 
+    def out_keys(self):
+        """Row keys of the float64 and the int32 output block (alloc_outputs), results first."""
+        i32 = list(self.RESULT_I32) + [k for k in P1_KEEP if k.startswith("n_") and k not in self.RESULT_I32]
+        f64 = list(self.RESULT_F64) + [k for k in P1_KEEP + K.P2_F64 if k not in i32 and k not in self.RESULT_F64]
+        return f64, i32
+
     def alloc_outputs(self):
-        """Per-reflection output tensors of one run (NaN until written)."""
+        """Per-reflection output tensors of one run (NaN until written): (o1, o2, blocks), the per-key tensors being
+        rows of one float64 and one int32 block (``blocks``), the results first, so that the rows of a phi block move
+        to the host in one 2-D copy per dtype (copy_results)."""
         N, device = self.sp.n, self.device
-        f64 = lambda: torch.full((N,), float("nan"), dtype=torch.float64, device=device)
-        o1 = {k: (torch.zeros(N, dtype=torch.int32, device=device) if k.startswith("n_") else f64()) for k in P1_KEEP}
-        o2 = {k: f64() for k in K.P2_F64}
-        return o1, o2
+        f64, i32 = self.out_keys()
+        F = torch.full((len(f64), N), float("nan"), dtype=torch.float64, device=device)
+        I = torch.zeros((len(i32), N), dtype=torch.int32, device=device)
+        rows = {k: F[i] for i, k in enumerate(f64)}
+        rows.update({k: I[i] for i, k in enumerate(i32)})
+        return {k: rows[k] for k in P1_KEEP}, {k: rows[k] for k in K.P2_F64}, (F, I)
 
     def enqueue(self, timer=None, host=None):
         """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
@@ -966,11 +990,7 @@ class Sweep:
             if host is not None:        # the block's rows (contiguous in the table) to the host, behind its pass 2
                 cs = self.copy_stream()
                 cs.wait_stream(side)
-                s0, n_b = fit[0], fit[1]
-                src = {**rb.o1, **rb.o2}
-                with torch.cuda.stream(cs):
-                    for k in self.RESULT_KEYS:
-                        host[k][s0:s0 + n_b].copy_(src[k][s0:s0 + n_b], non_blocking=True)
+                self.copy_results(rb.result(), host, (fit[0], fit[1]), cs)
 
         if defer:
             return pass2
@@ -1052,7 +1072,10 @@ class Sweep:
         N = sp.n
         inv = np.empty(N, np.int64)
         inv[sp.perm] = np.arange(N)
-        host = {k: v.cpu().numpy()[inv] for k, v in {**run["o1"], **run["o2"]}.items()}
+        F, I = (b.cpu().numpy() for b in run["blocks"])      # two copies for every per-reflection output
+        f64, i32 = self.out_keys()
+        host = {k: F[i][inv] for i, k in enumerate(f64)}
+        host.update({k: I[i][inv] for i, k in enumerate(i32)})
         refl = dict(hp["refl"])
         for k in OUT_F64:
             refl[k] = host[k].astype(np.float64)
@@ -1078,7 +1101,7 @@ class RunBufs:
     the normalised profiles of every block, with the raw pointers the launches take (no per-launch tensor views)."""
 
     def __init__(self, sweep):
-        self.o1, self.o2 = sweep.alloc_outputs()
+        self.o1, self.o2, self.blocks = sweep.alloc_outputs()
         dev, G = sweep.device, K.PROFILE_GRID
         nb, nreg = max(len(sweep.order), 1), sweep.o["nreg"]
         self.learn = bool(sweep.o["learn_profiles"])
@@ -1099,13 +1122,18 @@ class RunBufs:
         return base + es * start
 
     def result(self):
-        return dict(o1=self.o1, o2=self.o2, prof_by_block=self.prof_by_block)
+        return dict(o1=self.o1, o2=self.o2, blocks=self.blocks, prof_by_block=self.prof_by_block)
 
 
 def _record(stream):
     e = torch.cuda.Event()
     e.record(stream)
     return e
+
+
+class _Rows(dict):
+    """Per-key views of result blocks (host_results); ``blocks`` holds the blocks."""
+    blocks = ()
 
 
 class _Null:
