@@ -307,7 +307,7 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
     nbytes = copy_mib * 2 ** 20
     f_fac = statistics.mean(fetch[calib]) * 1024 / nbytes
     w_fac = statistics.mean(write[calib]) * 1024 / nbytes
-    pat = re.compile(r"\b" + re.escape(kernel) + r"\b")
+    pat = re.compile(r"\b" + re.escape(kernel) + (r"\b" if re.match(r"\w", kernel[-1]) else ""))
     fk = [v for k, vs in fetch.items() if pat.search(k) for v in vs]
     wk = [v for k, vs in write.items() if pat.search(k) for v in vs]
     f_b = statistics.mean(fk) * 1024 / f_fac
@@ -315,7 +315,8 @@ def summarize_pmc(fetch_csv, write_csv, copy_mib, kernel, workload, command):
     rec = dict(kernel=kernel, workload=workload, bytes_per_launch=round(f_b + w_b), fetch_bytes=round(f_b),
                write_bytes=round(w_b), dispatches=len(fk), copy_calibration=dict(
                    mib=copy_mib, fetch_factor=round(f_fac, 4), write_factor=round(w_fac, 4)), command=command)
-    base = ROOT / "profiles" / f"traffic_{kernel[2:] if kernel.startswith('k_') else kernel}_{workload}"
+    label = re.sub(r"\W+", "_", kernel[2:] if kernel.startswith("k_") else kernel).strip("_")
+    base = ROOT / "profiles" / f"traffic_{label}_{workload}"
     base.with_suffix(".json").write_text(json.dumps(rec, indent=1) + "\n")
     base.with_suffix(".md").write_text(
         f"# HBM traffic of {kernel} ({workload})\n\nCommand (each counter in its own run): `{command}`\n\n"
@@ -532,6 +533,137 @@ def bench_decode(args, dev):
     print(json.dumps(outj), flush=True)
 
 
+def bench_forward(args, dev):
+    """Forward (window) model passes (xpg/forward/kernels.py:34-134) at the full-size sweep's density: a window of the
+    whole 4148 x 4362 detector over 100 frames (1.81 G voxels: mu, A, M float32 and NS int16 resident, 25 GB), the
+    reflections whose supports meet it (896k over 1800 frames -> 52.8k), supports 7 frames x 9 x 9 pixels (the central
+    0.8 of the 7 x 11 x 11 shoeboxes), two terms (halo).  A step is one iteration of the intensity solve: scatter
+    (mu += I f + beta), gather mode 1 (Newton gradient and Fisher diagonal), gather mode 0 (Fisher-vector product).
+    Roofline per kernel: algorithmic bytes = factors (R, P) + per-row scalars + per in-window voxel 8 B (scatter:
+    the float atomic's read and write), 6 B (mode 1: M, NS) or 10 B (mode 0: A, M, NS); over the kernel's HIP-event
+    time on its launch stream."""
+    from types import SimpleNamespace
+
+    import numpy as np
+    import torch
+    from xpg_amd import forward as F
+
+    torch.cuda.set_device(dev)
+    nf = args.frames or 100
+    ny, nx, Dz, b, T = 4362, 4148, 7, 9, 2
+    n = int(round(896_000 * (nf + Dz - 1) / 1800))
+    win = SimpleNamespace(f0=0, f1=nf, y0=0, ny=ny, x0=0, nx=nx)
+    nv = nf * ny * nx
+    rng = np.random.default_rng(7)
+    iz0 = rng.integers(-Dz + 1, nf, n)
+    iy0 = rng.integers(-b // 2, ny - b // 2, n)
+    ix0 = rng.integers(-b // 2, nx - b // 2, n)
+    o = np.argsort(iz0 * (ny * nx) + iy0 * nx + ix0, kind="stable")    # the model's row order: by support origin
+    iz0, iy0, ix0 = iz0[o], iy0[o], ix0[o]
+    kz = np.clip(iz0[:, None] + np.arange(Dz), -1, nf)
+    ky = iy0[:, None] + np.arange(b)
+    kx = ix0[:, None] + np.arange(b)
+    in_vox = (((kz >= 0) & (kz < nf)).sum(1) * ((ky >= 0) & (ky < ny)).sum(1) * ((kx >= 0) & (kx < nx)).sum(1))
+    n_in = int(in_vox.sum())
+    g = torch.Generator(device=dev).manual_seed(11)
+    R = torch.rand((T, n, Dz), device=dev, generator=g)
+    P = torch.rand((T, n, b, b), device=dev, generator=g) / (b * b)
+    I = torch.rand(n, device=dev, generator=g) * 1000
+    beta = torch.rand(n, device=dev, generator=g)
+    t_iz, t_iy, t_ix = (torch.as_tensor(a, dtype=torch.int32, device=dev) for a in (iz0, iy0, ix0))
+    rows = torch.arange(n, device=dev)
+    B = torch.full((nv,), 2.0, device=dev)
+    NS = torch.randint(0, 9, (nv,), dtype=torch.int16, device=dev, generator=g)
+    A = torch.rand(nv, device=dev, generator=g)
+    mu = torch.empty(nv, device=dev)
+    gN = torch.zeros(n, device=dev)
+    fd = torch.zeros(n, device=dev)
+    gR = torch.zeros(n, device=dev)
+    torch.cuda.synchronize(dev)
+    log(f"forward window {nf}x{ny}x{nx} ({nv / 1e9:.2f} G voxels), {n} reflections, {n_in / 1e6:.1f} M in-window "
+        f"support voxels")
+    st = torch.cuda.current_stream(dev)
+
+    def step(ev=None):
+        mu.copy_(B)
+        gN.zero_(), fd.zero_(), gR.zero_()
+        if ev:
+            ev[0].record(st)
+        F.scatter(mu, R, P, I, beta, t_iz, t_iy, t_ix, win)
+        if ev:
+            ev[1].record(st)
+        F.gather(gN, fd, R, P, rows, t_iz, t_iy, t_ix, win, None, mu, NS, 1)
+        if ev:
+            ev[2].record(st)
+        F.gather(gR, None, R, P, rows, t_iz, t_iy, t_ix, win, A, mu, NS, 0)
+        if ev:
+            ev[3].record(st)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    first = (gN.clone(), fd.clone(), gR.clone())
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    t0 = time.time()
+    for ev in evs:
+        step(ev)
+    torch.cuda.synchronize(dev)
+    wall = (time.time() - t0) / args.steps
+    # mu's float atomics add in a varying order, so the gathers agree between steps to rounding only
+    assert args.warmup == 0 or all(torch.allclose(a, c, rtol=1e-4, atol=1e-6) for a, c in zip(first, (gN, fd, gR))), \
+        "gathers differ between steps"
+    ms = {k: sum(ev[i].elapsed_time(ev[i + 1]) for ev in evs) / args.steps
+          for i, k in enumerate(("k_fm_scatter", "k_fm_gather<newton>", "k_fm_gather<ratio>"))}
+    fac = T * (Dz + b * b) * 4 + 20
+    alg = {"k_fm_scatter": n * fac + 8 * n_in, "k_fm_gather<newton>": n * (fac + 8) + 6 * n_in,
+           "k_fm_gather<ratio>": n * (fac + 4) + 10 * n_in}
+    kern = {k: dict(avg_launch_ms=round(ms[k], 4), alg_bytes_per_launch=alg[k],
+                    achieved=round(alg[k] / (ms[k] * 1e-3) / 1e9, 1),
+                    frac=round(alg[k] / (ms[k] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)) for k in ms}
+    # HBM bytes per launch from the committed counter summaries (the int16-count instantiations k_fm_gather<0, MODE>)
+    for k, f in (("k_fm_scatter", "fm_scatter"), ("k_fm_gather<newton>", "fm_gather_0_1"),
+                 ("k_fm_gather<ratio>", "fm_gather_0_0")):
+        tf = ROOT / "profiles" / f"traffic_{f}_forward.json"
+        kern[k]["traffic"] = json.loads(tf.read_text()).get("bytes_per_launch") if tf.exists() else None
+    dom = max(ms, key=ms.get)
+    roof = dict(bound="hbm", kernel=dom, achieved=kern[dom]["achieved"], peak=PEAK_HBM_GBS, unit="GB/s",
+                frac=kern[dom]["frac"], traffic=kern[dom]["traffic"], kernels=kern)
+    t_step = sum(ms.values())
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (numpy float64, one core) on a 512 x 512 tile of the same window at the same density
+        from oracle import forward as FO
+        ty, tx = 512, 512
+        m = (iy0 >= 0) & (iy0 < ty) & (ix0 >= 0) & (ix0 < tx)
+        sw = SimpleNamespace(f0=0, f1=nf, y0=0, ny=ty, x0=0, nx=tx)
+        Rh, Ph = R[:, torch.as_tensor(m, device=dev)].cpu().numpy(), P[:, torch.as_tensor(m, device=dev)].cpu().numpy()
+        Ih, bh = I.cpu().numpy()[m], beta.cpu().numpy()[m]
+        nvs = nf * ty * tx
+        As = rng.random(nvs).astype(np.float32)
+        NSs = rng.integers(0, 9, nvs).astype(np.int16)
+        ns = int(m.sum())
+        t = time.time()
+        reps = 0
+        while reps < 1 or time.time() - t < min(args.cpu_seconds, 10.0):
+            mus = FO.scatter(np.full(nvs, 2.0), Rh, Ph, Ih, bh, iz0[m], iy0[m], ix0[m], sw)
+            FO.gather(np.zeros(ns), np.zeros(ns), Rh, Ph, np.arange(ns), iz0[m], iy0[m], ix0[m], sw, None, mus, NSs, 1)
+            FO.gather(np.zeros(ns), None, Rh, Ph, np.arange(ns), iz0[m], iy0[m], ix0[m], sw, As, mus, NSs, 0)
+            reps += 1
+        dt = (time.time() - t) / reps
+        cpu = dict(value=round(ns / dt, 1), unit="reflections/s", cores=1, kind="port",
+                   sample=f"{ns} reflections of a 512x512x{nf} tile of the same window, oracle/forward.py (numpy "
+                          f"float64) scatter + both gathers, {reps} reps, {dt:.2f} s each")
+        log(f"cpu baseline {cpu}")
+    outj = dict(metric="forward-model window passes: reflections/s on 1 MI355X", value=round(n / (t_step * 1e-3), 1),
+                unit="reflections/s", n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(t_step, 4),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f32", data="synthetic",
+                config=dict(workload=f"forward_window_{nx}x{ny}x{nf}", reflections=n, support=[Dz, b, b], terms=T,
+                            in_window_voxels=n_in, window_voxels=nv, ns_dtype="int16",
+                            wall_ms_per_step=round(wall * 1e3, 3), parallelism="single GPU"),
+                roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(outj), flush=True)
+
+
 def bench_cbf(args, dev):
     """CBF byte-offset decode of PILATUS 6M frames (2463 x 2527 int32: Poisson background, module gaps at -1, bad
     pixels at -2, hot pixels up to the count cut-off), encoded as read_cbf reads them (encode_byte_offset); the streams
@@ -621,7 +753,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode", "cbf"])
+    ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode", "cbf",
+                                                                 "forward"])
     ap.add_argument("--serve-datasets", type=int, default=32)
     ap.add_argument("--serve-frame-scale", type=float, default=0.1,
                     help="serving workload: frame counts U{50..3600} times this factor (at least 20 frames)")
@@ -693,6 +826,9 @@ def main():
         return
     if args.workload == "cbf":
         bench_cbf(args, dev)
+        return
+    if args.workload == "forward":
+        bench_forward(args, dev)
         return
     spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
                 stress=synth.spec_stress)[args.workload]()

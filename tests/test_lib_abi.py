@@ -1,4 +1,4 @@
-"""The C ABI library loads and exports every entry point include/xpg_integrate.h declares, and the ctypes structs of
+"""The C ABI library loads and exports every entry point include/*.h declares, and the ctypes structs of
 xpg_amd/_lib.py have the C layout (checked against gcc on the header).  No GPU needed."""
 import re
 import subprocess
@@ -7,7 +7,8 @@ from pathlib import Path
 ROOT = Path(__file__).resolve().parent.parent
 STRUCTS = dict(Detector="xpg_detector", Call="xpg_call", Refl="xpg_refl", Pass1Out="xpg_pass1_out",
                Pass2Out="xpg_pass2_out", ProfileAcc="xpg_profile_acc", Frames="xpg_frames", Widths="xpg_widths",
-               Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch", CbfBatch="xpg_cbf_batch")
+               Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch", CbfBatch="xpg_cbf_batch",
+               FmSupports="xpg_fm_supports", FmWindow="xpg_fm_window")
 
 
 def _declared():
@@ -67,7 +68,7 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     import ctypes
     from xpg_amd import _lib
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', '#include "xpg_decode.h"', '#include "xpg_cbf.h"',
-             "int main(void) {"]
+             '#include "xpg_forward.h"', "int main(void) {"]
     for py, c in STRUCTS.items():
         lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
         for f, _ in getattr(_lib, py)._fields_:

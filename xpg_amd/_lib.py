@@ -1,5 +1,5 @@
 """ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_cbf.h,
-include/xpg_synth.h).
+include/xpg_synth.h, include/xpg_forward.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
 is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
@@ -101,6 +101,15 @@ class CbfBatch(C.Structure):
                 ("status", C.c_void_p)]
 
 
+class FmSupports(C.Structure):
+    _fields_ = [("R", C.c_void_p), ("P", C.c_void_p), ("iz0", C.c_void_p), ("iy0", C.c_void_p), ("ix0", C.c_void_p),
+                ("T", C.c_int32), ("n", C.c_int32), ("Dz", C.c_int32), ("b", C.c_int32)]
+
+
+class FmWindow(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("f0", "nf", "y0", "ny", "x0", "nx")]
+
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -136,6 +145,9 @@ _SIGS = {
     "xpg_bslz4_index_host": ([C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_int32, C.c_void_p], C.c_int32),
     "xpg_bslz4_index": ([P(Bslz4Batch), C.c_void_p], C.c_int),
     "xpg_bslz4_decode": ([P(Bslz4Batch), C.c_void_p], C.c_int),
+    "xpg_fm_scatter": ([C.c_void_p, P(FmSupports), C.c_void_p, C.c_void_p, P(FmWindow), C.c_void_p], C.c_int),
+    "xpg_fm_gather": ([C.c_void_p, C.c_void_p, P(FmSupports), C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_int32, C.c_int32, P(FmWindow), C.c_void_p], C.c_int),
     "xpg_cbf_workspace_bytes": ([C.c_int64, C.c_int64], C.c_int64),
     "xpg_cbf_decode": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_decode_static": ([P(CbfBatch), C.c_void_p], C.c_int),
