@@ -79,11 +79,16 @@ typedef struct xpg_refl {
    * its pixels (a row's geo / geo_p block starts at element 3 poff, its per-pixel pass-1 outputs at poff).
    * wave_lds: LDS bytes per wavefront of the launch, at least xpg_lds_bytes of every row's shape (0: sized from
    * Dz x Dy x Dx); a row whose layout exceeds it is skipped, its outputs left as they were.
-   * wg_waves: at most this many wavefronts (rows) per workgroup (0: as many as fit, up to 4). */
+   * wg_waves: at most this many wavefronts (rows) per workgroup (0: as many as fit, up to 4).
+   * lds_rows: pass 2 of prepared rows whose boxes all fit the register kernel (at most 8 frames and 128 pixels) runs
+   * the rows with an overlapping partner in the LDS kernel; lds_rows > 0 says that none of them lies past the first
+   * lds_rows rows of the table (the LDS kernel covers only those), -1 that there is none; 0 (the default)
+   * gives no bound. */
   const int32_t *rdz, *rdy, *rdx;
   const int64_t *voff, *poff;
   int32_t wave_lds;
   int32_t wg_waves;
+  int32_t lds_rows;
 } xpg_refl;
 
 /* pass 1 outputs (xpg/integrate_kernels.py:362-365).  Per-reflection float64 / int32 (n); fz (n, Dz); per-pixel

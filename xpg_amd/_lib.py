@@ -50,7 +50,8 @@ class Refl(C.Structure):
                 ("has_p", C.c_void_p), ("pS", C.c_void_p), ("pphi", C.c_void_p), ("K", C.c_int32),
                 ("nbx", C.c_void_p), ("nby", C.c_void_p), ("nbz", C.c_void_p), ("nbsx", C.c_void_p),
                 ("nbsz", C.c_void_p), ("rdz", C.c_void_p), ("rdy", C.c_void_p), ("rdx", C.c_void_p),
-                ("voff", C.c_void_p), ("poff", C.c_void_p), ("wave_lds", C.c_int32), ("wg_waves", C.c_int32)]
+                ("voff", C.c_void_p), ("poff", C.c_void_p), ("wave_lds", C.c_int32), ("wg_waves", C.c_int32),
+                ("lds_rows", C.c_int32)]
 
 
 class Pass1Out(C.Structure):

@@ -237,6 +237,7 @@ LDS_BINS = (8192, 12288, 16384, 24576, 32768, 49152, 65536, 98304, 163840)
 # copies of each block's fixed-point profile grids: pass 1's workgroups spread their integer increments over them
 # (fewer atomics on one address; the normalisation adds them up, exactly)
 ACC_COPIES = 8
+PREP_LEAD = 2           # batched form: blocks whose preparation the host submits before the chain starts
 
 
 class SweepPlan:
@@ -379,6 +380,11 @@ class SweepPlan:
         s.wg_waves = int(self.wg_waves)
         for k in ("x", "y", "z", "sx", "sz"):
             setattr(s, "nb" + k, C.c_void_p(self.t["nb" + k].data_ptr() + 4 * self.K * start))
+        # pass 2 of register-kernel rows: the partner rows lead each shape group, so the LDS kernel needs to cover only
+        # the rows up to the last one that has a partner or a box past the register kernel's (8 frames, 128 pixels)
+        need = (self.h["has_p"][start:start + n] > 0.5) | (self.rows_dz[start:start + n] > 8) | \
+            (self.rows_bx[start:start + n] ** 2 > 128)
+        s.lds_rows = int(np.flatnonzero(need)[-1]) + 1 if need.any() else -1
         if mixed:
             s.rdz = C.c_void_p(self.t["rdz"].data_ptr() + 4 * start)
             s.rdy = s.rdx = C.c_void_p(self.t["rdx"].data_ptr() + 4 * start)
@@ -792,8 +798,11 @@ class Sweep:
         st = L.stream_ptr(device)
         ready = {}
         # the widths-independent work of a block (shoebox gather, Kabsch frame, eps and J, neighbour masks) on the
-        # preparation stream: every block up front, or (opts "prep_ahead" = D) D blocks ahead of the chain, each after
-        # the previous block's pass 1, so that its output is still in the last-level cache when its pass 1 reads it
+        # preparation stream: every block ahead, or (opts "prep_ahead" = D) D blocks ahead of the chain, each after
+        # the previous block's pass 1, so that its output is still in the last-level cache when its pass 1 reads it.
+        # In the first form the host submits PREP_LEAD blocks' preparations, then one more after each block of the
+        # chain: the device still runs them as early as it can, and the chain's first launches are not held behind
+        # the host's submission of every block's preparation.
         ahead = o.get("prep_ahead")
 
         def prepare(b):
@@ -817,7 +826,7 @@ class Sweep:
         held = []
         if batched:
             self.xs.wait_stream(main)
-            for b in (self.order if ahead is None else self.order[:ahead + 1]):
+            for b in self.order[:(PREP_LEAD if ahead is None else ahead + 1)]:
                 prepare(b)
         for i, b in enumerate(self.order):
             if batched and ahead is not None and i + ahead + 1 < len(self.order):
@@ -851,6 +860,8 @@ class Sweep:
                                   defer=defer, host=host)
             if f is not None:
                 held.append(f)
+            if batched and ahead is None and i + PREP_LEAD < len(self.order):
+                prepare(self.order[i + PREP_LEAD])
         while held:
             held.pop()()
         if batched:
