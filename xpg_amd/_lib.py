@@ -1,5 +1,5 @@
 """ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_cbf.h,
-include/xpg_synth.h, include/xpg_forward.h).
+include/xpg_synth.h, include/xpg_forward.h, include/xpg_finalise.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
 is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
@@ -111,6 +111,21 @@ class FmWindow(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("f0", "nf", "y0", "ny", "x0", "nx")]
 
 
+class LpGeom(C.Structure):
+    _fields_ = [("e0", C.c_double * 3), ("axis", C.c_double * 3), ("pn", C.c_double * 3), ("pm", C.c_double * 3),
+                ("pfrac", C.c_double), ("ed3", C.c_double * 3), ("distance", C.c_double), ("tsens", C.c_double),
+                ("mu", C.c_double), ("mu_air", C.c_double), ("segmented", C.c_int32)]
+
+
+class FinalIn(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("S", "x", "y", "iprf", "vprf", "isum", "vsum", "p_valid", "p_total", "corr",
+                                          "coverage")]
+
+
+class FinalOut(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("rlp", "I", "sig", "Isum", "sigsum", "peak", "corr", "good")]
+
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -153,6 +168,9 @@ _SIGS = {
     "xpg_cbf_decode": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_decode_static": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_encode_host": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64], C.c_int64),
+    "xpg_finalise": ([P(LpGeom), P(Detector), C.c_int64, P(FinalIn), P(FinalOut), C.c_void_p], C.c_int),
+    "xpg_format_integrate_rows": ([C.c_void_p, C.c_int64, C.c_int64] + [C.c_void_p] * 12 + [C.c_double] * 2
+                                  + [C.c_void_p] * 2, C.c_int64),
     "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
                               C.c_int64),
 }

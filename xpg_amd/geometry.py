@@ -287,6 +287,43 @@ class Geometry:
     def resolution_of_pixel(self, x, y):
         return 1.0 / np.linalg.norm(self.pixel_to_s(x, y) - self.s0, axis=-1)
 
+    def panel_normal_at(self, x, y):
+        """Unit sensor normal (lab) of the segment containing raw pixel (x, y) (gaps: segment 0)."""
+        if not self.segmented:
+            return np.broadcast_to(self.ed3, np.shape(x) + (3,))
+        P0, Bx, By = self.panels()
+        un = np.cross(Bx / self.qx, By / self.qy)
+        return un[np.maximum(self.segment_of(x, y), 0)]
+
+
+# Photoelectric (sensors) and total (air) mass attenuation coefficients (cm^2/g) against energy (keV), with densities
+# (g/cm^3): the NIST XCOM values the reference tabulates (xpg/geometry.py:422-431), interpolated log-log.
+MU_TABLES = {
+    "SI": (2.33, ((5, 242.0), (6, 145.5), (8, 62.0), (10, 32.87), (15, 9.76), (20, 4.06), (30, 1.15), (40, 0.476),
+                  (50, 0.240), (60, 0.137), (80, 0.0566), (100, 0.0287))),
+    "CDTE": (5.85, ((8, 250.0), (10, 132.0), (15, 44.0), (20, 21.0), (25, 12.0), (26.70, 10.1), (26.72, 36.0),
+                    (31.80, 23.5), (31.82, 40.5), (40, 22.5), (50, 12.5), (60, 7.6), (80, 3.5), (100, 1.95))),
+    "GAAS": (5.32, ((8, 37.0), (10, 20.5), (11.0, 16.0), (11.1, 95.0), (11.8, 77.0), (11.9, 150.0), (15, 80.0),
+                    (20, 37.0), (30, 12.0), (40, 5.3), (60, 1.7), (100, 0.4))),
+    "AIR": (1.205e-3, ((5, 39.31), (6, 22.70), (8, 9.921), (10, 5.120), (15, 1.614), (20, 0.7779), (30, 0.3538),
+                       (40, 0.2485), (50, 0.2080), (60, 0.1875), (80, 0.1662), (100, 0.1541))),
+}
+HC_KEV_A = 12.398419843320026           # h c in keV Angstrom
+
+
+def mu_linear(wavelength, material="Si"):
+    """Linear attenuation coefficient (1/mm) of a material at a wavelength (Angstrom): log-log interpolation of the
+    mass attenuation table at E = hc / lambda, times the density (xpg/geometry.py:434-446); 0 for a material not
+    tabulated."""
+    key = str(material).upper().replace(" ", "")
+    key = "SI" if key == "SILICON" else key
+    if key not in MU_TABLES:
+        return 0.0
+    rho, tab = MU_TABLES[key]
+    e = np.log([t[0] for t in tab])
+    m = np.log([t[1] for t in tab])
+    return float(np.exp(np.interp(np.log(HC_KEV_A / wavelength), e, m))) * rho / 10.0
+
 
 @dataclass
 class Crystal:

@@ -26,6 +26,7 @@ from scipy.spatial import cKDTree
 
 from . import integrate_kernels as K
 from ._lib import XpgError
+from .finalise import FINAL_F64
 from .frames import DeviceFrames, from_imageset
 from .geometry import PIX0, Geometry, centring_allowed, detector_consts, generate_hkl, predict_reflections
 from .integrate_kernels import (FG_R, N_BOX, N_FG, PROFILE_GRID, RC_C, gaussian_profile, rc_basis_np,
@@ -298,7 +299,9 @@ class SweepPlan:
                       ix0=T(ix0[p], np.int32), iy0=T(iy0[p], np.int32), iz0=T(iz0[p], np.int32),
                       fb=T(fb[p], np.int32), pxs=T(pxs[p], np.int32), region=T(region[p], np.int32),
                       has_p=T(has_partner[p], np.float32), pS=T(refl["S"][pj[p]], np.float64),
-                      pphi=T(np.radians(refl["phi"][pj[p]]), np.float64))
+                      pphi=T(np.radians(refl["phi"][pj[p]]), np.float64), x=T(refl["x"][p], np.float64),
+                      y=T(refl["y"][p], np.float64),
+                      coverage=T(refl["coverage"][p] if "coverage" in refl else np.ones(p.size), np.float32))
         for k in ("x", "y", "z", "sx", "sz"):
             self.h["nb" + k] = T(nb[k][p], np.float32)
         # per-row shapes, and each row's voxel / pixel offset inside its bin's buffers
@@ -706,7 +709,7 @@ class Sweep:
             self.graph_out = self.enqueue(None, host)
         return self.graph_out
 
-    RESULT_F64 = OUT_F64
+    RESULT_F64 = OUT_F64 + FINAL_F64
     RESULT_I32 = ("n_over", "n_fg_masked")
     RESULT_KEYS = RESULT_F64 + RESULT_I32
 
@@ -764,6 +767,7 @@ class Sweep:
         """Row keys of the float64 and the int32 output block (alloc_outputs), results first."""
         i32 = list(self.RESULT_I32) + [k for k in P1_KEEP if k.startswith("n_") and k not in self.RESULT_I32]
         f64 = list(self.RESULT_F64) + [k for k in P1_KEEP + K.P2_F64 if k not in i32 and k not in self.RESULT_F64]
+        assert set(FINAL_F64) <= set(f64)
         return f64, i32
 
     def alloc_outputs(self):
@@ -776,7 +780,7 @@ class Sweep:
         I = torch.zeros((len(i32), N), dtype=torch.int32, device=device)
         rows = {k: F[i] for i, k in enumerate(f64)}
         rows.update({k: I[i] for i, k in enumerate(i32)})
-        return {k: rows[k] for k in P1_KEEP}, {k: rows[k] for k in K.P2_F64}, (F, I)
+        return {k: rows[k] for k in P1_KEEP}, {k: rows[k] for k in K.P2_F64 + FINAL_F64}, (F, I)
 
     def enqueue(self, timer=None, host=None):
         """Launches of the whole sweep on the current stream (and the form's side streams), with no synchronisation
@@ -998,6 +1002,7 @@ class Sweep:
                         L.check(lib.xpg_pass2_frames(C.byref(det.s), C.byref(call2), C.byref(view.s),
                                                      C.byref(self.store), L.ptr(prof), int(prof.shape[0]), bg, nbg,
                                                      0, C.byref(out), sst))
+            self._finalise_block(b, rb, fit[0], fit[1], sst)
             if host is not None:        # the block's rows (contiguous in the table) to the host, behind its pass 2
                 cs = self.copy_stream()
                 cs.wait_stream(side)
@@ -1007,6 +1012,34 @@ class Sweep:
             return pass2
         pass2()
         return None
+
+    def _finalise_block(self, b, rb, s0, n, stream):
+        """The block's rows [s0, s0 + n) (table order) corrected on the device behind its pass 2: Lorentz-polarisation,
+        absorption and the columns of xpg/stage_integrate.py:408-421 (xpg_finalise) into the FINAL_F64 rows."""
+        from . import _lib as L
+        from .finalise import lp_geom
+        g = self.exp_blocks[b].geom
+        key = id(g)
+        if not hasattr(self, "_lp"):
+            self._lp = {}
+        if key not in self._lp:
+            self._lp[key] = (g, lp_geom(g))
+        geom = self._lp[key][1]
+        t = self.sp.t
+        fin = L.FinalIn()
+        fin.S = C.c_void_p(t["S"].data_ptr() + 24 * s0)
+        fin.x, fin.y = C.c_void_p(t["x"].data_ptr() + 8 * s0), C.c_void_p(t["y"].data_ptr() + 8 * s0)
+        fin.coverage = C.c_void_p(t["coverage"].data_ptr() + 4 * s0)
+        for k in ("iprf", "vprf", "p_valid", "p_total", "corr"):
+            setattr(fin, k, C.c_void_p(rb.p2(k, s0)))
+        for k in ("isum", "vsum"):
+            setattr(fin, k, C.c_void_p(rb.p1(k, s0)))
+        fo = L.FinalOut()
+        for k in FINAL_F64:
+            setattr(fo, k, C.c_void_p(rb.p2(k, s0)))
+        det = self.det_for(g) if g.segmented else None
+        L.check(L.lib().xpg_finalise(C.byref(geom), C.byref(det.s) if det is not None else None, int(n),
+                                     C.byref(fin), C.byref(fo), stream))
 
     def arena(self):
         """Prepared-data buffers for one block at a time (the largest block's bins back to back): the single-stream
@@ -1088,7 +1121,7 @@ class Sweep:
         host = {k: F[i][inv] for i, k in enumerate(f64)}
         host.update({k: I[i][inv] for i, k in enumerate(i32)})
         refl = dict(hp["refl"])
-        for k in OUT_F64:
+        for k in OUT_F64 + FINAL_F64:
             refl[k] = host[k].astype(np.float64)
         refl["n_over"] = host["n_over"].astype(np.int32)
         refl["n_fg_masked"] = host["n_fg_masked"].astype(np.int32)
