@@ -566,8 +566,32 @@ def bench_forward(args, dev):
     in_vox = (((kz >= 0) & (kz < nf)).sum(1) * ((ky >= 0) & (ky < ny)).sum(1) * ((kx >= 0) & (kx < nx)).sum(1))
     n_in = int(in_vox.sum())
     g = torch.Generator(device=dev).manual_seed(11)
-    R = torch.rand((T, n, Dz), device=dev, generator=g)
-    P = torch.rand((T, n, b, b), device=dev, generator=g) / (b * b)
+    # the factors from the full-size geometry (xpg_fm_terms): each support centred on its reflection, learnt profiles
+    # of 9 regions, the 21-component rocking curve and the halo
+    from xpg_amd import synth
+    from xpg_amd.geometry import detector_consts
+    geo = synth.make_geometry(synth.spec_full())
+    xc, yc = (ix0 + b / 2 + 0.5).astype(float), (iy0 + b / 2 + 0.5).astype(float)
+    S_t = torch.as_tensor(geo.pixel_to_s(xc, yc), device=dev)
+    phi_t = torch.as_tensor(np.radians((iz0 + Dz / 2) * geo.osc), device=dev)
+    from xpg_amd.integrate_kernels import DeviceDetector, gaussian_profile, rc_default_weights
+    prof_t = torch.as_tensor(np.repeat(np.asarray(gaussian_profile(), np.float32)[None], 9, 0), device=dev)
+    reg_t = torch.as_tensor(np.clip((xc / nx * 3).astype(int), 0, 2) * 3 + np.clip((yc / ny * 3).astype(int), 0, 2),
+                            device=dev)
+    F_det = DeviceDetector(detector_consts(geo), dev)
+    org = [torch.as_tensor(a, dtype=torch.int32, device=dev) for a in (ix0, iy0, iz0)]
+    terms_args = (F_det, geo.axis, geo.s0, 0.0, np.radians(geo.osc), np.radians(0.02), np.radians(0.1), S_t, phi_t,
+                  *org, Dz, b)
+    terms_kw = dict(prof=prof_t, region=reg_t, umax=4.5, halo=(0.1, 3.0), rcw=rc_default_weights())
+    R, P = F.fraction_factors(*terms_args, **terms_kw)
+    torch.cuda.synchronize(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(5):
+        F.fraction_factors(*terms_args, **terms_kw)
+    e1.record()
+    torch.cuda.synchronize(dev)
+    ms_terms = e0.elapsed_time(e1) / 5
     I = torch.rand(n, device=dev, generator=g) * 1000
     beta = torch.rand(n, device=dev, generator=g)
     t_iz, t_iy, t_ix = (torch.as_tensor(a, dtype=torch.int32, device=dev) for a in (iz0, iy0, ix0))
@@ -626,6 +650,12 @@ def bench_forward(args, dev):
         tf = ROOT / "profiles" / f"traffic_{f}_forward.json"
         kern[k]["traffic"] = json.loads(tf.read_text()).get("bytes_per_launch") if tf.exists() else None
     dom = max(ms, key=ms.get)
+    # the factors: S, phi, origins in (40 B per reflection), R and P out, the profile grids (L2-resident)
+    alg_t = n * (40 + T * (Dz + b * b) * 4)
+    kern["k_fm_terms"] = dict(avg_launch_ms=round(ms_terms, 4), alg_bytes_per_launch=alg_t,
+                              achieved=round(alg_t / (ms_terms * 1e-3) / 1e9, 1),
+                              frac=round(alg_t / (ms_terms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                              note="per change of widths or geometry (the reference caches the factors between)")
     roof = dict(bound="hbm", kernel=dom, achieved=kern[dom]["achieved"], peak=PEAK_HBM_GBS, unit="GB/s",
                 frac=kern[dom]["frac"], traffic=kern[dom]["traffic"], kernels=kern)
     t_step = sum(ms.values())

@@ -43,6 +43,40 @@ int xpg_fm_gather(float *g, float *fd, const xpg_fm_supports *sup, const int64_t
                   const float *M, const void *NS, int32_t ns_type, int32_t mode, const xpg_fm_window *win,
                   void *stream);
 
+/* ---- the separable factors themselves (fraction_terms, xpg/forward/model.py:206-275) ------------------------------
+ * For each reflection r: the Kabsch frame of S, eps1 / eps2 at the centres of its b x b support pixels (super-pixels of
+ * pxs x pxs when pxs > 1), the pixel area J by second-order differences (b >= 3; else (qx pxs)^2), seff from sigma_D
+ * and the centre pixel's J, the rocking curve R over frames iz0 .. iz0 + Dz - 1 (the mixture with the given weights,
+ * else a Gaussian), and P = the learnt profile of the reflection's region sampled bilinearly at u = eps / seff times
+ * J / seff^2 (no profiles: the pixel-integrated Gaussian exp(-r^2 / 2 seff^2) J / (2 pi seff^2)).  With the halo the
+ * factors are ((1 - w) R, P) and (w R_h, P_h): R_h the Gaussian rocking curve of kappa sigma_M, P_h the Gaussian of
+ * width sqrt((kappa sigma_D)^2 + pix^2 / 12). */
+typedef struct xpg_fm_rows {
+  const double *S;                   /* (n, 3) diffracted beams, 1/A */
+  const double *phi;                 /* (n) predicted rotation angles, radians */
+  const int32_t *ix0, *iy0, *iz0;    /* (n) support origin: column, row (super-pixels when pxs > 1), frame */
+  const int32_t *region;             /* (n) profile region, or NULL without learnt profiles */
+  const double *sd, *sm;             /* (n) per-reflection widths (radians), or NULL: the model's scalars */
+  int32_t n, Dz, b, pxs;
+} xpg_fm_rows;
+
+typedef struct xpg_fm_model {
+  double axis[3], s0[3];
+  double frames_phi0, osc;           /* radians: phi at frame coordinate 0, per frame */
+  double sd, sm;                     /* radians */
+  const float *prof;                 /* (n_regions, G, G) learnt profiles (density in seff units), or NULL */
+  int32_t n_regions, G;
+  double umax;                       /* half-width of the profile grid, seff units */
+  int32_t halo;                      /* non-zero: the halo term, weight halo_w and broadening halo_kappa */
+  double halo_w, halo_kappa;
+  float rcw[21];                     /* rocking-curve mixture weights when use_rcw */
+  int32_t use_rcw;
+} xpg_fm_model;
+
+/* R (T, n, Dz) and P (T, n, b, b) float32 (T = 2 with the halo, else 1) of the reflections of `rows`. */
+int xpg_fm_terms(const struct xpg_detector *det, const xpg_fm_model *m, const xpg_fm_rows *rows, float *R, float *P,
+                 void *stream);
+
 #ifdef __cplusplus
 }
 #endif

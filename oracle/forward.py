@@ -94,3 +94,64 @@ def term_scale(R, P, rows, iz0, iy0, ix0, win, A, M, NS, mode, n_out):
     out = np.zeros(n_out)
     np.add.at(out, np.asarray(rows, np.int64), s)
     return out
+
+
+def fraction_terms(det, axis, s0, frames_phi0, osc, sd, sm, S, phi, ix0, iy0, iz0, Dz, Dx, prof=None, region=None,
+                   umax=None, halo=None, rcw=None, pxs=1):
+    """The separable factors [(R (n, Dz), P (n, Dx, Dx)), ...] of xpg/forward/model.py:206-275 in float64: Kabsch frame,
+    eps at the (super-)pixel centres, J by second-order differences (Dx >= 3; else (qx pxs)^2), seff from sigma_D and
+    the centre pixel, the rocking curve over frames iz0 + j, and P the learnt profile sampled bilinearly at
+    u = eps / seff times J / seff^2 (or the pixel-integrated Gaussian); with halo = (w, kappa) the factors
+    ((1 - w) R, P) and (w R_h, Gaussian of sqrt((kappa sd)^2 + pix^2 / 12)).  ``det`` as oracle.kernels.pixel_S takes
+    it; sd, sm scalars or (n,) arrays.  The geometry pieces are oracle.kernels' (pinned there)."""
+    from . import kernels as ok
+    S = np.asarray(S, np.float64)
+    n = S.shape[0]
+    Dy = Dx
+    rc = dict(ix0=np.asarray(ix0), iy0=np.asarray(iy0), pxs=np.full(n, float(pxs)))
+    x, y = ok.pixel_coords(rc, Dy, Dx)
+    s0 = np.asarray(s0, np.float64)
+    c1 = np.cross(S, np.broadcast_to(s0, S.shape))
+    e1 = c1 / np.linalg.norm(c1, axis=1, keepdims=True)
+    c2 = np.cross(S, e1)
+    e2 = c2 / np.linalg.norm(c2, axis=1, keepdims=True)
+    zeta = e1 @ np.asarray(axis, np.float64)
+    eps1, eps2 = ok.eps12(det, S, e1, e2, x, y)
+    if Dx >= 3:
+        g1x, g1y = ok.gradient2(eps1, 2), ok.gradient2(eps1, 1)
+        g2x, g2y = ok.gradient2(eps2, 2), ok.gradient2(eps2, 1)
+        J = np.abs(g1x * g2y - g2x * g1y)
+    else:
+        J = np.full_like(eps1, (float(det["qx"]) * pxs) ** 2)
+    k = np.asarray(iz0, np.float64)[:, None] + np.arange(Dz, dtype=np.float64)[None]
+    lo = frames_phi0 + osc * k
+    hi = lo + osc
+    lo, hi = np.minimum(lo, hi), np.maximum(lo, hi)
+    phb = np.broadcast_to(np.asarray(phi, np.float64)[:, None], lo.shape)
+    zb = np.broadcast_to(zeta[:, None], lo.shape)
+    sd = np.broadcast_to(np.asarray(sd, np.float64), (n,))
+    sm = np.broadcast_to(np.asarray(sm, np.float64), (n,))
+    R = ok.phi_rc(lo, hi, phb, zb, sm[:, None], rcw)
+    pix = np.sqrt(J[:, Dy // 2, Dx // 2])
+    seff = sd * np.sqrt(1.0 + (pix / sd) ** 2 / 12.0)
+    r2 = eps1 ** 2 + eps2 ** 2
+
+    def gauss2(se):
+        return np.exp(-0.5 * r2 / se[:, None, None] ** 2) / (2 * np.pi) * J / se[:, None, None] ** 2
+
+    if prof is None:
+        P = gauss2(seff)
+    else:
+        prof = np.asarray(prof, np.float64)
+        inside, i1, i2, t1, t2 = ok.bilinear(eps1 / seff[:, None, None], eps2 / seff[:, None, None], prof.shape[-1],
+                                            umax)
+        r = np.asarray(region)[:, None, None]
+        v = ((1 - t1) * (1 - t2) * prof[r, i2, i1] + t1 * (1 - t2) * prof[r, i2, i1 + 1]
+             + (1 - t1) * t2 * prof[r, i2 + 1, i1] + t1 * t2 * prof[r, i2 + 1, i1 + 1])
+        P = np.where(inside, v, 0.0) * J / seff[:, None, None] ** 2
+    if halo is None:
+        return [(R, P)]
+    w, kap = halo
+    sh = np.sqrt((kap * sd) ** 2 + pix ** 2 / 12.0)
+    Rh = ok.phi_rc(lo, hi, phb, zb, kap * sm[:, None])
+    return [((1 - w) * R, P), (w * Rh, gauss2(sh))]

@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 
 from oracle import forward as FO
+from oracle import kernels as ok
 
 
 def _case(seed, T, n=7, Dz=3, b=4):
@@ -99,3 +100,75 @@ def test_fraction_voxel_ids_orientation():
     assert f[0, 1, 2, 1] == 10.0 * 7.0
     mu = FO.scatter(np.zeros(30), R, P, [1.0], None, [2], [1], [0], win)
     assert mu[(1 * 3 + 2) * 5 + 1] == 70.0
+
+
+# ---- fraction_terms (the separable factors themselves) ---------------------------------------------------------------
+
+def _fm_geom(**kw):
+    from xpg_amd.geometry import Geometry
+    base = dict(wavelength=1.0, beam=np.array([0, 0, 1.0]), axis=np.array([1.0, 0, 0]), orgx=1024.5, orgy=1024.5,
+                distance=150, ed1=np.array([1.0, 0, 0]), ed2=np.array([0, 1.0, 0]), qx=0.075, qy=0.075, nx=2048,
+                ny=2048, phi0=0, osc=0.1)
+    base.update(kw)
+    return Geometry(**base)
+
+
+def _fm_rows(g, n, b, seed=0, pxs=1):
+    """n reflections on the detector with supports of b x b (super-)pixels centred on their predicted position."""
+    rng = np.random.default_rng(seed)
+    x, y = rng.uniform(300, 1700, n), rng.uniform(300, 1700, n)
+    S = g.pixel_to_s(x, y)
+    phi = np.radians(rng.uniform(2.0, 3.0, n))
+    z = np.degrees(phi) / g.osc
+    ix0 = np.floor(x / pxs - b / 2).astype(int)
+    iy0 = np.floor(y / pxs - b / 2).astype(int)
+    return S, phi, ix0, iy0, z
+
+
+def test_fraction_terms_gaussian_sums_to_one_and_rocking_curve_covers_the_range():
+    from xpg_amd.geometry import detector_consts
+    g = _fm_geom()
+    det = detector_consts(g)
+    n, b, Dz = 40, 31, 41
+    S, phi, ix0, iy0, z = _fm_rows(g, n, b)
+    iz0 = np.floor(z - Dz / 2).astype(int)
+    sd, sm = np.radians(0.03), np.radians(0.03)
+    for rcw in (None, ok.rc_default_weights()):
+        (R, P), = FO.fraction_terms(det, g.axis, g.s0, 0.0, np.radians(g.osc), sd, sm, S, phi, ix0, iy0, iz0, Dz, b,
+                                    rcw=rcw)
+        # the pixel-integrated Gaussian of a spot of ~3 pixels inside 31 pixels: sum 1 (midpoint rule)
+        np.testing.assert_allclose(P.sum((1, 2)), 1.0, atol=2e-3)
+        # the rocking curve over +-20 frames of 0.1 degree (sigma_M / |zeta| below 0.2 degree here): all of it
+        np.testing.assert_allclose(R.sum(1), 1.0, atol=1e-9)
+    # a box narrower than 3: J is the constant (qx pxs)^2
+    (_, P2), = FO.fraction_terms(det, g.axis, g.s0, 0.0, np.radians(g.osc), sd, sm, S, phi, ix0 + 14, iy0 + 14, iz0,
+                                 Dz, 2, pxs=2)
+    e = 2 * 0.075
+    assert np.all(P2 > 0) and np.isfinite(P2).all()
+    (_, P3), = FO.fraction_terms(det, g.axis, g.s0, 0.0, np.radians(g.osc), 1.0, sm, S, phi, ix0, iy0, iz0, Dz, 2)
+    np.testing.assert_allclose(P3.max(), e * e / 4 / (2 * np.pi * (1 + e * e / 4 / 12.0)), rtol=0.02)
+
+
+def test_fraction_terms_halo_and_learnt_profile_limits():
+    from xpg_amd.geometry import detector_consts
+    g = _fm_geom()
+    det = detector_consts(g)
+    n, b, Dz = 30, 15, 21
+    S, phi, ix0, iy0, z = _fm_rows(g, n, b, seed=2)
+    iz0 = np.floor(z - Dz / 2).astype(int)
+    sd, sm = np.radians(0.03), np.radians(0.08)
+    args = (det, g.axis, g.s0, 0.0, np.radians(g.osc), sd, sm, S, phi, ix0, iy0, iz0, Dz, b)
+    (R, P), = FO.fraction_terms(*args)
+    # kappa = 1 without mixture weights: the halo is the core itself, so the terms add up to the core's fraction
+    (R1, P1), (Rh, Ph) = FO.fraction_terms(*args, halo=(0.3, 1.0))
+    f = R1[:, :, None, None] * P1[:, None] + Rh[:, :, None, None] * Ph[:, None]
+    np.testing.assert_allclose(f, R[:, :, None, None] * P[:, None], rtol=1e-12, atol=1e-300)
+    # w = 0: the first term is the core
+    (R0, P0), (Rz, _) = FO.fraction_terms(*args, halo=(0.0, 2.0))
+    assert np.array_equal(R0, R) and np.array_equal(P0, P) and not Rz.any()
+    # a learnt profile that is the unit Gaussian on the grid reproduces the Gaussian factor to interpolation error
+    prof = np.repeat(ok.gaussian_profile()[None], 9, 0).astype(np.float32)
+    (Rp, Pp), = FO.fraction_terms(*args, prof=prof, region=np.arange(n) % 9, umax=ok.UMAX)
+    assert np.array_equal(Rp, R)
+    inner = P > 0.3 * P.max(axis=(1, 2), keepdims=True)
+    assert np.abs(Pp[inner] / P[inner] - 1).max() < 1.5e-2

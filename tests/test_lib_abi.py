@@ -9,7 +9,7 @@ STRUCTS = dict(Detector="xpg_detector", Call="xpg_call", Refl="xpg_refl", Pass1O
                Pass2Out="xpg_pass2_out", ProfileAcc="xpg_profile_acc", Frames="xpg_frames", Widths="xpg_widths",
                Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch", CbfBatch="xpg_cbf_batch",
                FmSupports="xpg_fm_supports", FmWindow="xpg_fm_window", LpGeom="xpg_lp_geom", FinalIn="xpg_final_in",
-               FinalOut="xpg_final_out")
+               FinalOut="xpg_final_out", FmRows="xpg_fm_rows", FmModel="xpg_fm_model")
 
 
 def _declared():

@@ -111,6 +111,19 @@ class FmWindow(C.Structure):
     _fields_ = [(k, C.c_int32) for k in ("f0", "nf", "y0", "ny", "x0", "nx")]
 
 
+class FmRows(C.Structure):
+    _fields_ = [("S", C.c_void_p), ("phi", C.c_void_p), ("ix0", C.c_void_p), ("iy0", C.c_void_p),
+                ("iz0", C.c_void_p), ("region", C.c_void_p), ("sd", C.c_void_p), ("sm", C.c_void_p),
+                ("n", C.c_int32), ("Dz", C.c_int32), ("b", C.c_int32), ("pxs", C.c_int32)]
+
+
+class FmModel(C.Structure):
+    _fields_ = [("axis", C.c_double * 3), ("s0", C.c_double * 3), ("frames_phi0", C.c_double), ("osc", C.c_double),
+                ("sd", C.c_double), ("sm", C.c_double), ("prof", C.c_void_p), ("n_regions", C.c_int32),
+                ("G", C.c_int32), ("umax", C.c_double), ("halo", C.c_int32), ("halo_w", C.c_double),
+                ("halo_kappa", C.c_double), ("rcw", C.c_float * RC_K), ("use_rcw", C.c_int32)]
+
+
 class LpGeom(C.Structure):
     _fields_ = [("e0", C.c_double * 3), ("axis", C.c_double * 3), ("pn", C.c_double * 3), ("pm", C.c_double * 3),
                 ("pfrac", C.c_double), ("ed3", C.c_double * 3), ("distance", C.c_double), ("tsens", C.c_double),
@@ -168,6 +181,7 @@ _SIGS = {
     "xpg_cbf_decode": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_decode_static": ([P(CbfBatch), C.c_void_p], C.c_int),
     "xpg_cbf_encode_host": ([C.c_void_p, C.c_int64, C.c_void_p, C.c_int64], C.c_int64),
+    "xpg_fm_terms": ([P(Detector), P(FmModel), P(FmRows), C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "xpg_finalise": ([P(LpGeom), P(Detector), C.c_int64, P(FinalIn), P(FinalOut), C.c_void_p], C.c_int),
     "xpg_format_integrate_rows": ([C.c_void_p, C.c_int64, C.c_int64] + [C.c_void_p] * 12 + [C.c_double] * 2
                                   + [C.c_void_p] * 2, C.c_int64),

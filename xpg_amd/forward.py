@@ -6,6 +6,9 @@ include/xpg_forward.h.
         mode 0: g[rows] += sum f A / M; mode 1: g[rows] += sum f (1 - NS / M), fd[rows] += sum f^2 / M (voxels with
         NS >= 0, inside the window); device rows outside g add nothing, host rows outside g raise
 
+    fraction_terms(det, axis, s0, frames_phi0, osc, sd, sm, S, phi, ix0, iy0, iz0, Dz, Dx, prof, region, umax, halo,
+                   rcw, pxs)                                   the separable factors [(R, P), ...] (xpg_fm_terms)
+
 R: (T, n, Dz), P: (T, n, b, b) float32 (T = 1, or 2 with the halo); iz0 / iy0 / ix0 the supports' origins; ``win``
 any object with f0, f1, y0, ny, x0, nx (xpg/forward/model.py's Window).  Asynchronous on the current stream; there is
 no CPU fallback (``available`` is False for the CPU, and it and the calls raise on a GPU without the library).
@@ -14,6 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -113,3 +117,73 @@ def gather(g, fd, R, P, rows, iz0, iy0, ix0, win, A, M, NS, mode):
                                   L.ptr(A) if mode == 0 else None, L.ptr(M), L.ptr(NS), NS_TYPES[NS.dtype], mode,
                                   C.byref(w), L.stream_ptr(g.device)))
     return g, fd
+
+
+def fraction_factors(det, axis, s0, frames_phi0, osc, sd, sm, S, phi, ix0, iy0, iz0, Dz, Dx, prof=None, region=None,
+                     umax=None, halo=None, rcw=None, pxs=1, device=None):
+    """The factors of fraction_terms stacked as the fused passes take them: R (T, n, Dz), P (T, n, Dx, Dx) float32
+    device tensors (xpg_fm_terms, include/xpg_forward.h).  ``det``: the integration kernels' DeviceDetector or the
+    reference's detector-constants dict (torch tensors or arrays); sd, sm scalars, 0-d or (n,) tensors / arrays
+    (radians); vectors may be device tensors."""
+    from .integrate_kernels import DeviceDetector, _dev, _host64
+    device = torch.device(device) if device is not None else (S.device if torch.is_tensor(S) else torch.device("cuda"))
+    if device.type != "cuda":
+        raise L.XpgError("fraction_terms runs on the GPU")
+    d = det if isinstance(det, DeviceDetector) else DeviceDetector(det, device)
+    n = int(S.shape[0])
+    Dz, b, pxs = int(Dz), int(Dx), int(pxs)
+    m = L.FmModel()
+    m.axis[:] = [float(v) for v in _host64(axis).reshape(3)]
+    m.s0[:] = [float(v) for v in _host64(s0).reshape(3)]
+    m.frames_phi0, m.osc = float(frames_phi0), float(osc)
+    keep = {}
+    rows = L.FmRows()
+    for k, v in (("sd", sd), ("sm", sm)):
+        a = _host64(v) if not torch.is_tensor(v) else v
+        if (torch.is_tensor(a) and a.numel() > 1) or (not torch.is_tensor(a) and np.size(a) > 1):
+            keep[k] = _dev(a, torch.float64, device).reshape(-1)
+            if keep[k].numel() != n:
+                raise L.XpgError(f"{k}: a scalar or one value per reflection")
+            setattr(rows, k, L.ptr(keep[k]))
+            setattr(m, k, 0.0)
+        else:
+            setattr(m, k, float(np.asarray(_host64(a)).reshape(-1)[0]))
+    keep["S"] = _dev(S, torch.float64, device).reshape(n, 3)
+    keep["phi"] = _dev(phi, torch.float64, device).reshape(n)
+    for k, v in (("ix0", ix0), ("iy0", iy0), ("iz0", iz0)):
+        keep[k] = _dev(v, torch.int32, device).reshape(n)
+    rows.S, rows.phi = L.ptr(keep["S"]), L.ptr(keep["phi"])
+    rows.ix0, rows.iy0, rows.iz0 = L.ptr(keep["ix0"]), L.ptr(keep["iy0"]), L.ptr(keep["iz0"])
+    rows.n, rows.Dz, rows.b, rows.pxs = n, Dz, b, pxs
+    if prof is not None:
+        keep["prof"] = _dev(prof, torch.float32, device)
+        G = int(keep["prof"].shape[-1])
+        keep["prof"] = keep["prof"].reshape(-1, G, G).contiguous()
+        if region is None:
+            raise L.XpgError("learnt profiles need each reflection's region")
+        keep["region"] = _dev(region, torch.int32, device).reshape(n)
+        m.prof, m.n_regions, m.G = L.ptr(keep["prof"]), int(keep["prof"].shape[0]), G
+        m.umax = float(umax) if umax is not None else 4.5
+        rows.region = L.ptr(keep["region"])
+    if halo is not None:
+        m.halo, m.halo_w, m.halo_kappa = 1, float(halo[0]), float(halo[1])
+    if rcw is not None:
+        m.rcw[:] = [float(v) for v in _host64(rcw).astype(np.float32).reshape(-1)]
+        m.use_rcw = 1
+    T = 2 if halo is not None else 1
+    R = torch.empty((T, n, Dz), dtype=torch.float32, device=device)
+    P = torch.empty((T, n, b, b), dtype=torch.float32, device=device)
+    with torch.cuda.device(device):
+        L.check(L.lib().xpg_fm_terms(C.byref(d.s), C.byref(m), C.byref(rows), L.ptr(R), L.ptr(P),
+                                     L.stream_ptr(device)))
+    return R, P
+
+
+def fraction_terms(det, axis, s0, frames_phi0, osc, sd, sm, S, phi, ix0, iy0, iz0, Dz, Dx, prof=None, region=None,
+                   umax=None, halo=None, rcw=None, pxs=1):
+    """fraction_terms of xpg/forward/model.py:206-275 on the device: the separable factors [(R (n, Dz),
+    P (n, Dx, Dx)), ...] of each reflection's fraction in its support (one term; two with the halo), float32 device
+    tensors, from the float64 device geometry of the integration kernels (xpg_fm_terms)."""
+    R, P = fraction_factors(det, axis, s0, frames_phi0, osc, sd, sm, S, phi, ix0, iy0, iz0, Dz, Dx, prof, region, umax,
+                            halo, rcw, pxs)
+    return [(R[t], P[t]) for t in range(R.shape[0])]
