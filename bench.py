@@ -823,7 +823,7 @@ def main():
         return
     if args.summarize_sq:
         print(json.dumps(summarize_sq(*args.summarize_sq, kernels=("k_prepare", "k_rocking", "k_pass1", "k_pass2_small",
-                                                                    "k_pass2", "k_normalise", "k_extract"))))
+                                                                    "k_pass2", "k_normalise", "k_extract", "k_finalise"))))
         return
     if args.summarize_pmc:
         f, w, mib, kern, wl, cmd = args.summarize_pmc
