@@ -900,6 +900,7 @@ def main():
         f"over {len(edges) - 1} blocks (host plan {t_host_plan:.1f}s, device setup {t_setup:.1f}s)")
 
     events = []
+    iso_events = []
 
     class Timer:
         """HIP events around one launch, recorded on the stream the launch goes to."""
@@ -973,6 +974,20 @@ def main():
             if not torch.equal(torch.nan_to_num(other["o2"]["iprf"], nan=-1.0), torch.nan_to_num(b_, nan=-1.0)):
                 graph_info["identical_results"] = False
         graph_info.setdefault("identical_results", True)
+        # one more eager run, outside the timed region, with the device synchronised around every launch: each
+        # kernel's duration alone (the timed sweep's overlapping streams stretch every launch by the kernels beside it)
+        class IsoTimer(Timer):
+            def __enter__(self):
+                torch.cuda.synchronize(dev)
+                super().__enter__()
+
+            def __exit__(self, *a):
+                self.e1.record(self.st)
+                torch.cuda.synchronize(dev)
+                iso_events.append((self.kind, self.gi, self.e0, self.e1))
+                return False
+        sweep.enqueue(IsoTimer, hostbuf)
+        torch.cuda.synchronize(dev)
     ms = t_step * 1e3
     value = integrated / t_step
 
@@ -997,15 +1012,18 @@ def main():
 
     # dominant kernel and its roofline, over every kernel of the sweep (HIP events on the launch stream, algorithmic
     # bytes per launch)
-    per = {}
+    def per_kind(evs):
+        per = {}
+        for kind, (start, m), e0, e1 in evs:
+            Bs = sweep.sp.rows_dz[start:start + m]
+            bxs = sweep.sp.rows_bx[start:start + m]
+            d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
+            d["ms"] += e0.elapsed_time(e1)
+            d["bytes"] += float(np.sum(per_refl_bytes(Bs * bxs * bxs, Bs, kind)))
+            d["n"] += 1
+        return per
+    per, iso = per_kind(events), per_kind(iso_events)
     n_timed = 1
-    for kind, (start, m), e0, e1 in events:
-        Bs = sweep.sp.rows_dz[start:start + m]
-        bxs = sweep.sp.rows_bx[start:start + m]
-        d = per.setdefault(kind, dict(ms=0.0, bytes=0.0, n=0))
-        d["ms"] += e0.elapsed_time(e1)
-        d["bytes"] += float(np.sum(per_refl_bytes(Bs * bxs * bxs, Bs, kind)))
-        d["n"] += 1
     dom = max(per, key=lambda k: per[k]["ms"])
     dd = per[dom]
     avg_ms = dd["ms"] / dd["n"]
@@ -1020,6 +1038,14 @@ def main():
                 alg_bytes_per_launch=round(avg_bytes), launches_per_step=dd["n"] // n_timed,
                 kernels={k: dict(ms_per_step=round(v["ms"] / n_timed, 3),
                                  GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1)) for k, v in per.items()})
+    if iso:
+        # the same kernels timed alone (IsoTimer): what each achieves without the kernels that share the device in the
+        # sweep; frac stays the in-sweep figure
+        for k, v in iso.items():
+            roof["kernels"].setdefault(k, {}).update(isolated_ms_per_step=round(v["ms"], 3),
+                                                     isolated_GBps=round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1))
+        if dom in iso:
+            roof["frac_isolated"] = round(iso[dom]["bytes"] / (iso[dom]["ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
 
     # the whole step against the HBM roofline: B_alg (every frame pixel once + two passes per shoebox) and the bound of
     # a design that reads only shoebox voxels, B_min (their exact union once + two passes)
