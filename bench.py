@@ -694,6 +694,173 @@ def bench_forward(args, dev):
     print(json.dumps(outj), flush=True)
 
 
+def refine_sweep(n_blocks=36, frames_per_block=50, every=6, seed=11):
+    """The block stage's input on the full-size sweep's geometry: spots of a crystal turning 0.02 deg per 5-deg block
+    (slippage) about a fixed axis, every `every`-th predicted reflection to the detector edge (the strong spots a spot
+    finder reports), 0.1 px / 0.05 frame noise, 2% displaced by 5-20 px; the starting model is the global one, its
+    detector origin 0.3 / -0.2 px and distance 0.05 mm off.  Returns (start, spots, data_range, delphi)."""
+    from dataclasses import replace
+
+    import numpy as np
+    from scipy.spatial.transform import Rotation
+    from xpg_amd import synth
+    from xpg_amd.geometry import Experiment, predict_reflections
+    from xpg_amd.models import Crystal, b_matrix
+
+    spec = synth.spec_full()
+    g = synth.make_geometry(spec)
+    c0 = Crystal.from_Astar(synth.make_experiment(spec).crystal.Astar, "o", space_group=spec.space_group)
+    dmin = synth.edge_resolution(g)
+    ax = np.array([0.3, -0.5, 0.8]) / np.linalg.norm([0.3, -0.5, 0.8])
+    rng = np.random.default_rng(seed)
+    xs, ys, zs = [], [], []
+    for b in range(n_blocks):
+        U = Rotation.from_rotvec(np.radians(0.02 * (b + 0.5)) * ax).as_matrix() @ c0.U
+        z0, z1 = b * frames_per_block, (b + 1) * frames_per_block
+        pr = predict_reflections(g, U @ b_matrix(c0.cell), dmin, z_range=(z0, z1))
+        k = np.arange(pr["x"].size)[(b % every)::every]
+        n = k.size
+        x, y = pr["x"][k] + rng.normal(scale=0.1, size=n), pr["y"][k] + rng.normal(scale=0.1, size=n)
+        out = rng.random(n) < 0.02
+        ang = rng.uniform(0, 2 * np.pi, n)
+        d = rng.uniform(5, 20, n)
+        x, y = np.where(out, x + d * np.cos(ang), x), np.where(out, y + d * np.sin(ang), y)
+        xs.append(x)
+        ys.append(y)
+        zs.append(np.clip(pr["z"][k] + rng.normal(scale=0.05, size=n), z0, z1 - 1e-6))
+    start = Experiment(replace(g, orgx=g.orgx + 0.3, orgy=g.orgy - 0.2, distance=g.distance + 0.05), c0)
+    return start, (np.concatenate(xs), np.concatenate(ys), np.concatenate(zs)), \
+        (1, n_blocks * frames_per_block), frames_per_block * spec.osc
+
+
+def bench_refine(args, dev):
+    """Per-phi-block geometry before integration (xpg/stage_integrate.py:28-212 with xpg/refine.py:26-443) on the
+    full-size sweep's geometry: 36 blocks of 5 deg (1800 frames of 0.1 deg), ~4k spots per block (refine_sweep).  A step
+    is the whole stage: the seed block, the tracking outwards with the weak position prior (three tolerances, up to 15
+    Levenberg-Marquardt iterations and 2 rejection cycles each), the empirical-Bayes prior, the restrained pass over all
+    blocks, then the smoothing of the orientation trajectory; flags POSITION, BEAM, ORIENTATION (8 parameters).
+    Roofline: k_rf_residuals at the restrained pass's shape (every block's parameter vector and its 8 perturbations in
+    one launch): algorithmic bytes = the spots read once (60 B each) + 24 B of residuals written per (spot, vector) +
+    the model rows, over its HIP-event time; the stage itself is bound by the host's round trips (the device share is
+    reported beside the wall time)."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+    from xpg_amd import _lib
+    from xpg_amd import blocks as xb
+    from xpg_amd import refine as xr
+
+    torch.cuda.set_device(dev)
+    nb = max(4, min(36, args.frames // 50)) if args.frames else 36
+    start, spots, drange, delphi = refine_sweep(n_blocks=nb)
+    n_spots = spots[0].size
+    flags = ["POSITION", "BEAM", "ORIENTATION"]
+    log(f"refine workload: {nb} blocks, {n_spots} spots")
+
+    def step():
+        state = {}
+        blocks, edges = xb.block_models(start, spots, drange, delphi, flags, state=state, device=dev)
+        return xb.smooth_blocks(blocks, start), state
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    xr.TIMING = []
+    t = time.time()
+    for _ in range(args.steps):
+        out, state = step()
+    torch.cuda.synchronize(dev)
+    wall = (time.time() - t) / args.steps
+    rounds = xr.TIMING
+    xr.TIMING = None
+    dev_ms = sum(a.elapsed_time(b) for a, b, _ in rounds) / args.steps
+    launches = sum(n for _, _, n in rounds) / args.steps
+    log(f"refine: {wall * 1e3:.1f} ms per stage, {len(rounds) / args.steps:.0f} device rounds, {launches:.0f} launches, "
+        f"device {dev_ms:.1f} ms; seed block {state['seed']}, {state['n_fallback']} fallbacks, prior sd "
+        f"{state.get('pos_prior_sd')}")
+    # the residual kernel alone at the restrained pass's shape: every block's vector and its 8 perturbations
+    edges = xb.block_edges(start.geom, drange, delphi)
+    par = xr.Parameterisation(start, flags)
+    reqs = []
+    for i in range(nb):
+        m = (spots[2] >= edges[i]) & (spots[2] < edges[i + 1])
+        p0 = start.geom.spot_to_p0(spots[0][m], spots[1][m], spots[2][m])
+        hkl, ok = xr.assign_indices(start.crystal.Astar, p0, tol=0.3)
+        sp, su_, _ = xr._spot_data(start, hkl[ok], spots[0][m][ok], spots[1][m][ok], spots[2][m][ok], 0.5, 0.05)
+        reqs.append(xr._Normal(xr._Problem(par, sp, su_), par.x0, np.ones(int(ok.sum()))))
+    pool = xr._Pool(dev)
+    for rq in reqs:
+        rq.prob.spot0 = pool.add(rq.prob.spots)
+    M = par.x0.size + 1
+    jobs = np.zeros(nb, xr._JOB)
+    t0 = r0 = 0
+    for j, rq in enumerate(reqs):
+        jobs[j] = (rq.prob.spot0, j * M, r0, t0, rq.prob.n, M)
+        t0, r0 = t0 + M * rq.prob.n, r0 + 3 * M * rq.prob.n
+    steps_ = np.concatenate([xr.steps_of(rq) for rq in reqs])
+    X = np.vstack([par.x0[None], par.x0 + np.diag(xr.steps_of(reqs[0]))])
+    models = torch.as_tensor(np.concatenate([par.models(X)] * nb), device=dev)
+    jobs_d = torch.from_numpy(jobs.view(np.uint8)).to(dev)
+    r_d = torch.empty(r0, dtype=torch.float64, device=dev)
+    w_d = torch.ones(pool.n, dtype=torch.float64, device=dev)
+    h_d = torch.as_tensor(steps_, device=dev)
+    o_d = torch.empty(nb * (M * M - M), dtype=torch.float64, device=dev)
+    st, su = pool.spots(), xr._setup(reqs[0].prob, M)
+    h = _lib.lib()
+    stream = _lib.stream_ptr(dev)
+    ms = {}
+    for name, call in (("k_rf_residuals", lambda: h.xpg_rf_residuals(C.byref(st), C.byref(su), _lib.ptr(jobs_d), nb,
+                                                                     t0, _lib.ptr(models), _lib.ptr(r_d), stream)),
+                       ("k_rf_normal", lambda: h.xpg_rf_normal(C.byref(st), C.byref(su), _lib.ptr(jobs_d), nb,
+                                                               _lib.ptr(r_d), _lib.ptr(h_d), _lib.ptr(w_d),
+                                                               _lib.ptr(o_d), stream))):
+        _lib.check(call())
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            _lib.check(call())
+        e1.record()
+        torch.cuda.synchronize(dev)
+        ms[name] = e0.elapsed_time(e1) / 20
+    n_res = pool.n
+    alg = {"k_rf_residuals": n_res * 60 + t0 * 24 + models.numel() * 8,
+           "k_rf_normal": 3 * n_res * M * 8 + n_res * 8 + nb * (M * M - M) * 8}
+    kern = {k: dict(avg_launch_ms=round(ms[k], 4), alg_bytes_per_launch=int(alg[k]),
+                    achieved=round(alg[k] / (ms[k] * 1e-3) / 1e9, 1),
+                    frac=round(alg[k] / (ms[k] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)) for k in ms}
+    kern["k_rf_residuals"]["evaluations"] = int(t0)
+    kern["k_rf_residuals"]["Gevals_per_s"] = round(t0 / (ms["k_rf_residuals"] * 1e-3) / 1e9, 2)
+    roof = dict(bound="hbm", kernel="k_rf_residuals", achieved=kern["k_rf_residuals"]["achieved"], peak=PEAK_HBM_GBS,
+                unit="GB/s", frac=kern["k_rf_residuals"]["frac"], traffic=None, kernels=kern,
+                stage=dict(wall_ms=round(wall * 1e3, 2), device_ms=round(dev_ms, 2), rounds=len(rounds) // args.steps,
+                           launches=int(launches),
+                           note="each round trip carries every waiting problem's requests; the host's p x p solves, "
+                                "index assignment and waits bound the stage"))
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (numpy float64, one core) on the first 4 blocks of the same sweep: a whole stage
+        from oracle import refine as orf
+        n4 = 4
+        sel = spots[2] < n4 * 50
+        t = time.time()
+        ob, _ = orf.block_models(start, tuple(v[sel] for v in spots), (1, n4 * 50), delphi, flags)
+        orf.smooth_blocks(ob, start)
+        dt = time.time() - t
+        cpu = dict(value=round(n4 / dt, 3), unit="blocks/s", cores=1, kind="port",
+                   sample=f"the whole block stage on the first {n4} blocks ({int(sel.sum())} spots), oracle/refine.py "
+                          f"(numpy float64, one core), {dt:.1f} s")
+        log(f"cpu baseline {cpu}")
+    outj = dict(metric="per-block geometry refinement: phi blocks/s on 1 MI355X", value=round(nb / wall, 3),
+                unit="blocks/s", n_gpus=1, steps=args.steps, warmup=args.warmup, ms_per_step=round(wall * 1e3, 2),
+                higher_is_better=True, scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload=f"refine_blocks_4148x4362x{nb * 50}", blocks=nb, spots=int(n_spots),
+                            flags=flags, parameters=int(par.x0.size), seed_block=int(state["seed"]),
+                            fallbacks=int(state["n_fallback"]), parallelism="single GPU"),
+                roofline=roof, cpu_baseline=cpu)
+    print(json.dumps(outj), flush=True)
+
+
 def bench_cbf(args, dev):
     """CBF byte-offset decode of PILATUS 6M frames (2463 x 2527 int32: Poisson background, module gaps at -1, bad
     pixels at -2, hot pixels up to the count cut-off), encoded as read_cbf reads them (encode_byte_offset); the streams
@@ -784,7 +951,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="full", choices=["full", "small", "uneven", "stress", "serve", "decode", "cbf",
-                                                                 "forward"])
+                                                                 "forward", "refine"])
     ap.add_argument("--serve-datasets", type=int, default=32)
     ap.add_argument("--serve-frame-scale", type=float, default=0.1,
                     help="serving workload: frame counts U{50..3600} times this factor (at least 20 frames)")
@@ -859,6 +1026,9 @@ def main():
         return
     if args.workload == "forward":
         bench_forward(args, dev)
+        return
+    if args.workload == "refine":
+        bench_refine(args, dev)
         return
     spec = dict(full=synth.spec_full, small=synth.spec_small, uneven=synth.spec_uneven,
                 stress=synth.spec_stress)[args.workload]()

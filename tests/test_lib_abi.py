@@ -9,7 +9,8 @@ STRUCTS = dict(Detector="xpg_detector", Call="xpg_call", Refl="xpg_refl", Pass1O
                Pass2Out="xpg_pass2_out", ProfileAcc="xpg_profile_acc", Frames="xpg_frames", Widths="xpg_widths",
                Prepared="xpg_prepared", Bslz4Batch="xpg_bslz4_batch", CbfBatch="xpg_cbf_batch",
                FmSupports="xpg_fm_supports", FmWindow="xpg_fm_window", LpGeom="xpg_lp_geom", FinalIn="xpg_final_in",
-               FinalOut="xpg_final_out", FmRows="xpg_fm_rows", FmModel="xpg_fm_model")
+               FinalOut="xpg_final_out", FmRows="xpg_fm_rows", FmModel="xpg_fm_model", RfSpots="xpg_rf_spots",
+               RfSetup="xpg_rf_setup", RfJob="xpg_rf_job")
 
 
 def _declared():
@@ -69,7 +70,7 @@ def test_ctypes_structs_match_c_layout(tmp_path):
     import ctypes
     from xpg_amd import _lib
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "xpg_integrate.h"', '#include "xpg_decode.h"', '#include "xpg_cbf.h"',
-             '#include "xpg_forward.h"', '#include "xpg_finalise.h"', "int main(void) {"]
+             '#include "xpg_forward.h"', '#include "xpg_finalise.h"', '#include "xpg_refine.h"', "int main(void) {"]
     for py, c in STRUCTS.items():
         lines.append(f'printf("{py} size %zu\\n", sizeof({c}));')
         for f, _ in getattr(_lib, py)._fields_:

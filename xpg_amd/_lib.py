@@ -1,5 +1,5 @@
 """ctypes binding of libxpg_integrate.so (include/xpg_integrate.h, include/xpg_decode.h, include/xpg_cbf.h,
-include/xpg_synth.h, include/xpg_forward.h, include/xpg_finalise.h).
+include/xpg_synth.h, include/xpg_forward.h, include/xpg_finalise.h, include/xpg_refine.h).
 
 The library is built in-tree by ``__graft_entry__.build()`` (hipcc, gfx950).  There is no fallback: when the library
 is missing or a HIP call fails, the calls raise.  Device memory and streams come from torch tensors; the structs
@@ -139,6 +139,22 @@ class FinalOut(C.Structure):
     _fields_ = [(k, C.c_void_p) for k in ("rlp", "I", "sig", "Isum", "sigsum", "peak", "corr", "good")]
 
 
+class RfSpots(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in ("h", "phi", "branch", "xo", "yo", "seg")] + [("n", C.c_int64)]
+
+
+class RfSetup(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("qx", "qy", "sxy", "sphi", "tsens", "mu", "plx")] + \
+        [(k, C.c_int32) for k in ("n_seg", "n_models")]
+
+
+class RfJob(C.Structure):
+    _fields_ = [(k, C.c_int64) for k in ("spot0", "model0", "r0", "t0")] + [("n_spots", C.c_int32),
+                                                                           ("n_models", C.c_int32)]
+
+
+RF_HEAD, RF_SEG = 17, 9
+
 P = C.POINTER
 _SIGS = {
     "xpg_version": ([], C.c_char_p),
@@ -185,6 +201,10 @@ _SIGS = {
     "xpg_finalise": ([P(LpGeom), P(Detector), C.c_int64, P(FinalIn), P(FinalOut), C.c_void_p], C.c_int),
     "xpg_format_integrate_rows": ([C.c_void_p, C.c_int64, C.c_int64] + [C.c_void_p] * 12 + [C.c_double] * 2
                                   + [C.c_void_p] * 2, C.c_int64),
+    "xpg_rf_residuals": ([P(RfSpots), P(RfSetup), C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
+                          C.c_void_p], C.c_int),
+    "xpg_rf_normal": ([P(RfSpots), P(RfSetup), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                       C.c_void_p], C.c_int),
     "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
                               C.c_int64),
 }

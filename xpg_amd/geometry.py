@@ -65,6 +65,7 @@ class Segment:
     distance: float = 0.0
     ex: np.ndarray = field(default_factory=lambda: np.array([1.0, 0.0, 0.0]))
     ey: np.ndarray = field(default_factory=lambda: np.array([0.0, 1.0, 0.0]))
+    refine: tuple = ()          # REFINE_SEGMENT= (POSITION / ORIENTATION); empty: both
 
     def axes(self, qx, qy):
         """(offset c, unit ex, unit ey) in the detector system: a raw pixel (x, y) is at qx x ex + qy y ey + c."""
@@ -77,6 +78,18 @@ class Segment:
     @property
     def rect(self):
         return (self.x1, self.x2, self.y1, self.y2)
+
+    @property
+    def centre(self):
+        return 0.5 * (self.x1 + self.x2), 0.5 * (self.y1 + self.y2)
+
+    def with_axes(self, c, ex, ey, qx, qy):
+        """This segment (same pixels) placed at detector-system offset c with axes ex, ey (xpg/detector.py:57-63)."""
+        ex = unit(ex)
+        ey = unit(ey - np.dot(ey, ex) * ex)
+        ez = np.cross(ex, ey)
+        return Segment(self.x1, self.x2, self.y1, self.y2, orgx=float(-(c @ ex) / qx), orgy=float(-(c @ ey) / qy),
+                       distance=float(c @ ez), ex=ex, ey=ey, refine=tuple(self.refine))
 
 
 def bilinear_table(tab, x, y):
@@ -223,6 +236,12 @@ class Geometry:
                 d1, d2 = self._parallax(X / np.linalg.norm(X, axis=-1, keepdims=True), u1, u2, un)
                 X = lab(xc - d1 / self.qx, yc - d2 / self.qy)
         return X
+
+    def spot_to_p0(self, x, y, z):
+        """Reciprocal-lattice vector at phi = 0 of an observed spot (x, y, z) (xpg/geometry.py:313-319)."""
+        p = self.pixel_to_s(x, y) - self.s0
+        R = rotation_matrices(self.axis, -np.radians(self.z_to_phi(z)))
+        return np.einsum("nij,nj->ni", R, p)
 
     def pixel_to_s(self, x, y):
         X = self.pixel_to_lab(x, y)
