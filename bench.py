@@ -790,13 +790,12 @@ def bench_refine(args, dev):
         sp, su_, _ = xr._spot_data(start, hkl[ok], spots[0][m][ok], spots[1][m][ok], spots[2][m][ok], 0.5, 0.05)
         reqs.append(xr._Normal(xr._Problem(par, sp, su_), par.x0, np.ones(int(ok.sum()))))
     pool = xr._Pool(dev)
-    for rq in reqs:
-        rq.prob.spot0 = pool.add(rq.prob.spots)
     M = par.x0.size + 1
     jobs = np.zeros(nb, xr._JOB)
-    t0 = r0 = 0
+    t0 = r0 = w0 = 0
     for j, rq in enumerate(reqs):
-        jobs[j] = (rq.prob.spot0, j * M, r0, t0, rq.prob.n, M)
+        jobs[j] = (pool.start_of(rq.prob), j * M, r0, t0, w0, rq.prob.n, M)
+        w0 += rq.prob.n
         t0, r0 = t0 + M * rq.prob.n, r0 + 3 * M * rq.prob.n
     steps_ = np.concatenate([xr.steps_of(rq) for rq in reqs])
     X = np.vstack([par.x0[None], par.x0 + np.diag(xr.steps_of(reqs[0]))])
@@ -814,7 +813,7 @@ def bench_refine(args, dev):
                                                                      t0, _lib.ptr(models), _lib.ptr(r_d), stream)),
                        ("k_rf_normal", lambda: h.xpg_rf_normal(C.byref(st), C.byref(su), _lib.ptr(jobs_d), nb,
                                                                _lib.ptr(r_d), _lib.ptr(h_d), _lib.ptr(w_d),
-                                                               _lib.ptr(o_d), stream))):
+                                                               _lib.ptr(o_d), None, stream))):
         _lib.check(call())
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()

@@ -39,18 +39,21 @@ typedef struct xpg_rf_spots {
 } xpg_rf_spots;
 
 /* Scalars: sigma_xy (pixels) and sigma_phi (radians) normalise the residuals; tsens, mu and plx the thick-sensor
- * parallax (plx = 0 or 1, as xpg/refine.py:172-178 multiplies by it); n_models is read by xpg_rf_normal only (every
- * job of a normal-equations launch has that many models, p = n_models - 1 parameters). */
+ * parallax (plx = 0 or 1, as xpg/refine.py:172-178 multiplies by it).  Read by xpg_rf_normal only: n_models (every job
+ * of a normal-equations launch has that many models, p = n_models - 1 parameters) and robust_c (> 0: each spot's
+ * weight is w / (1 + (|r_0| / robust_c)^2), |r_0| the norm of the spot's three residuals under model 0 -- the Cauchy
+ * weights of xpg/refine.py:366-370; 0: w as given). */
 typedef struct xpg_rf_setup {
-  double qx, qy, sxy, sphi, tsens, mu, plx;
+  double qx, qy, sxy, sphi, tsens, mu, plx, robust_c;
   int32_t n_seg, n_models;
 } xpg_rf_setup;
 
 /* One job: the spots [spot0, spot0 + n_spots) of one problem under n_models consecutive models starting at model
  * `model0` (models + model0 model_doubles); its residuals go to r + r0 as (n_models, 3, n_spots).  t0 is the job's
- * first thread, the sum of n_models n_spots over the jobs before it (the jobs are in launch order). */
+ * first thread, the sum of n_models n_spots over the jobs before it (the jobs are in launch order).  w0: where the
+ * job's n_spots weights start in xpg_rf_normal's w. */
 typedef struct xpg_rf_job {
-  int64_t spot0, model0, r0, t0;
+  int64_t spot0, model0, r0, t0, w0;
   int32_t n_spots, n_models;
 } xpg_rf_job;
 
@@ -63,10 +66,12 @@ int xpg_rf_residuals(const xpg_rf_spots *sp, const xpg_rf_setup *su, const xpg_r
 /* The normal equations of forward-difference Jacobians (xpg/refine.py:185-216, 218-222), one per job: r as written by
  * xpg_rf_residuals for the job, model 0 the parameter vector and model 1 + k that vector with parameter k moved by
  * h[j p + k]; J[row][k] = (r_{1+k}[row] - r_0[row]) / h[j p + k];
- * out[j] = (J^T W J (p x p, row-major), J^T W r_0 (p)) over the job's rows, W = w[spot row] on each of a spot's three
- * rows.  Sums run in a fixed order (the same result on every call).  out holds n_jobs (p p + p) doubles. */
+ * out[j] = (J^T W J (p x p, row-major), J^T W r_0 (p)) over the job's rows, W = the weight of the job's spot i (w[w0 + i],
+ * or its Cauchy form, robust_c) on each of its three rows.  Sums run in a fixed order (the same result on every call).
+ * out holds n_jobs (p p + p) doubles.  r0_out (or NULL): job j's model-0 residuals copied to r0_out + 3 w0 as
+ * (3, n_spots), so one launch answers "residuals at x" and "normal equations at x" together. */
 int xpg_rf_normal(const xpg_rf_spots *sp, const xpg_rf_setup *su, const xpg_rf_job *jobs, int32_t n_jobs,
-                  const double *r, const double *h, const double *w, double *out, void *stream);
+                  const double *r, const double *h, const double *w, double *out, double *r0_out, void *stream);
 
 #ifdef __cplusplus
 }

@@ -61,7 +61,8 @@ def test_residuals_and_normal_equations_match_the_oracle(kind):
         np.testing.assert_allclose(out[0][:, 3 * len(x):], np.array([opar.gauge(v) for v in X]), atol=1e-12)
     # the normal equations at one vector with Cauchy-like weights
     w1 = np.random.default_rng(7).uniform(0.1, 1.0, len(x))
-    (Ad, gd), = xr._drive_once([xr._Normal(p1, X[0], w1)])
+    (Ad, gd, r0), = xr._drive_once([xr._Normal(p1, X[0], w1)])
+    assert np.array_equal(r0[0, :3 * len(x)], out[0][0, :3 * len(x)])       # the same kernel, the same vector
     Ao, go = orf.normal(opar, X[0], data, np.concatenate([np.tile(w1, 3), np.ones(ne)]))
     assert np.abs(Ad - Ao).max() < 1e-6 * np.abs(Ao).max()
     assert np.abs(gd - go).max() < 1e-6 * np.abs(go).max()
@@ -149,8 +150,11 @@ def test_entry_points_reject_bad_arguments():
     st.n = 0
     su.sxy = 0.0
     assert h.xpg_rf_residuals(C.byref(st), C.byref(su), None, 0, 0, None, None, None) != 0
-    su.sxy, su.n_models = 1.0, 1
-    assert h.xpg_rf_normal(C.byref(st), C.byref(su), None, 1, None, None, None, None, None) != 0
+    su.sxy = 1.0
+    assert h.xpg_rf_normal(C.byref(st), C.byref(su), None, 1, None, None, None, None, None, None) != 0
+    assert b"job table" in h.xpg_last_error()
+    su.n_models = 1
+    assert h.xpg_rf_normal(C.byref(st), C.byref(su), None, 0, None, None, None, None, None, None) != 0
     assert b"parameter" in h.xpg_last_error()
     su.n_models = 3
     assert h.xpg_rf_residuals(C.byref(st), C.byref(su), None, 0, 0, None, None, None) == 0   # nothing to do

@@ -81,19 +81,25 @@ class FakeLib:
                 out[o + 2 * nj:o + 3 * nj] = (phi - sp["phi"][i]) / su.sphi
         return 0
 
-    def xpg_rf_normal(self, stp, sup, jobs, n_jobs, r, h, w, out, stream):
+    def xpg_rf_normal(self, stp, sup, jobs, n_jobs, r, h, w, out, r0_out, stream):
         self.calls["normal"] += 1
         st, su = stp._obj, sup._obj
         jb = self._jobs(jobs, n_jobs)
         p = su.n_models - 1
         rr = _arr(r, (jb["r0"] + 3 * jb["n_models"].astype(np.int64) * jb["n_spots"]).max())
-        hh, ww = _arr(h, n_jobs * p).reshape(n_jobs, p), _arr(w, st.n)
+        hh, ww = _arr(h, n_jobs * p).reshape(n_jobs, p), _arr(w, (jb["w0"] + jb["n_spots"]).max())
         oo = _arr(out, n_jobs * (p * p + p)).reshape(n_jobs, -1)
         for k, j in enumerate(jb):
             nj = int(j["n_spots"])
             R = rr[int(j["r0"]):int(j["r0"]) + 3 * (p + 1) * nj].reshape(p + 1, 3 * nj)
             J = ((R[1:] - R[0]) / hh[k][:, None]).T
-            wv = np.tile(ww[int(j["spot0"]):int(j["spot0"]) + nj], 3)
+            wj = ww[int(j["w0"]):int(j["w0"]) + nj]
+            if su.robust_c > 0:
+                rn = np.sqrt((R[0].reshape(3, nj) ** 2).sum(0))
+                wj = wj / (1.0 + (rn / su.robust_c) ** 2)
+            wv = np.tile(wj, 3)
+            if r0_out is not None and r0_out.value:
+                _arr(r0_out.value + 8 * 3 * int(j["w0"]), 3 * nj)[:] = R[0]
             JW = J * wv[:, None]
             oo[k, :p * p] = (JW.T @ J).ravel()
             oo[k, p * p:] = JW.T @ R[0]
@@ -243,3 +249,24 @@ def test_one_launch_serves_several_problems(fake):
     assert np.abs(out[0][:, :want.shape[1]] - want).max() < 1e-9 and np.array_equal(out[2], out[0][2:])
     Ao, go = orf.normal(opar, X[1], data, np.ones(want.shape[1] + 6))
     assert np.abs(out[1][0] - Ao).max() < 1e-6 * np.abs(Ao).max()
+    again = xr._drive_once([xr._Res(p2, X[:1]), xr._Res(p1, X[:1])], device="cpu")   # a new pool: problems re-enter it
+    assert np.array_equal(again[0], out[0][:1]) and np.array_equal(again[1], out[0][:1])
+
+
+def test_device_cauchy_weights_equal_the_hosts(fake):
+    """A normal request with robust_c weights its spots by the Cauchy form of its own residuals, as the host loop
+    would from the same residuals; its residual rows come back with it."""
+    g = _geom()
+    pr = predict_reflections(g, A, 2.6, z_range=(0, 20))
+    x, y, z = _noisy(pr, 9)
+    exp = Experiment(replace(g, orgx=513.0), Crystal.from_Astar(A, "m"))
+    par, opar = xr.Parameterisation(exp, FLAGS), orf.Param(exp, FLAGS)
+    spots, setup, _ = xr._spot_data(exp, pr["hkl"], x, y, z, 0.5, 0.25)
+    data, _ = orf.spot_data(exp, pr["hkl"], x, y, z, 0.5, 0.25)
+    inl = (np.arange(len(x)) % 7 != 0).astype(float)
+    (A_, g_, r), = xr._drive_once([xr._Normal(xr._Problem(par, spots, setup), par.x0, inl, 3.0)], device="cpu")
+    ro = orf.full_residuals(opar, par.x0, data)
+    assert np.abs(r[0] - ro).max() < 1e-9
+    w1 = inl / (1.0 + (np.sqrt((ro.reshape(3, -1) ** 2).sum(0)) / 3.0) ** 2)
+    Ao, go = orf.normal(opar, par.x0, data, np.tile(w1, 3))
+    assert np.abs(A_ - Ao).max() < 1e-6 * np.abs(Ao).max() and np.abs(g_ - go).max() < 1e-6 * np.abs(go).max()

@@ -144,13 +144,13 @@ class RfSpots(C.Structure):
 
 
 class RfSetup(C.Structure):
-    _fields_ = [(k, C.c_double) for k in ("qx", "qy", "sxy", "sphi", "tsens", "mu", "plx")] + \
+    _fields_ = [(k, C.c_double) for k in ("qx", "qy", "sxy", "sphi", "tsens", "mu", "plx", "robust_c")] + \
         [(k, C.c_int32) for k in ("n_seg", "n_models")]
 
 
 class RfJob(C.Structure):
-    _fields_ = [(k, C.c_int64) for k in ("spot0", "model0", "r0", "t0")] + [("n_spots", C.c_int32),
-                                                                           ("n_models", C.c_int32)]
+    _fields_ = [(k, C.c_int64) for k in ("spot0", "model0", "r0", "t0", "w0")] + [("n_spots", C.c_int32),
+                                                                                 ("n_models", C.c_int32)]
 
 
 RF_HEAD, RF_SEG = 17, 9
@@ -204,7 +204,7 @@ _SIGS = {
     "xpg_rf_residuals": ([P(RfSpots), P(RfSetup), C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,
                           C.c_void_p], C.c_int),
     "xpg_rf_normal": ([P(RfSpots), P(RfSetup), C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                       C.c_void_p], C.c_int),
+                       C.c_void_p, C.c_void_p], C.c_int),
     "xpg_bslz4_encode_host": ([C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64],
                               C.c_int64),
 }

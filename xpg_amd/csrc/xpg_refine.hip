@@ -102,8 +102,18 @@ constexpr int NT = 256;
 
 // One workgroup per (job, entry): entries 0 .. np - 1 are the pairs k <= l of J^T W J, then the p entries of J^T W r_0.
 // Each thread sums its spots in order, then a fixed tree over the workgroup.
+__device__ __forceinline__ double spot_weight(const double *wj, const double *r0, int64_t nj, int64_t q, double c) {
+  const int64_t i = q % nj;
+  const double wi = wj[i];
+  if (c <= 0.0) return wi;
+  const double x = r0[i], y = r0[nj + i], z = r0[2 * nj + i];
+  const double t = sqrt((x * x + y * y) + z * z) / c;
+  return wi / (1.0 + t * t);
+}
+
 __global__ void __launch_bounds__(NT) k_rf_normal(xpg_rf_spots sp, xpg_rf_setup su, const xpg_rf_job *jobs,
-                                                  const double *r, const double *hd, const double *w, double *out) {
+                                                  const double *r, const double *hd, const double *w, double *out,
+                                                  double *r0_out) {
   __shared__ double red[NT];
   const int p = su.n_models - 1, j = blockIdx.y;
   const int np = p * (p + 1) / 2;
@@ -122,16 +132,35 @@ __global__ void __launch_bounds__(NT) k_rf_normal(xpg_rf_spots sp, xpg_rf_setup 
   const double hk = hd[(int64_t)j * p + k], hl = l >= 0 ? hd[(int64_t)j * p + l] : 1.0;
   const double *r0 = r + jb.r0, *rk = r0 + (int64_t)(1 + k) * 3 * nj;
   const double *rl = l >= 0 ? r0 + (int64_t)(1 + l) * 3 * nj : r0;
+  // the job's 3 n rows (row o: spot o mod n), four rows per thread in flight before any is used
+  const int64_t rows = 3 * nj;
+  const double *wj = w + jb.w0;
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < nj; i += NT) {
-    const double wi = w[jb.spot0 + i];
-    for (int c = 0; c < 3; ++c) {
-      const int64_t o = c * nj + i;
-      const double jk = (rk[o] - r0[o]) / hk;
-      const double other = l >= 0 ? (rl[o] - r0[o]) / hl : r0[o];
-      acc += (jk * wi) * other;
+  int64_t o = threadIdx.x;
+  for (; o + 3 * NT < rows; o += 4 * NT) {
+    double vk[4], vl[4], v0[4], vw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t q = o + u * NT;
+      vk[u] = rk[q];
+      vl[u] = rl[q];
+      v0[u] = r0[q];
+      vw[u] = spot_weight(wj, r0, nj, q, su.robust_c);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double jk = (vk[u] - v0[u]) / hk;
+      const double other = l >= 0 ? (vl[u] - v0[u]) / hl : v0[u];
+      acc += (jk * vw[u]) * other;
     }
   }
+  for (; o < rows; o += NT) {
+    const double jk = (rk[o] - r0[o]) / hk;
+    const double other = l >= 0 ? (rl[o] - r0[o]) / hl : r0[o];
+    acc += (jk * spot_weight(wj, r0, nj, o, su.robust_c)) * other;
+  }
+  if (r0_out && l < 0 && k == 0)                 // one workgroup per job returns the model-0 rows
+    for (int64_t q = threadIdx.x; q < rows; q += NT) r0_out[3 * jb.w0 + q] = r0[q];
   red[threadIdx.x] = acc;
   __syncthreads();
   for (int s = NT / 2; s > 0; s >>= 1) {
@@ -169,7 +198,8 @@ int launched() {
 }  // namespace
 
 // The job table lives in device memory, so its ranges are the caller's contract (the Python host checks them before
-// it uploads the table): spot0 + n_spots <= n, model0 + n_models within models, r0 + 3 n_models n_spots within r.
+// it uploads the table): spot0 + n_spots <= n, model0 + n_models within models, r0 + 3 n_models n_spots within r,
+// w0 + n_spots within w.
 extern "C" int xpg_rf_residuals(const xpg_rf_spots *sp, const xpg_rf_setup *su, const xpg_rf_job *jobs, int32_t n_jobs,
                                 int64_t total, const double *models, double *r, void *stream) {
   const int rc = check_common(sp, su, jobs, n_jobs);
@@ -185,7 +215,8 @@ extern "C" int xpg_rf_residuals(const xpg_rf_spots *sp, const xpg_rf_setup *su, 
 }
 
 extern "C" int xpg_rf_normal(const xpg_rf_spots *sp, const xpg_rf_setup *su, const xpg_rf_job *jobs, int32_t n_jobs,
-                             const double *r, const double *h, const double *w, double *out, void *stream) {
+                             const double *r, const double *h, const double *w, double *out, double *r0_out,
+                             void *stream) {
   const int rc = check_common(sp, su, jobs, n_jobs);
   if (rc != XPG_OK) return rc;
   if (su->n_models < 2) return xpg::report_error(XPG_ERR_ARG, "rf_normal: needs at least one parameter");
@@ -196,6 +227,6 @@ extern "C" int xpg_rf_normal(const xpg_rf_spots *sp, const xpg_rf_setup *su, con
   const int p = su->n_models - 1;
   const int entries = p * (p + 1) / 2 + p;
   hipLaunchKernelGGL(k_rf_normal, dim3(entries, n_jobs), dim3(NT), 0, (hipStream_t)stream, *sp, *su, jobs, r, h, w,
-                     out);
+                     out, r0_out);
   return launched();
 }

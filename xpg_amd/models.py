@@ -69,22 +69,23 @@ def b_matrix(cell):
 def skew(u):
     """Cross-product matrices of (..., 3) vectors."""
     u = np.asarray(u, float)
-    z = np.zeros_like(u[..., 0])
-    return np.stack([np.stack([z, -u[..., 2], u[..., 1]], -1), np.stack([u[..., 2], z, -u[..., 0]], -1),
-                     np.stack([-u[..., 1], u[..., 0], z], -1)], -2)
+    k = np.zeros(u.shape[:-1] + (3, 3))
+    k[..., 0, 1], k[..., 0, 2], k[..., 1, 2] = -u[..., 2], u[..., 1], -u[..., 0]
+    k[..., 1, 0], k[..., 2, 0], k[..., 2, 1] = u[..., 2], -u[..., 1], u[..., 0]
+    return k
 
 
 def rotvec_to_matrix(r):
     """Rotation matrices of rotation vectors (axis x angle, (..., 3)) by Rodrigues' formula, with the series
-    1 - t^2/6 and 1/2 - t^2/24 below t^2 = 1e-12 (xpg/geometry.py:52-65)."""
+    1 - t^2/6 and 1/2 - t^2/24 below t^2 = 1e-12 (xpg/geometry.py:52-65).  (t = sqrt(t^2 + 1e-30) keeps both
+    quotients finite, so no floating-point warning can arise.)"""
     r = np.asarray(r, float)
     th2 = np.sum(r * r, -1)[..., None, None]
     th = np.sqrt(th2 + 1e-30)
     k = skew(r)
     small = th2 < 1e-12
-    with np.errstate(invalid="ignore", divide="ignore"):
-        a = np.where(small, 1.0 - th2 / 6.0, np.sin(th) / th)
-        b = np.where(small, 0.5 - th2 / 24.0, (1.0 - np.cos(th)) / (th2 + 1e-30))
+    a = np.where(small, 1.0 - th2 / 6.0, np.sin(th) / th)
+    b = np.where(small, 0.5 - th2 / 24.0, (1.0 - np.cos(th)) / (th2 + 1e-30))
     return np.eye(3) + a * k + b * (k @ k)
 
 
