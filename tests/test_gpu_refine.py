@@ -158,3 +158,20 @@ def test_entry_points_reject_bad_arguments():
     assert b"parameter" in h.xpg_last_error()
     su.n_models = 3
     assert h.xpg_rf_residuals(C.byref(st), C.byref(su), None, 0, 0, None, None, None) == 0   # nothing to do
+
+
+def test_centroid_refinement_matches_the_oracle_per_block():
+    from test_refine_host import _centroid_table
+    exp, spots, rng_, delphi, Us = drifting_sweep(n_blocks=6)
+    edges = xb.block_edges(exp.geom, rng_, delphi)
+    refl = _centroid_table(exp, spots, Us, edges)
+    prior = xb.pos_prior(["POSITION"], exp.geom)
+    print(f"centroid refinement: {len(refl['xo'])} rows, 6 blocks", flush=True)
+    got = xb.refine_blocks_on_centroids([exp] * 6, edges, refl, ["POSITION", "BEAM", "ORIENTATION"], prior=prior)
+    ok = (refl["isum"] / np.sqrt(refl["vsum"]) > 10) & (refl["peak"] > 90) & (refl["n_over"] == 0)
+    for i in range(6):
+        m = ok & (refl["zo"] >= edges[i]) & (refl["zo"] < edges[i + 1])
+        want = orf.refine(exp, refl["hkl"][m], refl["xo"][m], refl["yo"][m], refl["zo"][m],
+                          ["POSITION", "BEAM", "ORIENTATION"], max_iter=15, n_reject_cycles=2, prior=prior)
+        np.testing.assert_allclose(got[i].crystal.U, want.exp.crystal.U, atol=1e-8)
+        assert got[i].geom.orgx == pytest.approx(want.exp.geom.orgx, abs=1e-5)

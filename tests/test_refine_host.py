@@ -14,7 +14,7 @@ from xpg_amd import _lib
 from xpg_amd import blocks as xb
 from xpg_amd import refine as xr
 from xpg_amd.geometry import Experiment, predict_reflections
-from xpg_amd.models import Crystal
+from xpg_amd.models import Crystal, b_matrix
 
 from test_refine_oracle import A, FLAGS, _geom, _modules, _noisy, drifting_sweep
 
@@ -270,3 +270,40 @@ def test_device_cauchy_weights_equal_the_hosts(fake):
     w1 = inl / (1.0 + (np.sqrt((ro.reshape(3, -1) ** 2).sum(0)) / 3.0) ** 2)
     Ao, go = orf.normal(opar, par.x0, data, np.tile(w1, 3))
     assert np.abs(A_ - Ao).max() < 1e-6 * np.abs(Ao).max() and np.abs(g_ - go).max() < 1e-6 * np.abs(go).max()
+
+
+def _centroid_table(exp, spots, Us, edges):
+    """A previous pass's reflection table for the drifting sweep: indices from each block's true orientation,
+    observed centroids, and strong / full / clean flags (a few rows failing each criterion)."""
+    x, y, z = spots
+    hkl = np.zeros((x.size, 3), int)
+    for i, U in enumerate(Us):
+        m = (z >= edges[i]) & (z < edges[i + 1])
+        A_i = U @ b_matrix(exp.crystal.cell)                 # block i's true reciprocal basis
+        hkl[m] = np.rint(exp.geom.spot_to_p0(x[m], y[m], z[m]) @ np.linalg.inv(A_i).T)
+    rng = np.random.default_rng(3)
+    n = x.size
+    refl = dict(hkl=hkl, xo=x, yo=y, zo=z, z=z + rng.normal(scale=0.05, size=n), isum=np.full(n, 1e4),
+                vsum=np.full(n, 1e4), peak=np.full(n, 99.0), n_over=np.zeros(n, int), rho=np.zeros(n))
+    refl["peak"][::9] = 50.0
+    refl["isum"][::11] = 5.0
+    refl["n_over"][::13] = 1
+    return refl
+
+
+def test_centroid_refinement_matches_the_oracle_per_block(fake):
+    exp, spots, rng_, delphi, Us = drifting_sweep(n_blocks=5)
+    edges = xb.block_edges(exp.geom, rng_, delphi)
+    refl = _centroid_table(exp, spots, Us, edges)
+    prior = xb.pos_prior(["POSITION"], exp.geom)
+    blocks = [exp] * 5
+    got = xb.refine_blocks_on_centroids(blocks, edges, refl, ["POSITION", "BEAM", "ORIENTATION", "CELL"], prior=prior,
+                                        device="cpu")
+    isg = refl["isum"] / np.sqrt(refl["vsum"])
+    ok = (isg > 10) & (refl["peak"] > 90) & (refl["n_over"] == 0)
+    for i in range(5):
+        m = ok & (refl["zo"] >= edges[i]) & (refl["zo"] < edges[i + 1])
+        want = orf.refine(blocks[i], refl["hkl"][m], refl["xo"][m], refl["yo"][m], refl["zo"][m],
+                          ["POSITION", "BEAM", "ORIENTATION"], max_iter=15, n_reject_cycles=2, prior=prior)
+        np.testing.assert_allclose(got[i].crystal.U, want.exp.crystal.U, atol=1e-8)
+        np.testing.assert_allclose(got[i].crystal.U, Us[i], atol=2e-4)
